@@ -1,0 +1,1068 @@
+// The editable DAG: context, fully-filled chain, the edit pipeline (top-down
+// edit-SVO construction, bottom-up merge into the hash-consed DAG), terrain
+// build, voxel queries and export.
+//
+// reference: MyGPUHashDAG (my_gpu_hash_dag.{h,cu}), create_edit_svo.cu,
+// my_gpu_hash_dag_edits.cu, MyGPUHashDAGUndoRedo (my_gpu_base_dag.h:16-82).
+//
+// Execution model: every kernel reads its item count from device memory and
+// walks it with a grid-stride loop, so one edit is a fixed sequence of
+// launches on one stream with no host round trip until the final status
+// read.  Work items of all levels live in one array (level L+1 starts where
+// level L ends); the SVO node of work item w is nodes[w], the SVO leaf of
+// leaf work item i is leaves[i].
+#include "dag_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+
+namespace svdag {
+
+__device__ inline bool leaf_is_full(const uint32_t* leaf, uint32_t M, uint32_t& mat)
+{
+    if (leaf[0] != 0xFFFFFFFFu || leaf[1] != 0xFFFFFFFFu)
+        return false;
+    if (M == 0) {
+        mat = 0;
+        return true;
+    }
+    const uint32_t m = leaf[2] & ((1u << M) - 1u);
+    const uint32_t rep = (M == 4) ? m * 0x11111111u : m * 0x01010101u;
+    const uint32_t words = (64u * M) / 32u;
+    for (uint32_t i = 0; i < words; ++i)
+        if (leaf[2 + i] != rep)
+            return false;
+    mat = m;
+    return true;
+}
+
+__device__ inline uint32_t dedup_cap(uint32_t count, uint32_t maxCap)
+{
+    uint32_t cap = 64;
+    while (cap < 2u * count && cap < maxCap)
+        cap <<= 1;
+    return cap;
+}
+
+__device__ inline bool merge_skipped(const WsDev& ws, const DagDev& dag)
+{
+    return ws.status->overflow != 0 || ws.lvl[dag.leafLevel].count == 0;
+}
+
+// ---------------------------------------------------------------- fully-filled chain
+// my_gpu_hash_dag.cu:38-63: a full leaf per material, then per level a node
+// with 8 copies of the level below, flagged full.
+__global__ void init_fully_filled(DagDev dag)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    uint32_t item[kMaxItemWords];
+    for (uint32_t m = 0; m < dag.numMaterials; ++m) {
+        item[0] = 0xFFFFFFFFu;
+        item[1] = 0xFFFFFFFFu;
+        const uint32_t matWords = (64u * dag.M) / 32u;
+        const uint32_t rep = dag.M == 0 ? 0u : (dag.M == 4 ? m * 0x11111111u : m * 0x01010101u);
+        for (uint32_t i = 0; i < matWords; ++i)
+            item[2 + i] = rep;
+        const uint32_t s = 2 + matWords;
+        uint32_t h = 0;
+        for (uint32_t j = 0; j < s; ++j)
+            h ^= hash_part(item[j], j);
+        const TableDev& t = dag.tables[s - 2];
+        uint32_t r = table_find_lane(t, item, s, h);
+        if (r == kNotFound)
+            r = table_insert_lane(t, item, s, h);
+        dag.fullyFilled[dag.leafLevel * dag.numMaterials + m] = leaf_handle(dag, r, s);
+    }
+    for (int level = (int)dag.leafLevel - 1; level >= 0; --level) {
+        for (uint32_t m = 0; m < dag.numMaterials; ++m) {
+            item[0] = header_make_full(0xFFu, m, dag.M);
+            for (uint32_t c = 0; c < 8; ++c)
+                item[1 + c] = dag.fullyFilled[(level + 1) * dag.numMaterials + m];
+            uint32_t h = 0;
+            for (uint32_t j = 0; j < 9; ++j)
+                h ^= hash_part(item[j], j);
+            const TableDev& t = dag.tables[9 - 2];
+            uint32_t r = table_find_lane(t, item, 9, h);
+            if (r == kNotFound)
+                r = table_insert_lane(t, item, 9, h);
+            dag.fullyFilled[level * dag.numMaterials + m] = node_handle(r, 9);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- edit kernels
+__global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < kMaxLevels) {
+        LevelDesc d {};
+        if (t == 0)
+            d.count = 1;
+        ws.lvl[t] = d;
+    }
+    if (t == 0) {
+        const uint32_t r = *root;
+        EditStatus st {};
+        st.rootBefore = r;
+        st.rootAfter = r;
+        *ws.status = st;
+        ws.work[0] = make_uint4(0u, 0u, 0u, r);
+        if (isCopy) {
+            CopyState cs;
+            for (int k = 0; k < 8; ++k)
+                cs.p[k] = CopyParent { 0u, 0u, 0u, kSentinel };
+            cs.p[0].h = r;
+            ws.state[0] = cs;
+        }
+    }
+}
+
+// One work item per 8 lanes, one lane per child (create_edit_svo.cu:167-303).
+__global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, EditorDev ed, uint32_t L)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t c = lane & 7u;
+    const uint32_t gbase = lane & ~7u;
+    const LevelDesc me = ws.lvl[L];
+    const uint32_t childStart = me.start + me.count;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ws.lvl[L + 1].start = childStart;
+    const uint32_t childLevel = L + 1;
+    const uint32_t childDepth = dag.levels - childLevel;
+    const bool leafChildren = childLevel == dag.leafLevel;
+    const uint32_t groupsPerBlock = blockDim.x / 8u;
+    // After an overflow the count may run past the array; the edit is
+    // discarded then, but no lane may read or write beyond capWork.
+    const uint32_t count = me.start >= ws.capWork ? 0u : min(me.count, ws.capWork - me.start);
+    for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / 8u; i < count; i += gridDim.x * groupsPerBlock) {
+        const uint32_t w = me.start + i;
+        const uint4 item = ws.work[w];
+        uint32_t childHandle = kSentinel;
+        if (item.w != kSentinel) {
+            const uint32_t* node = dag_node_ptr(dag, item.w);
+            const uint32_t mask = node[0] & 0xFFu;
+            if ((mask >> c) & 1u)
+                childHandle = node[1 + __builtin_popcount(mask & ((1u << c) - 1u))];
+        }
+        const uint32_t cx = (item.x << 1) | ((c >> 2) & 1u);
+        const uint32_t cy = (item.y << 1) | ((c >> 1) & 1u);
+        const uint32_t cz = (item.z << 1) | (c & 1u);
+        NState st = NS_PARTIAL;
+        uint32_t stMat = 0;
+        if (childHandle == kSentinel)
+            st = NS_EMPTY;
+        else if (!leafChildren)
+            st = header_full(dag_node_ptr(dag, childHandle)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
+        uint32_t fillMat = 0;
+        Op op;
+        CopyState cs;
+        if (ed.kind == EK_COPY)
+            op = decide_copy(ed, dag, cx, cy, cz, childDepth, ws.state[w], cs);
+        else
+            op = decide(ed, cx, cy, cz, childDepth, st, fillMat);
+        const bool sub = op == OP_SUBDIVIDE;
+        const uint32_t subMask = group_ballot<8>(sub, gbase);
+        uint32_t base = 0;
+        if (c == 0 && subMask)
+            base = atomicAdd(&ws.lvl[L + 1].count, (uint32_t)__builtin_popcount(subMask));
+        base = group_bcast<8>(base, 0);
+        if (sub) {
+            const uint32_t local = base + __builtin_popcount(subMask & ((1u << c) - 1u));
+            const uint64_t g = (uint64_t)childStart + local;
+            if (g < ws.capWork && (!leafChildren || local < ws.capLeaf)) {
+                ws.work[g] = make_uint4(cx, cy, cz, childHandle);
+                if (ed.kind == EK_COPY)
+                    ws.state[g] = cs;
+            } else {
+                atomicOr(&ws.status->overflow, 1u);
+            }
+            childHandle = local;
+        } else if (op == OP_EMPTY) {
+            childHandle = kSentinel;
+        } else if (op == OP_FILL) {
+            childHandle = dag.fullyFilled[childLevel * dag.numMaterials + fillMat];
+        }
+        const bool out = childHandle != kSentinel;
+        const uint32_t cmask = group_ballot<8>(out, gbase);
+        uint32_t* rec = ws.nodes + (size_t)w * kNodeStride;
+        const uint32_t n = __builtin_popcount(cmask);
+        if (out)
+            rec[1 + __builtin_popcount(cmask & ((1u << c) - 1u))] = childHandle;
+        if (c >= n)
+            rec[1 + c] = 0u;
+        if (c == 0) {
+            rec[0] = cmask ? header_make(cmask, subMask, dag.M) : 0u;
+            rec[9] = 0u;
+        }
+    }
+}
+
+// One leaf per wave64, one lane per voxel (create_edit_svo.cu:320-426).
+__global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, EditorDev ed)
+{
+    __shared__ uint32_t lds[4][16];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = threadIdx.x >> 6;
+    const LevelDesc me = ws.lvl[dag.leafLevel];
+    const uint32_t wavesPerBlock = blockDim.x / 64u;
+    const uint32_t count = min(me.count, ws.capLeaf);
+    for (uint32_t i = blockIdx.x * wavesPerBlock + wv; i < count; i += gridDim.x * wavesPerBlock) {
+        const uint32_t w = me.start + i;
+        if (w >= ws.capWork)
+            break;
+        const uint4 item = ws.work[w];
+        bool hasOld = false;
+        uint32_t oldMat = 0;
+        if (item.w != kSentinel)
+            hasOld = leaf_material(dag_leaf_ptr(dag, item.w), lane, dag.M, oldMat);
+        const uint32_t vx = (item.x << 2) | (((lane >> 5) & 1u) << 1) | ((lane >> 2) & 1u);
+        const uint32_t vy = (item.y << 2) | (((lane >> 4) & 1u) << 1) | ((lane >> 1) & 1u);
+        const uint32_t vz = (item.z << 2) | (((lane >> 3) & 1u) << 1) | (lane & 1u);
+        uint32_t fm;
+        const Op op = decide(ed, vx, vy, vz, 0, hasOld ? NS_FILLED : NS_EMPTY, fm);
+        bool has = false;
+        uint32_t mat = 0;
+        if (op == OP_FILL || op == OP_SUBDIVIDE) {
+            if (ed.kind == EK_COPY)
+                has = copy_value(ed, dag, vx, vy, vz, ws.state[w], hasOld, oldMat, mat);
+            else
+                has = new_value(ed, hasOld, oldMat, mat);
+        } else if (op == OP_KEEP) {
+            has = hasOld;
+            mat = oldMat;
+        }
+        const unsigned long long mask = __ballot(has);
+        if (lane < 16)
+            lds[wv][lane] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (has && dag.M) {
+            const uint32_t off = (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull)) * dag.M;
+            atomicOr(&lds[wv][off >> 5], mat << (off & 31u));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t* rec = ws.leaves + (size_t)i * ws.leafStride;
+        if (lane < ws.leafStride) {
+            uint32_t v;
+            if (lane == 0)
+                v = (uint32_t)mask;
+            else if (lane == 1)
+                v = (uint32_t)(mask >> 32);
+            else
+                v = (lane - 2 < 16) ? lds[wv][lane - 2] : 0u;
+            rec[lane] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Bottom-up, per level: clear the level's dedup table; map SVO children to
+// DAG handles and drop empty ones (updateChildPointers,
+// my_gpu_hash_dag_edits.cu:44-95); flag full items when the level below had
+// any (discover_fully_filled, 96-150, gated at 337).
+__global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint32_t L)
+{
+    if (merge_skipped(ws, dag))
+        return;
+    const LevelDesc me = ws.lvl[L];
+    const uint32_t cap = dedup_cap(me.count, ws.dedupCap);
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = tid; i < cap; i += stride)
+        ws.dedup[i] = 0ull;
+    if (L == dag.leafLevel) {
+        for (uint32_t i = tid; i < me.count; i += stride) {
+            uint32_t m;
+            if (leaf_is_full(ws.leaves + (size_t)i * ws.leafStride, dag.M, m))
+                atomicAdd(&ws.lvl[L].fullCount, 1u);
+        }
+        return;
+    }
+    const bool gate = ws.lvl[L + 1].fullCount != 0;
+    const uint32_t childStart = ws.lvl[L + 1].start;
+    const bool leafChildren = L + 1 == dag.leafLevel;
+    for (uint32_t i = tid; i < me.count; i += stride) {
+        uint32_t* rec = ws.nodes + (size_t)(me.start + i) * kNodeStride;
+        const uint32_t hdr = rec[0];
+        const uint32_t mask = hdr & 0xFFu;
+        if (mask == 0)
+            continue;
+        const uint32_t imask = header_payload(hdr, dag.M) & 0xFFu;
+        uint32_t out[8];
+        uint32_t n = 0, newMask = 0, k = 0;
+        for (uint32_t c = 0; c < 8; ++c) {
+            if (!((mask >> c) & 1u))
+                continue;
+            uint32_t h = rec[1 + k++];
+            if ((imask >> c) & 1u) {
+                const uint32_t r = ws.rep[childStart + h];
+                h = (r == kSentinel) ? kSentinel : ws.handle[r];
+            }
+            if (h != kSentinel) {
+                out[n++] = h;
+                newMask |= 1u << c;
+            }
+        }
+        bool full = false;
+        uint32_t fm = 0;
+        if (gate && newMask == 0xFFu) {
+            full = true;
+            for (uint32_t j = 0; j < 8 && full; ++j) {
+                uint32_t m = 0;
+                const bool f = leafChildren ? leaf_is_full(dag_leaf_ptr(dag, out[j]), dag.M, m)
+                                            : header_full(dag_node_ptr(dag, out[j])[0], dag.M, m);
+                if (j == 0)
+                    fm = m;
+                full = f && m == fm;
+            }
+        }
+        rec[0] = full ? header_make_full(0xFFu, fm, dag.M) : header_make(newMask, 0u, dag.M);
+        for (uint32_t j = 0; j < 8; ++j)
+            rec[1 + j] = j < n ? out[j] : 0u;
+        if (full)
+            atomicAdd(&ws.lvl[L].fullCount, 1u);
+    }
+}
+
+// Per level: collapse equal SVO items onto one representative through a
+// transient open-addressing table of (hash, index) keys -- contents are
+// compared in place, so no slot ever waits on another writer -- and look
+// each representative up in the DAG (insertDuplicates / findUnique1 /
+// findUnique2, my_gpu_hash_dag_edits.cu:164-246).
+template <int G>
+__global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, uint32_t L)
+{
+    if (merge_skipped(ws, dag))
+        return;
+    const LevelDesc me = ws.lvl[L];
+    const bool isLeaf = L == dag.leafLevel;
+    const uint32_t capMask = dedup_cap(me.count, ws.dedupCap) - 1u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const uint32_t groupsPerBlock = blockDim.x / G;
+    for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / G; i < me.count; i += gridDim.x * groupsPerBlock) {
+        const uint32_t w = me.start + i;
+        const uint32_t* rec = isLeaf ? ws.leaves + (size_t)i * ws.leafStride : ws.nodes + (size_t)w * kNodeStride;
+        const uint32_t w0 = rec[0], w1 = rec[1];
+        const bool empty = isLeaf ? (w0 == 0u && w1 == 0u) : ((w0 & 0xFFu) == 0u);
+        if (empty) {
+            if (gl == 0)
+                ws.rep[w] = kSentinel;
+            continue;
+        }
+        const uint32_t s = isLeaf ? leaf_size_from_words(w0, w1, dag.M) : node_size_from_header(w0);
+        Needle<G> n;
+        needle_load<G>(n, rec, s, gl);
+        const uint32_t h = needle_hash<G>(n, s, gl);
+        const unsigned long long key = ((unsigned long long)h << 32) | (unsigned long long)(w + 1u);
+        uint32_t p = mix32(h) & capMask;
+        uint32_t rep = w;
+        while (true) {
+            unsigned long long v = 0;
+            if (gl == 0)
+                v = atomicCAS(&ws.dedup[p], 0ull, key);
+            const uint32_t vlo = group_bcast<G>((uint32_t)v, 0);
+            const uint32_t vhi = group_bcast<G>((uint32_t)(v >> 32), 0);
+            if (vlo == 0u && vhi == 0u)
+                break;
+            if (vhi == h) {
+                const uint32_t j = vlo - 1u;
+                const uint32_t* other = isLeaf ? ws.leaves + (size_t)(j - me.start) * ws.leafStride
+                                               : ws.nodes + (size_t)j * kNodeStride;
+                const bool eq = needle_equal_lane<G>(n, other, s, gl);
+                if (group_ballot<G>(!eq, gbase) == 0) {
+                    rep = j;
+                    break;
+                }
+            }
+            p = (p + 1u) & capMask;
+        }
+        if (gl == 0)
+            ws.rep[w] = rep;
+        if (rep == w) {
+            const uint32_t r = table_find_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
+            if (gl == 0)
+                ws.handle[w] = (r == kNotFound) ? kNotFound : (isLeaf ? leaf_handle(dag, r, s) : node_handle(r, s));
+        }
+    }
+}
+
+// Insert the representatives the DAG did not hold (insertUniqueInHashTableAsWarp,
+// my_gpu_hash_dag_edits.cu:248-280).  All contents are distinct, so inserts
+// never race on equal items, and no find runs in this kernel.
+template <int G>
+__global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32_t L)
+{
+    if (merge_skipped(ws, dag))
+        return;
+    const LevelDesc me = ws.lvl[L];
+    const bool isLeaf = L == dag.leafLevel;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const uint32_t groupsPerBlock = blockDim.x / G;
+    for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / G; i < me.count; i += gridDim.x * groupsPerBlock) {
+        const uint32_t w = me.start + i;
+        if (ws.rep[w] != w || ws.handle[w] != kNotFound)
+            continue;
+        const uint32_t* rec = isLeaf ? ws.leaves + (size_t)i * ws.leafStride : ws.nodes + (size_t)w * kNodeStride;
+        const uint32_t w0 = rec[0], w1 = rec[1];
+        const uint32_t s = isLeaf ? leaf_size_from_words(w0, w1, dag.M) : node_size_from_header(w0);
+        Needle<G> n;
+        needle_load<G>(n, rec, s, gl);
+        const uint32_t h = needle_hash<G>(n, s, gl);
+        const uint32_t r = table_insert_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
+        if (gl == 0) {
+            if (r == kNotFound) {
+                atomicOr(&ws.status->oom, 1u);
+                ws.handle[w] = kSentinel;
+            } else {
+                ws.handle[w] = isLeaf ? leaf_handle(dag, r, s) : node_handle(r, s);
+            }
+        }
+    }
+}
+
+// Publish the new root (my_gpu_hash_dag_edits.cu:626) unless the merge was
+// skipped (no leaf, hazard kept from :651) or failed.
+__global__ void edit_finish(WsDev ws, DagDev dag, uint32_t* root)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    EditStatus* st = ws.status;
+    st->leafCount = ws.lvl[dag.leafLevel].count;
+    if (st->overflow || st->leafCount == 0 || st->oom) {
+        st->applied = 0;
+        st->rootAfter = st->rootBefore;
+        return;
+    }
+    const uint32_t r = ws.rep[ws.lvl[0].start];
+    const uint32_t newRoot = (r == kSentinel) ? kSentinel : ws.handle[r];
+    st->applied = 1;
+    st->rootAfter = newRoot;
+    *root = newRoot;
+}
+
+// ---------------------------------------------------------------- queries
+// DAGUtils::get_value (dag_utils.h:146-182).
+__global__ void query_voxels_kernel(DagDev dag, const uint32_t* root, const uint32_t* xyz, uint32_t n, uint32_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    uint32_t h = *root;
+    uint32_t result = kSentinel;
+    if (h != kSentinel) {
+        for (uint32_t level = 0; level < dag.leafLevel; ++level) {
+            const uint32_t* node = dag_node_ptr(dag, h);
+            const uint32_t mask = node[0] & 0xFFu;
+            const uint32_t sh = dag.levels - (level + 1);
+            const uint32_t c = (((x >> sh) & 1u) << 2) | (((y >> sh) & 1u) << 1) | ((z >> sh) & 1u);
+            if (!((mask >> c) & 1u)) {
+                h = kSentinel;
+                break;
+            }
+            h = node[1 + __builtin_popcount(mask & ((1u << c) - 1u))];
+        }
+        if (h != kSentinel) {
+            const uint32_t v = ((x & 1u) << 2) | ((y & 1u) << 1) | (z & 1u) | ((x & 2u) << 4) | ((y & 2u) << 3)
+                | ((z & 2u) << 2);
+            uint32_t m;
+            if (leaf_material(dag_leaf_ptr(dag, h), v, dag.M, m))
+                result = m;
+        }
+    }
+    out[i] = result;
+}
+
+// ---------------------------------------------------------------- terrain
+// Seeded 2D Perlin fBm in double precision (only +, -, *, / and floor, so the
+// host and device evaluate it bit-identically with contraction off).
+struct PerlinDev {
+    uint32_t perm[512];
+};
+
+__host__ __device__ inline double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+__host__ __device__ inline double lerp_d(double t, double a, double b) { return a + t * (b - a); }
+__host__ __device__ inline double grad2(uint32_t hsh, double x, double z)
+{
+    switch (hsh & 7u) {
+    case 0: return x + z;
+    case 1: return -x + z;
+    case 2: return x - z;
+    case 3: return -x - z;
+    case 4: return x;
+    case 5: return -x;
+    case 6: return z;
+    default: return -z;
+    }
+}
+__host__ __device__ inline double perlin2(const uint32_t* P, double x, double z)
+{
+    const double X = floor(x), Z = floor(z);
+    const uint32_t xi = (uint32_t)(int64_t)X & 255u, zi = (uint32_t)(int64_t)Z & 255u;
+    const double xf = x - X, zf = z - Z;
+    const double u = fade(xf), v = fade(zf);
+    const uint32_t aa = P[P[xi] + zi], ab = P[P[xi] + zi + 1], ba = P[P[xi + 1] + zi], bb = P[P[xi + 1] + zi + 1];
+    const double x1 = lerp_d(u, grad2(aa, xf, zf), grad2(ba, xf - 1.0, zf));
+    const double x2 = lerp_d(u, grad2(ab, xf, zf - 1.0), grad2(bb, xf - 1.0, zf - 1.0));
+    return lerp_d(v, x1, x2);
+}
+
+__global__ void terrain_heights_kernel(PerlinDev perlin, svdag_terrain tp, uint32_t side, int32_t* out)
+{
+    const uint64_t n = (uint64_t)side * side;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % side), z = (uint32_t)(i / side);
+        double sum = 0.0, amp = 1.0, freq = 1.0, norm = 0.0;
+        for (uint32_t o = 0; o < tp.octaves; ++o) {
+            sum = sum + amp * perlin2(perlin.perm, ((double)x * tp.frequency) * freq, ((double)z * tp.frequency) * freq);
+            norm = norm + amp;
+            amp = amp * 0.5;
+            freq = freq * 2.0;
+        }
+        const double hgt = tp.base + tp.amplitude * (sum / norm);
+        out[i] = (int32_t)floor(hgt);
+    }
+}
+
+__global__ void pyramid_level_kernel(const int32_t* srcMin, const int32_t* srcMax, uint32_t srcSide, int32_t* dstMin,
+                                     int32_t* dstMax)
+{
+    const uint32_t side = srcSide / 2;
+    const uint64_t n = (uint64_t)side * side;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % side), z = (uint32_t)(i / side);
+        const uint64_t a = (uint64_t)(2 * z) * srcSide + 2 * x, b = a + srcSide;
+        dstMin[i] = min(min(srcMin[a], srcMin[a + 1]), min(srcMin[b], srcMin[b + 1]));
+        dstMax[i] = max(max(srcMax[a], srcMax[a + 1]), max(srcMax[b], srcMax[b + 1]));
+    }
+}
+
+// splitmix64-driven Fisher-Yates shuffle of 0..255, duplicated to 512 entries.
+void perlin_perm(uint64_t seed, uint32_t* perm)
+{
+    uint32_t p[256];
+    for (uint32_t i = 0; i < 256; ++i)
+        p[i] = i;
+    uint64_t s = seed;
+    for (uint32_t i = 255; i >= 1; --i) {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const uint32_t j = (uint32_t)(z % (i + 1));
+        std::swap(p[i], p[j]);
+    }
+    for (uint32_t i = 0; i < 512; ++i)
+        perm[i] = p[i & 255u];
+}
+
+} // namespace svdag
+
+// ======================================================================== host
+namespace svdag {
+
+static uint32_t bit_width(uint32_t v)
+{
+    uint32_t n = 0;
+    while (v) {
+        ++n;
+        v >>= 1;
+    }
+    return n;
+}
+
+static void ws_release(WsDev& ws)
+{
+    for (void* p : { (void*)ws.work, (void*)ws.state, (void*)ws.nodes, (void*)ws.leaves, (void*)ws.rep,
+                     (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status })
+        if (p)
+            (void)hipFree(p);
+    ws = WsDev {};
+}
+
+static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
+{
+    WsDev old = ctx->ws;
+    const bool hadState = old.state != nullptr;
+    ws_release(ctx->ws);
+    (void)old;
+    WsDev& ws = ctx->ws;
+    if (capWork > 0x7FFFFFFFull)
+        capWork = 0x7FFFFFFFull;
+    ws.capWork = (uint32_t)capWork;
+    ws.capLeaf = (uint32_t)capWork;
+    uint64_t dc = 64;
+    while (dc < 2 * capWork)
+        dc <<= 1;
+    ws.dedupCap = (uint32_t)std::min<uint64_t>(dc, 1ull << 31);
+    ws.leafStride = ctx->dag.maxLeafWords;
+    SVDAG_HIP(hipMalloc(&ws.work, (size_t)capWork * sizeof(uint4)));
+    if (withState || hadState)
+        SVDAG_HIP(hipMalloc(&ws.state, (size_t)capWork * sizeof(CopyState)));
+    SVDAG_HIP(hipMalloc(&ws.nodes, (size_t)capWork * kNodeStride * 4));
+    SVDAG_HIP(hipMalloc(&ws.leaves, (size_t)capWork * ws.leafStride * 4));
+    SVDAG_HIP(hipMalloc(&ws.rep, (size_t)capWork * 4));
+    SVDAG_HIP(hipMalloc(&ws.handle, (size_t)capWork * 4));
+    SVDAG_HIP(hipMalloc(&ws.dedup, (size_t)ws.dedupCap * 8));
+    SVDAG_HIP(hipMalloc(&ws.lvl, kMaxLevels * sizeof(LevelDesc)));
+    SVDAG_HIP(hipMalloc(&ws.status, sizeof(EditStatus)));
+    SVDAG_HIP(hipMemsetAsync(ws.dedup, 0, (size_t)ws.dedupCap * 8, ctx->stream));
+    return SVDAG_OK;
+}
+
+int ctx_refresh_tables(svdag_ctx* ctx)
+{
+    SVDAG_TRY(ctx->ts.upload(ctx->stream));
+    ctx->dag.tables = ctx->ts.devTables;
+    return SVDAG_OK;
+}
+
+// Voxel bounds [lo, hi] of an edit, used only to size launch grids.
+struct Bounds {
+    int64_t lo[3], hi[3];
+};
+
+static Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e)
+{
+    const int64_t side = (int64_t)1 << ctx->dag.levels;
+    Bounds b {};
+    for (int a = 0; a < 3; ++a) {
+        double lo = 0, hi = (double)(side - 1);
+        if (e.kind == EK_SPHERE_ADD || e.kind == EK_SPHERE_REMOVE || e.kind == EK_PAINT) {
+            const double c = a == 0 ? e.cx : (a == 1 ? e.cy : e.cz);
+            lo = c - e.radius - 2.0;
+            hi = c + e.radius + 2.0;
+        } else if (e.kind == EK_BOX_ADD || e.kind == EK_BOX_REMOVE || e.kind == EK_COPY) {
+            lo = e.bmin[a] - 2.0;
+            hi = e.bmax[a] + 2.0;
+        }
+        b.lo[a] = std::max<int64_t>(0, (int64_t)std::floor(lo));
+        b.hi[a] = std::min<int64_t>(side - 1, (int64_t)std::ceil(hi));
+        if (b.hi[a] < b.lo[a])
+            b.hi[a] = b.lo[a];
+    }
+    return b;
+}
+
+static uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
+{
+    const uint32_t sh = ctx->dag.levels - L;
+    uint64_t n = 1;
+    for (int a = 0; a < 3; ++a)
+        n *= (uint64_t)((b.hi[a] >> sh) - (b.lo[a] >> sh) + 1);
+    return n;
+}
+
+static int launch_edit(svdag_ctx* ctx, const EditorDev& ed)
+{
+    const DagDev& dag = ctx->dag;
+    const WsDev& ws = ctx->ws;
+    const hipStream_t st = ctx->stream;
+    const Bounds b = edit_bounds(ctx, ed);
+    const uint32_t kCap = 8192;
+    edit_begin<<<1, 64, 0, st>>>(ws, ctx->dRoot, ed.kind == EK_COPY ? 1u : 0u);
+    for (uint32_t L = 0; L < dag.leafLevel; ++L) {
+        const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, L), ws.capWork);
+        svo_inner_level<<<grid_for(items, 32, kCap), 256, 0, st>>>(ws, dag, ed, L);
+    }
+    {
+        const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, dag.leafLevel), ws.capLeaf);
+        svo_leaf_level<<<grid_for(items, 4, kCap), 256, 0, st>>>(ws, dag, ed);
+    }
+    const uint32_t G = ctx->groupSize;
+    for (int L = (int)dag.leafLevel; L >= 0; --L) {
+        const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork);
+        merge_prepare<<<grid_for(std::max<uint64_t>(items, 2 * items), 256, kCap), 256, 0, st>>>(ws, dag, (uint32_t)L);
+        const uint32_t gridG = grid_for(items, 256 / G, kCap);
+        switch (G) {
+        case 4:
+            merge_dedup_find<4><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            merge_insert<4><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            break;
+        case 8:
+            merge_dedup_find<8><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            merge_insert<8><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            break;
+        case 32:
+            merge_dedup_find<32><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            merge_insert<32><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            break;
+        default:
+            merge_dedup_find<16><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            merge_insert<16><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
+            break;
+        }
+    }
+    edit_finish<<<1, 64, 0, st>>>(ws, dag, ctx->dRoot);
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+// Run one edit to completion: retries with a larger workspace on overflow
+// and with larger slab arenas on OOM (nothing is merged in either case).
+static int run_edit(svdag_ctx* ctx, const EditorDev& ed, svdag_edit_stats* stats)
+{
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        if (ed.kind == EK_COPY && !ctx->ws.state)
+            SVDAG_TRY(ws_alloc(ctx, ctx->ws.capWork, true));
+        SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        SVDAG_TRY(launch_edit(ctx, ed));
+        SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(ctx->hStatus, ctx->ws.status, sizeof(EditStatus), hipMemcpyDeviceToHost, ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(ctx->hLevels, ctx->ws.lvl, kMaxLevels * sizeof(LevelDesc), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+        const EditStatus s = *ctx->hStatus;
+        if (s.overflow) {
+            uint64_t need = 0;
+            for (uint32_t L = 0; L <= ctx->dag.leafLevel; ++L)
+                need = std::max<uint64_t>(need, (uint64_t)ctx->hLevels[L].start + ctx->hLevels[L].count);
+            uint64_t cap = std::max<uint64_t>(2ull * ctx->ws.capWork, need + need / 4);
+            SVDAG_TRY(ws_alloc(ctx, cap, ed.kind == EK_COPY));
+            continue;
+        }
+        if (s.oom) {
+            for (auto& t : ctx->ts.tables) {
+                AllocState a {};
+                SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
+                if (a.oom)
+                    SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, 1ull << 25), ctx->stream));
+            }
+            SVDAG_TRY(ctx_refresh_tables(ctx));
+            continue;
+        }
+        ctx->root = s.rootAfter;
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            for (uint32_t L = 0; L <= ctx->dag.leafLevel && L < 32; ++L)
+                stats->svo_items[L] = ctx->hLevels[L].count;
+            stats->applied = s.applied;
+            stats->root_before = s.rootBefore;
+            stats->root_after = s.rootAfter;
+            float ms = 0;
+            SVDAG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            stats->gpu_ms = ms;
+        }
+        return SVDAG_OK;
+    }
+    set_error("edit did not fit after repeated workspace / arena growth");
+    return SVDAG_E_CAPACITY;
+}
+
+static int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e)
+{
+    e = EditorDev {};
+    e.kind = d.kind;
+    e.material = d.material;
+    if (d.kind > SVDAG_COPY) {
+        set_error("unknown edit kind");
+        return SVDAG_E_INVALID;
+    }
+    if (d.material >= ctx->dag.numMaterials) {
+        set_error("material does not fit the material bits");
+        return SVDAG_E_INVALID;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(d.center[a]) || !std::isfinite(d.src[a]) || !std::isfinite(d.dst[a])) {
+            set_error("non-finite edit coordinates");
+            return SVDAG_E_INVALID;
+        }
+    if (!std::isfinite(d.radius) || d.radius < 0.0f) {
+        set_error("radius must be finite and >= 0");
+        return SVDAG_E_INVALID;
+    }
+    // MyGpuSphereEditorBase (my_gpu_dag_editors.h:406-411).
+    e.cx = d.center[0] + 0.5f;
+    e.cy = d.center[1] + 0.5f;
+    e.cz = d.center[2] + 0.5f;
+    e.radius = std::max(d.radius, 0.1f);
+    e.r2 = e.radius * e.radius;
+    // MyGpuBoxEditorBase (50-55); the copy tool centres it at dst + (center - src) (157-164).
+    float c[3] = { d.center[0], d.center[1], d.center[2] };
+    if (d.kind == SVDAG_COPY) {
+        for (int a = 0; a < 3; ++a) {
+            c[a] = d.dst[a] + (d.center[a] - d.src[a]);
+            e.d2s[a] = -d.dst[a] + d.src[a];
+            e.id2s[a] = (int32_t)e.d2s[a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        e.bmin[a] = c[a] - d.radius;
+        e.bmax[a] = d.radius == 0.0f ? c[a] + 1.0f : c[a] + d.radius;
+    }
+    return SVDAG_OK;
+}
+
+} // namespace svdag
+
+using namespace svdag;
+
+extern "C" {
+
+int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
+{
+    if (!cfgIn || !out) {
+        set_error("svdag_create: null argument");
+        return SVDAG_E_INVALID;
+    }
+    svdag_config cfg = *cfgIn;
+    if (cfg.levels < 4 || cfg.levels > 20) {
+        set_error("levels must be in [4, 20]");
+        return SVDAG_E_INVALID;
+    }
+    if (cfg.material_bits != 0 && cfg.material_bits != 4 && cfg.material_bits != 8) {
+        set_error("material_bits must be 0, 4 or 8");
+        return SVDAG_E_INVALID;
+    }
+    if (cfg.group_size != 0 && cfg.group_size != 4 && cfg.group_size != 8 && cfg.group_size != 16 && cfg.group_size != 32) {
+        set_error("group_size must be 0, 4, 8, 16 or 32");
+        return SVDAG_E_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device visible");
+        return SVDAG_E_NO_DEVICE;
+    }
+    if (cfg.load_factor == 0)
+        cfg.load_factor = 96;
+    if (cfg.expected_items == 0)
+        cfg.expected_items = 1ull << 20;
+    if (cfg.workspace_items == 0)
+        cfg.workspace_items = 1ull << 22;
+    auto* ctx = new svdag_ctx();
+    ctx->cfg = cfg;
+    ctx->groupSize = cfg.group_size ? cfg.group_size : 16;
+    SVDAG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    SVDAG_HIP(hipEventCreate(&ctx->ev0));
+    SVDAG_HIP(hipEventCreate(&ctx->ev1));
+    SVDAG_HIP(hipHostMalloc((void**)&ctx->hStatus, sizeof(EditStatus) + kMaxLevels * sizeof(LevelDesc)));
+    ctx->hLevels = reinterpret_cast<LevelDesc*>(ctx->hStatus + 1);
+
+    DagDev& d = ctx->dag;
+    d.levels = cfg.levels;
+    d.leafLevel = cfg.levels - 2;
+    d.M = cfg.material_bits;
+    d.numMaterials = 1u << d.M;
+    d.minLeafWords = 2 + (d.M ? 1 : 0);
+    d.maxLeafWords = 2 + (64 * d.M + 31) / 32;
+    d.leafSizeBits = bit_width(d.maxLeafWords - d.minLeafWords);
+    const uint32_t maxWords = std::max(9u, d.maxLeafWords);
+    d.numTables = maxWords - 2 + 1;
+    const uint32_t indexBits = std::min(32u - kNodeSizeBits, 32u - d.leafSizeBits);
+    const uint64_t slabLimit = 1ull << (indexBits - kSlotBits);
+
+    const uint64_t perTable = std::max<uint64_t>(64, cfg.expected_items / (d.numTables * (uint64_t)cfg.load_factor));
+    const uint32_t buckets = (uint32_t)std::min<uint64_t>(perTable, slabLimit / 2);
+    ctx->ts.minWords = 2;
+    ctx->ts.maxWords = maxWords;
+    ctx->ts.tables.resize(d.numTables);
+    for (uint32_t i = 0; i < d.numTables; ++i) {
+        uint64_t cap = buckets + (cfg.arena_slabs ? cfg.arena_slabs : buckets / 2 + 4096);
+        cap = std::min<uint64_t>(cap, slabLimit);
+        SVDAG_TRY(table_alloc(ctx->ts.tables[i], 2 + i, buckets, (uint32_t)cap, ctx->stream));
+    }
+    SVDAG_TRY(ctx_refresh_tables(ctx));
+    SVDAG_HIP(hipMalloc(&ctx->dFullyFilled, (size_t)d.levels * d.numMaterials * 4));
+    d.fullyFilled = ctx->dFullyFilled;
+    SVDAG_HIP(hipMalloc(&ctx->dRoot, 4));
+    SVDAG_HIP(hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream));
+    init_fully_filled<<<1, 64, 0, ctx->stream>>>(d);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_TRY(ws_alloc(ctx, cfg.workspace_items, false));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->root = kSentinel;
+    *out = ctx;
+    return SVDAG_OK;
+}
+
+int svdag_destroy(svdag_ctx* ctx)
+{
+    if (!ctx)
+        return SVDAG_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ts.release();
+    ws_release(ctx->ws);
+    for (void* p : { (void*)ctx->dFullyFilled, (void*)ctx->dRoot, (void*)ctx->hmin, (void*)ctx->hmax,
+                     (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits })
+        if (p)
+            (void)hipFree(p);
+    (void)hipHostFree(ctx->hStatus);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SVDAG_OK;
+}
+
+int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)
+{
+    if (!ctx || !tp || tp->octaves == 0 || tp->octaves > 16 || !std::isfinite(tp->base) || !std::isfinite(tp->amplitude)
+        || !std::isfinite(tp->frequency)) {
+        set_error("svdag_build_terrain: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    if (tp->material >= ctx->dag.numMaterials) {
+        set_error("terrain material does not fit the material bits");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const uint32_t D = ctx->dag.levels;
+    const uint32_t side = 1u << D;
+    // Pyramid: level k holds (side >> k)^2 cells.
+    uint64_t off = 0;
+    for (uint32_t k = 0; k <= D; ++k) {
+        ctx->pyrOffset[k] = off;
+        off += (uint64_t)(side >> k) * (side >> k);
+    }
+    for (uint32_t k = D + 1; k < kMaxLevels; ++k)
+        ctx->pyrOffset[k] = off;
+    if (ctx->hmin)
+        (void)hipFree(ctx->hmin);
+    if (ctx->hmax)
+        (void)hipFree(ctx->hmax);
+    ctx->hmin = ctx->hmax = nullptr;
+    SVDAG_HIP(hipMalloc(&ctx->hmin, off * 4));
+    SVDAG_HIP(hipMalloc(&ctx->hmax, off * 4));
+    PerlinDev perlin;
+    perlin_perm(tp->seed, perlin.perm);
+    terrain_heights_kernel<<<grid_for((uint64_t)side * side, 256, 1u << 16), 256, 0, ctx->stream>>>(perlin, *tp, side, ctx->hmin);
+    SVDAG_HIP(hipMemcpyAsync(ctx->hmax, ctx->hmin, (uint64_t)side * side * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    for (uint32_t k = 1; k <= D; ++k)
+        pyramid_level_kernel<<<grid_for((uint64_t)(side >> k) * (side >> k), 256, 1u << 16), 256, 0, ctx->stream>>>(
+            ctx->hmin + ctx->pyrOffset[k - 1], ctx->hmax + ctx->pyrOffset[k - 1], side >> (k - 1),
+            ctx->hmin + ctx->pyrOffset[k], ctx->hmax + ctx->pyrOffset[k]);
+    SVDAG_HIP(hipGetLastError());
+    EditorDev e {};
+    e.kind = EK_TERRAIN;
+    e.material = tp->material;
+    e.levels = D;
+    e.hmin = ctx->hmin;
+    e.hmax = ctx->hmax;
+    for (uint32_t k = 0; k < kMaxLevels; ++k)
+        e.pyrOffset[k] = ctx->pyrOffset[k];
+    // The terrain replaces whatever the DAG held: build from the empty root.
+    ctx->root = kSentinel;
+    SVDAG_HIP(hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream));
+    svdag_edit_stats stats;
+    SVDAG_TRY(run_edit(ctx, e, &stats));
+    ctx->undo.clear();
+    ctx->redo.clear();
+    return SVDAG_OK;
+}
+
+int svdag_terrain_heights(svdag_ctx* ctx, uint32_t x0, uint32_t z0, uint32_t w, uint32_t d, int32_t* out)
+{
+    if (!ctx || !out || !ctx->hmin) {
+        set_error("svdag_terrain_heights: no terrain");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t side = 1u << ctx->dag.levels;
+    if ((uint64_t)x0 + w > side || (uint64_t)z0 + d > side)
+        return SVDAG_E_INVALID;
+    for (uint32_t z = 0; z < d; ++z)
+        SVDAG_HIP(hipMemcpyAsync(out + (size_t)z * w, ctx->hmin + (size_t)(z0 + z) * side + x0, (size_t)w * 4,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    return SVDAG_OK;
+}
+
+int svdag_root(svdag_ctx* ctx, uint32_t* out)
+{
+    if (!ctx || !out)
+        return SVDAG_E_INVALID;
+    *out = ctx->root;
+    return SVDAG_OK;
+}
+
+int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* stats)
+{
+    if (!ctx || !desc)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    EditorDev e;
+    SVDAG_TRY(make_editor(ctx, *desc, e));
+    // MyGPUHashDAGUndoRedo::add_frame (my_gpu_base_dag.h:23-34), recorded
+    // before every edit (my_gpu_hash_dag_edits.cu:638-640).
+    if (ctx->undo.empty() || ctx->undo.back() != ctx->root) {
+        ctx->redo.clear();
+        ctx->undo.push_back(ctx->root);
+    }
+    svdag_edit_stats local;
+    return run_edit(ctx, e, stats ? stats : &local);
+}
+
+static int set_root(svdag_ctx* ctx, uint32_t r)
+{
+    ctx->root = r;
+    SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    return SVDAG_OK;
+}
+
+int svdag_undo(svdag_ctx* ctx)
+{
+    if (!ctx)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->undo.empty())
+        return SVDAG_OK;
+    const uint32_t r = ctx->undo.back();
+    ctx->undo.pop_back();
+    ctx->redo.push_back(ctx->root);
+    return set_root(ctx, r);
+}
+
+int svdag_redo(svdag_ctx* ctx)
+{
+    if (!ctx)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->redo.empty())
+        return SVDAG_OK;
+    const uint32_t r = ctx->redo.back();
+    ctx->redo.pop_back();
+    ctx->undo.push_back(ctx->root);
+    return set_root(ctx, r);
+}
+
+int svdag_history(svdag_ctx* ctx, uint32_t out[2])
+{
+    if (!ctx || !out)
+        return SVDAG_E_INVALID;
+    out[0] = (uint32_t)ctx->undo.size();
+    out[1] = (uint32_t)ctx->redo.size();
+    return SVDAG_OK;
+}
+
+int svdag_query_voxels(svdag_ctx* ctx, const uint32_t* xyz, uint32_t n, uint32_t* out)
+{
+    if (!ctx || (n && (!xyz || !out)))
+        return SVDAG_E_INVALID;
+    if (n == 0)
+        return SVDAG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    uint32_t *dIn = nullptr, *dOut = nullptr;
+    SVDAG_HIP(hipMalloc(&dIn, (size_t)n * 12));
+    SVDAG_HIP(hipMalloc(&dOut, (size_t)n * 4));
+    SVDAG_HIP(hipMemcpyAsync(dIn, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    query_voxels_kernel<<<grid_for(n, 256, 1u << 20), 256, 0, ctx->stream>>>(ctx->dag, ctx->dRoot, dIn, n, dOut);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipMemcpyAsync(out, dOut, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dIn);
+    (void)hipFree(dOut);
+    return SVDAG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,271 @@
+// Edit tools evaluated on the device: node classification (should_edit) and
+// the per-voxel new value (get_new_value).
+//
+// The float arithmetic is the reference's, operation by operation, with FP
+// contraction off (every a*b and a-b rounded separately):
+//   sphere  my_gpu_dag_editors.h:401-496   (centre + 0.5, radius >= 0.1,
+//           overlap = r^2 - sum of squared axis gaps > 0, Fill/Empty when the
+//           box shrunk by one voxel is inside)
+//   box     my_gpu_dag_editors.h:45-119    ([c-r, c+r), r == 0 -> [c, c+1))
+//   paint   my_gpu_dag_editors.h:498-536   (Fill only over flagged full nodes)
+//   copy    my_gpu_dag_editors.h:127-253   (translation; up to 8 source parents)
+// The terrain tool (kind 6) has no reference counterpart: it classifies nodes
+// against a min/max height pyramid so that a heightfield builds through the
+// same pipeline as an edit.
+#pragma once
+#include "svdag_device.h"
+
+namespace svdag {
+
+enum EditKind : uint32_t {
+    EK_SPHERE_ADD = 0,
+    EK_SPHERE_REMOVE = 1,
+    EK_BOX_ADD = 2,
+    EK_BOX_REMOVE = 3,
+    EK_PAINT = 4,
+    EK_COPY = 5,
+    EK_TERRAIN = 6
+};
+enum Op : uint32_t { OP_SUBDIVIDE = 0, OP_FILL = 1, OP_EMPTY = 2, OP_KEEP = 3 };
+enum NState : uint32_t { NS_PARTIAL = 0, NS_FILLED = 1, NS_EMPTY = 2 };
+
+constexpr uint32_t kMaxLevels = 24;
+
+struct EditorDev {
+    uint32_t kind;
+    uint32_t material;
+    // sphere / paint
+    float cx, cy, cz;
+    float radius, r2;
+    // box and copy destination bounds
+    float bmin[3], bmax[3];
+    // copy
+    float d2s[3];
+    int32_t id2s[3];
+    // terrain: height pyramid, level k holds (2^D >> k)^2 cells, row-major in z
+    uint32_t levels;
+    const int32_t* hmin;
+    const int32_t* hmax;
+    uint64_t pyrOffset[kMaxLevels];
+};
+
+// One copy-editor source parent: path and handle (handle == kSentinel ends the list).
+struct CopyParent {
+    uint32_t x, y, z, h;
+};
+struct CopyState {
+    CopyParent p[8];
+};
+
+__host__ __device__ inline float fmax_ref(float a, float b) { return a > b ? a : b; }
+__host__ __device__ inline float fmin_ref(float a, float b) { return a < b ? a : b; }
+
+__host__ __device__ inline bool sphere_overlap(const EditorDev& e, const float mn[3], const float mx[3])
+{
+    float dist = e.r2;
+    const float c[3] = { e.cx, e.cy, e.cz };
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] < mn[a]) {
+            const float d = c[a] - mn[a];
+            dist = dist - d * d;
+        } else if (c[a] > mx[a]) {
+            const float d = c[a] - mx[a];
+            dist = dist - d * d;
+        }
+    }
+    return dist > 0.0f;
+}
+
+__host__ __device__ inline bool sphere_contained(const EditorDev& e, const float mn[3], const float mx[3])
+{
+    const float c[3] = { e.cx, e.cy, e.cz };
+    float m[3];
+    for (int a = 0; a < 3; ++a) {
+        const float lo = mn[a] - c[a];
+        const float hi = mx[a] - c[a];
+        const float l2 = lo * lo, h2 = hi * hi;
+        m[a] = l2 > h2 ? l2 : h2; // std::max(l2, h2)
+    }
+    return (m[0] + m[1]) + m[2] < e.r2;
+}
+
+__host__ __device__ inline bool box_overlap(const float bmin[3], const float bmax[3], const float mn[3], const float mx[3])
+{
+    return !(bmin[0] >= mx[0] || bmin[1] >= mx[1] || bmin[2] >= mx[2] || bmax[0] <= mn[0] || bmax[1] <= mn[1]
+             || bmax[2] <= mn[2] || mn[0] >= bmax[0] || mn[1] >= bmax[1] || mn[2] >= bmax[2] || mx[0] <= bmin[0]
+             || mx[1] <= bmin[1] || mx[2] <= bmin[2]);
+}
+
+__host__ __device__ inline bool box_contained(const float bmin[3], const float bmax[3], const float mn[3], const float mx[3])
+{
+    return bmin[0] <= mn[0] && bmin[1] <= mn[1] && bmin[2] <= mn[2] && bmax[0] >= mx[0] && bmax[1] >= mx[1]
+        && bmax[2] >= mx[2];
+}
+
+// Node box [p << depth, (p << depth) + 2^depth) in float, as Path::as_position.
+__host__ __device__ inline void node_box(uint32_t x, uint32_t y, uint32_t z, uint32_t depth, float mn[3], float mx[3])
+{
+    mn[0] = (float)(x << depth);
+    mn[1] = (float)(y << depth);
+    mn[2] = (float)(z << depth);
+    const float size = (float)(1u << depth);
+    mx[0] = mn[0] + size;
+    mx[1] = mn[1] + size;
+    mx[2] = mn[2] + size;
+}
+
+// should_edit for the stateless tools (and the copy tool's voxel level).
+__device__ inline Op decide(const EditorDev& e, uint32_t x, uint32_t y, uint32_t z, uint32_t depth, NState st,
+                            uint32_t& fillMat)
+{
+    float mn[3], mx[3];
+    node_box(x, y, z, depth, mn, mx);
+    fillMat = e.material;
+    switch (e.kind) {
+    case EK_SPHERE_ADD:
+    case EK_SPHERE_REMOVE: {
+        if (!sphere_overlap(e, mn, mx))
+            return OP_KEEP;
+        const float smn[3] = { mn[0] + 1.0f, mn[1] + 1.0f, mn[2] + 1.0f };
+        const float smx[3] = { mx[0] - 1.0f, mx[1] - 1.0f, mx[2] - 1.0f };
+        if (sphere_contained(e, smn, smx))
+            return e.kind == EK_SPHERE_ADD ? OP_FILL : OP_EMPTY;
+        return OP_SUBDIVIDE;
+    }
+    case EK_BOX_ADD:
+    case EK_BOX_REMOVE:
+        if (box_contained(e.bmin, e.bmax, mn, mx))
+            return e.kind == EK_BOX_ADD ? OP_FILL : OP_EMPTY;
+        if (box_overlap(e.bmin, e.bmax, mn, mx))
+            return OP_SUBDIVIDE;
+        return OP_KEEP;
+    case EK_PAINT: {
+        const bool overlaps = sphere_overlap(e, mn, mx);
+        const bool contained = sphere_contained(e, mn, mx);
+        if (!overlaps)
+            return OP_KEEP;
+        if (contained && st == NS_FILLED)
+            return OP_FILL;
+        if (st == NS_EMPTY)
+            return OP_EMPTY;
+        return OP_SUBDIVIDE;
+    }
+    case EK_COPY:
+        return box_overlap(e.bmin, e.bmax, mn, mx) ? OP_SUBDIVIDE : OP_KEEP;
+    case EK_TERRAIN: {
+        const uint64_t width = (uint64_t)1 << (e.levels - depth);
+        const uint64_t idx = e.pyrOffset[depth] + (uint64_t)z * width + x;
+        const int64_t lo = e.hmin[idx], hi = e.hmax[idx];
+        const int64_t y0 = (int64_t)y << depth;
+        const int64_t y1 = y0 + ((int64_t)1 << depth) - 1;
+        if (y1 <= lo)
+            return OP_FILL;
+        if (y0 > hi)
+            return OP_EMPTY;
+        return OP_SUBDIVIDE;
+    }
+    default:
+        return OP_KEEP;
+    }
+}
+
+// get_new_value for the stateless tools: true and a material when the voxel
+// ends up set.
+__device__ inline bool new_value(const EditorDev& e, bool hasOld, uint32_t oldMat, uint32_t& mat)
+{
+    switch (e.kind) {
+    case EK_SPHERE_ADD:
+    case EK_BOX_ADD:
+    case EK_PAINT:
+    case EK_TERRAIN:
+        mat = e.material;
+        return true;
+    default:
+        (void)hasOld;
+        (void)oldMat;
+        return false;
+    }
+}
+
+__host__ __device__ inline bool aabb_overlap(const float lmn[3], const float lmx[3], const float rmn[3], const float rmx[3])
+{
+    return !(lmn[0] >= rmx[0] || lmn[1] >= rmx[1] || lmn[2] >= rmx[2] || lmx[0] <= rmn[0] || lmx[1] <= rmn[1]
+             || lmx[2] <= rmn[2] || rmn[0] >= lmx[0] || rmn[1] >= lmx[1] || rmn[2] >= lmx[2] || rmx[0] <= lmn[0]
+             || rmx[1] <= lmn[1] || rmx[2] <= lmn[2]);
+}
+
+// Copy tool, stateful should_edit for an inner-level child at childDepth:
+// collects the source children of the parent's source nodes that overlap the
+// shifted destination region (my_gpu_dag_editors.h:173-217).
+__device__ inline Op decide_copy(const EditorDev& e, const DagDev& dag, uint32_t x, uint32_t y, uint32_t z,
+                                 uint32_t depth, const CopyState& in, CopyState& out)
+{
+    float mn[3], mx[3];
+    node_box(x, y, z, depth, mn, mx);
+    for (int k = 0; k < 8; ++k)
+        out.p[k].h = kSentinel;
+    if (!box_overlap(e.bmin, e.bmax, mn, mx))
+        return OP_KEEP;
+    const float size = (float)(1u << depth);
+    float smn[3], smx[3];
+    for (int a = 0; a < 3; ++a) {
+        smn[a] = fmax_ref(mn[a], e.bmin[a]) + e.d2s[a];
+        smx[a] = fmin_ref(smn[a] + size, e.bmax[a] + e.d2s[a]);
+    }
+    int nOut = 0;
+    for (int k = 0; k < 8; ++k) {
+        const CopyParent& par = in.p[k];
+        if (par.h == kSentinel)
+            break;
+        const uint32_t* node = dag_node_ptr(dag, par.h);
+        const uint32_t mask = node[0] & 0xFFu;
+        for (uint32_t c = 0; c < 8; ++c) {
+            if (!(mask & (1u << c)))
+                continue;
+            const uint32_t cx = (par.x << 1) | ((c >> 2) & 1u);
+            const uint32_t cy = (par.y << 1) | ((c >> 1) & 1u);
+            const uint32_t cz = (par.z << 1) | (c & 1u);
+            float cmn[3], cmx[3];
+            node_box(cx, cy, cz, depth, cmn, cmx);
+            if (aabb_overlap(cmn, cmx, smn, smx) && nOut < 8) {
+                out.p[nOut].x = cx;
+                out.p[nOut].y = cy;
+                out.p[nOut].z = cz;
+                out.p[nOut].h = node[1 + __builtin_popcount(mask & ((1u << c) - 1u))];
+                ++nOut;
+            }
+        }
+    }
+    return nOut == 0 ? OP_KEEP : OP_SUBDIVIDE;
+}
+
+// Copy tool voxel value (my_gpu_dag_editors.h:225-252): the source voxel's
+// material if set, else the old value; the source position clamps at 0.
+__device__ inline bool copy_value(const EditorDev& e, const DagDev& dag, uint32_t vx, uint32_t vy, uint32_t vz,
+                                  const CopyState& st, bool hasOld, uint32_t oldMat, uint32_t& mat)
+{
+    int32_t s[3] = { (int32_t)vx + e.id2s[0], (int32_t)vy + e.id2s[1], (int32_t)vz + e.id2s[2] };
+    for (int a = 0; a < 3; ++a)
+        s[a] = s[a] > 0 ? s[a] : 0;
+    const uint32_t px = (uint32_t)s[0] >> 2, py = (uint32_t)s[1] >> 2, pz = (uint32_t)s[2] >> 2;
+    const CopyParent* found = nullptr;
+    for (int k = 0; k < 8; ++k) {
+        if (st.p[k].h == kSentinel)
+            break;
+        if (st.p[k].x == px && st.p[k].y == py && st.p[k].z == pz)
+            found = &st.p[k];
+    }
+    mat = oldMat;
+    if (!found)
+        return hasOld;
+    const uint32_t bit = (((uint32_t)s[0] & 1u) << 2) | (((uint32_t)s[1] & 1u) << 1) | ((uint32_t)s[2] & 1u)
+        | (((uint32_t)s[0] & 2u) << 4) | (((uint32_t)s[1] & 2u) << 3) | (((uint32_t)s[2] & 2u) << 2);
+    uint32_t m;
+    if (leaf_material(dag_leaf_ptr(dag, found->h), bit, dag.M, m)) {
+        mat = m;
+        return true;
+    }
+    return hasOld;
+}
+
+} // namespace svdag

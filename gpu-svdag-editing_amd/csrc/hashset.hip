@@ -1,0 +1,464 @@
+// Slab hash tables on the device: allocation, initialisation, growth and the
+// bulk insert / find kernels (per-lane and wave64-group variants).
+//
+// Stands in for CompactAccelerationHashTable / GpuHashTableBase
+// (hash_tables/compact_acceleration_hash_table.{h,cu}, gpu_hash_table_base.{h,cu})
+// and the HashMapBench bulk kernels (extras/hashmap_bench/src/main.cu:124-216).
+#include "svdag_internal.h"
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+namespace svdag {
+
+static thread_local std::string g_lastError;
+void set_error(const std::string& msg) { g_lastError = msg; }
+
+// Head slabs: empty tag line, no successor, no active bits.
+__global__ void table_init_heads(TableDev t)
+{
+    const size_t n = (size_t)t.numBuckets * kTagWords;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        t.tags[i] = ((i & (kTagWords - 1)) == kNextWord) ? kNoSlab : 0u;
+        if ((i & (kTagWords - 1)) < 4)
+            t.active[(i / kTagWords) * 4 + (i & 3)] = 0u;
+    }
+}
+
+int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t slabCap, hipStream_t stream)
+{
+    if (itemWords == 0 || itemWords > kMaxItemWords || numBuckets == 0 || slabCap < numBuckets) {
+        set_error("table_alloc: bad size");
+        return SVDAG_E_INVALID;
+    }
+    if (((uint64_t)slabCap << kSlotBits) > (1ull << 32)) {
+        set_error("table_alloc: slab capacity exceeds the pointer range");
+        return SVDAG_E_INVALID;
+    }
+    t.itemWords = itemWords;
+    t.numBuckets = numBuckets;
+    t.slabCap = slabCap;
+    TableDev& d = t.dev;
+    d.itemWords = itemWords;
+    d.numBuckets = numBuckets;
+    d.slabCap = slabCap;
+    SVDAG_HIP(hipMalloc(&d.tags, (size_t)slabCap * kTagWords * 4));
+    SVDAG_HIP(hipMalloc(&d.items, (size_t)slabCap * kSlots * itemWords * 4));
+    SVDAG_HIP(hipMalloc(&d.active, (size_t)slabCap * 4 * 4));
+    SVDAG_HIP(hipMalloc(&d.freeList, (size_t)slabCap * 4));
+    SVDAG_HIP(hipMalloc(&d.alloc, sizeof(AllocState)));
+    AllocState a { 0xFFFFFFFFull, numBuckets, 0u };
+    SVDAG_HIP(hipMemcpyAsync(d.alloc, &a, sizeof(a), hipMemcpyHostToDevice, stream));
+    table_init_heads<<<grid_for((uint64_t)numBuckets * kTagWords, 256, 4096), 256, 0, stream>>>(d);
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+void table_free(HostTable& t)
+{
+    TableDev& d = t.dev;
+    if (d.tags) (void)hipFree(d.tags);
+    if (d.items) (void)hipFree(d.items);
+    if (d.active) (void)hipFree(d.active);
+    if (d.freeList) (void)hipFree(d.freeList);
+    if (d.alloc) (void)hipFree(d.alloc);
+    d = TableDev {};
+    t = HostTable {};
+}
+
+// Arena growth: allocate the larger arena, copy the used slabs (ids are
+// preserved, so every handle stays valid), release the old arena.
+int table_grow(HostTable& t, uint32_t newCap, hipStream_t stream)
+{
+    if (newCap <= t.slabCap)
+        return SVDAG_OK;
+    if (((uint64_t)newCap << kSlotBits) > (1ull << 32)) {
+        set_error("table_grow: slab capacity exceeds the pointer range");
+        return SVDAG_E_OOM;
+    }
+    TableDev& d = t.dev;
+    const uint32_t old = t.slabCap;
+    uint32_t *tags = nullptr, *items = nullptr, *active = nullptr, *freeList = nullptr;
+    SVDAG_HIP(hipMalloc(&tags, (size_t)newCap * kTagWords * 4));
+    SVDAG_HIP(hipMalloc(&items, (size_t)newCap * kSlots * t.itemWords * 4));
+    SVDAG_HIP(hipMalloc(&active, (size_t)newCap * 4 * 4));
+    SVDAG_HIP(hipMalloc(&freeList, (size_t)newCap * 4));
+    SVDAG_HIP(hipMemcpyAsync(tags, d.tags, (size_t)old * kTagWords * 4, hipMemcpyDeviceToDevice, stream));
+    SVDAG_HIP(hipMemcpyAsync(items, d.items, (size_t)old * kSlots * t.itemWords * 4, hipMemcpyDeviceToDevice, stream));
+    SVDAG_HIP(hipMemcpyAsync(active, d.active, (size_t)old * 4 * 4, hipMemcpyDeviceToDevice, stream));
+    SVDAG_HIP(hipMemcpyAsync(freeList, d.freeList, (size_t)old * 4, hipMemcpyDeviceToDevice, stream));
+    SVDAG_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(d.tags);
+    (void)hipFree(d.items);
+    (void)hipFree(d.active);
+    (void)hipFree(d.freeList);
+    d.tags = tags;
+    d.items = items;
+    d.active = active;
+    d.freeList = freeList;
+    d.slabCap = newCap;
+    t.slabCap = newCap;
+    // The OOM flag and the over-counted fresh watermark are reset so that the
+    // next allocation continues from the first never-used slab.
+    AllocState a {};
+    SVDAG_HIP(hipMemcpyAsync(&a, d.alloc, sizeof(a), hipMemcpyDeviceToHost, stream));
+    SVDAG_HIP(hipStreamSynchronize(stream));
+    a.fresh = std::min(a.fresh, old);
+    a.oom = 0;
+    SVDAG_HIP(hipMemcpyAsync(d.alloc, &a, sizeof(a), hipMemcpyHostToDevice, stream));
+    return SVDAG_OK;
+}
+
+int table_read_alloc(const HostTable& t, AllocState& out, hipStream_t stream)
+{
+    SVDAG_HIP(hipMemcpyAsync(&out, t.dev.alloc, sizeof(out), hipMemcpyDeviceToHost, stream));
+    SVDAG_HIP(hipStreamSynchronize(stream));
+    return SVDAG_OK;
+}
+
+int TableSet::upload(hipStream_t stream)
+{
+    std::vector<TableDev> h(tables.size());
+    for (size_t i = 0; i < tables.size(); ++i)
+        h[i] = tables[i].dev;
+    if (!devTables)
+        SVDAG_HIP(hipMalloc(&devTables, sizeof(TableDev) * std::max<size_t>(h.size(), 1)));
+    SVDAG_HIP(hipMemcpyAsync(devTables, h.data(), sizeof(TableDev) * h.size(), hipMemcpyHostToDevice, stream));
+    return SVDAG_OK;
+}
+
+void TableSet::release()
+{
+    for (auto& t : tables)
+        table_free(t);
+    tables.clear();
+    if (devTables)
+        (void)hipFree(devTables);
+    devTables = nullptr;
+}
+
+// ------------------------------------------------------------------ bulk kernels
+// Segment descriptors live in device memory as 5 words each:
+// [item_words, count, word_offset, out_offset, first_global_index].
+__device__ inline uint32_t seg_of(const uint32_t* segs, uint32_t nseg, uint32_t idx)
+{
+    uint32_t s = 0;
+    while (s + 1 < nseg && idx >= segs[(s + 1) * 5 + 4])
+        ++s;
+    return s;
+}
+
+template <int G, bool kInsert>
+__global__ void __launch_bounds__(256) bulk_group_kernel(const TableDev* tables, uint32_t minWords, const uint32_t* words,
+                                                         const uint32_t* segs, uint32_t nseg, uint32_t total, uint32_t* out)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const uint32_t groupsPerBlock = blockDim.x / G;
+    for (uint32_t idx = blockIdx.x * groupsPerBlock + threadIdx.x / G; idx < total; idx += gridDim.x * groupsPerBlock) {
+        const uint32_t sg = seg_of(segs, nseg, idx);
+        const uint32_t s = segs[sg * 5 + 0];
+        const uint32_t local = idx - segs[sg * 5 + 4];
+        const TableDev t = tables[s - minWords];
+        Needle<G> n;
+        needle_load<G>(n, words + segs[sg * 5 + 2] + (size_t)local * s, s, gl);
+        const uint32_t h = needle_hash<G>(n, s, gl);
+        uint32_t r;
+        if constexpr (kInsert)
+            r = table_insert_group<G>(t, n, s, h, gl, gbase);
+        else
+            r = table_find_group<G>(t, n, s, h, gl, gbase);
+        if (gl == 0)
+            out[segs[sg * 5 + 3] + local] = r;
+    }
+}
+
+template <bool kInsert>
+__global__ void __launch_bounds__(256) bulk_lane_kernel(const TableDev* tables, uint32_t minWords, const uint32_t* words,
+                                                        const uint32_t* segs, uint32_t nseg, uint32_t total, uint32_t* out)
+{
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+        const uint32_t sg = seg_of(segs, nseg, idx);
+        const uint32_t s = segs[sg * 5 + 0];
+        const uint32_t local = idx - segs[sg * 5 + 4];
+        const TableDev t = tables[s - minWords];
+        const uint32_t* p = words + segs[sg * 5 + 2] + (size_t)local * s;
+        uint32_t h = 0;
+        for (uint32_t j = 0; j < s; ++j)
+            h ^= hash_part(p[j], j);
+        out[segs[sg * 5 + 3] + local] = kInsert ? table_insert_lane(t, p, s, h) : table_find_lane(t, p, s, h);
+    }
+}
+
+// Copy the words of n items (pointers already validated on the host).
+__global__ void gather_items(TableDev t, const uint32_t* ptrs, uint32_t n, uint32_t* out)
+{
+    const uint64_t total = (uint64_t)n * t.itemWords;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = (uint32_t)(i / t.itemWords), j = (uint32_t)(i % t.itemWords);
+        out[i] = table_item_ptr(t, ptrs[k])[j];
+    }
+}
+
+int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* segs, uint32_t nseg, uint32_t* out,
+                int variant, bool insert, hipStream_t stream, uint32_t* segBuf)
+{
+    if (nseg == 0)
+        return SVDAG_OK;
+    if (nseg > 64) {
+        set_error("too many segments (max 64)");
+        return SVDAG_E_INVALID;
+    }
+    uint32_t host[64 * 5];
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nseg; ++i) {
+        if (segs[i].item_words < ts.minWords || segs[i].item_words > ts.maxWords) {
+            set_error("segment item size outside the table set");
+            return SVDAG_E_INVALID;
+        }
+        host[i * 5 + 0] = segs[i].item_words;
+        host[i * 5 + 1] = segs[i].count;
+        host[i * 5 + 2] = segs[i].word_offset;
+        host[i * 5 + 3] = segs[i].out_offset;
+        host[i * 5 + 4] = (uint32_t)total;
+        total += segs[i].count;
+    }
+    if (total >= (1ull << 32)) {
+        set_error("too many items");
+        return SVDAG_E_INVALID;
+    }
+    SVDAG_HIP(hipMemcpyAsync(segBuf, host, nseg * 5 * 4, hipMemcpyHostToDevice, stream));
+    const uint32_t n = (uint32_t)total;
+    if (n == 0)
+        return SVDAG_OK;
+    const uint32_t block = 256;
+    switch (variant) {
+    case 0: {
+        const uint32_t grid = grid_for(n, block, 1u << 16);
+        if (insert)
+            bulk_lane_kernel<true><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out);
+        else
+            bulk_lane_kernel<false><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out);
+        break;
+    }
+#define SVDAG_BULK_CASE(G)                                                                                              \
+    case G: {                                                                                                           \
+        const uint32_t grid = grid_for((uint64_t)n * G, block, 1u << 16);                                               \
+        if (insert)                                                                                                     \
+            bulk_group_kernel<G, true><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out); \
+        else                                                                                                            \
+            bulk_group_kernel<G, false><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out); \
+        break;                                                                                                          \
+    }
+        SVDAG_BULK_CASE(4)
+        SVDAG_BULK_CASE(8)
+        SVDAG_BULK_CASE(16)
+        SVDAG_BULK_CASE(32)
+#undef SVDAG_BULK_CASE
+    default:
+        set_error("variant must be 0, 4, 8, 16 or 32");
+        return SVDAG_E_INVALID;
+    }
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+} // namespace svdag
+
+using namespace svdag;
+
+struct svdag_hashset {
+    TableSet ts;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t* segBuf = nullptr;
+};
+
+extern "C" {
+
+const char* svdag_last_error(void) { return g_lastError.c_str(); }
+uint32_t svdag_abi_version(void) { return 1u; }
+
+int svdag_device_count(int* out)
+{
+    if (!out)
+        return SVDAG_E_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        n = 0;
+    *out = n;
+    return SVDAG_OK;
+}
+
+static int require_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device visible");
+        return SVDAG_E_NO_DEVICE;
+    }
+    return SVDAG_OK;
+}
+
+int svdag_dev_alloc(size_t bytes, void** out)
+{
+    if (!out)
+        return SVDAG_E_INVALID;
+    SVDAG_TRY(require_device());
+    SVDAG_HIP(hipMalloc(out, bytes ? bytes : 4));
+    return SVDAG_OK;
+}
+int svdag_dev_free(void* p)
+{
+    if (p)
+        SVDAG_HIP(hipFree(p));
+    return SVDAG_OK;
+}
+int svdag_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    SVDAG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SVDAG_OK;
+}
+int svdag_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    SVDAG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SVDAG_OK;
+}
+int svdag_device_synchronize(void)
+{
+    SVDAG_HIP(hipDeviceSynchronize());
+    return SVDAG_OK;
+}
+
+int svdag_hashset_create(uint32_t minWords, uint32_t maxWords, const uint32_t* buckets, const uint32_t* reserve,
+                         svdag_hashset** out)
+{
+    if (!out || !buckets || minWords == 0 || maxWords < minWords || maxWords > kMaxItemWords) {
+        set_error("svdag_hashset_create: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    SVDAG_TRY(require_device());
+    auto* hs = new svdag_hashset();
+    SVDAG_HIP(hipStreamCreateWithFlags(&hs->stream, hipStreamNonBlocking));
+    SVDAG_HIP(hipEventCreate(&hs->ev0));
+    SVDAG_HIP(hipEventCreate(&hs->ev1));
+    SVDAG_HIP(hipMalloc(&hs->segBuf, 64 * 5 * 4));
+    hs->ts.minWords = minWords;
+    hs->ts.maxWords = maxWords;
+    hs->ts.tables.resize(maxWords - minWords + 1);
+    for (uint32_t s = minWords; s <= maxWords; ++s) {
+        const uint32_t nb = std::max(1u, buckets[s - minWords]);
+        const uint64_t res = reserve ? reserve[s - minWords] : 0;
+        // Overflow reserve: every reserved item may need a slab of its own at
+        // worst, but a chain only overflows past 124 items, so size the
+        // reserve at one slab per 123 items plus one eighth of the buckets.
+        uint64_t cap = (uint64_t)nb + res / 123 + nb / 8 + 64;
+        cap = std::min<uint64_t>(cap, (1ull << (32 - kSlotBits)));
+        int r = table_alloc(hs->ts.tables[s - minWords], s, nb, (uint32_t)cap, hs->stream);
+        if (r != SVDAG_OK) {
+            hs->ts.release();
+            delete hs;
+            return r;
+        }
+    }
+    SVDAG_TRY(hs->ts.upload(hs->stream));
+    SVDAG_HIP(hipStreamSynchronize(hs->stream));
+    *out = hs;
+    return SVDAG_OK;
+}
+
+int svdag_hashset_destroy(svdag_hashset* hs)
+{
+    if (!hs)
+        return SVDAG_OK;
+    (void)hipStreamSynchronize(hs->stream);
+    hs->ts.release();
+    (void)hipFree(hs->segBuf);
+    (void)hipEventDestroy(hs->ev0);
+    (void)hipEventDestroy(hs->ev1);
+    (void)hipStreamDestroy(hs->stream);
+    delete hs;
+    return SVDAG_OK;
+}
+
+static int hashset_bulk(svdag_hashset* hs, const uint32_t* words, const svdag_segment* segs, uint32_t nseg,
+                        uint32_t* out, int variant, float* ms, bool insert)
+{
+    if (!hs || !words || !segs || !out)
+        return SVDAG_E_INVALID;
+    if (ms)
+        SVDAG_HIP(hipEventRecord(hs->ev0, hs->stream));
+    SVDAG_TRY(launch_bulk(hs->ts, words, segs, nseg, out, variant, insert, hs->stream, hs->segBuf));
+    if (ms)
+        SVDAG_HIP(hipEventRecord(hs->ev1, hs->stream));
+    SVDAG_HIP(hipStreamSynchronize(hs->stream));
+    if (ms)
+        SVDAG_HIP(hipEventElapsedTime(ms, hs->ev0, hs->ev1));
+    if (insert) {
+        for (auto& t : hs->ts.tables) {
+            AllocState a {};
+            SVDAG_TRY(table_read_alloc(t, a, hs->stream));
+            if (a.oom) {
+                set_error("slab arena exhausted during bulk insert");
+                return SVDAG_E_OOM;
+            }
+        }
+    }
+    return SVDAG_OK;
+}
+
+int svdag_hashset_add_dev(svdag_hashset* hs, const uint32_t* words, const svdag_segment* segs, uint32_t nseg,
+                          uint32_t* out, int variant, float* ms)
+{
+    return hashset_bulk(hs, words, segs, nseg, out, variant, ms, true);
+}
+
+int svdag_hashset_find_dev(svdag_hashset* hs, const uint32_t* words, const svdag_segment* segs, uint32_t nseg,
+                           uint32_t* out, int variant, float* ms)
+{
+    return hashset_bulk(hs, words, segs, nseg, out, variant, ms, false);
+}
+
+int svdag_hashset_read(svdag_hashset* hs, uint32_t s, const uint32_t* ptrs, uint32_t n, uint32_t* outWords)
+{
+    if (!hs || s < hs->ts.minWords || s > hs->ts.maxWords || (n && (!ptrs || !outWords)))
+        return SVDAG_E_INVALID;
+    if (n == 0)
+        return SVDAG_OK;
+    const HostTable& t = hs->ts.tables[s - hs->ts.minWords];
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t slab = ptrs[i] >> kSlotBits, slot = ptrs[i] & ((1u << kSlotBits) - 1u);
+        if (ptrs[i] == kNotFound || slab >= t.slabCap || slot >= kSlots) {
+            set_error("svdag_hashset_read: invalid pointer");
+            return SVDAG_E_INVALID;
+        }
+    }
+    uint32_t *dPtrs = nullptr, *dOut = nullptr;
+    SVDAG_HIP(hipMalloc(&dPtrs, (size_t)n * 4));
+    SVDAG_HIP(hipMalloc(&dOut, (size_t)n * s * 4));
+    SVDAG_HIP(hipMemcpyAsync(dPtrs, ptrs, (size_t)n * 4, hipMemcpyHostToDevice, hs->stream));
+    gather_items<<<grid_for((uint64_t)n * s, 256, 1u << 16), 256, 0, hs->stream>>>(t.dev, dPtrs, n, dOut);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipMemcpyAsync(outWords, dOut, (size_t)n * s * 4, hipMemcpyDeviceToHost, hs->stream));
+    SVDAG_HIP(hipStreamSynchronize(hs->stream));
+    (void)hipFree(dPtrs);
+    (void)hipFree(dOut);
+    return SVDAG_OK;
+}
+
+int svdag_hashset_usage(svdag_hashset* hs, uint32_t s, uint32_t out[4])
+{
+    if (!hs || !out || s < hs->ts.minWords || s > hs->ts.maxWords)
+        return SVDAG_E_INVALID;
+    const HostTable& t = hs->ts.tables[s - hs->ts.minWords];
+    AllocState a {};
+    SVDAG_TRY(table_read_alloc(t, a, hs->stream));
+    out[0] = t.numBuckets;
+    out[1] = t.slabCap;
+    out[2] = std::min(a.fresh, t.slabCap); // slabs ever handed out (head + overflow)
+    out[3] = a.oom;
+    return SVDAG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,560 @@
+// Device-side primitives shared by every kernel of the SVDAG editing back-end:
+// item encoding, the wave64 slab hash table (find / insert, group-cooperative
+// and per-lane forms), and the slab allocator.
+//
+// Encoding follows GPUBaseDAG (dags/my_gpu_dags/my_gpu_base_dag.h:86-351):
+//   node header = mask[0:8] | fullFlag[8] | fullMaterial[9:9+M] | payload[9+M:32]
+//   node        = header + one handle per set mask bit (2..9 words)
+//   leaf        = [mask lo][mask hi][packed M-bit materials in voxel-bit order]
+//   handle      = index[0:32-b] | (size - minSize)[32-b:32]
+// Tables are indexed by item size - 2, nodes and leaves of equal word count
+// share one table (my_gpu_hash_dag.h:173, 240).
+//
+// Table layout (own design for wave64, not the reference's 32-slot slab):
+//   slab  = one 128-byte tag line (31 words x 4 one-byte tags = 124 slots,
+//           word 31 = next slab id) kept in a dense tag array, plus a separate
+//           item array [slab][124][itemWords].  Head slabs 0..numBuckets-1 are
+//           the buckets; overflow slabs come from the slab allocator.
+//   The dense tag array (128 B per bucket) is what every probe touches first;
+//   keeping it apart from the item words keeps it cache resident.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svdag {
+
+constexpr uint32_t kSlots = 124;        // item slots per slab
+constexpr uint32_t kTagWords = 32;      // words per tag line (128 B)
+constexpr uint32_t kNextWord = 31;      // tag-line word that holds the next slab id
+constexpr uint32_t kSlotBits = 7;       // pointer = slab << 7 | slot
+constexpr uint32_t kNoSlab = 0xFFFFFFFFu;
+constexpr uint32_t kNotFound = 0xFFFFFFFFu;
+constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+constexpr uint32_t kMaxItemWords = 18;  // 2 + 64*8/32 (8 material bits)
+
+// Bucket hash: SlabHash-style xor of per-word modular hashes (utils.h:386-403).
+constexpr uint32_t kHashX = 2717583853u;
+constexpr uint32_t kHashY = 1117415409u;
+constexpr uint32_t kPrime = 4294967291u;
+
+__host__ __device__ inline uint32_t hash_part(uint32_t w, uint32_t j)
+{
+    const uint32_t x = kHashX * w + j * kHashY; // u32 wrap, then mod prime
+    return x >= kPrime ? x - kPrime : x;
+}
+// One-byte tag stored per slot; 0 marks an empty slot.
+__host__ __device__ inline uint32_t make_tag(uint32_t h)
+{
+    const uint32_t t = (h * 0x9E3779B1u) >> 24;
+    return t ? t : 1u;
+}
+__host__ __device__ inline uint32_t mix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+// Exact per-byte equality-to-zero mask: bit 7 of every zero byte.
+__host__ __device__ inline uint32_t zero_bytes(uint32_t x)
+{
+    const uint32_t y = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return ~(y | x | 0x7F7F7F7Fu);
+}
+// Compact bits 7/15/23/31 into bits 0..3.
+__host__ __device__ inline uint32_t compact_bytes(uint32_t z)
+{
+    return ((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u);
+}
+
+// ---------------------------------------------------------------- allocator
+// Slab ids come from a lock-free free stack (64-bit head = ABA tag << 32 |
+// slab id, links in freeList[id]) and, when it is empty, from the never-used
+// tail of the pre-reserved arena (bump counter `fresh`).
+struct AllocState {
+    unsigned long long freeHead; // tag << 32 | id; id == kNoSlab when empty
+    uint32_t fresh;              // next never-used slab id (starts at numBuckets)
+    uint32_t oom;                // set when an allocation fails
+};
+
+struct TableDev {
+    uint32_t* tags;      // [slabCap][32]
+    uint32_t* items;     // [slabCap][kSlots][itemWords]
+    uint32_t* active;    // [slabCap][4] GC mark bits
+    uint32_t* freeList;  // [slabCap] free-stack links
+    AllocState* alloc;
+    uint32_t itemWords;
+    uint32_t numBuckets;
+    uint32_t slabCap;
+    uint32_t pad;
+};
+
+__device__ inline uint32_t ld_relaxed(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_relaxed(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long ld_relaxed64(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pop a recycled slab or bump the arena; never waits on another thread.
+__device__ inline uint32_t slab_alloc(const TableDev& t)
+{
+    AllocState* a = t.alloc;
+    unsigned long long old = ld_relaxed64(&a->freeHead);
+    while ((uint32_t)old != kNoSlab) {
+        const uint32_t id = (uint32_t)old;
+        const uint32_t nxt = ld_relaxed(&t.freeList[id]);
+        const unsigned long long nv = (((old >> 32) + 1ull) << 32) | nxt;
+        const unsigned long long r = atomicCAS(&a->freeHead, old, nv);
+        if (r == old)
+            return id;
+        old = r;
+    }
+    const uint32_t id = atomicAdd(&a->fresh, 1u);
+    if (id >= t.slabCap) {
+        atomicOr(&a->oom, 1u);
+        return kNoSlab;
+    }
+    return id;
+}
+
+// Push a slab nobody references any more.
+__device__ inline void slab_free(const TableDev& t, uint32_t id)
+{
+    AllocState* a = t.alloc;
+    unsigned long long old = ld_relaxed64(&a->freeHead);
+    while (true) {
+        st_relaxed(&t.freeList[id], (uint32_t)old);
+        __threadfence();
+        const unsigned long long nv = (((old >> 32) + 1ull) << 32) | id;
+        const unsigned long long r = atomicCAS(&a->freeHead, old, nv);
+        if (r == old)
+            return;
+        old = r;
+    }
+}
+
+// The chain of `bucket` was seen full with head successor `observed`.  A
+// fresh slab is linked after the head only if nobody grew the chain since,
+// so a full chain grows by one slab per round however many inserts race on
+// it; a loser returns its slab and rescans.  Returns the slab to continue
+// with: the new slab, the head (rescan), or kNoSlab when the arena is empty.
+__device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32_t observed)
+{
+    uint32_t* headNext = t.tags + (size_t)bucket * kTagWords + kNextWord;
+    if (ld_relaxed(headNext) != observed)
+        return bucket;
+    const uint32_t id = slab_alloc(t);
+    if (id == kNoSlab)
+        return kNoSlab;
+    uint32_t* nl = t.tags + (size_t)id * kTagWords;
+    for (uint32_t i = 0; i < kNextWord; ++i)
+        st_relaxed(nl + i, 0u);
+    st_relaxed(nl + kNextWord, observed);
+    for (uint32_t i = 0; i < 4; ++i)
+        t.active[(size_t)id * 4 + i] = 0u;
+    __threadfence();
+    if (atomicCAS(headNext, observed, id) == observed)
+        return id;
+    slab_free(t, id);
+    return bucket;
+}
+
+// ---------------------------------------------------------------- wave64 groups
+// A "group" is G consecutive lanes of a wave64 that cooperate on one item.
+template <int G>
+__device__ inline uint32_t group_ballot(bool p, uint32_t gbase)
+{
+    if constexpr (G == 1) {
+        return p ? 1u : 0u;
+    } else {
+        const unsigned long long b = __ballot(p);
+        constexpr unsigned long long m = (G >= 32) ? 0xFFFFFFFFull : ((1ull << G) - 1ull);
+        return (uint32_t)((b >> gbase) & m);
+    }
+}
+template <int G>
+__device__ inline uint32_t group_bcast(uint32_t v, uint32_t src)
+{
+    if constexpr (G == 1)
+        return v;
+    else
+        return (uint32_t)__shfl((int)v, (int)src, G);
+}
+template <int G>
+__device__ inline uint32_t group_xor(uint32_t v)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1)
+        v ^= (uint32_t)__shfl_xor((int)v, m, G);
+    return v;
+}
+
+// Needle words held by a group: lane gl holds words gl, gl+G, gl+2G, ...
+template <int G>
+struct Needle {
+    static constexpr int K = (kMaxItemWords + G - 1) / G;
+    uint32_t w[K];
+};
+
+template <int G>
+__device__ inline void needle_load(Needle<G>& n, const uint32_t* p, uint32_t s, uint32_t gl)
+{
+#pragma unroll
+    for (int k = 0; k < Needle<G>::K; ++k) {
+        const uint32_t j = gl + (uint32_t)k * G;
+        n.w[k] = (j < s) ? p[j] : 0u;
+    }
+}
+template <int G>
+__device__ inline uint32_t needle_hash(const Needle<G>& n, uint32_t s, uint32_t gl)
+{
+    uint32_t h = 0;
+#pragma unroll
+    for (int k = 0; k < Needle<G>::K; ++k) {
+        const uint32_t j = gl + (uint32_t)k * G;
+        if (j < s)
+            h ^= hash_part(n.w[k], j);
+    }
+    return group_xor<G>(h);
+}
+template <int G>
+__device__ inline bool needle_equal_lane(const Needle<G>& n, const uint32_t* item, uint32_t s, uint32_t gl)
+{
+    bool eq = true;
+#pragma unroll
+    for (int k = 0; k < Needle<G>::K; ++k) {
+        const uint32_t j = gl + (uint32_t)k * G;
+        if (j < s)
+            eq &= (item[j] == n.w[k]);
+    }
+    return eq;
+}
+
+// Load this lane's W = 32/G words of a tag line.
+template <int G>
+__device__ inline void tagline_load(uint32_t (&words)[32 / G], const uint32_t* line, uint32_t gl)
+{
+    constexpr int W = 32 / G;
+    const uint32_t* p = line + gl * W;
+    if constexpr (W == 1) {
+        words[0] = p[0];
+    } else if constexpr (W == 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(p);
+        words[0] = v.x;
+        words[1] = v.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < W / 4; ++q) {
+            const uint4 v = reinterpret_cast<const uint4*>(p)[q];
+            words[4 * q + 0] = v.x;
+            words[4 * q + 1] = v.y;
+            words[4 * q + 2] = v.z;
+            words[4 * q + 3] = v.w;
+        }
+    }
+}
+
+// Group-cooperative find (G in {4, 8, 16, 32}): the tag line is read once by
+// the whole group (W words per lane), candidate slots are resolved one at a
+// time with the group comparing the candidate's words in parallel.
+template <int G>
+__device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G>& n, uint32_t s, uint32_t h,
+                                            uint32_t gl, uint32_t gbase)
+{
+    static_assert(G >= 4 && G <= 32, "group size");
+    constexpr int W = 32 / G;
+    const uint32_t tag4 = make_tag(h) * 0x01010101u;
+    uint32_t slab = h % t.numBuckets;
+    while (true) {
+        uint32_t words[W];
+        tagline_load<G>(words, t.tags + (size_t)slab * kTagWords, gl);
+        uint32_t cand = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            if (gl * W + i != kNextWord)
+                cand |= compact_bytes(zero_bytes(words[i] ^ tag4)) << (4 * i);
+        }
+        const uint32_t next = group_bcast<G>(words[W - 1], G - 1);
+        while (true) {
+            const uint32_t bal = group_ballot<G>(cand != 0, gbase);
+            if (!bal)
+                break;
+            const uint32_t leader = __ffs(bal) - 1u;
+            const uint32_t mySlot = gl * W * 4 + (cand ? (__ffs(cand) - 1u) : 0u);
+            const uint32_t slot = group_bcast<G>(mySlot, leader);
+            if (gl == leader)
+                cand &= cand - 1u;
+            const uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
+            const bool eq = needle_equal_lane<G>(n, item, s, gl);
+            if (group_ballot<G>(!eq, gbase) == 0)
+                return (slab << kSlotBits) | slot;
+        }
+        if (next == kNoSlab)
+            return kNotFound;
+        slab = next;
+    }
+}
+
+// Per-lane find: one lane walks the whole 128-byte tag line with 16-byte loads.
+__device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* needle, uint32_t s, uint32_t h)
+{
+    const uint32_t tag4 = make_tag(h) * 0x01010101u;
+    uint32_t slab = h % t.numBuckets;
+    while (true) {
+        const uint4* line = reinterpret_cast<const uint4*>(t.tags + (size_t)slab * kTagWords);
+        uint32_t next = kNoSlab;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint4 v = line[q];
+            const uint32_t ws[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (q * 4 + i == (int)kNextWord) {
+                    next = ws[i];
+                    continue;
+                }
+                uint32_t c = compact_bytes(zero_bytes(ws[i] ^ tag4));
+                while (c) {
+                    const uint32_t slot = (uint32_t)(q * 4 + i) * 4 + (__ffs(c) - 1u);
+                    c &= c - 1u;
+                    const uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
+                    bool eq = true;
+                    for (uint32_t j = 0; j < s && eq; ++j)
+                        eq = item[j] == needle[j];
+                    if (eq)
+                        return (slab << kSlotBits) | slot;
+                }
+            }
+        }
+        if (next == kNoSlab)
+            return kNotFound;
+        slab = next;
+    }
+}
+
+// Group-cooperative insert of an item the caller knows to be absent (or, for
+// bulk multiset inserts, allowed to be present).  Claims a slot with a CAS on
+// its tag word; a full chain grows through chain_grow.
+template <int G>
+__device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G>& n, uint32_t s, uint32_t h,
+                                              uint32_t gl, uint32_t gbase)
+{
+    static_assert(G >= 4 && G <= 32, "group size");
+    constexpr int W = 32 / G;
+    const uint32_t tag = make_tag(h);
+    const uint32_t bucket = h % t.numBuckets;
+    uint32_t slab = bucket;
+    uint32_t headNext = kNoSlab;
+    while (true) {
+        uint32_t* line = t.tags + (size_t)slab * kTagWords;
+        uint32_t words[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            words[i] = ld_relaxed(line + gl * W + i);
+        uint32_t emp = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            if (gl * W + i != kNextWord)
+                emp |= compact_bytes(zero_bytes(words[i])) << (4 * i);
+        }
+        const uint32_t next = group_bcast<G>(words[W - 1], G - 1);
+        const uint32_t bal = group_ballot<G>(emp != 0, gbase);
+        if (bal) {
+            const uint32_t leader = __ffs(bal) - 1u;
+            uint32_t ok = 0, slot = 0;
+            if (gl == leader) {
+                const uint32_t bit = __ffs(emp) - 1u;
+                const uint32_t i = bit >> 2, b = bit & 3u;
+                uint32_t old = 0;
+#pragma unroll
+                for (int q = 0; q < W; ++q)
+                    if ((uint32_t)q == i)
+                        old = words[q];
+                const uint32_t r = atomicCAS(line + gl * W + i, old, old | (tag << (8u * b)));
+                ok = (r == old) ? 1u : 0u;
+                slot = (gl * W + i) * 4u + b;
+            }
+            ok = group_bcast<G>(ok, leader);
+            slot = group_bcast<G>(slot, leader);
+            if (ok) {
+                uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
+#pragma unroll
+                for (int k = 0; k < Needle<G>::K; ++k) {
+                    const uint32_t j = gl + (uint32_t)k * G;
+                    if (j < s)
+                        item[j] = n.w[k];
+                }
+                return (slab << kSlotBits) | slot;
+            }
+            continue;
+        }
+        if (slab == bucket)
+            headNext = next;
+        if (next != kNoSlab) {
+            slab = next;
+            continue;
+        }
+        uint32_t grown = 0;
+        if (gl == 0)
+            grown = chain_grow(t, bucket, headNext);
+        slab = group_bcast<G>(grown, 0);
+        if (slab == kNoSlab)
+            return kNotFound;
+    }
+}
+
+// Per-lane insert.
+__device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* item, uint32_t s, uint32_t h)
+{
+    const uint32_t tag = make_tag(h);
+    const uint32_t bucket = h % t.numBuckets;
+    uint32_t slab = bucket;
+    uint32_t headNext = kNoSlab;
+    while (true) {
+        uint32_t* line = t.tags + (size_t)slab * kTagWords;
+        bool retry = false;
+        for (uint32_t w = 0; w < kNextWord; ++w) {
+            const uint32_t v = ld_relaxed(line + w);
+            uint32_t e = compact_bytes(zero_bytes(v));
+            if (!e)
+                continue;
+            const uint32_t b = __ffs(e) - 1u;
+            const uint32_t r = atomicCAS(line + w, v, v | (tag << (8u * b)));
+            if (r != v) {
+                retry = true;
+                break;
+            }
+            const uint32_t slot = w * 4u + b;
+            uint32_t* dst = t.items + ((size_t)slab * kSlots + slot) * s;
+            for (uint32_t j = 0; j < s; ++j)
+                dst[j] = item[j];
+            return (slab << kSlotBits) | slot;
+        }
+        if (retry)
+            continue;
+        const uint32_t next = ld_relaxed(line + kNextWord);
+        if (slab == bucket)
+            headNext = next;
+        if (next != kNoSlab) {
+            slab = next;
+            continue;
+        }
+        slab = chain_grow(t, bucket, headNext);
+        if (slab == kNoSlab)
+            return kNotFound;
+    }
+}
+
+__host__ __device__ inline const uint32_t* table_item_ptr(const TableDev& t, uint32_t ptr)
+{
+    const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
+    return t.items + ((size_t)slab * kSlots + slot) * t.itemWords;
+}
+
+// ---------------------------------------------------------------- DAG encoding
+struct DagDev {
+    const TableDev* tables;   // [numTables], index = item words - 2
+    uint32_t* fullyFilled;    // [levels][numMaterials]
+    uint32_t levels;          // total levels D (voxel grid 2^D)
+    uint32_t leafLevel;       // D - 2
+    uint32_t M;               // material bits (0, 4 or 8)
+    uint32_t numMaterials;    // 1 << M
+    uint32_t minLeafWords;    // 2 + (M ? 1 : 0)
+    uint32_t maxLeafWords;    // 2 + 64*M/32
+    uint32_t leafSizeBits;    // bit_width(maxLeaf - minLeaf)
+    uint32_t numTables;
+};
+
+constexpr uint32_t kNodeSizeBits = 3; // bit_width(9 - 2)
+
+__host__ __device__ inline uint32_t node_size_from_header(uint32_t header)
+{
+    return 1u + (uint32_t)__builtin_popcount(header & 0xFFu);
+}
+__host__ __device__ inline uint32_t leaf_size_from_words(uint32_t w0, uint32_t w1, uint32_t M)
+{
+    const uint32_t n = (uint32_t)__builtin_popcount(w0) + (uint32_t)__builtin_popcount(w1);
+    return 2u + ((n * M + 31u) >> 5);
+}
+__host__ __device__ inline uint32_t node_handle(uint32_t ptr, uint32_t size)
+{
+    return ptr | ((size - 2u) << (32u - kNodeSizeBits));
+}
+__host__ __device__ inline uint32_t node_handle_size(uint32_t h)
+{
+    return 2u + (h >> (32u - kNodeSizeBits));
+}
+__host__ __device__ inline uint32_t node_handle_ptr(uint32_t h)
+{
+    return h & ((1u << (32u - kNodeSizeBits)) - 1u);
+}
+__host__ __device__ inline uint32_t leaf_handle(const DagDev& d, uint32_t ptr, uint32_t size)
+{
+    if (d.leafSizeBits == 0)
+        return ptr;
+    return ptr | ((size - d.minLeafWords) << (32u - d.leafSizeBits));
+}
+__host__ __device__ inline uint32_t leaf_handle_size(const DagDev& d, uint32_t h)
+{
+    if (d.leafSizeBits == 0)
+        return d.minLeafWords;
+    return d.minLeafWords + (h >> (32u - d.leafSizeBits));
+}
+__host__ __device__ inline uint32_t leaf_handle_ptr(const DagDev& d, uint32_t h)
+{
+    if (d.leafSizeBits == 0)
+        return h;
+    return h & ((1u << (32u - d.leafSizeBits)) - 1u);
+}
+__device__ inline const uint32_t* dag_node_ptr(const DagDev& d, uint32_t h)
+{
+    const uint32_t s = node_handle_size(h);
+    return table_item_ptr(d.tables[s - 2u], node_handle_ptr(h));
+}
+__device__ inline const uint32_t* dag_leaf_ptr(const DagDev& d, uint32_t h)
+{
+    const uint32_t s = leaf_handle_size(d, h);
+    return table_item_ptr(d.tables[s - 2u], leaf_handle_ptr(d, h));
+}
+__host__ __device__ inline bool header_full(uint32_t header, uint32_t M, uint32_t& mat)
+{
+    mat = M ? ((header >> 9) & ((1u << M) - 1u)) : 0u;
+    return (header >> 8) & 1u;
+}
+__host__ __device__ inline uint32_t header_payload(uint32_t header, uint32_t M)
+{
+    return header >> (9u + M);
+}
+__host__ __device__ inline uint32_t header_make(uint32_t mask, uint32_t payload, uint32_t M)
+{
+    return (mask & 0xFFu) | (payload << (9u + M));
+}
+__host__ __device__ inline uint32_t header_make_full(uint32_t mask, uint32_t mat, uint32_t M)
+{
+    return (mask & 0xFFu) | (1u << 8) | (M ? (mat << 9) : 0u);
+}
+// get_material (my_gpu_base_dag.h:293-316), M in {0, 4, 8} (no straddling).
+__host__ __device__ inline bool leaf_material(const uint32_t* leaf, uint32_t voxel, uint32_t M, uint32_t& mat)
+{
+    const uint64_t bits = (uint64_t)leaf[0] | ((uint64_t)leaf[1] << 32);
+    if (!((bits >> voxel) & 1ull))
+        return false;
+    if (M == 0) {
+        mat = 0;
+        return true;
+    }
+    const uint32_t off = (uint32_t)__builtin_popcountll(bits & ((1ull << voxel) - 1ull)) * M;
+    mat = (leaf[2 + (off >> 5)] >> (off & 31u)) & ((1u << M) - 1u);
+    return true;
+}
+
+} // namespace svdag

@@ -1,0 +1,312 @@
+"""ctypes binding of libsvdag (include/svdag.h) used by the tests and bench.py.
+
+Loads gpu-svdag-editing_amd/lib/libsvdag.so and fails loudly if it is missing:
+there is no fallback path.  The classes mirror the reference's interface
+names (MyGPUHashDAG, MyGPUHashDAGUndoRedo, DAGTracer) over the C ABI.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsvdag.so")
+
+SVDAG_OK = 0
+ERRORS = {-1: "INVALID", -2: "OOM", -3: "DEVICE", -4: "CAPACITY", -5: "EMPTY", -6: "NO_DEVICE"}
+
+SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE, PAINT, COPY = range(6)
+NOT_FOUND = 0xFFFFFFFF
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class SvdagError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"svdag error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Segment(ctypes.Structure):
+    _fields_ = [("item_words", ctypes.c_uint32), ("count", ctypes.c_uint32),
+                ("word_offset", ctypes.c_uint32), ("out_offset", ctypes.c_uint32)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("material_bits", ctypes.c_uint32),
+                ("load_factor", ctypes.c_uint32), ("group_size", ctypes.c_uint32),
+                ("expected_items", ctypes.c_uint64), ("arena_slabs", ctypes.c_uint64),
+                ("workspace_items", ctypes.c_uint64)]
+
+
+class Terrain(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("base", ctypes.c_double), ("amplitude", ctypes.c_double),
+                ("frequency", ctypes.c_double), ("octaves", ctypes.c_uint32), ("material", ctypes.c_uint32)]
+
+
+class EditDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("material", ctypes.c_uint32), ("center", ctypes.c_float * 3),
+                ("radius", ctypes.c_float), ("src", ctypes.c_float * 3), ("dst", ctypes.c_float * 3)]
+
+
+class EditStats(ctypes.Structure):
+    _fields_ = [("svo_items", ctypes.c_uint32 * 32), ("applied", ctypes.c_uint32),
+                ("root_before", ctypes.c_uint32), ("root_after", ctypes.c_uint32), ("gpu_ms", ctypes.c_float)]
+
+
+class GcStats(ctypes.Structure):
+    _fields_ = [("items_freed", ctypes.c_uint64), ("slabs_freed", ctypes.c_uint64),
+                ("items_live", ctypes.c_uint64), ("gpu_ms", ctypes.c_float)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_double * 3), ("rotation", ctypes.c_double * 9),
+                ("fov_degrees", ctypes.c_double), ("bounds_min", ctypes.c_double * 3),
+                ("bounds_max", ctypes.c_double * 3)]
+
+
+HIT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("material", "<u4"),
+                      ("t", "<f4"), ("px", "<f4"), ("py", "<f4"), ("pz", "<f4")])
+
+# Every symbol include/svdag.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "svdag_last_error", "svdag_abi_version", "svdag_device_count", "svdag_dev_alloc", "svdag_dev_free",
+    "svdag_memcpy_h2d", "svdag_memcpy_d2h", "svdag_device_synchronize",
+    "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
+    "svdag_hashset_read", "svdag_hashset_usage",
+    "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_terrain_heights", "svdag_root",
+    "svdag_edit", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
+    "svdag_trace_primary", "svdag_export_canonical", "svdag_export_tables",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with gpu-svdag-editing_amd/build_lib.py")
+        L = ctypes.CDLL(LIB_PATH)
+        L.svdag_last_error.restype = ctypes.c_char_p
+        L.svdag_abi_version.restype = ctypes.c_uint32
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != SVDAG_OK:
+        raise SvdagError(code, lib().svdag_last_error().decode(errors="replace"))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().svdag_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class DeviceBuffer:
+    """Device allocation owned by the caller (plumbing only)."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        check(lib().svdag_dev_alloc(ctypes.c_size_t(self.nbytes), ctypes.byref(self.ptr)))
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        if arr.nbytes:
+            check(lib().svdag_memcpy_h2d(b.ptr, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(arr.nbytes)))
+        return b
+
+    def to_array(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().svdag_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(out.nbytes)))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().svdag_dev_free(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_segments(groups):
+    """groups: list of (item_words, uint32 array [count, item_words]) -> (flat words, Segment array)."""
+    segs = (Segment * len(groups))()
+    flat = []
+    woff = 0
+    ooff = 0
+    for i, (s, arr) in enumerate(groups):
+        arr = np.ascontiguousarray(arr, dtype=np.uint32).reshape(-1, s)
+        segs[i].item_words = s
+        segs[i].count = arr.shape[0]
+        segs[i].word_offset = woff
+        segs[i].out_offset = ooff
+        flat.append(arr.reshape(-1))
+        woff += arr.size
+        ooff += arr.shape[0]
+    words = np.concatenate(flat) if flat else np.zeros(0, np.uint32)
+    return words, segs, ooff
+
+
+class HashSet:
+    """Slab hash tables, one per item size (CompactAccelerationHashTable per size)."""
+
+    def __init__(self, min_words, max_words, buckets, reserve=None):
+        n = max_words - min_words + 1
+        b = (ctypes.c_uint32 * n)(*[int(x) for x in buckets])
+        r = (ctypes.c_uint32 * n)(*[int(x) for x in (reserve if reserve is not None else [0] * n)])
+        self.h = ctypes.c_void_p()
+        check(lib().svdag_hashset_create(min_words, max_words, b, r, ctypes.byref(self.h)))
+        self.min_words, self.max_words = min_words, max_words
+
+    def _bulk(self, fn, words_dev, segs, out_dev, variant):
+        ms = ctypes.c_float(0)
+        check(fn(self.h, ctypes.cast(words_dev.ptr, u32p), segs, len(segs), ctypes.cast(out_dev.ptr, u32p),
+                 int(variant), ctypes.byref(ms)))
+        return ms.value
+
+    def add_dev(self, words_dev, segs, out_dev, variant=16):
+        return self._bulk(lib().svdag_hashset_add_dev, words_dev, segs, out_dev, variant)
+
+    def find_dev(self, words_dev, segs, out_dev, variant=16):
+        return self._bulk(lib().svdag_hashset_find_dev, words_dev, segs, out_dev, variant)
+
+    def read(self, item_words, ptrs):
+        ptrs = np.ascontiguousarray(ptrs, dtype=np.uint32)
+        out = np.zeros((len(ptrs), item_words), np.uint32)
+        check(lib().svdag_hashset_read(self.h, item_words, ptrs.ctypes.data_as(u32p), len(ptrs),
+                                       out.ctypes.data_as(u32p)))
+        return out
+
+    def usage(self, item_words):
+        out = (ctypes.c_uint32 * 4)()
+        check(lib().svdag_hashset_usage(self.h, item_words, out))
+        return list(out)
+
+    def close(self):
+        if self.h:
+            check(lib().svdag_hashset_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def look_at(position, target, up=(0.0, 1.0, 0.0)):
+    """CameraView rotation rows (row 0 = -right, row 1 = up, row 2 = forward)."""
+    p = np.asarray(position, float)
+    f = np.asarray(target, float) - p
+    f /= np.linalg.norm(f)
+    r = np.cross(f, np.asarray(up, float))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    return np.concatenate([-r, u, f])
+
+
+class MyGPUHashDAG:
+    """Editable DAG on the GPU (reference: MyGPUHashDAG<GPU_Malloc> with
+    MyGPUHashDAGUndoRedo, editMyHashDag and DAGTracer::resolve_paths)."""
+
+    def __init__(self, levels, material_bits=4, load_factor=96, group_size=0, expected_items=0,
+                 arena_slabs=0, workspace_items=0):
+        cfg = Config(levels, material_bits, load_factor, group_size, expected_items, arena_slabs, workspace_items)
+        self.h = ctypes.c_void_p()
+        check(lib().svdag_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+        self.levels = levels
+        self.material_bits = material_bits
+
+    def build_terrain(self, seed, base, amplitude, frequency, octaves, material):
+        t = Terrain(seed, base, amplitude, frequency, octaves, material)
+        check(lib().svdag_build_terrain(self.h, ctypes.byref(t)))
+
+    def terrain_heights(self, x0, z0, w, d):
+        out = np.empty((d, w), np.int32)
+        check(lib().svdag_terrain_heights(self.h, x0, z0, w, d, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    @property
+    def root(self):
+        r = ctypes.c_uint32()
+        check(lib().svdag_root(self.h, ctypes.byref(r)))
+        return r.value
+
+    def edit(self, kind, material, center, radius, src=(0, 0, 0), dst=(0, 0, 0)):
+        d = EditDesc(kind, material, (ctypes.c_float * 3)(*center), float(radius),
+                     (ctypes.c_float * 3)(*src), (ctypes.c_float * 3)(*dst))
+        st = EditStats()
+        check(lib().svdag_edit(self.h, ctypes.byref(d), ctypes.byref(st)))
+        return st
+
+    def undo(self):
+        check(lib().svdag_undo(self.h))
+
+    def redo(self):
+        check(lib().svdag_redo(self.h))
+
+    def history(self):
+        out = (ctypes.c_uint32 * 2)()
+        check(lib().svdag_history(self.h, out))
+        return tuple(out)
+
+    def garbage_collect(self):
+        st = GcStats()
+        check(lib().svdag_gc(self.h, ctypes.byref(st)))
+        return st
+
+    def query(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
+        out = np.empty(len(xyz), np.uint32)
+        check(lib().svdag_query_voxels(self.h, xyz.ctypes.data_as(u32p), len(xyz), out.ctypes.data_as(u32p)))
+        return out
+
+    def trace(self, position, rotation, fov, bmin, bmax, w, h):
+        cam = Camera((ctypes.c_double * 3)(*position), (ctypes.c_double * 9)(*rotation), float(fov),
+                     (ctypes.c_double * 3)(*bmin), (ctypes.c_double * 3)(*bmax))
+        out = np.empty(w * h, HIT_DTYPE)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_trace_primary(self.h, ctypes.byref(cam), w, h, out.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(ms)))
+        return out, ms.value
+
+    def canonical(self, dump=False):
+        hsh = ctypes.c_uint64()
+        n = ctypes.c_size_t(0)
+        counts = (ctypes.c_uint32 * 32)()
+        check(lib().svdag_export_canonical(self.h, ctypes.byref(hsh), None, ctypes.byref(n), counts))
+        if not dump:
+            return hsh.value, list(counts)
+        buf = np.empty(n.value, np.uint32)
+        check(lib().svdag_export_canonical(self.h, ctypes.byref(hsh), buf.ctypes.data_as(u32p), ctypes.byref(n),
+                                           counts))
+        return hsh.value, list(counts), buf
+
+    def export_records(self):
+        n = ctypes.c_size_t(0)
+        check(lib().svdag_export_tables(self.h, None, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint32)
+        check(lib().svdag_export_tables(self.h, buf.ctypes.data_as(u32p), ctypes.byref(n)))
+        return buf
+
+    def close(self):
+        if self.h:
+            check(lib().svdag_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

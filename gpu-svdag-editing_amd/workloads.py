@@ -1,0 +1,110 @@
+"""Seeded synthetic workloads.
+
+slab_hash_workload: the slab-hash microbenchmark input -- DAG interior nodes
+of 2..9 words (child mask uniform over the 255 non-empty masks, child words
+uniform u32), grouped by size, plus a find set with a given hit rate (hits
+drawn uniformly from the inserted items, misses fresh random items with the
+same size distribution).  Item seed 3, hit seed 4, miss seed 5.
+"""
+import numpy as np
+
+POPC8 = np.array([bin(i).count("1") for i in range(256)], dtype=np.int64)
+
+
+def _items_of_size(rng, s, n):
+    masks_with = np.nonzero(POPC8 == s - 1)[0]
+    out = np.empty((n, s), dtype=np.uint32)
+    out[:, 0] = masks_with[rng.integers(0, len(masks_with), n)]
+    if s > 1:
+        out[:, 1:] = rng.integers(0, 1 << 32, size=(n, s - 1), dtype=np.uint64).astype(np.uint32)
+    return out
+
+
+def slab_hash_items(n_items, seed=3):
+    """Returns {size: uint32[count, size]} for n_items DAG interior nodes."""
+    rng = np.random.default_rng(seed)
+    masks = rng.integers(1, 256, n_items)
+    sizes = 1 + POPC8[masks]
+    out = {}
+    for s in range(2, 10):
+        sel = masks[sizes == s]
+        n = len(sel)
+        if n == 0:
+            continue
+        arr = np.empty((n, s), dtype=np.uint32)
+        arr[:, 0] = sel
+        arr[:, 1:] = rng.integers(0, 1 << 32, size=(n, s - 1), dtype=np.uint64).astype(np.uint32)
+        out[s] = arr
+    return out
+
+
+def slab_hash_finds(items, n_finds, hit_rate=0.5, hit_seed=4, miss_seed=5):
+    """Find set with the inserted size distribution: {size: uint32[count, size]}."""
+    total = sum(len(a) for a in items.values())
+    rh = np.random.default_rng(hit_seed)
+    rm = np.random.default_rng(miss_seed)
+    out = {}
+    for s, arr in items.items():
+        n_s = int(round(n_finds * len(arr) / total))
+        hits = int(round(n_s * hit_rate))
+        h = arr[rh.integers(0, len(arr), hits)]
+        m = _items_of_size(rm, s, n_s - hits)
+        both = np.concatenate([h, m]) if len(m) else h
+        perm = rh.permutation(len(both))
+        out[s] = np.ascontiguousarray(both[perm])
+    return out
+
+
+def buckets_for(items, load_factor=96):
+    """Bucket count per size, ceil(n_s / load_factor) (reference TARGET_LOAD_FACTOR)."""
+    return {s: max(1, -(-len(a) // load_factor)) for s, a in items.items()}
+
+
+# ---------------------------------------------------------------- editing workloads
+SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE, PAINT, COPY = range(6)
+
+
+def terrain_params(levels):
+    """fBm heightfield terrain: 5 octaves at depth 10 (base 512, amplitude 200,
+    frequency 1/256); at depth >= 15 the same shape scaled to the grid (base
+    2^(D-1), amplitude 6000 * 2^(D-15), 8 octaves, 4 periods across)."""
+    side = 1 << levels
+    if levels < 15:
+        scale = side / 1024.0
+        return dict(seed=0x5EED, base=512.0 * scale, amplitude=200.0 * scale, frequency=1.0 / (256.0 * scale),
+                    octaves=5, material=1)
+    return dict(seed=0x5EED, base=side / 2.0, amplitude=6000.0 * side / 32768.0, frequency=4.0 / side,
+                octaves=8, material=1)
+
+
+def config1_edits():
+    """Sphere-add r=64 (centre seed 1), then sphere-remove r=48 (centre seed 2), in [0,1024)^3."""
+    c1 = np.random.default_rng(1).uniform(0, 1024, 3).astype(np.float32)
+    c2 = np.random.default_rng(2).uniform(0, 1024, 3).astype(np.float32)
+    return [(SPHERE_ADD, 2, tuple(float(x) for x in c1), 64.0),
+            (SPHERE_REMOVE, 0, tuple(float(x) for x in c2), 48.0)]
+
+
+def surface_edits(heights, n, seed, levels, rmin=8.0, rmax=512.0, kinds=(0.4, 0.4, 0.2), near=1024.0):
+    """Edit stream near the terrain surface: sphere-add / sphere-remove / box-fill
+    with the given probabilities, radius log-uniform in [rmin, rmax], centre
+    (x, h(x,z) + U[-near, near], z)."""
+    rng = np.random.default_rng(seed)
+    side = 1 << levels
+    out = []
+    for _ in range(n):
+        k = rng.choice(3, p=kinds)
+        kind = (SPHERE_ADD, SPHERE_REMOVE, BOX_ADD)[k]
+        r = float(np.exp(rng.uniform(np.log(rmin), np.log(rmax))))
+        x, z = rng.integers(0, side, 2)
+        y = float(heights(int(x), int(z))) + float(rng.uniform(-near, near))
+        y = min(max(y, 0.0), side - 1.0)
+        mat = 0 if kind == SPHERE_REMOVE else int(rng.integers(2, 15))
+        out.append((kind, mat, (float(x), y, float(z)), r))
+    return out
+
+
+def config1_camera(levels=10):
+    side = float(1 << levels)
+    return dict(position=(-0.1 * side, 0.85 * side, -0.1 * side), target=(0.5 * side, 0.35 * side, 0.5 * side),
+                fov=60.0, bmin=(0.0, 0.0, 0.0), bmax=(side, side, side))

@@ -1,0 +1,223 @@
+/*
+ * svdag.h - C ABI of the MI355X (gfx950) SVDAG editing back-end.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (SVDAG_OK = 0, negative = error; svdag_last_error() gives the message).
+ * Buffers are host memory unless the parameter name ends in _dev.
+ * One context = one HIP stream; calls on one context must not overlap.
+ *
+ * Each entry point names the reference interface (mathijs727/GPU-SVDAG-Editing,
+ * file:line) it stands in for.
+ */
+#ifndef SVDAG_H
+#define SVDAG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SVDAG_OK = 0,
+    SVDAG_E_INVALID = -1,   /* bad argument */
+    SVDAG_E_OOM = -2,       /* slab arena exhausted (device flag); DAG unchanged */
+    SVDAG_E_DEVICE = -3,    /* HIP runtime error */
+    SVDAG_E_CAPACITY = -4,  /* edit workspace overflow; DAG unchanged */
+    SVDAG_E_EMPTY = -5,     /* operation on an empty DAG */
+    SVDAG_E_NO_DEVICE = -6  /* no GPU visible */
+};
+
+/* Message of the last failed call on this thread. */
+const char* svdag_last_error(void);
+/* ABI version (bumped on incompatible changes). */
+uint32_t svdag_abi_version(void);
+
+/* ------------------------------------------------------------------ device memory
+ * Thin plumbing for callers without their own device allocator.
+ * (reference: Memory::malloc / GpuMemoryPool, memory.h:29-63, 192-224)            */
+int svdag_device_count(int* out_count);
+int svdag_dev_alloc(size_t bytes, void** out_ptr_dev);
+int svdag_dev_free(void* ptr_dev);
+int svdag_memcpy_h2d(void* dst_dev, const void* src, size_t bytes);
+int svdag_memcpy_d2h(void* dst, const void* src_dev, size_t bytes);
+int svdag_device_synchronize(void);
+
+/* ------------------------------------------------------------------ slab hash tables
+ * A set of slab hash tables, one per item size in [min_words, max_words].
+ * Stands in for CompactAccelerationHashTable + GpuHashTableBase
+ * (hash_tables/compact_acceleration_hash_table.h:13-402, gpu_hash_table_base.h:51-326)
+ * and the HashMapBench bulk harness (extras/hashmap_bench/src/main.cu:124-370).      */
+typedef struct svdag_hashset svdag_hashset;
+
+/* A run of items of one size packed in a word buffer. */
+typedef struct {
+    uint32_t item_words;   /* words per item */
+    uint32_t count;        /* number of items */
+    uint32_t word_offset;  /* first word in the word buffer */
+    uint32_t out_offset;   /* first result slot */
+} svdag_segment;
+
+/* Variant of a bulk operation: 0 = one lane per item (cf. add/find),
+ * 4/8/16/32 = that many lanes of a wave64 per item (cf. addAsWarp/findAsWarp). */
+enum { SVDAG_LANE = 0 };
+
+/* reference: CompactAccelerationHashTable::allocate(numBuckets, numElements, itemSize)
+ * (compact_acceleration_hash_table.cu:5-13) for each size.                           */
+int svdag_hashset_create(uint32_t min_words, uint32_t max_words, const uint32_t* buckets_per_size,
+                         const uint32_t* reserve_items_per_size, svdag_hashset** out);
+int svdag_hashset_destroy(svdag_hashset* hs);
+/* Bulk insert (multiset semantics); out_ptrs_dev[i] = table pointer of item i.
+ * reference: add / addAsWarp (compact_acceleration_hash_table.h:27-121).
+ * elapsed_ms (optional) receives the kernel time measured with HIP events.           */
+int svdag_hashset_add_dev(svdag_hashset* hs, const uint32_t* words_dev, const svdag_segment* segs, uint32_t nseg,
+                          uint32_t* out_ptrs_dev, int variant, float* elapsed_ms);
+/* Bulk find; out_ptrs_dev[i] = pointer of a stored equal item or 0xFFFFFFFF.
+ * reference: find / findAsWarp (compact_acceleration_hash_table.h:205-280).          */
+int svdag_hashset_find_dev(svdag_hashset* hs, const uint32_t* words_dev, const svdag_segment* segs, uint32_t nseg,
+                           uint32_t* out_ptrs_dev, int variant, float* elapsed_ms);
+/* Decode pointers to item words (host out buffer n*item_words).
+ * reference: decodePointer (compact_acceleration_hash_table.h:398-402).              */
+int svdag_hashset_read(svdag_hashset* hs, uint32_t item_words, const uint32_t* ptrs, uint32_t n, uint32_t* out_words);
+/* Slab usage of one table: out[0]=buckets, out[1]=slab capacity, out[2]=slabs ever
+ * handed out (head + overflow, recycled ones counted once), out[3]=oom flag.  reference: memory_used_by_slabs (gpu_hash_table_base.cu:187-206). */
+int svdag_hashset_usage(svdag_hashset* hs, uint32_t item_words, uint32_t out[4]);
+
+/* ------------------------------------------------------------------ editable DAG
+ * reference: EDag::MyGpuDag = MyGPUHashDAG<EMemoryType::GPU_Malloc>
+ * (dags/my_gpu_dags/my_gpu_hash_dag/my_gpu_hash_dag.h:59-339; engine.h:31-37).       */
+typedef struct svdag_ctx svdag_ctx;
+
+typedef struct {
+    uint32_t levels;           /* D: the voxel grid is 2^D per side (reference MAX_LEVELS) */
+    uint32_t material_bits;    /* 0, 4 or 8 (reference HASH_DAG_MATERIAL_BITS) */
+    uint32_t load_factor;      /* items per bucket when sizing tables (reference TARGET_LOAD_FACTOR=96) */
+    uint32_t group_size;       /* lanes per item in DAG find/insert: 0 = default, 1, 4, 8, 16, 32 */
+    uint64_t expected_items;   /* expected DAG items, sizes the bucket count (0 = 1<<20) */
+    uint64_t arena_slabs;      /* overflow slab reserve per table (0 = buckets/2 + 4096) */
+    uint64_t workspace_items;  /* edit workspace capacity in work items (0 = 1<<22) */
+} svdag_config;
+
+/* reference: MyGPUHashDAG::allocate (my_gpu_hash_dag.cu:24-66) incl. the fully-filled chain. */
+int svdag_create(const svdag_config* cfg, svdag_ctx** out);
+/* reference: MyGPUHashDAG::free (my_gpu_hash_dag.cu:68-76). */
+int svdag_destroy(svdag_ctx* ctx);
+
+/* Seeded fBm heightfield terrain (no reference counterpart: the reference
+ * imports scene files through MyGPUHashDAGFactory::load_from_DAG,
+ * my_gpu_hash_dag_factory.cpp:426-508, whose fully-filled rule (357-360,
+ * 406-407) the built DAG follows).  Voxel (x,y,z) is solid iff y <= h(x,z),
+ * h = floor(base + amplitude * fbm(x*freq, z*freq)); material = material.     */
+typedef struct {
+    uint64_t seed;
+    double base;
+    double amplitude;
+    double frequency;          /* cycles per voxel of octave 0 */
+    uint32_t octaves;
+    uint32_t material;
+} svdag_terrain;
+int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* terrain);
+/* Heights of a rectangle [x0,x0+w) x [z0,z0+d), row-major in z (out: w*d int32). */
+int svdag_terrain_heights(svdag_ctx* ctx, uint32_t x0, uint32_t z0, uint32_t w, uint32_t d, int32_t* out);
+
+/* reference: MyGPUHashDAG::get_first_node_index / firstNodeIndex (my_gpu_hash_dag.h:72, 290-293). */
+int svdag_root(svdag_ctx* ctx, uint32_t* out_root);
+
+/* Edit tools.  reference: my_gpu_dag_editors.h
+ *   SPHERE_ADD/REMOVE  MyGpuSphereEditor<bool>(center, radius, material)  (457-496)
+ *   BOX_ADD/REMOVE     MyGpuBoxEditor<bool>(center, radius, material)     (82-119)
+ *   PAINT              MyGpuSpherePaintEditor(center, radius, color, mat) (498-536)
+ *   COPY               MyGpuCopyEditor(dag, src, dst, center, radius)     (127-253)  */
+enum {
+    SVDAG_SPHERE_ADD = 0,
+    SVDAG_SPHERE_REMOVE = 1,
+    SVDAG_BOX_ADD = 2,
+    SVDAG_BOX_REMOVE = 3,
+    SVDAG_PAINT = 4,
+    SVDAG_COPY = 5
+};
+typedef struct {
+    uint32_t kind;
+    uint32_t material;
+    float center[3];
+    float radius;
+    float src[3];   /* COPY only */
+    float dst[3];   /* COPY only */
+} svdag_edit_desc;
+
+typedef struct {
+    uint32_t svo_items[32];    /* edit-SVO items per level (index = level) */
+    uint32_t applied;          /* 0 when the edit made no leaf (reference skips the merge) */
+    uint32_t root_before;
+    uint32_t root_after;
+    float gpu_ms;              /* device time of the edit (HIP events) */
+} svdag_edit_stats;
+
+/* Apply one edit synchronously and record the previous root as an undo frame.
+ * reference: editMyHashDag<Editor>(editor, dag, undoRedo, stats, memPool, stream)
+ * (my_gpu_hash_dag_edits.h:12-13; my_gpu_hash_dag_edits.cu:633-671).  stats may be NULL. */
+int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* edit, svdag_edit_stats* stats);
+
+/* reference: MyGPUHashDAGUndoRedo::undo / redo (my_gpu_base_dag.h:35-64). */
+int svdag_undo(svdag_ctx* ctx);
+int svdag_redo(svdag_ctx* ctx);
+/* Undo/redo stack depths: out[0] = undo frames, out[1] = redo frames. */
+int svdag_history(svdag_ctx* ctx, uint32_t out[2]);
+
+/* reference: MyGPUHashDAGUndoRedo::garbageCollect -> MyGPUHashDAG::garbageCollect
+ * (my_gpu_base_dag.h:66-77; my_gpu_hash_dag.cu:364-475).  Clears the history,
+ * marks from the root and the fully-filled chain, and sweeps (always). */
+typedef struct {
+    uint64_t items_freed;
+    uint64_t slabs_freed;
+    uint64_t items_live;
+    float gpu_ms;
+} svdag_gc_stats;
+int svdag_gc(svdag_ctx* ctx, svdag_gc_stats* stats);
+
+/* Voxel queries: xyz = n*3 uint32 (host); out_material[i] = material or 0xFFFFFFFF.
+ * reference: DAGUtils::get_value (dag_utils.h:146-182) / DAGTracer::get_voxel_values
+ * (dag_tracer.cu:361-390). */
+int svdag_query_voxels(svdag_ctx* ctx, const uint32_t* xyz, uint32_t n, uint32_t* out_material);
+
+/* Primary rays.  reference: DAGTracer::resolve_paths -> Tracer::trace_paths
+ * (dag_tracer.cu:59-134; tracer.cu:186-273, 353-411).  Camera as CameraView
+ * (camera_view.h): position, 3x3 rotation (row 0 = -right, row 1 = up,
+ * row 2 = forward), fov in degrees; bounds as DAGInfo (dag_info.h). */
+typedef struct {
+    double position[3];
+    double rotation[9];
+    double fov_degrees;
+    double bounds_min[3];
+    double bounds_max[3];
+} svdag_camera;
+/* One hit record per pixel, stored at row (height-1-y) like the reference. */
+typedef struct {
+    uint32_t x, y, z;          /* hit voxel path (0,0,0 on a miss) */
+    uint32_t material;         /* material id, 0xFFFFFFFF on a miss */
+    float t;                   /* ray parameter of the entry face */
+    float px, py, pz;          /* hit position */
+} svdag_hit;
+int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* camera, uint32_t width, uint32_t height,
+                        svdag_hit* out_hits, float* gpu_ms);
+
+/* Canonical form of the reachable DAG (nodes renumbered per level by sorted
+ * content, children replaced by canonical ids).  hash = FNV-1a 64 of the dump.
+ * If buf is non-NULL and *len_words is large enough the dump is written;
+ * *len_words always receives the dump length.  out_counts (optional, 32
+ * entries) receives unique items per level. */
+int svdag_export_canonical(svdag_ctx* ctx, uint64_t* out_hash, uint32_t* buf, size_t* len_words,
+                           uint32_t* out_counts);
+
+/* Reachable DAG as flat records (for host-side checkers).  *len_words
+ * receives the number of words needed; the dump is written if buf is large
+ * enough.  Layout: [levels, material_bits, root, n_records] then per record
+ * [level, handle, size, words[size]] for every reachable (level, handle). */
+int svdag_export_tables(svdag_ctx* ctx, uint32_t* buf, size_t* len_words);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVDAG_H */

@@ -1,0 +1,233 @@
+"""ctypes loader of liboracle.so -- test infrastructure only (tests/, smoke(),
+bench.py's cpu_baseline leg).  Never imported by the product."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+u32p = ctypes.POINTER(ctypes.c_uint32)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ora_table_create.restype = ctypes.c_void_p
+        L.ora_table_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.ora_table_destroy.argtypes = [ctypes.c_void_p]
+        for f in ("ora_table_add", "ora_table_decode"):
+            getattr(L, f).argtypes = [ctypes.c_void_p, u32p, ctypes.c_uint32, u32p]
+        L.ora_table_find.argtypes = [ctypes.c_void_p, u32p, ctypes.c_uint32, u32p, ctypes.c_int]
+        L.ora_xor_hash.restype = ctypes.c_uint32
+        L.ora_xor_hash.argtypes = [u32p, ctypes.c_uint32]
+        L.ora_accel_hash.restype = ctypes.c_uint32
+        L.ora_accel_hash.argtypes = [u32p, ctypes.c_uint32]
+        L.ora_table_overflow_slabs.restype = ctypes.c_uint32
+        L.ora_table_overflow_slabs.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(u32p)
+
+
+class OracleTable:
+    """The reference CompactAccelerationHashTable (host path), restated."""
+
+    def __init__(self, item_words, num_buckets):
+        self.s = item_words
+        self.h = lib().ora_table_create(item_words, num_buckets)
+
+    def add(self, items):
+        items = np.ascontiguousarray(items, dtype=np.uint32).reshape(-1, self.s)
+        out = np.empty(len(items), np.uint32)
+        lib().ora_table_add(self.h, _p(items), len(items), _p(out))
+        return out
+
+    def find(self, items, threads=1):
+        items = np.ascontiguousarray(items, dtype=np.uint32).reshape(-1, self.s)
+        out = np.empty(len(items), np.uint32)
+        lib().ora_table_find(self.h, _p(items), len(items), _p(out), threads)
+        return out
+
+    def decode(self, ptrs):
+        ptrs = np.ascontiguousarray(ptrs, dtype=np.uint32)
+        out = np.empty((len(ptrs), self.s), np.uint32)
+        lib().ora_table_decode(self.h, _p(ptrs), len(ptrs), _p(out))
+        return out
+
+    def overflow_slabs(self):
+        return lib().ora_table_overflow_slabs(self.h)
+
+    def __del__(self):
+        try:
+            lib().ora_table_destroy(self.h)
+        except Exception:
+            pass
+
+
+def xor_hash(item):
+    a = np.ascontiguousarray(item, dtype=np.uint32)
+    return lib().ora_xor_hash(_p(a), len(a))
+
+
+def accel_hash(item):
+    a = np.ascontiguousarray(item, dtype=np.uint32)
+    return lib().ora_accel_hash(_p(a), len(a))
+
+
+def _dag_sigs(L):
+    vp = ctypes.c_void_p
+    f32p = ctypes.POINTER(ctypes.c_float)
+    f64p = ctypes.POINTER(ctypes.c_double)
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    L.ora_dag_create.restype = vp
+    L.ora_dag_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.ora_dag_destroy.argtypes = [vp]
+    L.ora_dag_root.restype = ctypes.c_uint32
+    L.ora_dag_root.argtypes = [vp]
+    L.ora_dag_set_root.argtypes = [vp, ctypes.c_uint32]
+    L.ora_dag_item_count.restype = ctypes.c_uint64
+    L.ora_dag_item_count.argtypes = [vp]
+    L.ora_terrain_heights.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]
+    L.ora_dag_build_terrain.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint32, ctypes.c_uint32]
+    L.ora_dag_edit.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, f32p, ctypes.c_float, f32p, f32p, u32p]
+    L.ora_dag_query.argtypes = [vp, u32p, ctypes.c_uint32, u32p]
+    L.ora_dag_canonical.restype = ctypes.c_uint64
+    L.ora_dag_canonical.argtypes = [vp, u32p, szp, u32p]
+    L.ora_canonical_from_records.restype = ctypes.c_uint64
+    L.ora_canonical_from_records.argtypes = [u32p, ctypes.c_size_t, u32p, szp, u32p]
+    L.ora_dag_live_items.restype = ctypes.c_uint64
+    L.ora_dag_live_items.argtypes = [vp]
+    L.ora_dag_trace.argtypes = [vp, f64p, f64p, ctypes.c_double, f64p, f64p, ctypes.c_uint32, ctypes.c_uint32, u32p]
+    L.ora_leaf_build.restype = ctypes.c_uint32
+    L.ora_leaf_build.argtypes = [ctypes.c_uint32, u32p, u32p, u32p]
+    L.ora_leaf_material.argtypes = [ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
+    L.ora_leaf_size.restype = ctypes.c_uint32
+    L.ora_leaf_size.argtypes = [ctypes.c_uint32, u32p]
+
+
+_dag_ready = False
+
+
+def dlib():
+    global _dag_ready
+    L = lib()
+    if not _dag_ready:
+        _dag_sigs(L)
+        _dag_ready = True
+    return L
+
+
+SENT = 0xFFFFFFFF
+HIT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("material", "<u4"),
+                      ("t", "<f4"), ("px", "<f4"), ("py", "<f4"), ("pz", "<f4")])
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def _d(v):
+    return (ctypes.c_double * len(v))(*[float(x) for x in v])
+
+
+def terrain_heights(seed, base, amp, freq, octaves, x0, z0, w, d):
+    out = np.empty((d, w), np.int32)
+    dlib().ora_terrain_heights(seed, base, amp, freq, octaves, x0, z0, w, d,
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out
+
+
+class OracleDag:
+    """The reference's GPU editing path, restated on the CPU."""
+
+    def __init__(self, levels, material_bits=4):
+        self.levels = levels
+        self.h = dlib().ora_dag_create(levels, material_bits)
+        if not self.h:
+            raise ValueError("bad oracle DAG parameters")
+
+    def build_terrain(self, seed, base, amp, freq, octaves, material):
+        dlib().ora_dag_build_terrain(self.h, seed, base, amp, freq, octaves, material)
+
+    @property
+    def root(self):
+        return dlib().ora_dag_root(self.h)
+
+    @root.setter
+    def root(self, r):
+        dlib().ora_dag_set_root(self.h, r)
+
+    def edit(self, kind, material, center, radius, src=(0, 0, 0), dst=(0, 0, 0)):
+        counts = (ctypes.c_uint32 * 32)()
+        applied = dlib().ora_dag_edit(self.h, kind, material, _f3(center), float(radius), _f3(src), _f3(dst), counts)
+        return applied, list(counts)
+
+    def query(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
+        out = np.empty(len(xyz), np.uint32)
+        dlib().ora_dag_query(self.h, _p(xyz), len(xyz), _p(out))
+        return out
+
+    def canonical(self, dump=False):
+        n = ctypes.c_size_t(0)
+        counts = (ctypes.c_uint32 * 32)()
+        h = dlib().ora_dag_canonical(self.h, None, ctypes.byref(n), counts)
+        if not dump:
+            return h, list(counts)
+        buf = np.empty(n.value, np.uint32)
+        dlib().ora_dag_canonical(self.h, _p(buf), ctypes.byref(n), counts)
+        return h, list(counts), buf
+
+    def live_items(self):
+        return dlib().ora_dag_live_items(self.h)
+
+    def trace(self, position, rotation, fov, bmin, bmax, w, h):
+        out = np.empty(w * h, HIT_DTYPE)
+        dlib().ora_dag_trace(self.h, _d(position), _d(rotation), float(fov), _d(bmin), _d(bmax), w, h,
+                             out.ctypes.data_as(u32p))
+        return out
+
+    def __del__(self):
+        try:
+            dlib().ora_dag_destroy(self.h)
+        except Exception:
+            pass
+
+
+def canonical_from_records(records):
+    records = np.ascontiguousarray(records, dtype=np.uint32)
+    n = ctypes.c_size_t(0)
+    counts = (ctypes.c_uint32 * 32)()
+    h = dlib().ora_canonical_from_records(_p(records), len(records), None, ctypes.byref(n), counts)
+    return h, list(counts)
+
+
+def leaf_build(material_bits, present, materials):
+    present = np.ascontiguousarray(present, dtype=np.uint32)
+    materials = np.ascontiguousarray(materials, dtype=np.uint32)
+    out = np.zeros(20, np.uint32)
+    n = dlib().ora_leaf_build(material_bits, _p(present), _p(materials), _p(out))
+    return out[:n]
+
+
+def leaf_material(material_bits, leaf, voxel):
+    leaf = np.ascontiguousarray(leaf, dtype=np.uint32)
+    m = ctypes.c_uint32(0)
+    ok = dlib().ora_leaf_material(material_bits, _p(leaf), voxel, ctypes.byref(m))
+    return m.value if ok else None
+
+
+def leaf_size(material_bits, leaf):
+    leaf = np.ascontiguousarray(leaf, dtype=np.uint32)
+    return dlib().ora_leaf_size(material_bits, _p(leaf))

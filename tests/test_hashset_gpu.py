@@ -1,0 +1,108 @@
+"""GPU slab hash tables (lane and wave64-group variants) against the oracle's
+restatement of CompactAccelerationHashTable: set semantics, bit-exact words."""
+import numpy as np
+import pytest
+
+import oracle_py
+import svdag_py
+import workloads
+
+VARIANTS = [0, 4, 8, 16, 32]
+
+
+def _build(items, load_factor=96, reserve_scale=1):
+    sizes = list(range(2, 10))
+    b = workloads.buckets_for(items, load_factor)
+    hs = svdag_py.HashSet(2, 9, [b.get(s, 1) for s in sizes],
+                          [reserve_scale * len(items.get(s, [])) for s in sizes])
+    return hs, b
+
+
+def _oracle_found(items, finds, buckets):
+    out = {}
+    for s, a in items.items():
+        t = oracle_py.OracleTable(s, buckets[s])
+        t.add(a)
+        out[s] = t.find(finds[s]) != 0xFFFFFFFF
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_bulk_add_find_parity(variant):
+    items = workloads.slab_hash_items(1 << 17)
+    finds = workloads.slab_hash_finds(items, 1 << 17)
+    hs, buckets = _build(items)
+    groups = sorted(items.items())
+    words, segs, n = svdag_py.make_segments(groups)
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    hs.add_dev(wdev, segs, odev, variant)
+    ptrs = odev.to_array(np.uint32, n)
+    assert np.all(ptrs != svdag_py.NOT_FOUND)
+    # Every insert decodes back to its own words.
+    for i, (s, a) in enumerate(groups):
+        seg = segs[i]
+        got = hs.read(s, ptrs[seg.out_offset:seg.out_offset + seg.count])
+        assert np.array_equal(got, a), s
+    # Finds: present exactly when the oracle finds the item, and the found
+    # pointer decodes to the needle.
+    fgroups = [(s, finds[s]) for s, _ in groups]
+    fwords, fsegs, fn = svdag_py.make_segments(fgroups)
+    fdev = svdag_py.DeviceBuffer.from_array(fwords)
+    rdev = svdag_py.DeviceBuffer(4 * fn)
+    hs.find_dev(fdev, fsegs, rdev, variant)
+    res = rdev.to_array(np.uint32, fn)
+    expect = _oracle_found(items, finds, buckets)
+    for i, (s, a) in enumerate(fgroups):
+        seg = fsegs[i]
+        r = res[seg.out_offset:seg.out_offset + seg.count]
+        hit = r != svdag_py.NOT_FOUND
+        assert np.array_equal(hit, expect[s]), s
+        assert np.array_equal(hs.read(s, r[hit]), a[hit]), s
+    hs.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [0, 16])
+def test_long_chains_and_duplicates(variant):
+    # Few buckets force multi-slab chains; a small alphabet forces duplicates.
+    rng = np.random.default_rng(7)
+    items = {3: rng.integers(0, 40, size=(20000, 3)).astype(np.uint32)}
+    items[3][:, 0] = 3  # valid two-child header
+    hs = svdag_py.HashSet(3, 3, [5], [len(items[3]) * 30])
+    words, segs, n = svdag_py.make_segments([(3, items[3])])
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    hs.add_dev(wdev, segs, odev, variant)
+    ptrs = odev.to_array(np.uint32, n)
+    assert len(set(ptrs.tolist())) == n  # multiset insert: one slot per item
+    assert np.array_equal(hs.read(3, ptrs), items[3])
+    probes = rng.integers(0, 45, size=(5000, 3)).astype(np.uint32)
+    probes[:, 0] = 3
+    pw, ps, pn = svdag_py.make_segments([(3, probes)])
+    pdev = svdag_py.DeviceBuffer.from_array(pw)
+    rdev = svdag_py.DeviceBuffer(4 * pn)
+    hs.find_dev(pdev, ps, rdev, variant)
+    r = rdev.to_array(np.uint32, pn)
+    present = {tuple(x) for x in items[3].tolist()}
+    assert [bool(x != svdag_py.NOT_FOUND) for x in r] == [tuple(p) in present for p in probes.tolist()]
+    hs.close()
+
+
+@pytest.mark.gpu
+def test_empty_batch_and_oom_is_reported():
+    hs = svdag_py.HashSet(2, 2, [1], [0])
+    usage = hs.usage(2)
+    assert usage[0] == 1 and usage[2] == 1
+    # 1 head slab + a reserve of 64 overflow slabs: 124 * 65 items fit, more do not.
+    cap_items = 124 * usage[1]
+    items = np.stack([np.full(cap_items + 500, 1, np.uint32),
+                      np.arange(cap_items + 500, dtype=np.uint32)], axis=1)
+    words, segs, n = svdag_py.make_segments([(2, items)])
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    with pytest.raises(svdag_py.SvdagError) as e:
+        hs.add_dev(wdev, segs, odev, 16)
+    assert e.value.code == -2
+    hs.close()
