@@ -25,14 +25,23 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-svdag-editing_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "voxel edits/s & p99 edit latency (depth 15); slab-hash find Mops/s vs HBM"
 HBM_PEAK_GBS = 8000.0
-VARIANTS = [0, 4, 8, 16, 32]
+VARIANTS = [0, 4, 8, 16, 32, 204, 404, 208, 408]
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def variant_name(v):
+    if v == 0:
+        return "per-lane"
+    if v >= 100:
+        return f"wave64 group of {v % 100} lanes, {v // 100} finds in flight per group"
+    return f"wave64 group of {v} lanes"
 
 
 def read_traffic(path):
@@ -70,10 +79,22 @@ def slab_hash_bench(args):
     hs = svdag_py.HashSet(2, 9, [buckets.get(s, 1) for s in sizes], [len(items.get(s, [])) for s in sizes])
     add_ms = hs.add_dev(wdev, segs, odev, 16)
 
+    if args.pmc_mode:
+        # Counter runs: one device copy of a known size (calibrates FETCH_SIZE /
+        # WRITE_SIZE) followed by a few launches of one find variant only.
+        nbytes = 256 << 20
+        a = svdag_py.DeviceBuffer(nbytes)
+        b = svdag_py.DeviceBuffer(nbytes)
+        svdag_py.check(svdag_py.lib().svdag_memcpy_d2d(b.ptr, a.ptr, svdag_py.ctypes.c_size_t(nbytes)))
+        for _ in range(args.steps):
+            hs.find_dev(fdev, fsegs, rdev, args.variant)
+        log(f"[bench] pmc mode: copied {nbytes} bytes, {args.steps} find launches of variant {args.variant}")
+        return None
+
     # Algorithmic bytes of one find launch.
     res = None
     per_variant = {}
-    for v in VARIANTS:
+    for v in ([args.variant] if args.variant is not None else VARIANTS):
         ms = [hs.find_dev(fdev, fsegs, rdev, v) for _ in range(max(2, args.warmup))]
         per_variant[v] = min(ms)
         log(f"[bench] variant {v}: {min(ms):.3f} ms")
@@ -117,7 +138,7 @@ def slab_hash_bench(args):
         "config": {
             "workload": f"slab-hash find: 2^{args.log2_items} DAG node items of 2-9 u32 in per-size tables "
                         f"at 96 items/bucket, 2^{args.log2_items} finds, 50% hits",
-            "variant": ("per-lane" if best == 0 else f"wave64 group of {best} lanes"),
+            "variant": variant_name(best),
             "find_kernel_ms_by_variant": {str(k): round(v, 4) for k, v in per_variant.items()},
             "bulk_insert_ms": round(add_ms, 3),
             "hit_rate": round(float(hits.mean()), 4),
@@ -175,13 +196,16 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="skip the variant sweep and use this one")
+    ap.add_argument("--pmc-mode", action="store_true", help="calibration copy + find launches only (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "
               "(see the sharding notes in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     out = slab_hash_bench(args)
-    print(json.dumps(out), flush=True)
+    if out is not None:
+        print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -359,8 +359,8 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
         }
         const uint32_t s = isLeaf ? leaf_size_from_words(w0, w1, dag.M) : node_size_from_header(w0);
         Needle<G> n;
-        needle_load<G>(n, rec, s, gl);
-        const uint32_t h = needle_hash<G>(n, s, gl);
+        needle_load(n, rec, s, gl);
+        const uint32_t h = needle_hash(n, s, gl);
         const unsigned long long key = ((unsigned long long)h << 32) | (unsigned long long)(w + 1u);
         uint32_t p = mix32(h) & capMask;
         uint32_t rep = w;
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
                 const uint32_t j = vlo - 1u;
                 const uint32_t* other = isLeaf ? ws.leaves + (size_t)(j - me.start) * ws.leafStride
                                                : ws.nodes + (size_t)j * kNodeStride;
-                const bool eq = needle_equal_lane<G>(n, other, s, gl);
+                const bool eq = needle_equal_lane(n, other, s, gl);
                 if (group_ballot<G>(!eq, gbase) == 0) {
                     rep = j;
                     break;
@@ -416,8 +416,8 @@ __global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32
         const uint32_t w0 = rec[0], w1 = rec[1];
         const uint32_t s = isLeaf ? leaf_size_from_words(w0, w1, dag.M) : node_size_from_header(w0);
         Needle<G> n;
-        needle_load<G>(n, rec, s, gl);
-        const uint32_t h = needle_hash<G>(n, s, gl);
+        needle_load(n, rec, s, gl);
+        const uint32_t h = needle_hash(n, s, gl);
         const uint32_t r = table_insert_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
         if (gl == 0) {
             if (r == kNotFound) {

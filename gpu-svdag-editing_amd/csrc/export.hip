@@ -18,12 +18,19 @@ namespace svdag {
 struct HostDag {
     uint32_t levels = 0, leafLevel = 0, M = 0, minLeaf = 0, leafSizeBits = 0;
     std::vector<std::vector<uint32_t>> items; // per table: slots [fresh][kSlots][words]
+    std::vector<std::vector<uint32_t>> tags;  // per table: tag lines [fresh][32]
     std::vector<uint32_t> fresh;
+    std::vector<uint32_t> fullyFilledLevel0;  // level-0 fully-filled node per material
 
+    // Words of a stored item; nullptr if the pointer is out of range or its
+    // slot is free (tag 0), i.e. the handle dangles.
     const uint32_t* item(uint32_t s, uint32_t ptr) const
     {
         const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
         if (s < 2 || s - 2 >= items.size() || slab >= fresh[s - 2] || slot >= kSlots)
+            return nullptr;
+        const uint32_t tag = (tags[s - 2][(size_t)slab * kTagWords + slot / 4] >> (8 * (slot % 4))) & 0xFFu;
+        if (!tag)
             return nullptr;
         return items[s - 2].data() + ((size_t)slab * kSlots + slot) * s;
     }
@@ -38,7 +45,11 @@ static int copy_dag_to_host(svdag_ctx* ctx, HostDag& h)
     h.minLeaf = d.minLeafWords;
     h.leafSizeBits = d.leafSizeBits;
     h.items.resize(ctx->ts.tables.size());
+    h.tags.resize(ctx->ts.tables.size());
     h.fresh.resize(ctx->ts.tables.size());
+    h.fullyFilledLevel0.resize(d.numMaterials);
+    SVDAG_HIP(hipMemcpyAsync(h.fullyFilledLevel0.data(), ctx->dFullyFilled, d.numMaterials * 4, hipMemcpyDeviceToHost,
+                             ctx->stream));
     for (size_t i = 0; i < ctx->ts.tables.size(); ++i) {
         const HostTable& t = ctx->ts.tables[i];
         AllocState a {};
@@ -46,7 +57,10 @@ static int copy_dag_to_host(svdag_ctx* ctx, HostDag& h)
         const uint32_t fresh = std::min(a.fresh, t.slabCap);
         h.fresh[i] = fresh;
         h.items[i].resize((size_t)fresh * kSlots * t.itemWords);
+        h.tags[i].resize((size_t)fresh * kTagWords);
         SVDAG_HIP(hipMemcpyAsync(h.items[i].data(), t.dev.items, h.items[i].size() * 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(h.tags[i].data(), t.dev.tags, h.tags[i].size() * 4, hipMemcpyDeviceToHost,
                                  ctx->stream));
     }
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));
@@ -67,13 +81,14 @@ static uint32_t h_leaf_ptr(const HostDag& h, uint32_t handle)
     return h.leafSizeBits ? handle & ((1u << (32u - h.leafSizeBits)) - 1u) : handle;
 }
 
-// Breadth-first walk of every reachable (level, handle).
-static int reachable(const HostDag& h, uint32_t root, std::vector<std::vector<Rec>>& levels)
+// Breadth-first walk of every (level, handle) reachable from the roots.
+static int reachable(const HostDag& h, const std::vector<uint32_t>& roots, std::vector<std::vector<Rec>>& levels)
 {
     levels.assign(h.leafLevel + 1, {});
-    if (root == kSentinel)
-        return SVDAG_OK;
-    std::vector<uint32_t> cur { root };
+    std::vector<uint32_t> cur;
+    for (uint32_t r : roots)
+        if (r != kSentinel)
+            cur.push_back(r);
     for (uint32_t L = 0; L <= h.leafLevel; ++L) {
         std::unordered_set<uint32_t> seen;
         std::vector<uint32_t> next;
@@ -84,7 +99,7 @@ static int reachable(const HostDag& h, uint32_t root, std::vector<std::vector<Re
             const uint32_t s = leaf ? h_leaf_size(h, hd) : node_handle_size(hd);
             const uint32_t* p = h.item(s, leaf ? h_leaf_ptr(h, hd) : node_handle_ptr(hd));
             if (!p) {
-                set_error("export: dangling handle");
+                set_error("export: reachable handle points at a free or out-of-range slot");
                 return SVDAG_E_INVALID;
             }
             Rec r { L, hd, std::vector<uint32_t>(p, p + s) };
@@ -165,7 +180,9 @@ int svdag_export_tables(svdag_ctx* ctx, uint32_t* buf, size_t* len)
     HostDag h;
     SVDAG_TRY(copy_dag_to_host(ctx, h));
     std::vector<std::vector<Rec>> levels;
-    SVDAG_TRY(reachable(h, ctx->root, levels));
+    std::vector<uint32_t> roots { ctx->root };
+    roots.insert(roots.end(), h.fullyFilledLevel0.begin(), h.fullyFilledLevel0.end());
+    SVDAG_TRY(reachable(h, roots, levels));
     std::vector<uint32_t> out { h.levels, h.M, ctx->root, 0u };
     uint32_t n = 0;
     for (const auto& lv : levels)
@@ -192,7 +209,7 @@ int svdag_export_canonical(svdag_ctx* ctx, uint64_t* hash, uint32_t* buf, size_t
     HostDag h;
     SVDAG_TRY(copy_dag_to_host(ctx, h));
     std::vector<std::vector<Rec>> levels;
-    SVDAG_TRY(reachable(h, ctx->root, levels));
+    SVDAG_TRY(reachable(h, { ctx->root }, levels));
     std::vector<uint32_t> dump, cnt;
     canonicalize(h, ctx->root, levels, dump, cnt);
     *hash = fnv1a(dump);

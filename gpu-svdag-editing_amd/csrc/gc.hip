@@ -5,12 +5,13 @@
 // (gpu_hash_table_base.cu:230-374).  Roots are the current root plus the
 // level-0 fully-filled node of every material (377-380).
 //
-// Mark: one frontier per level.  The atomicOr that sets an item's active bit
-// returns the old bit, so only the first visitor of an item expands its
-// children -- this replaces the reference's per-level radix sort + unique.
-// Levels are processed in order, so an item first met at its shallowest
-// level is expanded there, which reaches every item its deeper occurrences
-// would reach.
+// Mark: one frontier per level.  Every slot has a 32-bit mask of the levels
+// it was reached at; the atomicOr that sets bit L returns the old mask, so
+// only the first visitor of an (item, level) pair expands it -- this replaces
+// the reference's per-level radix sort + unique.  Deduplication must be per
+// level: one item can occur at several levels, and the same child handle
+// decodes to a node at one depth and to a leaf (another table) next to the
+// leaf level.
 // Sweep (always on; the reference sweeps only under CAPTURE_MEMORY_STATS_SLOW):
 // tags of unmarked items are cleared and wholly unmarked overflow slabs are
 // unlinked and pushed on the slab free stack.
@@ -26,12 +27,15 @@ struct GcCounters {
     unsigned long long itemsFreed, slabsFreed, itemsLive;
 };
 
-__global__ void gc_clear_active(TableDev t)
+__global__ void gc_clear_active(TableDev t, uint32_t* levelMask)
 {
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
-    const uint64_t n = (uint64_t)fresh * 4;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        t.active[i] = 0u;
+    const uint64_t n = (uint64_t)fresh * kSlots;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        levelMask[i] = 0u;
+        if (i < (uint64_t)fresh * 4)
+            t.active[i] = 0u;
+    }
 }
 
 __global__ void gc_seed(DagDev dag, const uint32_t* root, uint32_t* front, GcCounters* ctr)
@@ -46,7 +50,8 @@ __global__ void gc_seed(DagDev dag, const uint32_t* root, uint32_t* front, GcCou
     ctr->count[0] = n;
 }
 
-__global__ void gc_mark_level(DagDev dag, uint32_t L, const uint32_t* in, uint32_t* out, uint32_t outCap, GcCounters* ctr)
+__global__ void gc_mark_level(DagDev dag, uint32_t* const* levelMask, uint32_t L, const uint32_t* in, uint32_t* out,
+                              uint32_t outCap, GcCounters* ctr)
 {
     const uint32_t n = min(ctr->count[L], outCap); // both frontier buffers hold outCap entries
     const bool leaf = L == dag.leafLevel;
@@ -56,9 +61,12 @@ __global__ void gc_mark_level(DagDev dag, uint32_t L, const uint32_t* in, uint32
         const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h);
         const TableDev& t = dag.tables[s - 2];
         const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
-        const uint32_t bit = 1u << (slot & 31u);
-        const uint32_t old = atomicOr(&t.active[(size_t)slab * 4 + (slot >> 5)], bit);
-        if ((old & bit) || leaf)
+        const uint32_t seen = atomicOr(&levelMask[s - 2][(size_t)slab * kSlots + slot], 1u << L);
+        if (seen & (1u << L))
+            continue;
+        if (!seen)
+            atomicOr(&t.active[(size_t)slab * 4 + (slot >> 5)], 1u << (slot & 31u));
+        if (leaf)
             continue;
         const uint32_t* node = table_item_ptr(t, ptr);
         const uint32_t mask = node[0] & 0xFFu;
@@ -131,6 +139,26 @@ static int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
     if (!ctx->gcCounters)
         SVDAG_HIP(hipMalloc(&ctx->gcCounters, sizeof(GcCounters)));
     GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
+    // Per-slot level masks, sized to each table's current slab capacity.
+    const size_t nt = ctx->ts.tables.size();
+    std::vector<uint32_t*> masks(nt, nullptr);
+    uint32_t** dMasks = nullptr;
+    for (size_t i = 0; i < nt; ++i)
+        SVDAG_HIP(hipMalloc(&masks[i], (size_t)ctx->ts.tables[i].slabCap * kSlots * 4));
+    SVDAG_HIP(hipMalloc(&dMasks, nt * sizeof(uint32_t*)));
+    SVDAG_HIP(hipMemcpyAsync(dMasks, masks.data(), nt * sizeof(uint32_t*), hipMemcpyHostToDevice, st));
+    struct Release {
+        std::vector<uint32_t*>& m;
+        uint32_t**& d;
+        ~Release()
+        {
+            for (auto* p : m)
+                if (p)
+                    (void)hipFree(p);
+            if (d)
+                (void)hipFree(d);
+        }
+    } release { masks, dMasks };
     for (int attempt = 0; attempt < 6; ++attempt) {
         if (cap > ctx->gcCap) {
             for (int k = 0; k < 2; ++k) {
@@ -142,11 +170,12 @@ static int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
         }
         SVDAG_HIP(hipEventRecord(ctx->ev0, st));
         SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
-        for (auto& t : ctx->ts.tables)
-            gc_clear_active<<<grid_for((uint64_t)t.slabCap * 4, 256, 8192), 256, 0, st>>>(t.dev);
+        for (size_t i = 0; i < nt; ++i)
+            gc_clear_active<<<grid_for((uint64_t)ctx->ts.tables[i].slabCap * kSlots, 256, 8192), 256, 0, st>>>(
+                ctx->ts.tables[i].dev, masks[i]);
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L <= dag.leafLevel; ++L)
-            gc_mark_level<<<4096, 256, 0, st>>>(dag, L, ctx->gcFront[L & 1], ctx->gcFront[(L + 1) & 1],
+            gc_mark_level<<<4096, 256, 0, st>>>(dag, dMasks, L, ctx->gcFront[L & 1], ctx->gcFront[(L + 1) & 1],
                                                  (uint32_t)std::min<uint64_t>(ctx->gcCap, 0xFFFFFFFFull), ctr);
         GcCounters h {};
         SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -160,6 +189,7 @@ static int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
         SVDAG_HIP(hipEventRecord(ctx->ev1, st));
         SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         SVDAG_HIP(hipStreamSynchronize(st));
+        (void)h;
         if (stats) {
             stats->items_freed = h.itemsFreed;
             stats->slabs_freed = h.slabsFreed;

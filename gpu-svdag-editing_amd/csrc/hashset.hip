@@ -46,6 +46,8 @@ int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t 
     SVDAG_HIP(hipMalloc(&d.items, (size_t)slabCap * kSlots * itemWords * 4));
     SVDAG_HIP(hipMalloc(&d.active, (size_t)slabCap * 4 * 4));
     SVDAG_HIP(hipMalloc(&d.freeList, (size_t)slabCap * 4));
+    SVDAG_HIP(hipMalloc(&d.grow, (size_t)numBuckets * 4));
+    SVDAG_HIP(hipMemsetAsync(d.grow, 0, (size_t)numBuckets * 4, stream));
     SVDAG_HIP(hipMalloc(&d.alloc, sizeof(AllocState)));
     AllocState a { 0xFFFFFFFFull, numBuckets, 0u };
     SVDAG_HIP(hipMemcpyAsync(d.alloc, &a, sizeof(a), hipMemcpyHostToDevice, stream));
@@ -61,6 +63,7 @@ void table_free(HostTable& t)
     if (d.items) (void)hipFree(d.items);
     if (d.active) (void)hipFree(d.active);
     if (d.freeList) (void)hipFree(d.freeList);
+    if (d.grow) (void)hipFree(d.grow);
     if (d.alloc) (void)hipFree(d.alloc);
     d = TableDev {};
     t = HostTable {};
@@ -148,7 +151,7 @@ __device__ inline uint32_t seg_of(const uint32_t* segs, uint32_t nseg, uint32_t 
     return s;
 }
 
-template <int G, bool kInsert>
+template <int G, bool kInsert, int MW>
 __global__ void __launch_bounds__(256) bulk_group_kernel(const TableDev* tables, uint32_t minWords, const uint32_t* words,
                                                          const uint32_t* segs, uint32_t nseg, uint32_t total, uint32_t* out)
 {
@@ -161,9 +164,9 @@ __global__ void __launch_bounds__(256) bulk_group_kernel(const TableDev* tables,
         const uint32_t s = segs[sg * 5 + 0];
         const uint32_t local = idx - segs[sg * 5 + 4];
         const TableDev t = tables[s - minWords];
-        Needle<G> n;
-        needle_load<G>(n, words + segs[sg * 5 + 2] + (size_t)local * s, s, gl);
-        const uint32_t h = needle_hash<G>(n, s, gl);
+        Needle<G, MW> n;
+        needle_load(n, words + segs[sg * 5 + 2] + (size_t)local * s, s, gl);
+        const uint32_t h = needle_hash(n, s, gl);
         uint32_t r;
         if constexpr (kInsert)
             r = table_insert_group<G>(t, n, s, h, gl, gbase);
@@ -171,6 +174,97 @@ __global__ void __launch_bounds__(256) bulk_group_kernel(const TableDev* tables,
             r = table_find_group<G>(t, n, s, h, gl, gbase);
         if (gl == 0)
             out[segs[sg * 5 + 3] + local] = r;
+    }
+}
+
+// Find with N needles per group in flight: the needle loads, hashes and tag
+// line loads of all N are issued before any candidate is compared, so one
+// group keeps N independent probes outstanding (memory-level parallelism
+// without more waves).
+template <int G, int N, int MW>
+__global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* tables, uint32_t minWords,
+                                                              const uint32_t* words, const uint32_t* segs, uint32_t nseg,
+                                                              uint32_t total, uint32_t* out)
+{
+    constexpr int W = 32 / G;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const uint32_t groupsPerBlock = blockDim.x / G;
+    const uint32_t stride = gridDim.x * groupsPerBlock * N;
+    for (uint32_t base = (blockIdx.x * groupsPerBlock + threadIdx.x / G) * N; base < total; base += stride) {
+        Needle<G, MW> n[N];
+        uint32_t s[N], h[N], slab[N], outIdx[N], tag4[N];
+        const TableDev* tp[N];
+        bool valid[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const uint32_t idx = base + k;
+            valid[k] = idx < total;
+            const uint32_t sg = seg_of(segs, nseg, valid[k] ? idx : base);
+            s[k] = segs[sg * 5 + 0];
+            const uint32_t local = (valid[k] ? idx : base) - segs[sg * 5 + 4];
+            outIdx[k] = segs[sg * 5 + 3] + local;
+            tp[k] = tables + (s[k] - minWords);
+            needle_load(n[k], words + segs[sg * 5 + 2] + (size_t)local * s[k], s[k], gl);
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            h[k] = needle_hash(n[k], s[k], gl);
+            tag4[k] = make_tag(h[k]) * 0x01010101u;
+            slab[k] = h[k] % tp[k]->numBuckets;
+        }
+        uint32_t words_[N][W];
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            tagline_load<G>(words_[k], tp[k]->tags + (size_t)slab[k] * kTagWords, gl);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            uint32_t r = kNotFound;
+            const TableDev t = *tp[k];
+            uint32_t sl = slab[k];
+            bool first = true;
+            while (true) {
+                uint32_t wv[W];
+                if (first) {
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        wv[i] = words_[k][i];
+                } else {
+                    tagline_load<G>(wv, t.tags + (size_t)sl * kTagWords, gl);
+                }
+                first = false;
+                uint32_t cand = 0;
+#pragma unroll
+                for (int i = 0; i < W; ++i)
+                    if (gl * W + i != kNextWord)
+                        cand |= compact_bytes(zero_bytes(wv[i] ^ tag4[k])) << (4 * i);
+                const uint32_t next = group_bcast<G>(wv[W - 1], G - 1);
+                bool done = false;
+                while (true) {
+                    const uint32_t bal = group_ballot<G>(cand != 0, gbase);
+                    if (!bal)
+                        break;
+                    const uint32_t leader = __ffs(bal) - 1u;
+                    const uint32_t mySlot = gl * W * 4 + (cand ? (__ffs(cand) - 1u) : 0u);
+                    const uint32_t slot = group_bcast<G>(mySlot, leader);
+                    if (gl == leader)
+                        cand &= cand - 1u;
+                    const uint32_t* item = t.items + ((size_t)sl * kSlots + slot) * s[k];
+                    const bool eq = needle_equal_lane(n[k], item, s[k], gl);
+                    if (group_ballot<G>(!eq, gbase) == 0) {
+                        r = (sl << kSlotBits) | slot;
+                        done = true;
+                        break;
+                    }
+                }
+                if (done || next == kNoSlab)
+                    break;
+                sl = next;
+            }
+            if (gl == 0 && valid[k])
+                out[outIdx[k]] = r;
+        }
     }
 }
 
@@ -233,6 +327,8 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
     if (n == 0)
         return SVDAG_OK;
     const uint32_t block = 256;
+    // Items of at most 10 words need fewer needle registers per lane.
+    const bool narrow = ts.maxWords <= 10;
     switch (variant) {
     case 0: {
         const uint32_t grid = grid_for(n, block, 1u << 16);
@@ -242,13 +338,16 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
             bulk_lane_kernel<false><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out);
         break;
     }
+#define SVDAG_BULK_LAUNCH(G, INS, MW) \
+    bulk_group_kernel<G, INS, MW><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out)
 #define SVDAG_BULK_CASE(G)                                                                                              \
     case G: {                                                                                                           \
         const uint32_t grid = grid_for((uint64_t)n * G, block, 1u << 16);                                               \
-        if (insert)                                                                                                     \
-            bulk_group_kernel<G, true><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out); \
-        else                                                                                                            \
-            bulk_group_kernel<G, false><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out); \
+        if (insert) {                                                                                                   \
+            if (narrow) SVDAG_BULK_LAUNCH(G, true, 10); else SVDAG_BULK_LAUNCH(G, true, 18);                            \
+        } else {                                                                                                        \
+            if (narrow) SVDAG_BULK_LAUNCH(G, false, 10); else SVDAG_BULK_LAUNCH(G, false, 18);                          \
+        }                                                                                                               \
         break;                                                                                                          \
     }
         SVDAG_BULK_CASE(4)
@@ -256,8 +355,27 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
         SVDAG_BULK_CASE(16)
         SVDAG_BULK_CASE(32)
 #undef SVDAG_BULK_CASE
+#undef SVDAG_BULK_LAUNCH
+#define SVDAG_MULTI_CASE(V, G, N)                                                                                       \
+    case V: {                                                                                                           \
+        if (insert) {                                                                                                   \
+            set_error("multi-needle variants are find-only");                                                           \
+            return SVDAG_E_INVALID;                                                                                     \
+        }                                                                                                               \
+        const uint32_t grid = grid_for(((uint64_t)n + N - 1) / N * G, block, 1u << 16);                                 \
+        if (narrow)                                                                                                     \
+            bulk_find_multi_kernel<G, N, 10><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out); \
+        else                                                                                                            \
+            bulk_find_multi_kernel<G, N, 18><<<grid, block, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, out); \
+        break;                                                                                                          \
+    }
+        SVDAG_MULTI_CASE(204, 4, 2)
+        SVDAG_MULTI_CASE(404, 4, 4)
+        SVDAG_MULTI_CASE(208, 8, 2)
+        SVDAG_MULTI_CASE(408, 8, 4)
+#undef SVDAG_MULTI_CASE
     default:
-        set_error("variant must be 0, 4, 8, 16 or 32");
+        set_error("variant must be 0, 4, 8, 16, 32 (lanes per item) or 204, 404, 208, 408 (N*100 + G, find only)");
         return SVDAG_E_INVALID;
     }
     SVDAG_HIP(hipGetLastError());
@@ -323,6 +441,11 @@ int svdag_memcpy_h2d(void* dst, const void* src, size_t bytes)
 int svdag_memcpy_d2h(void* dst, const void* src, size_t bytes)
 {
     SVDAG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SVDAG_OK;
+}
+int svdag_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    SVDAG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return SVDAG_OK;
 }
 int svdag_device_synchronize(void)

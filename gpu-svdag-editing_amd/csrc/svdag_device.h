@@ -84,6 +84,7 @@ struct TableDev {
     uint32_t* items;     // [slabCap][kSlots][itemWords]
     uint32_t* active;    // [slabCap][4] GC mark bits
     uint32_t* freeList;  // [slabCap] free-stack links
+    uint32_t* grow;      // [numBuckets] chain-growth flags
     AllocState* alloc;
     uint32_t itemWords;
     uint32_t numBuckets;
@@ -142,19 +143,28 @@ __device__ inline void slab_free(const TableDev& t, uint32_t id)
     }
 }
 
-// The chain of `bucket` was seen full with head successor `observed`.  A
-// fresh slab is linked after the head only if nobody grew the chain since,
-// so a full chain grows by one slab per round however many inserts race on
-// it; a loser returns its slab and rescans.  Returns the slab to continue
-// with: the new slab, the head (rescan), or kNoSlab when the arena is empty.
+// The chain of `bucket` was seen full with head successor `observed`.  The
+// bucket's grow flag (taken by CAS, never waited on) admits one grower at a
+// time: it allocates a slab and links it right after the head, which cannot
+// race because only flag holders move a head's successor during inserts.
+// Everyone else rescans.  Returns the slab to continue with: the new slab,
+// the head (rescan), or kNoSlab when the arena is exhausted.
 __device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32_t observed)
 {
     uint32_t* headNext = t.tags + (size_t)bucket * kTagWords + kNextWord;
     if (ld_relaxed(headNext) != observed)
         return bucket;
+    if (atomicCAS(&t.grow[bucket], 0u, 1u) != 0u)
+        return bucket;
+    if (ld_relaxed(headNext) != observed) {
+        atomicExch(&t.grow[bucket], 0u);
+        return bucket;
+    }
     const uint32_t id = slab_alloc(t);
-    if (id == kNoSlab)
+    if (id == kNoSlab) {
+        atomicExch(&t.grow[bucket], 0u);
         return kNoSlab;
+    }
     uint32_t* nl = t.tags + (size_t)id * kTagWords;
     for (uint32_t i = 0; i < kNextWord; ++i)
         st_relaxed(nl + i, 0u);
@@ -162,10 +172,10 @@ __device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32
     for (uint32_t i = 0; i < 4; ++i)
         t.active[(size_t)id * 4 + i] = 0u;
     __threadfence();
-    if (atomicCAS(headNext, observed, id) == observed)
-        return id;
-    slab_free(t, id);
-    return bucket;
+    atomicExch(headNext, id);
+    __threadfence();
+    atomicExch(&t.grow[bucket], 0u);
+    return id;
 }
 
 // ---------------------------------------------------------------- wave64 groups
@@ -199,39 +209,39 @@ __device__ inline uint32_t group_xor(uint32_t v)
 }
 
 // Needle words held by a group: lane gl holds words gl, gl+G, gl+2G, ...
-template <int G>
+template <int G, int MW = (int)kMaxItemWords>
 struct Needle {
-    static constexpr int K = (kMaxItemWords + G - 1) / G;
+    static constexpr int K = (MW + G - 1) / G;
     uint32_t w[K];
 };
 
-template <int G>
-__device__ inline void needle_load(Needle<G>& n, const uint32_t* p, uint32_t s, uint32_t gl)
+template <int G, int MW>
+__device__ inline void needle_load(Needle<G, MW>& n, const uint32_t* p, uint32_t s, uint32_t gl)
 {
 #pragma unroll
-    for (int k = 0; k < Needle<G>::K; ++k) {
+    for (int k = 0; k < Needle<G, MW>::K; ++k) {
         const uint32_t j = gl + (uint32_t)k * G;
         n.w[k] = (j < s) ? p[j] : 0u;
     }
 }
-template <int G>
-__device__ inline uint32_t needle_hash(const Needle<G>& n, uint32_t s, uint32_t gl)
+template <int G, int MW>
+__device__ inline uint32_t needle_hash(const Needle<G, MW>& n, uint32_t s, uint32_t gl)
 {
     uint32_t h = 0;
 #pragma unroll
-    for (int k = 0; k < Needle<G>::K; ++k) {
+    for (int k = 0; k < Needle<G, MW>::K; ++k) {
         const uint32_t j = gl + (uint32_t)k * G;
         if (j < s)
             h ^= hash_part(n.w[k], j);
     }
     return group_xor<G>(h);
 }
-template <int G>
-__device__ inline bool needle_equal_lane(const Needle<G>& n, const uint32_t* item, uint32_t s, uint32_t gl)
+template <int G, int MW>
+__device__ inline bool needle_equal_lane(const Needle<G, MW>& n, const uint32_t* item, uint32_t s, uint32_t gl)
 {
     bool eq = true;
 #pragma unroll
-    for (int k = 0; k < Needle<G>::K; ++k) {
+    for (int k = 0; k < Needle<G, MW>::K; ++k) {
         const uint32_t j = gl + (uint32_t)k * G;
         if (j < s)
             eq &= (item[j] == n.w[k]);
@@ -266,8 +276,8 @@ __device__ inline void tagline_load(uint32_t (&words)[32 / G], const uint32_t* l
 // Group-cooperative find (G in {4, 8, 16, 32}): the tag line is read once by
 // the whole group (W words per lane), candidate slots are resolved one at a
 // time with the group comparing the candidate's words in parallel.
-template <int G>
-__device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G>& n, uint32_t s, uint32_t h,
+template <int G, int MW>
+__device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
                                             uint32_t gl, uint32_t gbase)
 {
     static_assert(G >= 4 && G <= 32, "group size");
@@ -344,8 +354,8 @@ __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* ne
 // Group-cooperative insert of an item the caller knows to be absent (or, for
 // bulk multiset inserts, allowed to be present).  Claims a slot with a CAS on
 // its tag word; a full chain grows through chain_grow.
-template <int G>
-__device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G>& n, uint32_t s, uint32_t h,
+template <int G, int MW>
+__device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
                                               uint32_t gl, uint32_t gbase)
 {
     static_assert(G >= 4 && G <= 32, "group size");
@@ -388,7 +398,7 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G>
             if (ok) {
                 uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
 #pragma unroll
-                for (int k = 0; k < Needle<G>::K; ++k) {
+                for (int k = 0; k < Needle<G, MW>::K; ++k) {
                     const uint32_t j = gl + (uint32_t)k * G;
                     if (j < s)
                         item[j] = n.w[k];
@@ -403,8 +413,10 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G>
             slab = next;
             continue;
         }
-        uint32_t grown = 0;
-        if (gl == 0)
+        // One group per wave grows a chain per round; the others rescan.
+        const unsigned long long growers = __ballot(gl == 0);
+        uint32_t grown = bucket;
+        if (gl == 0 && (uint32_t)(__ffsll((long long)growers) - 1) == __lane_id())
             grown = chain_grow(t, bucket, headNext);
         slab = group_bcast<G>(grown, 0);
         if (slab == kNoSlab)
@@ -448,7 +460,11 @@ __device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* 
             slab = next;
             continue;
         }
-        slab = chain_grow(t, bucket, headNext);
+        // One lane per wave grows a chain per round; the others rescan.
+        const unsigned long long growers = __ballot(1);
+        slab = bucket;
+        if ((uint32_t)(__ffsll((long long)growers) - 1) == __lane_id())
+            slab = chain_grow(t, bucket, headNext);
         if (slab == kNoSlab)
             return kNotFound;
     }

@@ -1,0 +1,59 @@
+// Drives the C++ facade the way Engine::edit<T> drives the reference
+// (engine.h:104-163): build a terrain DAG, apply edits, undo/redo, collect
+// garbage, trace.  Prints "<canonical hash hex> <hits>" for a test to check.
+#include "my_gpu_hash_dag.hpp"
+#include <cstdio>
+#include <cstdlib>
+
+int main(int argc, char** argv)
+{
+    using namespace svdag;
+    const uint32_t levels = argc > 1 ? (uint32_t)std::atoi(argv[1]) : 10;
+    try {
+        MyGPUHashDAG dag(levels, 4);
+        const double side = double(1u << levels);
+        svdag_terrain t { 0x5EED, 512.0 * side / 1024.0, 200.0 * side / 1024.0, 1.0 / (256.0 * side / 1024.0), 5, 1 };
+        dag.build_terrain(t);
+        MyGPUHashDAGUndoRedo undoRedo;
+        const float s = (float)(side / 1024.0);
+        editMyHashDag(MyGpuSphereEditor<true>({ 300 * s, 520 * s, 400 * s }, 40 * s, 2), dag, undoRedo);
+        editMyHashDag(MyGpuBoxEditor<false>({ 310 * s, 530 * s, 410 * s }, 20 * s, 0), dag, undoRedo);
+        editMyHashDag(MyGpuSpherePaintEditor({ 600 * s, 500 * s, 600 * s }, 50 * s, { 1, 0, 0 }, 7), dag, undoRedo);
+        editMyHashDag(MyGpuCopyEditor<MyGPUHashDAG>(dag, { 300 * s, 520 * s, 400 * s }, { 700 * s, 530 * s, 200 * s },
+                                                    { 300 * s, 520 * s, 400 * s }, 30 * s),
+                      dag, undoRedo);
+        undoRedo.undo(dag);
+        undoRedo.redo(dag);
+        undoRedo.garbageCollect(dag);
+        DAGTracer tracer(128, 128);
+        CameraView view;
+        const double pos[3] = { -0.1 * side, 0.85 * side, -0.1 * side };
+        for (int i = 0; i < 3; ++i)
+            view.position[i] = pos[i];
+        // Looking along +x+z and down (rows: -right, up, forward), as look_at().
+        const double f[3] = { 0.6 * side, -0.5 * side, 0.6 * side };
+        double fl = 0;
+        for (double v : f)
+            fl += v * v;
+        fl = __builtin_sqrt(fl);
+        const double fw[3] = { f[0] / fl, f[1] / fl, f[2] / fl };
+        double r[3] = { fw[1] * 0 - fw[2] * 1, fw[2] * 0 - fw[0] * 0, fw[0] * 1 - fw[1] * 0 };
+        double rl = __builtin_sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        for (double& v : r)
+            v /= rl;
+        const double u[3] = { r[1] * fw[2] - r[2] * fw[1], r[2] * fw[0] - r[0] * fw[2], r[0] * fw[1] - r[1] * fw[0] };
+        const double rot[9] = { -r[0], -r[1], -r[2], u[0], u[1], u[2], fw[0], fw[1], fw[2] };
+        for (int i = 0; i < 9; ++i)
+            view.rotation[i] = rot[i];
+        DAGInfo info { { 0, 0, 0 }, { side, side, side } };
+        tracer.resolve_paths(view, info, dag);
+        uint32_t hits = 0;
+        for (const auto& h : tracer.hits())
+            hits += h.material != 0xFFFFFFFFu;
+        std::printf("%016llx %u\n", (unsigned long long)dag.canonical_hash(), hits);
+        return 0;
+    } catch (const Error& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 2;
+    }
+}

@@ -1,0 +1,260 @@
+// C++ host facade over the C ABI (include/svdag.h) with the reference's
+// interface names, so a caller written against EDag::MyGpuDag reads the same:
+//
+//   reference                                          here
+//   MyGPUHashDAG<EMemoryType::GPU_Malloc>              svdag::MyGPUHashDAG
+//     (my_gpu_hash_dag.h:59-339)
+//   MyGpuSphereEditor<bool>, MyGpuBoxEditor<bool>,     same names
+//   MyGpuSpherePaintEditor, MyGpuCopyEditor<TDAG>
+//     (my_gpu_dag_editors.h:82-253, 457-536)
+//   editMyHashDag(editor, dag, undoRedo, stats,        svdag::editMyHashDag(editor, dag, undoRedo)
+//     memPool, stream) (my_gpu_hash_dag_edits.h:12-13)
+//   MyGPUHashDAGUndoRedo::{undo,redo,garbageCollect}   svdag::MyGPUHashDAGUndoRedo
+//     (my_gpu_base_dag.h:16-82)
+//   DAGTracer::resolve_paths(camera, dagInfo, dag)     svdag::DAGTracer::resolve_paths
+//     (dag_tracer.cu:112-134)
+//
+// Ownership differs in one respect, by design: the DAG object owns its
+// device memory (RAII) where the reference needs an explicit free().  The
+// undo/redo history lives in the DAG context (the reference keeps it in a
+// separate MyGPUHashDAGUndoRedo object that swaps the root); the facade's
+// MyGPUHashDAGUndoRedo forwards to it.  Errors throw svdag::Error (the
+// reference aborts on CUDA errors and traps on failed checks).
+#pragma once
+#include "svdag.h"
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace svdag {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m)
+        : std::runtime_error("svdag error " + std::to_string(c) + ": " + m)
+        , code(c)
+    {
+    }
+};
+
+inline void check(int code)
+{
+    if (code != SVDAG_OK)
+        throw Error(code, svdag_last_error());
+}
+
+struct float3 {
+    float x, y, z;
+};
+
+class MyGPUHashDAG {
+public:
+    static constexpr uint32_t invalid_handle = 0xFFFFFFFFu;
+
+    explicit MyGPUHashDAG(uint32_t levels, uint32_t materialBits = 4, uint32_t loadFactor = 96)
+    {
+        svdag_config cfg {};
+        cfg.levels = levels;
+        cfg.material_bits = materialBits;
+        cfg.load_factor = loadFactor;
+        check(svdag_create(&cfg, &ctx_));
+        levels_ = levels;
+    }
+    MyGPUHashDAG(const MyGPUHashDAG&) = delete;
+    MyGPUHashDAG& operator=(const MyGPUHashDAG&) = delete;
+    MyGPUHashDAG(MyGPUHashDAG&& o) noexcept
+        : ctx_(o.ctx_)
+        , levels_(o.levels_)
+    {
+        o.ctx_ = nullptr;
+    }
+    ~MyGPUHashDAG()
+    {
+        if (ctx_)
+            svdag_destroy(ctx_);
+    }
+
+    uint32_t levels() const { return levels_; }
+    uint32_t leaf_level() const { return levels_ - 2; }
+    uint32_t get_first_node_index() const
+    {
+        uint32_t r = 0;
+        check(svdag_root(ctx_, &r));
+        return r;
+    }
+
+    void build_terrain(const svdag_terrain& t) { check(svdag_build_terrain(ctx_, &t)); }
+
+    // DAGUtils::get_value over many voxels: 0xFFFFFFFF marks an empty voxel.
+    std::vector<uint32_t> get_values(const std::vector<std::array<uint32_t, 3>>& xyz) const
+    {
+        std::vector<uint32_t> out(xyz.size());
+        check(svdag_query_voxels(ctx_, reinterpret_cast<const uint32_t*>(xyz.data()), (uint32_t)xyz.size(), out.data()));
+        return out;
+    }
+
+    uint64_t canonical_hash() const
+    {
+        uint64_t h = 0;
+        check(svdag_export_canonical(ctx_, &h, nullptr, nullptr, nullptr));
+        return h;
+    }
+
+    svdag_ctx* context() const { return ctx_; }
+
+private:
+    svdag_ctx* ctx_ = nullptr;
+    uint32_t levels_ = 0;
+};
+
+// ------------------------------------------------------------------ editors
+template <bool isAdding>
+struct MyGpuSphereEditor {
+    float3 center;
+    float radius;
+    uint32_t fillMaterial;
+    MyGpuSphereEditor(float3 c, float r, uint32_t material)
+        : center(c)
+        , radius(r)
+        , fillMaterial(material)
+    {
+    }
+    svdag_edit_desc desc() const
+    {
+        return { isAdding ? (uint32_t)SVDAG_SPHERE_ADD : (uint32_t)SVDAG_SPHERE_REMOVE, fillMaterial,
+                 { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 } };
+    }
+};
+
+template <bool isAdding>
+struct MyGpuBoxEditor {
+    float3 center;
+    float radius;
+    uint32_t fillMaterial;
+    MyGpuBoxEditor(float3 c, float r, uint32_t material)
+        : center(c)
+        , radius(r)
+        , fillMaterial(material)
+    {
+    }
+    svdag_edit_desc desc() const
+    {
+        return { isAdding ? (uint32_t)SVDAG_BOX_ADD : (uint32_t)SVDAG_BOX_REMOVE, fillMaterial,
+                 { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 } };
+    }
+};
+
+struct MyGpuSpherePaintEditor {
+    float3 center;
+    float radius;
+    float3 paintColor; // carried for interface parity; shading is not part of this back-end
+    uint32_t paintMaterial;
+    MyGpuSpherePaintEditor(float3 c, float r, float3 color, uint32_t material)
+        : center(c)
+        , radius(r)
+        , paintColor(color)
+        , paintMaterial(material)
+    {
+    }
+    svdag_edit_desc desc() const
+    {
+        return { (uint32_t)SVDAG_PAINT, paintMaterial, { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 } };
+    }
+};
+
+template <typename TDag>
+struct MyGpuCopyEditor {
+    float3 source, dest, center;
+    float radius;
+    // The reference copies the DAG value into the editor to read the source;
+    // here the source is the same live DAG the edit is applied to.
+    MyGpuCopyEditor(const TDag&, float3 src, float3 dst, float3 c, float r)
+        : source(src)
+        , dest(dst)
+        , center(c)
+        , radius(r)
+    {
+    }
+    svdag_edit_desc desc() const
+    {
+        return { (uint32_t)SVDAG_COPY, 0u, { center.x, center.y, center.z }, radius, { source.x, source.y, source.z },
+                 { dest.x, dest.y, dest.z } };
+    }
+};
+
+// ------------------------------------------------------------------ undo / redo / GC
+class MyGPUHashDAGUndoRedo {
+public:
+    void undo(MyGPUHashDAG& dag) { check(svdag_undo(dag.context())); }
+    void redo(MyGPUHashDAG& dag) { check(svdag_redo(dag.context())); }
+    // Clears the history, then marks from the root and the fully-filled
+    // chain and sweeps everything else.
+    svdag_gc_stats garbageCollect(MyGPUHashDAG& dag)
+    {
+        svdag_gc_stats st {};
+        check(svdag_gc(dag.context(), &st));
+        return st;
+    }
+    std::array<uint32_t, 2> depth(const MyGPUHashDAG& dag) const
+    {
+        std::array<uint32_t, 2> out {};
+        check(svdag_history(dag.context(), out.data()));
+        return out;
+    }
+};
+
+// Applies one edit and records the previous root as an undo frame.
+template <typename Editor>
+svdag_edit_stats editMyHashDag(const Editor& editor, MyGPUHashDAG& dag, MyGPUHashDAGUndoRedo&)
+{
+    const svdag_edit_desc d = editor.desc();
+    svdag_edit_stats st {};
+    check(svdag_edit(dag.context(), &d, &st));
+    return st;
+}
+
+// ------------------------------------------------------------------ tracer
+struct CameraView {
+    double position[3] = { 0, 0, 0 };
+    double rotation[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 }; // rows: -right, up, forward
+    double fov = 60.0;
+};
+struct DAGInfo {
+    double boundsAABBMin[3];
+    double boundsAABBMax[3];
+};
+
+class DAGTracer {
+public:
+    DAGTracer(uint32_t width, uint32_t height)
+        : width_(width)
+        , height_(height)
+        , hits_((size_t)width * height)
+    {
+    }
+    // Returns the device time in milliseconds; hit records land in hits().
+    float resolve_paths(const CameraView& view, const DAGInfo& info, const MyGPUHashDAG& dag)
+    {
+        svdag_camera cam {};
+        for (int i = 0; i < 3; ++i) {
+            cam.position[i] = view.position[i];
+            cam.bounds_min[i] = info.boundsAABBMin[i];
+            cam.bounds_max[i] = info.boundsAABBMax[i];
+        }
+        for (int i = 0; i < 9; ++i)
+            cam.rotation[i] = view.rotation[i];
+        cam.fov_degrees = view.fov;
+        float ms = 0.0f;
+        check(svdag_trace_primary(dag.context(), &cam, width_, height_, hits_.data(), &ms));
+        return ms;
+    }
+    const std::vector<svdag_hit>& hits() const { return hits_; }
+
+private:
+    uint32_t width_, height_;
+    std::vector<svdag_hit> hits_;
+};
+
+} // namespace svdag

@@ -71,7 +71,7 @@ HIT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("material", "<u
 # Every symbol include/svdag.h declares (checked by the CPU test suite).
 EXPORTED = [
     "svdag_last_error", "svdag_abi_version", "svdag_device_count", "svdag_dev_alloc", "svdag_dev_free",
-    "svdag_memcpy_h2d", "svdag_memcpy_d2h", "svdag_device_synchronize",
+    "svdag_memcpy_h2d", "svdag_memcpy_d2h", "svdag_memcpy_d2d", "svdag_device_synchronize",
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_terrain_heights", "svdag_root",

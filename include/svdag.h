@@ -42,6 +42,7 @@ int svdag_dev_alloc(size_t bytes, void** out_ptr_dev);
 int svdag_dev_free(void* ptr_dev);
 int svdag_memcpy_h2d(void* dst_dev, const void* src, size_t bytes);
 int svdag_memcpy_d2h(void* dst, const void* src_dev, size_t bytes);
+int svdag_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes);
 int svdag_device_synchronize(void);
 
 /* ------------------------------------------------------------------ slab hash tables
@@ -60,7 +61,9 @@ typedef struct {
 } svdag_segment;
 
 /* Variant of a bulk operation: 0 = one lane per item (cf. add/find),
- * 4/8/16/32 = that many lanes of a wave64 per item (cf. addAsWarp/findAsWarp). */
+ * 4/8/16/32 = that many lanes of a wave64 per item (cf. addAsWarp/findAsWarp),
+ * N*100 + G (204, 404, 208, 408) = find only, G lanes per group with N
+ * needles per group in flight. */
 enum { SVDAG_LANE = 0 };
 
 /* reference: CompactAccelerationHashTable::allocate(numBuckets, numElements, itemSize)
@@ -213,7 +216,9 @@ int svdag_export_canonical(svdag_ctx* ctx, uint64_t* out_hash, uint32_t* buf, si
 /* Reachable DAG as flat records (for host-side checkers).  *len_words
  * receives the number of words needed; the dump is written if buf is large
  * enough.  Layout: [levels, material_bits, root, n_records] then per record
- * [level, handle, size, words[size]] for every reachable (level, handle). */
+ * [level, handle, size, words[size]] for every (level, handle) reachable from
+ * the root or from a level-0 fully-filled node (the garbage-collection roots).
+ * Fails with SVDAG_E_INVALID if a reachable handle points at a free slot. */
 int svdag_export_tables(svdag_ctx* ctx, uint32_t* buf, size_t* len_words);
 
 #ifdef __cplusplus

@@ -1092,3 +1092,76 @@ int ora_leaf_material(uint32_t M, const uint32_t* leaf, uint32_t voxel, uint32_t
 uint32_t ora_leaf_size(uint32_t M, const uint32_t* leaf) { return Enc { M }.leaf_size(leaf); }
 
 } // extern "C"
+
+extern "C" {
+
+// Node header encode/decode (my_gpu_base_dag.h:152-189).
+uint32_t ora_header_make(uint32_t M, uint32_t mask, uint32_t payload) { return Enc { M }.header_payload(mask, payload); }
+void ora_header_decode(uint32_t M, uint32_t header, uint32_t* mask, uint32_t* payload, uint32_t* full, uint32_t* mat)
+{
+    const Enc e { M };
+    *mask = header & 0xFFu;
+    *payload = e.payload(header);
+    *full = e.is_full(header, *mat) ? 1u : 0u;
+}
+
+// MyGpuSphereEditorBase::test_node_overlap for a box (my_gpu_dag_editors.h:420-440).
+int ora_sphere_overlap(const float* center, float radius, const float* mn, const float* mx)
+{
+    const float zero[3] = { 0, 0, 0 };
+    const Editor e = make_editor(SPHERE_ADD, 0, center, radius, zero, zero);
+    return sphere_overlap(e, mn, mx) ? 1 : 0;
+}
+
+// Factory rule (my_gpu_hash_dag_factory.cpp:317-424) over a dense voxel grid:
+// vox[(x * side + y) * side + z] = material, or 0xFF for empty.
+uint32_t ora_dag_build_dense(void* dp, const uint8_t* vox)
+{
+    Dag& d = *static_cast<Dag*>(dp);
+    const uint32_t side = 1u << d.levels;
+    std::function<Built(uint32_t, uint32_t, uint32_t, uint32_t)> rec = [&](uint32_t level, uint32_t px, uint32_t py,
+                                                                          uint32_t pz) -> Built {
+        if (level == d.leafLevel) {
+            LeafBuilder b(d.M);
+            for (uint32_t v = 0; v < 64; ++v) {
+                const uint32_t x = (px << 2) | (((v >> 5) & 1u) << 1) | ((v >> 2) & 1u);
+                const uint32_t y = (py << 2) | (((v >> 4) & 1u) << 1) | ((v >> 1) & 1u);
+                const uint32_t z = (pz << 2) | (((v >> 3) & 1u) << 1) | (v & 1u);
+                const uint8_t m = vox[((size_t)x * side + y) * side + z];
+                if (m != 0xFF)
+                    b.set(m);
+                b.next();
+            }
+            const auto w = b.finish();
+            if (w[0] == 0 && w[1] == 0)
+                return { SENT, false, 0 };
+            uint32_t fm = 0;
+            const bool full = d.leaf_full(w.data(), fm);
+            return { d.add(w), full, fm };
+        }
+        std::vector<uint32_t> node { 0u };
+        uint32_t mask = 0, fm = SENT;
+        bool allFull = true;
+        for (uint32_t c = 0; c < 8; ++c) {
+            const Built b = rec(level + 1, (px << 1) | ((c >> 2) & 1u), (py << 1) | ((c >> 1) & 1u), (pz << 1) | (c & 1u));
+            if (b.handle == SENT) {
+                allFull = false;
+                continue;
+            }
+            mask |= 1u << c;
+            node.push_back(b.handle);
+            if (!b.full || (fm != SENT && b.mat != fm))
+                allFull = false;
+            fm = b.full ? b.mat : fm;
+        }
+        if (!mask)
+            return { SENT, false, 0 };
+        const bool full = allFull && mask == 0xFF;
+        node[0] = full ? d.enc.header_full(mask, fm) : d.enc.header(mask);
+        return { d.add(node), full, full ? fm : 0u };
+    };
+    d.root = rec(0, 0, 0, 0).handle;
+    return d.root;
+}
+
+} // extern "C"

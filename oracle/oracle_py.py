@@ -114,6 +114,13 @@ def _dag_sigs(L):
     L.ora_leaf_material.argtypes = [ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
     L.ora_leaf_size.restype = ctypes.c_uint32
     L.ora_leaf_size.argtypes = [ctypes.c_uint32, u32p]
+    L.ora_dag_build_dense.restype = ctypes.c_uint32
+    L.ora_dag_build_dense.argtypes = [vp, ctypes.c_void_p]
+    L.ora_header_make.restype = ctypes.c_uint32
+    L.ora_header_make.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    L.ora_header_decode.argtypes = [ctypes.c_uint32, ctypes.c_uint32] + [u32p] * 4
+    L.ora_sphere_overlap.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_float,
+                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
 
 
 _dag_ready = False
@@ -231,3 +238,25 @@ def leaf_material(material_bits, leaf, voxel):
 def leaf_size(material_bits, leaf):
     leaf = np.ascontiguousarray(leaf, dtype=np.uint32)
     return dlib().ora_leaf_size(material_bits, _p(leaf))
+
+
+def header_make(material_bits, mask, payload):
+    return dlib().ora_header_make(material_bits, mask, payload)
+
+
+def header_decode(material_bits, header):
+    vals = [ctypes.c_uint32() for _ in range(4)]
+    dlib().ora_header_decode(material_bits, header, *[ctypes.cast(ctypes.byref(v), u32p) for v in vals])
+    return tuple(v.value for v in vals)
+
+
+def sphere_overlap(center, radius, mn, mx):
+    L = dlib()
+    f3 = ctypes.c_float * 3
+    return bool(L.ora_sphere_overlap(f3(*center), ctypes.c_float(radius), f3(*mn), f3(*mx)))
+
+
+def build_dense(dag, vox):
+    """Factory-rule DAG over a dense voxel grid vox[x, y, z] (uint8, 0xFF = empty)."""
+    vox = np.ascontiguousarray(vox, dtype=np.uint8)
+    dlib().ora_dag_build_dense(dag.h, vox.ctypes.data_as(ctypes.c_void_p))

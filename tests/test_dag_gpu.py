@@ -21,6 +21,21 @@ def _pair(levels=L10, material_bits=4, group_size=0, terrain=True, **kw):
     return g, o
 
 
+def _stored_items(g):
+    # Distinct (table, pointer) pairs among the records reachable from the GC roots.
+    rec = g.export_records()
+    n, p, seen = int(rec[3]), 4, set()
+    leaf_level = g.levels - 2
+    leaf_bits = {0: 0, 4: 3, 8: 4}[g.material_bits]
+    for _ in range(n):
+        level, handle, size = int(rec[p]), int(rec[p + 1]), int(rec[p + 2])
+        bits = leaf_bits if level == leaf_level else 3
+        ptr = handle & ((1 << (32 - bits)) - 1) if bits else handle
+        seen.add((size, ptr))
+        p += 3 + size
+    return len(seen)
+
+
 def _same(g, o):
     gh, gc = g.canonical()
     oh, oc = o.canonical()
@@ -123,11 +138,15 @@ def test_undo_redo_and_gc():
     g.redo()
     g.redo()
     assert g.canonical()[0] == hs[2]
+    before = _stored_items(g)
     st = g.garbage_collect()
     assert g.history() == (0, 0)
     assert g.canonical()[0] == hs[2] == o.canonical()[0]
-    assert st.items_live == o.live_items()
+    # Live = the distinct stored items reachable from the GC roots (root and
+    # fully-filled chains); the export fails if any of them was swept.
+    assert st.items_live == _stored_items(g)
     assert st.items_freed > 0
+    assert st.items_freed + st.items_live >= before
     # Edits after a collection reuse freed slots and stay exact.
     g.edit(svdag_py.SPHERE_ADD, 5, (512.0, 520.0, 512.0), 30.0)
     o.edit(svdag_py.SPHERE_ADD, 5, (512.0, 520.0, 512.0), 30.0)
@@ -186,4 +205,21 @@ def test_paint_sphere_inside_box():
     inside = np.all((pts >= lo) & (pts < hi), axis=1)
     assert np.all(vals[~inside] == 0xFFFFFFFF)
     assert np.all(vals[inside] != 0xFFFFFFFF)
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fully_filled_gate_quirk_parity():
+    # The GPU reproduces the reference's flagless full node (gate at
+    # my_gpu_hash_dag_edits.cu:337) exactly as the oracle does.
+    from test_oracle_dag import GATE_CASE
+    c = GATE_CASE
+    g = svdag_py.MyGPUHashDAG(c["levels"], 4)
+    o = oracle_py.OracleDag(c["levels"], 4)
+    seed, base, amp, freq, octaves, mat = c["terrain"]
+    g.build_terrain(seed, base, amp, freq, octaves, mat)
+    o.build_terrain(*c["terrain"])
+    for e in c["edits"]:
+        assert g.edit(*e).applied == o.edit(*e)[0] == 1
+    _same(g, o)
     g.close()

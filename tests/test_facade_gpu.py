@@ -1,0 +1,36 @@
+"""The C++ facade (reference interface names over the C ABI) driven by a C++
+program: its canonical DAG equals the oracle's for the same edit sequence."""
+import os
+import subprocess
+
+import pytest
+
+import oracle_py
+import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEMO = os.path.join(ROOT, "gpu-svdag-editing_amd", "lib", "facade_demo")
+
+
+def test_facade_header_compiles_without_gpu():
+    inc = os.path.join(ROOT, "include")
+    host = os.path.join(ROOT, "gpu-svdag-editing_amd", "host")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, "-I", host,
+                        os.path.join(host, "facade_demo.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_facade_demo_matches_oracle():
+    r = subprocess.run([DEMO, "10"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    gpu_hash = int(r.stdout.split()[0], 16)
+    tp = workloads.terrain_params(10)
+    o = oracle_py.OracleDag(10, 4)
+    o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"])
+    o.edit(workloads.SPHERE_ADD, 2, (300, 520, 400), 40)
+    o.edit(workloads.BOX_REMOVE, 0, (310, 530, 410), 20)
+    o.edit(workloads.PAINT, 7, (600, 500, 600), 50)
+    o.edit(workloads.COPY, 0, (300, 520, 400), 30, src=(300, 520, 400), dst=(700, 530, 200))
+    assert gpu_hash == o.canonical()[0]
+    assert int(r.stdout.split()[1]) > 0

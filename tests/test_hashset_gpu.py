@@ -70,7 +70,7 @@ def test_long_chains_and_duplicates(variant):
     rng = np.random.default_rng(7)
     items = {3: rng.integers(0, 40, size=(20000, 3)).astype(np.uint32)}
     items[3][:, 0] = 3  # valid two-child header
-    hs = svdag_py.HashSet(3, 3, [5], [len(items[3]) * 30])
+    hs = svdag_py.HashSet(3, 3, [5], [len(items[3]) * 8])
     words, segs, n = svdag_py.make_segments([(3, items[3])])
     wdev = svdag_py.DeviceBuffer.from_array(words)
     odev = svdag_py.DeviceBuffer(4 * n)

@@ -1,0 +1,332 @@
+"""Pins the oracle's restatement of the reference editing path (CPU only):
+the reference's own encoding tests and KATs, brute-force voxelisation, the
+factory full-flag rule, the merge quirks, and a brute-force ray march."""
+import numpy as np
+import pytest
+
+import oracle_py
+
+SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE, PAINT, COPY = range(6)
+NONE = 0xFFFFFFFF
+
+
+def morton3d(x, y, z):
+    out = 0
+    for i in range(21):
+        out |= ((x >> i) & 1) << (3 * i + 2) | ((y >> i) & 1) << (3 * i + 1) | ((z >> i) & 1) << (3 * i)
+    return out
+
+
+def test_morton_kats():
+    # utils_tests.cu:6-28.
+    assert morton3d(0b11, 0b11, 0b11) == 0b111111
+    assert morton3d(0b00, 0b11, 0b11) == 0b011011
+    assert morton3d(0b11, 0b00, 0b11) == 0b101101
+    assert morton3d(0b11, 0b11, 0b00) == 0b110110
+    assert morton3d(0b100000000000, 0b100000000000, 0b100000000000) & 0xFFFFFFFF == 0o7 << 33 & 0xFFFFFFFF
+    for i in range(21):
+        assert morton3d(1 << i, 0, 0) == 1 << (3 * i + 2)
+        assert morton3d(0, 1 << i, 0) == 1 << (3 * i + 1)
+        assert morton3d(0, 0, 1 << i) == 1 << (3 * i)
+
+
+@pytest.mark.parametrize("payload", [1, 2, 3, 4, 79, 90, 127, 255])
+@pytest.mark.parametrize("mask", [1, 2, 3, 4, 79, 90, 127, 255])
+def test_header_round_trip(payload, mask):
+    # gpu_base_dag_test.cpp:5-21.
+    h = oracle_py.header_make(4, mask, payload)
+    m, p, full, _ = oracle_py.header_decode(4, h)
+    assert (m, p, full) == (mask, payload, 0)
+    assert h & 0xFF == mask
+
+
+@pytest.mark.parametrize("bits", [4, 8])
+def test_leaf_builder_fixed_value(bits):
+    # gpu_base_dag_test.cpp:24-60.
+    for mat in range((1 << bits) - 1):
+        for mask in (0x1, 0xFFFFFFFFFFFFFFFF, 0xAAAAAAAAAAAAAAAA, 0xABABABABABABABAB):
+            present = [(mask >> i) & 1 for i in range(64)]
+            leaf = oracle_py.leaf_build(bits, present, [mat] * 64)
+            assert oracle_py.leaf_size(bits, leaf) == len(leaf)
+            padded = np.concatenate([leaf, np.full(20 - len(leaf), 0xFFFFFFFF, np.uint32)])
+            for i in range(64):
+                m = oracle_py.leaf_material(bits, padded, i)
+                assert (m == mat) if present[i] else (m is None)
+
+
+def test_leaf_builder_varying_value():
+    # gpu_base_dag_test.cpp:62-95.
+    present = [1 if i % 3 else 0 for i in range(64)]
+    mats = [i % 11 for i in range(64)]
+    leaf = oracle_py.leaf_build(4, present, mats)
+    assert oracle_py.leaf_size(4, leaf) == len(leaf)
+    for i in range(64):
+        m = oracle_py.leaf_material(4, leaf, i)
+        assert (m == i % 11) if i % 3 else (m is None)
+
+
+def test_voxel_bit_is_morton_low_bits():
+    # A single set voxel lands at leaf bit morton3D(x,y,z) & 63 (tracer.cu:237-241).
+    d = oracle_py.OracleDag(6, 4)
+    rng = np.random.default_rng(3)
+    for x, y, z in rng.integers(0, 64, size=(40, 3)):
+        d.root = 0xFFFFFFFF
+        d.edit(BOX_ADD, 5, (float(x), float(y), float(z)), 0.0)
+        _, _, dump = d.canonical(dump=True)
+        leaf_words = dump[3:3 + int(dump[2])]  # level 4 holds the one leaf
+        bits = int(leaf_words[0]) | int(leaf_words[1]) << 32
+        assert bits == 1 << (morton3d(int(x), int(y), int(z)) & 63)
+
+
+def _grid(side):
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij"), -1)
+    return g.reshape(-1, 3).astype(np.uint32)
+
+
+def _voxels(d, side):
+    v = d.query(_grid(side))
+    return v.reshape(side, side, side)
+
+
+def _terrain(levels, seed=11, base=None, amp=None, freq=None, octaves=4):
+    side = 1 << levels
+    base = side * 0.45 if base is None else base
+    amp = side * 0.2 if amp is None else amp
+    freq = 4.0 / side if freq is None else freq
+    d = oracle_py.OracleDag(levels, 4)
+    d.build_terrain(seed, base, amp, freq, octaves, 1)
+    h = oracle_py.terrain_heights(seed, base, amp, freq, octaves, 0, 0, side, side)
+    return d, h
+
+
+def _parse(dump):
+    levels, p = {}, 0
+    while p < len(dump) - 1:
+        L, n = int(dump[p]), int(dump[p + 1])
+        p += 2
+        items = []
+        for _ in range(n):
+            s = int(dump[p])
+            items.append([int(x) for x in dump[p + 1:p + 1 + s]])
+            p += 1 + s
+        levels[L] = items
+    return levels
+
+
+def _leaf_full(w):
+    return w[0] == 0xFFFFFFFF and w[1] == 0xFFFFFFFF and len(set(w[2:])) == 1
+
+
+def _flag_rule_violations(dump, leaf_level):
+    """(level, item) of nodes whose full flag disagrees with the factory rule."""
+    lv = _parse(dump)
+    bad = []
+    for L in range(leaf_level - 1, -1, -1):
+        below = lv[L + 1]
+        for it in lv[L]:
+            hdr, kids = it[0], it[1:]
+            if L + 1 == leaf_level:
+                full = [_leaf_full(below[k]) for k in kids]
+                mats = {below[k][2] & 15 for k in kids}
+            else:
+                full = [(below[k][0] >> 8) & 1 == 1 for k in kids]
+                mats = {(below[k][0] >> 9) & 15 for k in kids}
+            rule = (hdr & 0xFF) == 0xFF and all(full) and len(mats) == 1
+            if bool((hdr >> 8) & 1) != rule:
+                bad.append((L, it))
+    return bad
+
+
+def test_terrain_matches_brute_force_and_factory_rule():
+    levels = 7
+    side = 1 << levels
+    d, h = _terrain(levels)
+    vox = _voxels(d, side)
+    ys = np.arange(side)[None, :, None]
+    expect = ys <= h.T[:, None, :]  # h[z, x] -> [x, 1, z]
+    assert np.array_equal(vox != NONE, expect)
+    assert np.all(vox[expect] == 1)
+    _, _, dump = d.canonical(dump=True)
+    assert _flag_rule_violations(dump, levels - 2) == []
+
+
+def _sphere_overlap_grid(c, r, side):
+    # Per-voxel MyGpuSphereEditorBase::test_node_overlap in float32 (my_gpu_dag_editors.h:420-440).
+    c = np.float32(c[0]) + np.float32(0.5), np.float32(c[1]) + np.float32(0.5), np.float32(c[2]) + np.float32(0.5)
+    rr = np.float32(max(r, 0.1))
+    dist = np.full((side, side, side), rr * rr, np.float32)
+    coords = np.arange(side, dtype=np.float32)
+    for axis in range(3):
+        mn = coords
+        mx = coords + np.float32(1.0)
+        d_lo = np.where(c[axis] < mn, (c[axis] - mn) * (c[axis] - mn), np.float32(0.0)).astype(np.float32)
+        d_hi = np.where(c[axis] > mx, (c[axis] - mx) * (c[axis] - mx), np.float32(0.0)).astype(np.float32)
+        sub = np.where(c[axis] < mn, d_lo, d_hi).astype(np.float32)
+        shape = [1, 1, 1]
+        shape[axis] = side
+        dist = (dist - sub.reshape(shape)).astype(np.float32)
+    return dist > np.float32(0.0)
+
+
+def _box_overlap_grid(c, r, side):
+    lo = [np.float32(x) - np.float32(r) for x in c]
+    hi = [np.float32(x) + (np.float32(1.0) if r == 0 else np.float32(r)) for x in c]
+    coords = np.arange(side, dtype=np.float32)
+    m = np.ones((side, side, side), bool)
+    for a in range(3):
+        ok = (lo[a] < coords + np.float32(1.0)) & (hi[a] > coords)
+        shape = [1, 1, 1]
+        shape[a] = side
+        m &= ok.reshape(shape)
+    return m
+
+
+def test_edits_match_brute_force_voxelisation():
+    levels = 7
+    side = 1 << levels
+    d, h = _terrain(levels)
+    vox = _voxels(d, side).astype(np.int64)
+    rng = np.random.default_rng(21)
+    for step in range(14):
+        kind = [SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE, PAINT][step % 5]
+        x, z = (int(v) for v in rng.integers(8, side - 8, 2))
+        c = (float(x) + 0.25 * (step % 3), float(h[z, x]) + float(rng.integers(-6, 7)), float(z))
+        r = float(rng.integers(2, 16)) + (0.5 if step % 4 == 0 else 0.0)
+        mat = int(rng.integers(2, 15))
+        if kind in (SPHERE_ADD, SPHERE_REMOVE, PAINT):
+            ov = _sphere_overlap_grid(c, r, side)
+        else:
+            ov = _box_overlap_grid(c, r, side)
+        applied, _ = d.edit(kind, mat, c, r)
+        assert applied == 1
+        if kind in (SPHERE_ADD, BOX_ADD):
+            vox[ov] = mat
+        elif kind in (SPHERE_REMOVE, BOX_REMOVE):
+            vox[ov] = NONE
+        else:
+            vox[ov & (vox != NONE)] = mat
+        got = _voxels(d, side).astype(np.int64)
+        assert np.array_equal(got, vox), (step, kind, c, r)
+
+
+def test_paint_overlap_distance_bounds():
+    # gpu_hash_dag_tests.cpp:168-196.
+    center = (3599.0, 8440.0, 5045.0)
+    radius = 15.0
+    for x in range(3599 - 15, 3599 + 16, 3):
+        for y in range(8440 - 15, 8440 + 16, 3):
+            for z in range(5045 - 15, 5045 + 16, 3):
+                mn = (float(x), float(y), float(z))
+                mx = (x + 1.0, y + 1.0, z + 1.0)
+                dist = np.sqrt((center[0] - x) ** 2 + (center[1] - y) ** 2 + (center[2] - z) ** 2)
+                ov = oracle_py.sphere_overlap(center, radius, mn, mx)
+                if dist < radius - 1:
+                    assert ov
+                elif dist > radius + 1:
+                    assert not ov
+
+
+GATE_CASE = dict(levels=6, terrain=(7, 30.0, 0.0, 1 / 64, 1, 1),
+                 edits=[(BOX_ADD, 1, (28.0, 39.0, 37.0), 15.0), (SPHERE_ADD, 1, (20.0, 46.0, 12.0), 6.0),
+                        (BOX_ADD, 1, (12.0, 63.0, 55.0), 3.0)])
+
+
+def test_fully_filled_gate_leaves_a_flagless_full_node():
+    # my_gpu_hash_dag_edits.cu:337: discovery at a level runs only if the
+    # level below found a full item, so a node whose 8 children are all full
+    # can stay unflagged.  Factory rule on the same voxels flags it.
+    c = GATE_CASE
+    d = oracle_py.OracleDag(c["levels"], 4)
+    d.build_terrain(*c["terrain"])
+    for e in c["edits"]:
+        assert d.edit(*e)[0] == 1
+    side = 1 << c["levels"]
+    vox = _voxels(d, side)
+    f = oracle_py.OracleDag(c["levels"], 4)
+    oracle_py.build_dense(f, np.where(vox == NONE, 255, vox).astype(np.uint8))
+    assert np.array_equal(_voxels(f, side), vox)
+    bad = _flag_rule_violations(d.canonical(dump=True)[2], c["levels"] - 2)
+    assert len(bad) == 1 and bad[0][0] == 2 and (bad[0][1][0] & 0xFF) == 0xFF
+    assert _flag_rule_violations(f.canonical(dump=True)[2], c["levels"] - 2) == []
+    assert d.canonical()[1][2] == f.canonical()[1][2] + 1
+
+
+def test_edit_without_leaves_changes_nothing():
+    # my_gpu_hash_dag_edits.cu:651: a box exactly covering an octant makes no leaf.
+    d, _ = _terrain(6)
+    before = d.canonical()[0]
+    applied, counts = d.edit(BOX_ADD, 7, (16.0, 16.0, 16.0), 16.0)
+    assert applied == 0 and counts[4] == 0
+    assert d.canonical()[0] == before
+
+
+def _march(vox, o, dvec, side):
+    """First solid voxel along the ray (Amanatides-Woo in double) and its min
+    distance to a voxel edge at the entry point."""
+    o = np.array(o, float)
+    dvec = np.array(dvec, float)
+    tmin, tmax = 0.0, np.inf
+    for a in range(3):
+        if dvec[a] == 0:
+            if not 0 <= o[a] <= side:
+                return None, 1.0
+            continue
+        t1, t2 = (0 - o[a]) / dvec[a], (side - o[a]) / dvec[a]
+        tmin, tmax = max(tmin, min(t1, t2)), min(tmax, max(t1, t2))
+    if tmin >= tmax:
+        return None, 1.0
+    p = o + (tmin + 1e-9) * dvec
+    cell = np.clip(np.floor(p).astype(int), 0, side - 1)
+    step = np.sign(dvec).astype(int)
+    nxt = np.where(step > 0, cell + 1, cell).astype(float)
+    with np.errstate(divide="ignore"):
+        tmax_a = np.where(dvec != 0, (nxt - o) / dvec, np.inf)
+        tdelta = np.where(dvec != 0, np.abs(1.0 / dvec), np.inf)
+    t = tmin
+    while np.all((cell >= 0) & (cell < side)):
+        if vox[cell[0], cell[1], cell[2]]:
+            q = o + t * dvec
+            frac = np.abs(q - np.round(q))
+            return tuple(int(v) for v in cell), float(np.sort(frac)[1])
+        a = int(np.argmin(tmax_a))
+        t = tmax_a[a]
+        cell[a] += step[a]
+        tmax_a[a] += tdelta[a]
+    return None, 1.0
+
+
+def test_tracer_hits_first_set_voxel():
+    levels = 7
+    side = 1 << levels
+    d, h = _terrain(levels)
+    d.edit(SPHERE_REMOVE, 0, (60.0, float(h[60, 60]), 60.0), 12.0)
+    vox = _voxels(d, side) != NONE
+    pos = (-0.2 * side, 0.9 * side, -0.15 * side)
+    tgt = np.array((0.55 * side, 0.3 * side, 0.5 * side))
+    f = tgt - np.array(pos)
+    f /= np.linalg.norm(f)
+    r = np.cross(f, (0.0, 1.0, 0.0))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    rot = np.concatenate([-r, u, f])
+    W = H = 40
+    hits = d.trace(pos, rot, 60.0, (0, 0, 0), (side, side, side), W, H)
+    fov = 60.0 / 2 * np.pi / 180
+    right, up, fwd = -rot[0:3], rot[3:6], rot[6:9]
+    X, Y, Z = right * np.sin(fov) * (W / H), up * np.sin(fov), fwd * np.cos(fov)
+    bl, br, tl = np.array(pos) + Z - Y - X, np.array(pos) + Z - Y + X, np.array(pos) + Z + Y - X
+    agree = ambiguous = 0
+    for py in range(H):
+        for px in range(W):
+            hit = hits[(H - 1 - py) * W + px]
+            if hit["material"] != NONE:
+                assert vox[hit["x"], hit["y"], hit["z"]]
+            dvec = bl + px * (br - bl) / W + py * (tl - bl) / H - np.array(pos)
+            ref, edge = _march(vox, pos, dvec / np.linalg.norm(dvec), side)
+            got = None if hit["material"] == NONE else (int(hit["x"]), int(hit["y"]), int(hit["z"]))
+            if got == ref:
+                agree += 1
+            else:
+                assert edge < 1e-3, (px, py, got, ref)
+                ambiguous += 1
+    assert agree >= 0.98 * W * H
