@@ -122,8 +122,9 @@ __global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy)
 }
 
 // One work item per 8 lanes, one lane per child (create_edit_svo.cu:167-303).
-__global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, EditorDev ed, uint32_t L)
+__global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, const EditorDev* edp, uint32_t L)
 {
+    const EditorDev& ed = *edp;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t c = lane & 7u;
     const uint32_t gbase = lane & ~7u;
@@ -202,8 +203,9 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, Edi
 }
 
 // One leaf per wave64, one lane per voxel (create_edit_svo.cu:320-426).
-__global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, EditorDev ed)
+__global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, const EditorDev* edp)
 {
+    const EditorDev& ed = *edp;
     __shared__ uint32_t lds[4][16];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
@@ -664,26 +666,28 @@ static uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
     return n;
 }
 
-static int launch_edit(svdag_ctx* ctx, const EditorDev& ed)
+// Enqueue one edit: ed is the host copy (for grid sizing), edDev the same
+// editor in device memory, ws the workspace view whose status / level table
+// slots this edit uses.
+static int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, const WsDev& ws)
 {
     const DagDev& dag = ctx->dag;
-    const WsDev& ws = ctx->ws;
     const hipStream_t st = ctx->stream;
     const Bounds b = edit_bounds(ctx, ed);
     const uint32_t kCap = 8192;
     edit_begin<<<1, 64, 0, st>>>(ws, ctx->dRoot, ed.kind == EK_COPY ? 1u : 0u);
     for (uint32_t L = 0; L < dag.leafLevel; ++L) {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, L), ws.capWork);
-        svo_inner_level<<<grid_for(items, 32, kCap), 256, 0, st>>>(ws, dag, ed, L);
+        svo_inner_level<<<grid_for(items, 32, kCap), 256, 0, st>>>(ws, dag, edDev, L);
     }
     {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, dag.leafLevel), ws.capLeaf);
-        svo_leaf_level<<<grid_for(items, 4, kCap), 256, 0, st>>>(ws, dag, ed);
+        svo_leaf_level<<<grid_for(items, 4, kCap), 256, 0, st>>>(ws, dag, edDev);
     }
     const uint32_t G = ctx->groupSize;
     for (int L = (int)dag.leafLevel; L >= 0; --L) {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork);
-        merge_prepare<<<grid_for(std::max<uint64_t>(items, 2 * items), 256, kCap), 256, 0, st>>>(ws, dag, (uint32_t)L);
+        merge_prepare<<<grid_for(2 * items, 256, kCap), 256, 0, st>>>(ws, dag, (uint32_t)L);
         const uint32_t gridG = grid_for(items, 256 / G, kCap);
         switch (G) {
         case 4:
@@ -709,30 +713,96 @@ static int launch_edit(svdag_ctx* ctx, const EditorDev& ed)
     return SVDAG_OK;
 }
 
-// Run one edit to completion: retries with a larger workspace on overflow
-// and with larger slab arenas on OOM (nothing is merged in either case).
-static int run_edit(svdag_ctx* ctx, const EditorDev& ed, svdag_edit_stats* stats)
+// Per-edit device slots for a batch of up to n edits.
+static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
 {
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        if (ed.kind == EK_COPY && !ctx->ws.state)
+    if (n <= ctx->batchCap)
+        return SVDAG_OK;
+    for (void* p : { (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr })
+        if (p)
+            (void)hipFree(p);
+    if (ctx->hBatch)
+        (void)hipHostFree(ctx->hBatch);
+    SVDAG_HIP(hipMalloc(&ctx->dEditors, (size_t)n * sizeof(EditorDev)));
+    SVDAG_HIP(hipMalloc(&ctx->dStatusArr, (size_t)n * sizeof(EditStatus)));
+    SVDAG_HIP(hipMalloc(&ctx->dLevelArr, (size_t)n * kMaxLevels * sizeof(LevelDesc)));
+    SVDAG_HIP(hipHostMalloc(&ctx->hBatch, (size_t)n * (sizeof(EditorDev) + sizeof(EditStatus) + kMaxLevels * sizeof(LevelDesc))));
+    ctx->batchCap = n;
+    return SVDAG_OK;
+}
+
+// Run edits [0, n) in order with no host round trip between them: edit k's
+// first kernel reads the root edit k-1's last kernel wrote.  On overflow or
+// OOM at edit k the root is put back to edit k's input, the workspace or
+// arena grows, and edits k.. run again (later results are unreachable).
+int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats* stats, float* totalMs)
+{
+    SVDAG_TRY(ensure_batch_slots(ctx, n));
+    EditorDev* hEd = reinterpret_cast<EditorDev*>(ctx->hBatch);
+    EditStatus* hSt = reinterpret_cast<EditStatus*>(hEd + ctx->batchCap);
+    LevelDesc* hLv = reinterpret_cast<LevelDesc*>(hSt + ctx->batchCap);
+    float ms = 0.0f;
+    uint32_t first = 0;
+    for (int attempt = 0; attempt < 8 && first < n; ++attempt) {
+        bool needState = false;
+        for (uint32_t k = first; k < n; ++k) {
+            hEd[k] = eds[k];
+            needState |= eds[k].kind == EK_COPY;
+        }
+        if (needState && !ctx->ws.state)
             SVDAG_TRY(ws_alloc(ctx, ctx->ws.capWork, true));
-        SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        SVDAG_TRY(launch_edit(ctx, ed));
-        SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        SVDAG_HIP(hipMemcpyAsync(ctx->hStatus, ctx->ws.status, sizeof(EditStatus), hipMemcpyDeviceToHost, ctx->stream));
-        SVDAG_HIP(hipMemcpyAsync(ctx->hLevels, ctx->ws.lvl, kMaxLevels * sizeof(LevelDesc), hipMemcpyDeviceToHost,
+        SVDAG_HIP(hipMemcpyAsync(ctx->dEditors + first, hEd + first, (n - first) * sizeof(EditorDev), hipMemcpyHostToDevice,
                                  ctx->stream));
+        SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        for (uint32_t k = first; k < n; ++k) {
+            WsDev ws = ctx->ws;
+            ws.status = ctx->dStatusArr + k;
+            ws.lvl = ctx->dLevelArr + (size_t)k * kMaxLevels;
+            SVDAG_TRY(launch_edit(ctx, eds[k], ctx->dEditors + k, ws));
+        }
+        SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(hSt + first, ctx->dStatusArr + first, (n - first) * sizeof(EditStatus),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(hLv + (size_t)first * kMaxLevels, ctx->dLevelArr + (size_t)first * kMaxLevels,
+                                 (size_t)(n - first) * kMaxLevels * sizeof(LevelDesc), hipMemcpyDeviceToHost, ctx->stream));
         SVDAG_HIP(hipStreamSynchronize(ctx->stream));
-        const EditStatus s = *ctx->hStatus;
+        float part = 0.0f;
+        SVDAG_HIP(hipEventElapsedTime(&part, ctx->ev0, ctx->ev1));
+        ms += part;
+        uint32_t bad = n;
+        for (uint32_t k = first; k < n; ++k)
+            if (hSt[k].overflow || hSt[k].oom) {
+                bad = k;
+                break;
+            }
+        for (uint32_t k = first; k < bad; ++k) {
+            ctx->root = hSt[k].rootAfter;
+            if (stats) {
+                svdag_edit_stats& o = stats[k];
+                std::memset(&o, 0, sizeof(o));
+                for (uint32_t L = 0; L <= ctx->dag.leafLevel && L < 32; ++L)
+                    o.svo_items[L] = hLv[(size_t)k * kMaxLevels + L].count;
+                o.applied = hSt[k].applied;
+                o.root_before = hSt[k].rootBefore;
+                o.root_after = hSt[k].rootAfter;
+                o.gpu_ms = part / (float)(n - first);
+            }
+        }
+        if (bad == n) {
+            first = n;
+            break;
+        }
+        const EditStatus s = hSt[bad];
+        ctx->root = s.rootBefore;
+        SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
         if (s.overflow) {
             uint64_t need = 0;
-            for (uint32_t L = 0; L <= ctx->dag.leafLevel; ++L)
-                need = std::max<uint64_t>(need, (uint64_t)ctx->hLevels[L].start + ctx->hLevels[L].count);
-            uint64_t cap = std::max<uint64_t>(2ull * ctx->ws.capWork, need + need / 4);
-            SVDAG_TRY(ws_alloc(ctx, cap, ed.kind == EK_COPY));
-            continue;
-        }
-        if (s.oom) {
+            for (uint32_t L = 0; L <= ctx->dag.leafLevel; ++L) {
+                const LevelDesc& d = hLv[(size_t)bad * kMaxLevels + L];
+                need = std::max<uint64_t>(need, (uint64_t)d.start + d.count);
+            }
+            SVDAG_TRY(ws_alloc(ctx, std::max<uint64_t>(2ull * ctx->ws.capWork, need + need / 4), ctx->ws.state != nullptr));
+        } else {
             for (auto& t : ctx->ts.tables) {
                 AllocState a {};
                 SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
@@ -740,27 +810,29 @@ static int run_edit(svdag_ctx* ctx, const EditorDev& ed, svdag_edit_stats* stats
                     SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, 1ull << 25), ctx->stream));
             }
             SVDAG_TRY(ctx_refresh_tables(ctx));
-            continue;
         }
-        ctx->root = s.rootAfter;
-        if (stats) {
-            std::memset(stats, 0, sizeof(*stats));
-            for (uint32_t L = 0; L <= ctx->dag.leafLevel && L < 32; ++L)
-                stats->svo_items[L] = ctx->hLevels[L].count;
-            stats->applied = s.applied;
-            stats->root_before = s.rootBefore;
-            stats->root_after = s.rootAfter;
-            float ms = 0;
-            SVDAG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            stats->gpu_ms = ms;
-        }
-        return SVDAG_OK;
+        first = bad;
     }
-    set_error("edit did not fit after repeated workspace / arena growth");
-    return SVDAG_E_CAPACITY;
+    if (first < n) {
+        set_error("edit did not fit after repeated workspace / arena growth");
+        return SVDAG_E_CAPACITY;
+    }
+    if (totalMs)
+        *totalMs = ms;
+    return SVDAG_OK;
 }
 
-static int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e)
+// Run one edit to completion.
+static int run_edit(svdag_ctx* ctx, const EditorDev& ed, svdag_edit_stats* stats)
+{
+    float ms = 0.0f;
+    SVDAG_TRY(run_edits(ctx, &ed, 1, stats, &ms));
+    if (stats)
+        stats->gpu_ms = ms;
+    return SVDAG_OK;
+}
+
+int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e)
 {
     e = EditorDev {};
     e.kind = d.kind;
@@ -894,9 +966,12 @@ int svdag_destroy(svdag_ctx* ctx)
     ctx->ts.release();
     ws_release(ctx->ws);
     for (void* p : { (void*)ctx->dFullyFilled, (void*)ctx->dRoot, (void*)ctx->hmin, (void*)ctx->hmax,
-                     (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits })
+                     (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits,
+                     (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr })
         if (p)
             (void)hipFree(p);
+    if (ctx->hBatch)
+        (void)hipHostFree(ctx->hBatch);
     (void)hipHostFree(ctx->hStatus);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -1000,6 +1075,29 @@ int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* st
     }
     svdag_edit_stats local;
     return run_edit(ctx, e, stats ? stats : &local);
+}
+
+int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, svdag_edit_stats* stats, float* gpu_ms)
+{
+    if (!ctx || (n && !descs))
+        return SVDAG_E_INVALID;
+    if (n == 0)
+        return SVDAG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::vector<EditorDev> eds(n);
+    for (uint32_t k = 0; k < n; ++k)
+        SVDAG_TRY(make_editor(ctx, descs[k], eds[k]));
+    std::vector<svdag_edit_stats> local(stats ? 0 : n);
+    svdag_edit_stats* st = stats ? stats : local.data();
+    SVDAG_TRY(run_edits(ctx, eds.data(), n, st, gpu_ms));
+    // Undo frames in edit order, as add_frame before every editMyHashDag.
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ctx->undo.empty() || ctx->undo.back() != st[k].root_before) {
+            ctx->redo.clear();
+            ctx->undo.push_back(st[k].root_before);
+        }
+    }
+    return SVDAG_OK;
 }
 
 static int set_root(svdag_ctx* ctx, uint32_t r)

@@ -65,6 +65,13 @@ struct svdag_ctx {
     uint32_t* gcFront[2] = { nullptr, nullptr };
     uint64_t gcCap = 0;
     uint32_t* gcCounters = nullptr;
+    // Per-edit device slots of a batch (editors, statuses, level tables) and
+    // their pinned host mirror.
+    svdag::EditorDev* dEditors = nullptr;
+    svdag::EditStatus* dStatusArr = nullptr;
+    svdag::LevelDesc* dLevelArr = nullptr;
+    void* hBatch = nullptr;
+    uint32_t batchCap = 0;
     // Tracer output buffer.
     void* dHits = nullptr;
     size_t hitCap = 0;
@@ -73,4 +80,10 @@ struct svdag_ctx {
 
 namespace svdag {
 int ctx_refresh_tables(svdag_ctx* ctx);
+int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
+                 svdag_hit* outDev);
+// Run edits [0, n) in order on ctx->stream (one sync at the end), with the
+// overflow / OOM retry; used by svdag_edit, svdag_edit_batch and the server.
+int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats* stats, float* totalMs);
+int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e);
 }

@@ -98,8 +98,7 @@ __device__ inline uint32_t intersection_mask(uint32_t levels, uint32_t level, ui
 
 constexpr uint32_t kTraceBlock = 256;
 
-__global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, const uint32_t* rootPtr, TraceParams p,
-                                                            svdag_hit* out)
+__global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t root, TraceParams p, svdag_hit* out)
 {
     __shared__ uint2 stack[20][kTraceBlock]; // levels <= 20
     const uint32_t tid = threadIdx.x;
@@ -130,7 +129,6 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, const ui
     uint32_t level = 0;
     uint32_t x = 0, y = 0, z = 0;
     uint32_t material = kSentinel;
-    const uint32_t root = *rootPtr;
 
     if (root != kSentinel) {
         uint32_t cIndex = root;
@@ -255,6 +253,18 @@ static TraceParams trace_params(const svdag_camera& c, uint32_t levels, uint32_t
     return p;
 }
 
+// Enqueue a trace of the DAG rooted at `root` (a value, so concurrent edits
+// that publish new roots cannot change what this frame sees).
+int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
+                 svdag_hit* outDev)
+{
+    const TraceParams p = trace_params(cam, ctx->dag.levels, W, H);
+    const uint32_t tiles = ((W + 15) / 16) * ((H + 15) / 16);
+    trace_kernel<<<tiles, kTraceBlock, 0, stream>>>(ctx->dag, root, p, outDev);
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
 } // namespace svdag
 
 using namespace svdag;
@@ -266,10 +276,6 @@ extern "C" int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* cam, uint
         set_error("svdag_trace_primary: bad arguments");
         return SVDAG_E_INVALID;
     }
-    if (ctx->dag.levels > 20) {
-        set_error("tracer supports at most 20 levels");
-        return SVDAG_E_INVALID;
-    }
     std::lock_guard<std::mutex> lock(ctx->mu);
     const size_t n = (size_t)W * H;
     if (ctx->hitCap < n) {
@@ -279,11 +285,8 @@ extern "C" int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* cam, uint
         SVDAG_HIP(hipMalloc(&ctx->dHits, n * sizeof(svdag_hit)));
         ctx->hitCap = n;
     }
-    const TraceParams p = trace_params(*cam, ctx->dag.levels, W, H);
-    const uint32_t tiles = ((W + 15) / 16) * ((H + 15) / 16);
     SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    trace_kernel<<<tiles, kTraceBlock, 0, ctx->stream>>>(ctx->dag, ctx->dRoot, p, (svdag_hit*)ctx->dHits);
-    SVDAG_HIP(hipGetLastError());
+    SVDAG_TRY(launch_trace(ctx, *cam, W, H, ctx->root, ctx->stream, (svdag_hit*)ctx->dHits));
     SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     SVDAG_HIP(hipMemcpyAsync(out, ctx->dHits, n * sizeof(svdag_hit), hipMemcpyDeviceToHost, ctx->stream));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));

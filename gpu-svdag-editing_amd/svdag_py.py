@@ -75,7 +75,7 @@ EXPORTED = [
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_terrain_heights", "svdag_root",
-    "svdag_edit", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
+    "svdag_edit", "svdag_edit_batch", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
     "svdag_trace_primary", "svdag_export_canonical", "svdag_export_tables",
 ]
 
@@ -249,6 +249,21 @@ class MyGPUHashDAG:
         st = EditStats()
         check(lib().svdag_edit(self.h, ctypes.byref(d), ctypes.byref(st)))
         return st
+
+    def edit_batch(self, edits):
+        """edits: list of (kind, material, center, radius[, src, dst]); returns (stats list, gpu ms)."""
+        n = len(edits)
+        descs = (EditDesc * n)()
+        for i, e in enumerate(edits):
+            kind, mat, c, r = e[:4]
+            src = e[4] if len(e) > 4 else (0, 0, 0)
+            dst = e[5] if len(e) > 5 else (0, 0, 0)
+            descs[i] = EditDesc(kind, mat, (ctypes.c_float * 3)(*c), float(r), (ctypes.c_float * 3)(*src),
+                                (ctypes.c_float * 3)(*dst))
+        stats = (EditStats * n)()
+        ms = ctypes.c_float(0)
+        check(lib().svdag_edit_batch(self.h, descs, n, stats, ctypes.byref(ms)))
+        return list(stats), ms.value
 
     def undo(self):
         check(lib().svdag_undo(self.h))

@@ -108,3 +108,46 @@ def config1_camera(levels=10):
     side = float(1 << levels)
     return dict(position=(-0.1 * side, 0.85 * side, -0.1 * side), target=(0.5 * side, 0.35 * side, 0.5 * side),
                 fov=60.0, bmin=(0.0, 0.0, 0.0), bmax=(side, side, side))
+
+
+def voxel_edit_count(kind, center, radius, levels):
+    """Voxels whose unit box satisfies the tool's overlap predicate, by per-row
+    interval counting (the unit of "voxel edits/s").  Spheres use the tool's
+    centre + 0.5 and radius >= 0.1; boxes use [c - r, c + r) (r = 0 -> one voxel)."""
+    side = 1 << levels
+    if kind in (SPHERE_ADD, SPHERE_REMOVE, PAINT):
+        c = np.asarray(center, np.float64) + 0.5
+        R = max(float(radius), 0.1)
+        ys = np.arange(max(0, int(np.floor(c[1] - R)) - 1), min(side, int(np.ceil(c[1] + R)) + 1))
+        zs = np.arange(max(0, int(np.floor(c[2] - R)) - 1), min(side, int(np.ceil(c[2] + R)) + 1))
+        if len(ys) == 0 or len(zs) == 0:
+            return 0
+
+        def gap(cc, v):
+            return np.where(cc < v, v - cc, np.where(cc > v + 1, cc - (v + 1), 0.0))
+
+        gy = gap(c[1], ys.astype(np.float64))[:, None]
+        gz = gap(c[2], zs.astype(np.float64))[None, :]
+        T = R * R - gy * gy - gz * gz
+        s = np.sqrt(np.maximum(T, 0.0))
+        # x overlaps iff the gap from c.x to [x, x+1] is < s: x in (c.x - s - 1, c.x + s).
+        lo = np.maximum(np.floor(c[0] - s - 1.0) + 1.0, 0.0)
+        hi = np.minimum(np.ceil(c[0] + s) - 1.0, side - 1.0)
+        n = np.where(T > 0, np.maximum(hi - lo + 1.0, 0.0), 0.0)
+        return int(n.sum())
+    r = float(radius)
+    total = 1
+    for a in range(3):
+        bmin = center[a] - r
+        bmax = center[a] + (1.0 if r == 0 else r)
+        lo = max(int(np.floor(bmin - 1.0)) + 1, 0)
+        hi = min(int(np.ceil(bmax)) - 1, side - 1)
+        total *= max(hi - lo + 1, 0)
+    return total
+
+
+def config3_edits(height_at, levels, n=256, seed=6):
+    """Edit stream of the batched-editing workload: 40% sphere-add, 40%
+    sphere-remove, 20% axis-aligned box fill; radius log-uniform in [8, 512];
+    centre (x, h(x,z) + U[-1024, 1024], z), x, z ~ U[0, 2^D)."""
+    return surface_edits(height_at, n, seed, levels, rmin=8.0, rmax=512.0, kinds=(0.4, 0.4, 0.2), near=1024.0)

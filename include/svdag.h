@@ -162,6 +162,13 @@ typedef struct {
  * (my_gpu_hash_dag_edits.h:12-13; my_gpu_hash_dag_edits.cu:633-671).  stats may be NULL. */
 int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* edit, svdag_edit_stats* stats);
 
+/* Apply n edits in order with no host round trip between them (each edit's
+ * kernels read the root the previous edit wrote on the device); one sync at
+ * the end.  Same result and undo frames as n calls of svdag_edit.  No
+ * reference counterpart (the reference syncs several times per edit,
+ * my_gpu_hash_dag_edits.cu:526-626).  stats: n entries or NULL. */
+int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* edits, uint32_t n, svdag_edit_stats* stats, float* gpu_ms);
+
 /* reference: MyGPUHashDAGUndoRedo::undo / redo (my_gpu_base_dag.h:35-64). */
 int svdag_undo(svdag_ctx* ctx);
 int svdag_redo(svdag_ctx* ctx);

@@ -223,3 +223,26 @@ def test_fully_filled_gate_quirk_parity():
         assert g.edit(*e).applied == o.edit(*e)[0] == 1
     _same(g, o)
     g.close()
+
+
+@pytest.mark.gpu
+def test_batched_edits_equal_sequential_edits():
+    heights = oracle_py.terrain_heights(workloads.terrain_params(L10)["seed"], 512.0, 200.0, 1 / 256, 5, 0, 0, 1024, 1024)
+    edits = workloads.surface_edits(lambda x, z: heights[z, x], 20, seed=9, levels=L10, rmin=2.0, rmax=120.0, near=60.0)
+    edits.append((svdag_py.BOX_ADD, 7, (256.0, 256.0, 256.0), 256.0))  # makes no leaf: skipped in both forms
+    edits.append((svdag_py.COPY, 0, (200.0, 500.0, 200.0), 30.0, (200.0, 500.0, 200.0), (500.0, 520.0, 700.0)))
+    a, o = _pair()
+    b, _ = _pair(terrain=True)
+    seq = [a.edit(*e) for e in edits]
+    for e in edits:
+        o.edit(*e)
+    bat, ms = b.edit_batch(edits)
+    assert ms > 0
+    for s1, s2 in zip(seq, bat):
+        assert s1.applied == s2.applied
+        assert list(s1.svo_items) == list(s2.svo_items)
+    assert a.canonical() == b.canonical()
+    assert b.canonical()[0] == o.canonical()[0]
+    assert a.history() == b.history()
+    a.close()
+    b.close()
