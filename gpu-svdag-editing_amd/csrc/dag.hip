@@ -21,24 +21,6 @@
 
 namespace svdag {
 
-__device__ inline bool leaf_is_full(const uint32_t* leaf, uint32_t M, uint32_t& mat)
-{
-    if (leaf[0] != 0xFFFFFFFFu || leaf[1] != 0xFFFFFFFFu)
-        return false;
-    if (M == 0) {
-        mat = 0;
-        return true;
-    }
-    const uint32_t m = leaf[2] & ((1u << M) - 1u);
-    const uint32_t rep = (M == 4) ? m * 0x11111111u : m * 0x01010101u;
-    const uint32_t words = (64u * M) / 32u;
-    for (uint32_t i = 0; i < words; ++i)
-        if (leaf[2 + i] != rep)
-            return false;
-    mat = m;
-    return true;
-}
-
 __device__ inline uint32_t dedup_cap(uint32_t count, uint32_t maxCap)
 {
     uint32_t cap = 64;
@@ -122,9 +104,11 @@ __global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy)
 }
 
 // One work item per 8 lanes, one lane per child (create_edit_svo.cu:167-303).
-__global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, const EditorDev* edp, uint32_t L)
+// kMulti: items of several edits share the level; item w uses eds[editOf[w]]
+// and its children inherit the edit.
+template <bool kMulti>
+__global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, const EditorDev* eds, uint32_t L)
 {
-    const EditorDev& ed = *edp;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t c = lane & 7u;
     const uint32_t gbase = lane & ~7u;
@@ -141,6 +125,8 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
     const uint32_t count = me.start >= ws.capWork ? 0u : min(me.count, ws.capWork - me.start);
     for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / 8u; i < count; i += gridDim.x * groupsPerBlock) {
         const uint32_t w = me.start + i;
+        const uint32_t e = kMulti ? ws.editOf[w] : 0u;
+        const EditorDev& ed = eds[e];
         const uint4 item = ws.work[w];
         uint32_t childHandle = kSentinel;
         if (item.w != kSentinel) {
@@ -168,14 +154,19 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
         const bool sub = op == OP_SUBDIVIDE;
         const uint32_t subMask = group_ballot<8>(sub, gbase);
         uint32_t base = 0;
-        if (c == 0 && subMask)
+        if (c == 0 && subMask) {
             base = atomicAdd(&ws.lvl[L + 1].count, (uint32_t)__builtin_popcount(subMask));
+            if (kMulti)
+                atomicAdd(&ws.edCount[e * kMaxLevels + L + 1], (uint32_t)__builtin_popcount(subMask));
+        }
         base = group_bcast<8>(base, 0);
         if (sub) {
             const uint32_t local = base + __builtin_popcount(subMask & ((1u << c) - 1u));
             const uint64_t g = (uint64_t)childStart + local;
             if (g < ws.capWork && (!leafChildren || local < ws.capLeaf)) {
                 ws.work[g] = make_uint4(cx, cy, cz, childHandle);
+                if (kMulti)
+                    ws.editOf[g] = e;
                 if (ed.kind == EK_COPY)
                     ws.state[g] = cs;
             } else {
@@ -203,9 +194,9 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
 }
 
 // One leaf per wave64, one lane per voxel (create_edit_svo.cu:320-426).
-__global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, const EditorDev* edp)
+template <bool kMulti>
+__global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, const EditorDev* eds)
 {
-    const EditorDev& ed = *edp;
     __shared__ uint32_t lds[4][16];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
@@ -216,6 +207,7 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
         const uint32_t w = me.start + i;
         if (w >= ws.capWork)
             break;
+        const EditorDev& ed = eds[kMulti ? ws.editOf[w] : 0u];
         const uint4 item = ws.work[w];
         bool hasOld = false;
         uint32_t oldMat = 0;
@@ -267,7 +259,10 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
 // Bottom-up, per level: clear the level's dedup table; map SVO children to
 // DAG handles and drop empty ones (updateChildPointers,
 // my_gpu_hash_dag_edits.cu:44-95); flag full items when the level below had
-// any (discover_fully_filled, 96-150, gated at 337).
+// any (discover_fully_filled, 96-150, gated at 337).  kMulti: the gate and
+// the full-item counts are per edit, and items of an edit that made no leaf
+// are emptied (that edit is dropped as a whole, :651).
+template <bool kMulti>
 __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint32_t L)
 {
     if (merge_skipped(ws, dag))
@@ -282,19 +277,25 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
         for (uint32_t i = tid; i < me.count; i += stride) {
             uint32_t m;
             if (leaf_is_full(ws.leaves + (size_t)i * ws.leafStride, dag.M, m))
-                atomicAdd(&ws.lvl[L].fullCount, 1u);
+                atomicAdd(kMulti ? &ws.edFull[ws.editOf[me.start + i] * kMaxLevels + L] : &ws.lvl[L].fullCount, 1u);
         }
         return;
     }
-    const bool gate = ws.lvl[L + 1].fullCount != 0;
+    const bool gateAll = ws.lvl[L + 1].fullCount != 0;
     const uint32_t childStart = ws.lvl[L + 1].start;
     const bool leafChildren = L + 1 == dag.leafLevel;
     for (uint32_t i = tid; i < me.count; i += stride) {
         uint32_t* rec = ws.nodes + (size_t)(me.start + i) * kNodeStride;
+        const uint32_t e = kMulti ? ws.editOf[me.start + i] : 0u;
+        const bool gate = kMulti ? ws.edFull[e * kMaxLevels + L + 1] != 0 : gateAll;
         const uint32_t hdr = rec[0];
         const uint32_t mask = hdr & 0xFFu;
         if (mask == 0)
             continue;
+        if (kMulti && ws.edCount[e * kMaxLevels + dag.leafLevel] == 0) {
+            rec[0] = 0u;
+            continue;
+        }
         const uint32_t imask = header_payload(hdr, dag.M) & 0xFFu;
         uint32_t out[8];
         uint32_t n = 0, newMask = 0, k = 0;
@@ -328,7 +329,7 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
         for (uint32_t j = 0; j < 8; ++j)
             rec[1 + j] = j < n ? out[j] : 0u;
         if (full)
-            atomicAdd(&ws.lvl[L].fullCount, 1u);
+            atomicAdd(kMulti ? &ws.edFull[e * kMaxLevels + L] : &ws.lvl[L].fullCount, 1u);
     }
 }
 
@@ -441,12 +442,14 @@ __global__ void edit_finish(WsDev ws, DagDev dag, uint32_t* root)
     EditStatus* st = ws.status;
     st->leafCount = ws.lvl[dag.leafLevel].count;
     if (st->overflow || st->leafCount == 0 || st->oom) {
+        st->done = st->overflow == 0 && st->oom == 0;
         st->applied = 0;
         st->rootAfter = st->rootBefore;
         return;
     }
     const uint32_t r = ws.rep[ws.lvl[0].start];
     const uint32_t newRoot = (r == kSentinel) ? kSentinel : ws.handle[r];
+    st->done = 1;
     st->applied = 1;
     st->rootAfter = newRoot;
     *root = newRoot;
@@ -587,7 +590,7 @@ static uint32_t bit_width(uint32_t v)
 static void ws_release(WsDev& ws)
 {
     for (void* p : { (void*)ws.work, (void*)ws.state, (void*)ws.nodes, (void*)ws.leaves, (void*)ws.rep,
-                     (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status })
+                     (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status, (void*)ws.editOf })
         if (p)
             (void)hipFree(p);
     ws = WsDev {};
@@ -616,6 +619,7 @@ static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
     SVDAG_HIP(hipMalloc(&ws.leaves, (size_t)capWork * ws.leafStride * 4));
     SVDAG_HIP(hipMalloc(&ws.rep, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.handle, (size_t)capWork * 4));
+    SVDAG_HIP(hipMalloc(&ws.editOf, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.dedup, (size_t)ws.dedupCap * 8));
     SVDAG_HIP(hipMalloc(&ws.lvl, kMaxLevels * sizeof(LevelDesc)));
     SVDAG_HIP(hipMalloc(&ws.status, sizeof(EditStatus)));
@@ -630,12 +634,9 @@ int ctx_refresh_tables(svdag_ctx* ctx)
     return SVDAG_OK;
 }
 
-// Voxel bounds [lo, hi] of an edit, used only to size launch grids.
-struct Bounds {
-    int64_t lo[3], hi[3];
-};
-
-static Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e)
+// Voxel bounds [lo, hi] of an edit: a 2-voxel margin around the editor's
+// box, so every node an editor can change intersects them.
+Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e)
 {
     const int64_t side = (int64_t)1 << ctx->dag.levels;
     Bounds b {};
@@ -649,7 +650,10 @@ static Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e)
             lo = e.bmin[a] - 2.0;
             hi = e.bmax[a] + 2.0;
         }
-        b.lo[a] = std::max<int64_t>(0, (int64_t)std::floor(lo));
+        // Clamp in double first: far-away editors must not overflow the cast.
+        lo = std::min(std::max(lo, -1.0), (double)side);
+        hi = std::min(std::max(hi, -1.0), (double)side);
+        b.lo[a] = std::min<int64_t>(side - 1, std::max<int64_t>(0, (int64_t)std::floor(lo)));
         b.hi[a] = std::min<int64_t>(side - 1, (int64_t)std::ceil(hi));
         if (b.hi[a] < b.lo[a])
             b.hi[a] = b.lo[a];
@@ -657,7 +661,7 @@ static Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e)
     return b;
 }
 
-static uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
+uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
 {
     const uint32_t sh = ctx->dag.levels - L;
     uint64_t n = 1;
@@ -666,10 +670,39 @@ static uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
     return n;
 }
 
+template <bool kMulti>
+static void launch_merge_level(svdag_ctx* ctx, const WsDev& ws, uint32_t L, uint64_t items)
+{
+    const DagDev& dag = ctx->dag;
+    const hipStream_t st = ctx->stream;
+    const uint32_t kCap = 8192;
+    merge_prepare<kMulti><<<grid_for(2 * items, 256, kCap), 256, 0, st>>>(ws, dag, L);
+    const uint32_t G = ctx->groupSize;
+    const uint32_t gridG = grid_for(items, 256 / G, kCap);
+    switch (G) {
+    case 4:
+        merge_dedup_find<4><<<gridG, 256, 0, st>>>(ws, dag, L);
+        merge_insert<4><<<gridG, 256, 0, st>>>(ws, dag, L);
+        break;
+    case 8:
+        merge_dedup_find<8><<<gridG, 256, 0, st>>>(ws, dag, L);
+        merge_insert<8><<<gridG, 256, 0, st>>>(ws, dag, L);
+        break;
+    case 32:
+        merge_dedup_find<32><<<gridG, 256, 0, st>>>(ws, dag, L);
+        merge_insert<32><<<gridG, 256, 0, st>>>(ws, dag, L);
+        break;
+    default:
+        merge_dedup_find<16><<<gridG, 256, 0, st>>>(ws, dag, L);
+        merge_insert<16><<<gridG, 256, 0, st>>>(ws, dag, L);
+        break;
+    }
+}
+
 // Enqueue one edit: ed is the host copy (for grid sizing), edDev the same
 // editor in device memory, ws the workspace view whose status / level table
 // slots this edit uses.
-static int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, const WsDev& ws)
+int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, const WsDev& ws)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
@@ -678,37 +711,31 @@ static int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edD
     edit_begin<<<1, 64, 0, st>>>(ws, ctx->dRoot, ed.kind == EK_COPY ? 1u : 0u);
     for (uint32_t L = 0; L < dag.leafLevel; ++L) {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, L), ws.capWork);
-        svo_inner_level<<<grid_for(items, 32, kCap), 256, 0, st>>>(ws, dag, edDev, L);
+        svo_inner_level<false><<<grid_for(items, 32, kCap), 256, 0, st>>>(ws, dag, edDev, L);
     }
     {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, dag.leafLevel), ws.capLeaf);
-        svo_leaf_level<<<grid_for(items, 4, kCap), 256, 0, st>>>(ws, dag, edDev);
+        svo_leaf_level<false><<<grid_for(items, 4, kCap), 256, 0, st>>>(ws, dag, edDev);
     }
-    const uint32_t G = ctx->groupSize;
-    for (int L = (int)dag.leafLevel; L >= 0; --L) {
-        const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork);
-        merge_prepare<<<grid_for(2 * items, 256, kCap), 256, 0, st>>>(ws, dag, (uint32_t)L);
-        const uint32_t gridG = grid_for(items, 256 / G, kCap);
-        switch (G) {
-        case 4:
-            merge_dedup_find<4><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            merge_insert<4><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            break;
-        case 8:
-            merge_dedup_find<8><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            merge_insert<8><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            break;
-        case 32:
-            merge_dedup_find<32><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            merge_insert<32><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            break;
-        default:
-            merge_dedup_find<16><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            merge_insert<16><<<gridG, 256, 0, st>>>(ws, dag, (uint32_t)L);
-            break;
-        }
-    }
+    for (int L = (int)dag.leafLevel; L >= 0; --L)
+        launch_merge_level<false>(ctx, ws, (uint32_t)L, std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork));
     edit_finish<<<1, 64, 0, st>>>(ws, dag, ctx->dRoot);
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems)
+{
+    const DagDev& dag = ctx->dag;
+    const hipStream_t st = ctx->stream;
+    const uint32_t kCap = 8192;
+    for (uint32_t L = S; L < dag.leafLevel; ++L)
+        svo_inner_level<true><<<grid_for(std::min<uint64_t>(levelItems[L], ws.capWork), 32, kCap), 256, 0, st>>>(
+            ws, dag, eds, L);
+    svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf), 4, kCap), 256, 0, st>>>(
+        ws, dag, eds);
+    for (int L = (int)dag.leafLevel; L >= (int)S; --L)
+        launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork));
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
@@ -718,11 +745,24 @@ static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
 {
     if (n <= ctx->batchCap)
         return SVDAG_OK;
-    for (void* p : { (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr })
+    for (void* p : { (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
+                     (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
+                     (void*)ctx->dWaveIdx })
         if (p)
             (void)hipFree(p);
-    if (ctx->hBatch)
-        (void)hipHostFree(ctx->hBatch);
+    for (void* p : { ctx->hBatch, ctx->hFused })
+        if (p)
+            (void)hipHostFree(p);
+    ctx->dEditors = nullptr;
+    ctx->hBatch = ctx->hFused = nullptr;
+    ctx->batchCap = 0;
+    SVDAG_HIP(hipMalloc(&ctx->dEdCount, (size_t)n * kMaxLevels * 4));
+    SVDAG_HIP(hipMalloc(&ctx->dEdFull, (size_t)n * kMaxLevels * 4));
+    SVDAG_HIP(hipMalloc(&ctx->dSeeds, (size_t)n * sizeof(SeedDesc)));
+    SVDAG_HIP(hipMalloc(&ctx->dSeedWork, (size_t)n * 8 * 4));
+    SVDAG_HIP(hipMalloc(&ctx->dWaves, (size_t)n * sizeof(uint2)));
+    SVDAG_HIP(hipMalloc(&ctx->dWaveIdx, 4));
+    SVDAG_HIP(hipHostMalloc(&ctx->hFused, (size_t)n * (sizeof(SeedDesc) + sizeof(uint2))));
     SVDAG_HIP(hipMalloc(&ctx->dEditors, (size_t)n * sizeof(EditorDev)));
     SVDAG_HIP(hipMalloc(&ctx->dStatusArr, (size_t)n * sizeof(EditStatus)));
     SVDAG_HIP(hipMalloc(&ctx->dLevelArr, (size_t)n * kMaxLevels * sizeof(LevelDesc)));
@@ -731,10 +771,13 @@ static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
     return SVDAG_OK;
 }
 
-// Run edits [0, n) in order with no host round trip between them: edit k's
-// first kernel reads the root edit k-1's last kernel wrote.  On overflow or
-// OOM at edit k the root is put back to edit k's input, the workspace or
-// arena grows, and edits k.. run again (later results are unreachable).
+// Run edits [0, n) in order with no host round trip between them.  In batch
+// mode 1 runs of geometric edits go through fused waves (fused.hip); other
+// edits (and all edits in mode 0) are enqueued one after another, each
+// reading the root the previous one wrote on the device.  On overflow or OOM
+// at edit k the root is put back to edit k's input, the workspace or arena
+// grows, and edits k.. run again (later results are unreachable).  An edit a
+// fused wave hands back runs on its own.
 int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats* stats, float* totalMs)
 {
     SVDAG_TRY(ensure_batch_slots(ctx, n));
@@ -743,7 +786,9 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
     LevelDesc* hLv = reinterpret_cast<LevelDesc*>(hSt + ctx->batchCap);
     float ms = 0.0f;
     uint32_t first = 0;
-    for (int attempt = 0; attempt < 8 && first < n; ++attempt) {
+    std::vector<uint8_t> alone(n, 0);
+    std::vector<BatchStep> steps;
+    for (int attempt = 0; attempt < 16 && first < n; ++attempt) {
         bool needState = false;
         for (uint32_t k = first; k < n; ++k) {
             hEd[k] = eds[k];
@@ -751,14 +796,37 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         }
         if (needState && !ctx->ws.state)
             SVDAG_TRY(ws_alloc(ctx, ctx->ws.capWork, true));
+        uint32_t S = 0;
+        if (ctx->batchMode == 1 && n - first > 1) {
+            S = plan_batch(ctx, eds, first, n, steps, alone);
+        } else {
+            steps.clear();
+            for (uint32_t k = first; k < n; ++k)
+                steps.push_back({ k, k + 1, false });
+        }
         SVDAG_HIP(hipMemcpyAsync(ctx->dEditors + first, hEd + first, (n - first) * sizeof(EditorDev), hipMemcpyHostToDevice,
                                  ctx->stream));
+        if (S) {
+            const SeedDesc* seeds = reinterpret_cast<const SeedDesc*>(ctx->hFused);
+            const uint2* waves = reinterpret_cast<const uint2*>(seeds + ctx->batchCap);
+            uint32_t nWaves = 0;
+            for (const BatchStep& b : steps)
+                nWaves += b.fused ? 1u : 0u;
+            SVDAG_HIP(hipMemcpyAsync(ctx->dSeeds + first, seeds + first, (n - first) * sizeof(SeedDesc),
+                                     hipMemcpyHostToDevice, ctx->stream));
+            SVDAG_HIP(hipMemcpyAsync(ctx->dWaves, waves, nWaves * sizeof(uint2), hipMemcpyHostToDevice, ctx->stream));
+            SVDAG_HIP(hipMemsetAsync(ctx->dWaveIdx, 0, 4, ctx->stream));
+        }
         SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        for (uint32_t k = first; k < n; ++k) {
+        for (const BatchStep& b : steps) {
+            if (b.fused) {
+                SVDAG_TRY(launch_wave(ctx, hEd, b, S));
+                continue;
+            }
             WsDev ws = ctx->ws;
-            ws.status = ctx->dStatusArr + k;
-            ws.lvl = ctx->dLevelArr + (size_t)k * kMaxLevels;
-            SVDAG_TRY(launch_edit(ctx, eds[k], ctx->dEditors + k, ws));
+            ws.status = ctx->dStatusArr + b.first;
+            ws.lvl = ctx->dLevelArr + (size_t)b.first * kMaxLevels;
+            SVDAG_TRY(launch_edit(ctx, eds[b.first], ctx->dEditors + b.first, ws));
         }
         SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         SVDAG_HIP(hipMemcpyAsync(hSt + first, ctx->dStatusArr + first, (n - first) * sizeof(EditStatus),
@@ -771,7 +839,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         ms += part;
         uint32_t bad = n;
         for (uint32_t k = first; k < n; ++k)
-            if (hSt[k].overflow || hSt[k].oom) {
+            if (!hSt[k].done) {
                 bad = k;
                 break;
             }
@@ -795,14 +863,17 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         const EditStatus s = hSt[bad];
         ctx->root = s.rootBefore;
         SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
-        if (s.overflow) {
+        if (s.fallback) {
+            alone[bad] = 1;
+        } else if (s.overflow) {
             uint64_t need = 0;
             for (uint32_t L = 0; L <= ctx->dag.leafLevel; ++L) {
                 const LevelDesc& d = hLv[(size_t)bad * kMaxLevels + L];
                 need = std::max<uint64_t>(need, (uint64_t)d.start + d.count);
             }
             SVDAG_TRY(ws_alloc(ctx, std::max<uint64_t>(2ull * ctx->ws.capWork, need + need / 4), ctx->ws.state != nullptr));
-        } else {
+        } else if (s.oom) {
+            FrameQuiesce q(ctx);
             for (auto& t : ctx->ts.tables) {
                 AllocState a {};
                 SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
@@ -810,6 +881,9 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
                     SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, 1ull << 25), ctx->stream));
             }
             SVDAG_TRY(ctx_refresh_tables(ctx));
+        } else {
+            set_error("edit " + std::to_string(bad) + " did not complete");
+            return SVDAG_E_DEVICE;
         }
         first = bad;
     }
@@ -962,16 +1036,21 @@ int svdag_destroy(svdag_ctx* ctx)
 {
     if (!ctx)
         return SVDAG_OK;
+    if (ctx->server)
+        (void)svdag_server_stop(ctx, nullptr);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->ts.release();
     ws_release(ctx->ws);
     for (void* p : { (void*)ctx->dFullyFilled, (void*)ctx->dRoot, (void*)ctx->hmin, (void*)ctx->hmax,
                      (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits,
-                     (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr })
+                     (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
+                     (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
+                     (void*)ctx->dWaveIdx, ctx->dLastFrame })
         if (p)
             (void)hipFree(p);
-    if (ctx->hBatch)
-        (void)hipHostFree(ctx->hBatch);
+    for (void* p : { ctx->hBatch, ctx->hFused })
+        if (p)
+            (void)hipHostFree(p);
     (void)hipHostFree(ctx->hStatus);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -1097,6 +1176,67 @@ int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, s
             ctx->undo.push_back(st[k].root_before);
         }
     }
+    return SVDAG_OK;
+}
+
+int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_result* out)
+{
+    if (!ctx || !out || n == 0 || !sample) {
+        set_error("svdag_autotune: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::vector<EditorDev> eds(n);
+    for (uint32_t k = 0; k < n; ++k)
+        SVDAG_TRY(make_editor(ctx, sample[k], eds[k]));
+    const uint32_t r0 = ctx->root;
+    const uint32_t g0 = ctx->groupSize;
+    std::vector<svdag_edit_stats> st(n);
+    auto restore = [&]() -> int {
+        ctx->root = r0;
+        SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
+        SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+        return SVDAG_OK;
+    };
+    float warm = 0.0f;
+    SVDAG_TRY(run_edits(ctx, eds.data(), n, st.data(), &warm));
+    SVDAG_TRY(restore());
+    const uint32_t cand[4] = { 4, 8, 16, 32 };
+    uint32_t best = g0;
+    float bestMs = 0.0f;
+    for (int c = 0; c < 4; ++c) {
+        ctx->groupSize = cand[c];
+        float m = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            float t = 0.0f;
+            const int rc = run_edits(ctx, eds.data(), n, st.data(), &t);
+            if (rc != SVDAG_OK) {
+                ctx->groupSize = g0;
+                (void)restore();
+                return rc;
+            }
+            SVDAG_TRY(restore());
+            m = rep == 0 ? t : std::min(m, t);
+        }
+        out->ms[c] = m;
+        if (c == 0 || m < bestMs) {
+            bestMs = m;
+            best = cand[c];
+        }
+    }
+    ctx->groupSize = best;
+    out->group_size = best;
+    return SVDAG_OK;
+}
+
+int svdag_set_batch_mode(svdag_ctx* ctx, uint32_t mode)
+{
+    if (!ctx || mode > SVDAG_BATCH_FUSED) {
+        set_error("svdag_set_batch_mode: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->batchMode = mode;
     return SVDAG_OK;
 }
 

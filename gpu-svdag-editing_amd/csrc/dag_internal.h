@@ -7,6 +7,7 @@
 
 namespace svdag {
 
+struct Server;
 constexpr uint32_t kNodeStride = 10; // SVO node record: header + 8 children + pad
 
 struct LevelDesc {
@@ -23,7 +24,8 @@ struct EditStatus {
     uint32_t rootBefore;
     uint32_t rootAfter;
     uint32_t leafCount;
-    uint32_t pad0, pad1;
+    uint32_t done;      // the edit ran to completion (applied or skipped)
+    uint32_t fallback;  // the fused pass could not take this edit: run it alone
 };
 
 // Edit workspace as seen by the kernels.
@@ -38,7 +40,50 @@ struct WsDev {
     LevelDesc* lvl;              // [kMaxLevels]
     EditStatus* status;
     uint32_t capWork, capLeaf, dedupCap, leafStride;
+    // Fused multi-edit pass only (null otherwise): batch-relative edit of
+    // each work item, and per-(edit, level) item and full-item counters.
+    uint32_t* editOf;            // [capWork]
+    uint32_t* edCount;           // [batch][kMaxLevels]
+    uint32_t* edFull;            // [batch][kMaxLevels]
 };
+
+// Level-S cell range of an edit in a fused wave: cells lo .. lo + n - 1 per
+// axis, n in {1, 2}.
+struct SeedDesc {
+    uint32_t lo[3];
+    uint32_t n[3];
+};
+
+// Fused-wave state: the waves of a batch are launched back to back and each
+// wave reads its edit range from waves[*waveIdx]; the last kernel of a wave
+// advances waveIdx.
+struct FusedDev {
+    const uint2* waves;     // (first edit, end edit) per wave
+    uint32_t* waveIdx;
+    const SeedDesc* seeds;  // [batch]
+    uint32_t* seedWork;     // [batch][8]: level-S work index of a cell, or kSentinel
+    uint32_t S;             // split level: the fused part covers levels S .. leaf
+};
+
+// A leaf is fully filled iff all 64 voxels are set with one material
+// (my_gpu_hash_dag_factory.cpp:357-360).
+__device__ inline bool leaf_is_full(const uint32_t* leaf, uint32_t M, uint32_t& mat)
+{
+    if (leaf[0] != 0xFFFFFFFFu || leaf[1] != 0xFFFFFFFFu)
+        return false;
+    if (M == 0) {
+        mat = 0;
+        return true;
+    }
+    const uint32_t m = leaf[2] & ((1u << M) - 1u);
+    const uint32_t rep = (M == 4) ? m * 0x11111111u : m * 0x01010101u;
+    const uint32_t words = (64u * M) / 32u;
+    for (uint32_t i = 0; i < words; ++i)
+        if (leaf[2 + i] != rep)
+            return false;
+    mat = m;
+    return true;
+}
 
 } // namespace svdag
 
@@ -72,18 +117,71 @@ struct svdag_ctx {
     svdag::LevelDesc* dLevelArr = nullptr;
     void* hBatch = nullptr;
     uint32_t batchCap = 0;
+    // Fused-pass slots (sized with the batch slots).
+    uint32_t* dEdCount = nullptr;
+    uint32_t* dEdFull = nullptr;
+    svdag::SeedDesc* dSeeds = nullptr;
+    uint32_t* dSeedWork = nullptr;
+    uint2* dWaves = nullptr;
+    uint32_t* dWaveIdx = nullptr;
+    void* hFused = nullptr;     // pinned: SeedDesc[batchCap] then uint2[batchCap]
+    uint32_t batchMode = 1;     // 0: edits one after another, 1: fused waves
     // Tracer output buffer.
     void* dHits = nullptr;
     size_t hitCap = 0;
     std::mutex mu;
+    // Serving (serve.hip).  While a server runs, frame traces read the tables
+    // on frameStream; whoever reallocates or sweeps tables first takes
+    // frameMu and drains that stream (FrameQuiesce).
+    svdag::Server* server = nullptr;
+    std::mutex frameMu;
+    hipStream_t frameStream = nullptr;
+    // The last frame of a stopped server.
+    void* dLastFrame = nullptr;
+    uint32_t lastFrameW = 0, lastFrameH = 0, lastFrameRoot = svdag::kSentinel;
 };
 
 namespace svdag {
+// Holds frame traces off the tables while they are reallocated or swept.
+struct FrameQuiesce {
+    std::unique_lock<std::mutex> lock;
+    explicit FrameQuiesce(svdag_ctx* ctx) : lock(ctx->frameMu)
+    {
+        if (ctx->frameStream)
+            (void)hipStreamSynchronize(ctx->frameStream);
+    }
+};
 int ctx_refresh_tables(svdag_ctx* ctx);
+int gc_run(svdag_ctx* ctx, svdag_gc_stats* st);
 int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
                  svdag_hit* outDev);
 // Run edits [0, n) in order on ctx->stream (one sync at the end), with the
 // overflow / OOM retry; used by svdag_edit, svdag_edit_batch and the server.
 int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats* stats, float* totalMs);
 int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e);
+
+// Voxel bounds [lo, hi] (inclusive) outside which an edit changes nothing.
+struct Bounds {
+    int64_t lo[3], hi[3];
+};
+Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e);
+uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L);
+
+// One edit's kernels on ctx->stream (dag.hip).
+int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, const WsDev& ws);
+// Levels S .. leaf of a fused wave: SVO build and merge with per-item editors
+// (dag.hip).  levelItems[L] bounds the items of level L (grid sizing only).
+int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems);
+
+// Fused multi-edit waves (fused.hip).
+struct BatchStep {
+    uint32_t first, end;  // edits [first, end)
+    bool fused;           // a fused wave (else a single edit on its own)
+};
+// Splits edits [first, n) into steps; fills the seeds / waves host mirrors
+// and returns the split level S (0 when nothing can be fused).
+// Edits flagged in `alone` run on their own.
+uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32_t n, std::vector<BatchStep>& steps,
+                    const std::vector<uint8_t>& alone);
+int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S);
 }

@@ -1,0 +1,487 @@
+// Fused multi-edit waves: many pending edits, one SVO build and one merge.
+//
+// Semantics are those of applying the edits one after another
+// (editMyHashDag, my_gpu_hash_dag_edits.cu:633-671, per edit); the fused form
+// changes only how the work is scheduled.
+//
+// Split level S.  Every edit of a batch spans at most two level-S cells per
+// axis (S is chosen from the largest edit).  An edit's SVO then has at most
+// eight items on every level above S (its "spine"); below S it owns the
+// subtrees of its level-S cells.
+//
+// Waves.  A wave is a run of consecutive edits whose level-S cells are
+// disjoint from the voxel bounds of every earlier edit of the wave.  For a
+// geometric editor (sphere / box add and remove, sphere paint) a node is
+// changed only if it intersects the editor's bounds, and its decision at a
+// node depends only on the node and the editor; so the subtrees below S that
+// edit k builds and merges are the same whether the earlier edits of the wave
+// have run or not.  One wave is therefore
+//   fused_seed   for each edit and level-S cell: walk from the wave's input
+//                root to the cell and decide it as the parent would
+//                (svo_inner_level); subdivided cells become level-S work
+//                items tagged with their edit;
+//   joint levels the SVO build and merge kernels over levels S .. leaf for all
+//                edits of the wave at once, with the fully-filled gate and
+//                the no-leaf rule kept per edit;
+//   fused_spine  one wave64 replays the edits in order on levels 0 .. S-1:
+//                top-down SVO decisions against the current root, then the
+//                bottom-up merge (payload cleared, gated full discovery,
+//                in-level dedup, DAG find / insert), and publishes each
+//                edit's root.
+// Copy edits read the live DAG outside their own region and run on their
+// own, as does an edit the spine cannot take (more than eight items on a
+// level, which the choice of S rules out).
+#include "dag_internal.h"
+#include <algorithm>
+#include <cstring>
+
+namespace svdag {
+
+constexpr uint32_t kSpineItems = 8;
+constexpr uint32_t kMaxWave = 256;
+
+// Reset the wave's joint level table and status and its edits' counters.
+__global__ void __launch_bounds__(256) wave_begin(WsDev ws, FusedDev f, EditStatus* status)
+{
+    const uint2 wv = f.waves[*f.waveIdx];
+    const uint32_t t = threadIdx.x;
+    if (t < kMaxLevels)
+        ws.lvl[t] = LevelDesc {};
+    if (t == 0)
+        *ws.status = EditStatus {};
+    for (uint32_t k = wv.x; k < wv.y; ++k) {
+        if (t < kMaxLevels) {
+            ws.edCount[k * kMaxLevels + t] = 0u;
+            ws.edFull[k * kMaxLevels + t] = 0u;
+        }
+        if (t < 8)
+            f.seedWork[k * 8 + t] = kSentinel;
+        if (t == 0)
+            status[k] = EditStatus {};
+    }
+}
+
+// One lane per (edit, level-S cell slot).
+__global__ void __launch_bounds__(256) fused_seed(WsDev ws, DagDev dag, const EditorDev* eds, FusedDev f,
+                                                  const uint32_t* rootPtr)
+{
+    const uint2 wv = f.waves[*f.waveIdx];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = wv.x + t / 8u, j = t & 7u;
+    if (k >= wv.y)
+        return;
+    const SeedDesc sd = f.seeds[k];
+    const uint32_t d[3] = { (j >> 2) & 1u, (j >> 1) & 1u, j & 1u };
+    if (d[0] >= sd.n[0] || d[1] >= sd.n[1] || d[2] >= sd.n[2])
+        return;
+    const uint32_t S = f.S;
+    const uint32_t cx = sd.lo[0] + d[0], cy = sd.lo[1] + d[1], cz = sd.lo[2] + d[2];
+    uint32_t h = *rootPtr;
+    for (uint32_t l = 0; l < S && h != kSentinel; ++l) {
+        const uint32_t sh = S - 1u - l;
+        const uint32_t c = (((cx >> sh) & 1u) << 2) | (((cy >> sh) & 1u) << 1) | ((cz >> sh) & 1u);
+        const uint32_t* node = dag_node_ptr(dag, h);
+        const uint32_t mask = node[0] & 0xFFu;
+        h = ((mask >> c) & 1u) ? node[1 + __builtin_popcount(mask & ((1u << c) - 1u))] : kSentinel;
+    }
+    NState st = NS_PARTIAL;
+    uint32_t stMat = 0;
+    if (h == kSentinel)
+        st = NS_EMPTY;
+    else if (S != dag.leafLevel)
+        st = header_full(dag_node_ptr(dag, h)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
+    uint32_t fillMat = 0;
+    if (decide(eds[k], cx, cy, cz, dag.levels - S, st, fillMat) != OP_SUBDIVIDE)
+        return;
+    const uint32_t g = atomicAdd(&ws.lvl[S].count, 1u);
+    atomicAdd(&ws.edCount[k * kMaxLevels + S], 1u);
+    if (g < ws.capWork && (S != dag.leafLevel || g < ws.capLeaf)) {
+        ws.work[g] = make_uint4(cx, cy, cz, h);
+        ws.editOf[g] = k;
+        f.seedWork[k * 8 + j] = g;
+    } else {
+        atomicOr(&ws.status->overflow, 1u);
+    }
+}
+
+// One wave64 (the whole block) replays the wave's edits on levels 0 .. S-1.
+__global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const EditorDev* eds, FusedDev f,
+                                                  uint32_t* rootPtr, EditStatus* status, LevelDesc* levels)
+{
+    __shared__ uint4 sItem[kMaxLevels][kSpineItems];
+    __shared__ uint32_t sRec[kMaxLevels][kSpineItems][kNodeStride];
+    __shared__ uint32_t sHandle[kMaxLevels][kSpineItems];
+    __shared__ uint32_t sCnt[kMaxLevels];
+    const uint32_t lane = threadIdx.x;
+    const uint2 wv = f.waves[*f.waveIdx];
+    const uint32_t S = f.S;
+    const uint32_t leafLevel = dag.leafLevel;
+    uint32_t root = *rootPtr;
+    const EditStatus joint = *ws.status;
+    if (joint.overflow || joint.oom) {
+        // Nothing of the wave is applied; the host grows what ran out and
+        // runs the edits again from the wave's first one.
+        if (lane == 0) {
+            EditStatus st {};
+            st.overflow = joint.overflow;
+            st.oom = joint.oom;
+            st.rootBefore = st.rootAfter = root;
+            status[wv.x] = st;
+        }
+        if (lane < kMaxLevels)
+            levels[wv.x * kMaxLevels + lane] = ws.lvl[lane];
+        __syncthreads();
+        if (lane == 0)
+            *f.waveIdx += 1u;
+        return;
+    }
+    const uint32_t i8 = lane >> 3, c8 = lane & 7u, gbase = lane & ~7u;
+    for (uint32_t k = wv.x; k < wv.y; ++k) {
+        const EditorDev& ed = eds[k];
+        const SeedDesc sd = f.seeds[k];
+        // ---- top-down: the edit's SVO on levels 0 .. S-1 (svo_inner_level).
+        if (lane == 0) {
+            sItem[0][0] = make_uint4(0u, 0u, 0u, root);
+            sCnt[0] = 1u;
+        }
+        __syncthreads();
+        bool bad = false;
+        for (uint32_t l = 0; l < S; ++l) {
+            const uint32_t n = sCnt[l];
+            const bool active = i8 < n;
+            const uint32_t childLevel = l + 1u;
+            const bool leafChildren = childLevel == leafLevel;
+            const uint4 item = active ? sItem[l][i8] : make_uint4(0u, 0u, 0u, kSentinel);
+            uint32_t childHandle = kSentinel;
+            if (item.w != kSentinel) {
+                const uint32_t* node = dag_node_ptr(dag, item.w);
+                const uint32_t mask = node[0] & 0xFFu;
+                if ((mask >> c8) & 1u)
+                    childHandle = node[1 + __builtin_popcount(mask & ((1u << c8) - 1u))];
+            }
+            const uint32_t cx = (item.x << 1) | ((c8 >> 2) & 1u);
+            const uint32_t cy = (item.y << 1) | ((c8 >> 1) & 1u);
+            const uint32_t cz = (item.z << 1) | (c8 & 1u);
+            NState st = NS_PARTIAL;
+            uint32_t stMat = 0;
+            if (childHandle == kSentinel)
+                st = NS_EMPTY;
+            else if (!leafChildren)
+                st = header_full(dag_node_ptr(dag, childHandle)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
+            uint32_t fillMat = 0;
+            const Op op = active ? decide(ed, cx, cy, cz, dag.levels - childLevel, st, fillMat) : OP_KEEP;
+            const bool sub = active && op == OP_SUBDIVIDE;
+            const unsigned long long subBal = __ballot(sub);
+            const uint32_t gsub = (uint32_t)(subBal >> gbase) & 0xFFu;
+            uint32_t ref = childHandle;
+            if (sub) {
+                if (childLevel < S) {
+                    const uint32_t idx = (uint32_t)__builtin_popcountll(subBal & ((1ull << lane) - 1ull));
+                    if (idx < kSpineItems)
+                        sItem[childLevel][idx] = make_uint4(cx, cy, cz, childHandle);
+                    ref = idx;
+                } else {
+                    const uint32_t dx = cx - sd.lo[0], dy = cy - sd.lo[1], dz = cz - sd.lo[2];
+                    const bool inRange = dx < sd.n[0] && dy < sd.n[1] && dz < sd.n[2];
+                    ref = inRange ? f.seedWork[k * 8 + ((dx << 2) | (dy << 1) | dz)] : kSentinel;
+                    bad |= ref == kSentinel;
+                }
+            } else if (op == OP_EMPTY) {
+                ref = kSentinel;
+            } else if (op == OP_FILL) {
+                ref = dag.fullyFilled[childLevel * dag.numMaterials + fillMat];
+            }
+            const bool out = active && ref != kSentinel;
+            const uint32_t cmask = (uint32_t)(__ballot(out) >> gbase) & 0xFFu;
+            if (active) {
+                uint32_t* rec = sRec[l][i8];
+                const uint32_t nOut = __builtin_popcount(cmask);
+                if (out)
+                    rec[1 + __builtin_popcount(cmask & ((1u << c8) - 1u))] = ref;
+                if (c8 >= nOut)
+                    rec[1 + c8] = 0u;
+                if (c8 == 0) {
+                    rec[0] = cmask ? header_make(cmask, gsub, dag.M) : 0u;
+                    rec[9] = 0u;
+                }
+            }
+            const uint32_t total = (uint32_t)__builtin_popcountll(subBal);
+            if (childLevel < S && total > kSpineItems)
+                bad = true;
+            if (lane == 0 && childLevel < S)
+                sCnt[childLevel] = min(total, kSpineItems);
+            __syncthreads();
+        }
+        if (__ballot(bad) != 0) {
+            // Hand the edit back to the host, which runs it on its own.
+            if (lane == 0) {
+                EditStatus st {};
+                st.fallback = 1u;
+                st.rootBefore = st.rootAfter = root;
+                status[k] = st;
+            }
+            break;
+        }
+        if (lane < kMaxLevels) {
+            LevelDesc d {};
+            d.count = lane < S ? sCnt[lane] : ws.edCount[k * kMaxLevels + lane];
+            levels[k * kMaxLevels + lane] = d;
+        }
+        const uint32_t leafCount = ws.edCount[k * kMaxLevels + leafLevel];
+        if (leafCount == 0) {
+            // No leaf: the edit is dropped (my_gpu_hash_dag_edits.cu:651).
+            if (lane == 0) {
+                EditStatus st {};
+                st.done = 1u;
+                st.rootBefore = st.rootAfter = root;
+                status[k] = st;
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- bottom-up merge of levels S-1 .. 0.
+        uint32_t prevFull = ws.edFull[k * kMaxLevels + S];
+        bool oom = false;
+        for (int l = (int)S - 1; l >= 0; --l) {
+            const uint32_t n = sCnt[l];
+            const bool childSeed = (uint32_t)l + 1u == S;
+            const bool leafChildren = (uint32_t)l + 1u == leafLevel;
+            // updateChildPointers + gated discover_fully_filled: one lane per item.
+            bool full = false;
+            if (lane < n) {
+                uint32_t* rec = sRec[l][lane];
+                const uint32_t hdr = rec[0];
+                const uint32_t mask = hdr & 0xFFu;
+                if (mask) {
+                    const uint32_t imask = header_payload(hdr, dag.M) & 0xFFu;
+                    uint32_t outH[8];
+                    uint32_t nOut = 0, newMask = 0, q = 0;
+                    for (uint32_t c = 0; c < 8; ++c) {
+                        if (!((mask >> c) & 1u))
+                            continue;
+                        uint32_t h = rec[1 + q++];
+                        if ((imask >> c) & 1u) {
+                            if (childSeed) {
+                                const uint32_t r = ws.rep[h];
+                                h = (r == kSentinel) ? kSentinel : ws.handle[r];
+                            } else {
+                                h = sHandle[l + 1][h];
+                            }
+                        }
+                        if (h != kSentinel) {
+                            outH[nOut++] = h;
+                            newMask |= 1u << c;
+                        }
+                    }
+                    uint32_t fm = 0;
+                    if (prevFull != 0 && newMask == 0xFFu) {
+                        full = true;
+                        for (uint32_t j = 0; j < 8 && full; ++j) {
+                            uint32_t m = 0;
+                            const bool fj = leafChildren ? leaf_is_full(dag_leaf_ptr(dag, outH[j]), dag.M, m)
+                                                         : header_full(dag_node_ptr(dag, outH[j])[0], dag.M, m);
+                            if (j == 0)
+                                fm = m;
+                            full = fj && m == fm;
+                        }
+                    }
+                    rec[0] = full ? header_make_full(0xFFu, fm, dag.M) : header_make(newMask, 0u, dag.M);
+                    for (uint32_t j = 0; j < 8; ++j)
+                        rec[1 + j] = j < nOut ? outH[j] : 0u;
+                }
+            }
+            prevFull = (uint32_t)__builtin_popcountll(__ballot(full));
+            __syncthreads();
+            // In-level dedup, then DAG find / insert: eight lanes per item.
+            bool empty = true;
+            uint32_t repIdx = i8;
+            if (i8 < n) {
+                const uint32_t* rec = sRec[l][i8];
+                empty = (rec[0] & 0xFFu) == 0u;
+                for (uint32_t j = 0; j < i8 && !empty; ++j) {
+                    bool eq = true;
+                    for (uint32_t w = 0; w < kNodeStride && eq; ++w)
+                        eq = sRec[l][j][w] == rec[w];
+                    if (eq) {
+                        repIdx = j;
+                        break;
+                    }
+                }
+            }
+            uint32_t handle = kSentinel;
+            if (i8 < n && !empty && repIdx == i8) {
+                const uint32_t* rec = sRec[l][i8];
+                const uint32_t s = node_size_from_header(rec[0]);
+                Needle<8> nd;
+                needle_load(nd, rec, s, c8);
+                const uint32_t h = needle_hash(nd, s, c8);
+                uint32_t r = table_find_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
+                if (r == kNotFound)
+                    r = table_insert_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
+                if (r == kNotFound)
+                    oom = true;
+                else
+                    handle = node_handle(r, s);
+            }
+            if (c8 == 0 && i8 < n)
+                sHandle[l][i8] = handle;
+            __syncthreads();
+            if (c8 == 0 && i8 < n && !empty && repIdx != i8)
+                sHandle[l][i8] = sHandle[l][repIdx];
+            __syncthreads();
+            if (__ballot(oom) != 0)
+                break;
+        }
+        if (__ballot(oom) != 0) {
+            if (lane == 0) {
+                EditStatus st {};
+                st.oom = 1u;
+                st.rootBefore = st.rootAfter = root;
+                status[k] = st;
+            }
+            break;
+        }
+        const uint32_t newRoot = sHandle[0][0];
+        if (lane == 0) {
+            EditStatus st {};
+            st.done = 1u;
+            st.applied = 1u;
+            st.rootBefore = root;
+            st.rootAfter = newRoot;
+            st.leafCount = leafCount;
+            status[k] = st;
+        }
+        root = newRoot;
+        __syncthreads();
+    }
+    __syncthreads();
+    if (lane == 0) {
+        *rootPtr = root;
+        *f.waveIdx += 1u;
+    }
+}
+
+// ======================================================================== host
+
+static uint32_t ceil_log2(uint64_t v)
+{
+    uint32_t p = 0;
+    while ((1ull << p) < v)
+        ++p;
+    return p;
+}
+
+// Edits whose bounds span more than a quarter of the volume on an axis run
+// on their own, so that one huge edit does not pull S up for the batch.
+constexpr int kMinSplit = 2;
+
+static uint64_t max_span(const Bounds& b)
+{
+    uint64_t m = 0;
+    for (int a = 0; a < 3; ++a)
+        m = std::max<uint64_t>(m, (uint64_t)(b.hi[a] - b.lo[a] + 1));
+    return m;
+}
+
+static bool fusable(const svdag_ctx* ctx, const EditorDev& e)
+{
+    if (!(e.kind == EK_SPHERE_ADD || e.kind == EK_SPHERE_REMOVE || e.kind == EK_BOX_ADD || e.kind == EK_BOX_REMOVE
+          || e.kind == EK_PAINT))
+        return false;
+    return (int)ctx->dag.levels - (int)ceil_log2(max_span(edit_bounds(ctx, e))) >= kMinSplit;
+}
+
+uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32_t n, std::vector<BatchStep>& steps,
+                    const std::vector<uint8_t>& alone)
+{
+    steps.clear();
+    const uint32_t D = ctx->dag.levels;
+    uint64_t maxSpan = 0;
+    for (uint32_t k = first; k < n; ++k)
+        if (fusable(ctx, eds[k]) && !alone[k])
+            maxSpan = std::max<uint64_t>(maxSpan, max_span(edit_bounds(ctx, eds[k])));
+    int S = maxSpan ? (int)D - (int)ceil_log2(maxSpan) : 0;
+    S = std::min<int>(S, (int)ctx->dag.leafLevel);
+    if (S < 1) {
+        for (uint32_t k = first; k < n; ++k)
+            steps.push_back({ k, k + 1, false });
+        return 0;
+    }
+    SeedDesc* seeds = reinterpret_cast<SeedDesc*>(ctx->hFused);
+    uint2* waves = reinterpret_cast<uint2*>(seeds + ctx->batchCap);
+    uint32_t nWaves = 0;
+    const uint32_t sh = D - (uint32_t)S;
+    std::vector<Bounds> members;
+    uint32_t waveStart = first;
+    bool open = false;
+    auto close = [&](uint32_t end) {
+        if (!open)
+            return;
+        steps.push_back({ waveStart, end, true });
+        waves[nWaves++] = make_uint2(waveStart, end);
+        members.clear();
+        open = false;
+    };
+    for (uint32_t k = first; k < n; ++k) {
+        if (!fusable(ctx, eds[k]) || alone[k]) {
+            close(k);
+            steps.push_back({ k, k + 1, false });
+            continue;
+        }
+        const Bounds b = edit_bounds(ctx, eds[k]);
+        SeedDesc sd {};
+        int64_t cellLo[3], cellHi[3];
+        for (int a = 0; a < 3; ++a) {
+            const int64_t lo = b.lo[a] >> sh, hi = b.hi[a] >> sh;
+            sd.lo[a] = (uint32_t)lo;
+            sd.n[a] = (uint32_t)(hi - lo + 1);
+            cellLo[a] = lo << sh;
+            cellHi[a] = ((hi + 1) << sh) - 1;
+        }
+        seeds[k] = sd;
+        bool conflict = false;
+        for (const Bounds& m : members) {
+            bool overlap = true;
+            for (int a = 0; a < 3; ++a)
+                overlap &= cellLo[a] <= m.hi[a] && m.lo[a] <= cellHi[a];
+            if (overlap) {
+                conflict = true;
+                break;
+            }
+        }
+        if (conflict || members.size() >= kMaxWave)
+            close(k);
+        if (!open) {
+            open = true;
+            waveStart = k;
+        }
+        members.push_back(b);
+    }
+    close(n);
+    return (uint32_t)S;
+}
+
+int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S)
+{
+    const DagDev& dag = ctx->dag;
+    const hipStream_t st = ctx->stream;
+    WsDev ws = ctx->ws;
+    ws.edCount = ctx->dEdCount;
+    ws.edFull = ctx->dEdFull;
+    const FusedDev f { ctx->dWaves, ctx->dWaveIdx, ctx->dSeeds, ctx->dSeedWork, S };
+    wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
+    const uint32_t lanes = 8u * (step.end - step.first);
+    fused_seed<<<(lanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
+    uint64_t levelItems[kMaxLevels] {};
+    for (uint32_t k = step.first; k < step.end; ++k) {
+        const Bounds b = edit_bounds(ctx, hEds[k]);
+        for (uint32_t L = S; L <= dag.leafLevel; ++L)
+            levelItems[L] += level_bound(ctx, b, L);
+    }
+    SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems));
+    fused_spine<<<1, 64, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+} // namespace svdag

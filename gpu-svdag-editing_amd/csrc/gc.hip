@@ -131,7 +131,7 @@ __global__ void gc_sweep(TableDev t, GcCounters* ctr)
     }
 }
 
-static int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
+int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
 {
     const hipStream_t st = ctx->stream;
     const DagDev& dag = ctx->dag;
@@ -217,5 +217,6 @@ extern "C" int svdag_gc(svdag_ctx* ctx, svdag_gc_stats* stats)
     ctx->undo.clear();
     ctx->redo.clear();
     svdag_gc_stats local {};
+    FrameQuiesce q(ctx);
     return gc_run(ctx, stats ? stats : &local);
 }

@@ -77,6 +77,8 @@ struct AllocState {
     unsigned long long freeHead; // tag << 32 | id; id == kNoSlab when empty
     uint32_t fresh;              // next never-used slab id (starts at numBuckets)
     uint32_t oom;                // set when an allocation fails
+    uint32_t freeCount;          // slabs on the free stack (for admission control)
+    uint32_t pad;
 };
 
 struct TableDev {
@@ -115,8 +117,10 @@ __device__ inline uint32_t slab_alloc(const TableDev& t)
         const uint32_t nxt = ld_relaxed(&t.freeList[id]);
         const unsigned long long nv = (((old >> 32) + 1ull) << 32) | nxt;
         const unsigned long long r = atomicCAS(&a->freeHead, old, nv);
-        if (r == old)
+        if (r == old) {
+            atomicSub(&a->freeCount, 1u);
             return id;
+        }
         old = r;
     }
     const uint32_t id = atomicAdd(&a->fresh, 1u);
@@ -137,8 +141,10 @@ __device__ inline void slab_free(const TableDev& t, uint32_t id)
         __threadfence();
         const unsigned long long nv = (((old >> 32) + 1ull) << 32) | id;
         const unsigned long long r = atomicCAS(&a->freeHead, old, nv);
-        if (r == old)
+        if (r == old) {
+            atomicAdd(&a->freeCount, 1u);
             return;
+        }
         old = r;
     }
 }

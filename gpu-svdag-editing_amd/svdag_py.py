@@ -65,6 +65,32 @@ class Camera(ctypes.Structure):
                 ("bounds_max", ctypes.c_double * 3)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("group_size", ctypes.c_uint32), ("ms", ctypes.c_float * 4)]
+
+
+class Memory(ctypes.Structure):
+    _fields_ = [("slab_capacity", ctypes.c_uint64), ("slabs_used", ctypes.c_uint64), ("slabs_free", ctypes.c_uint64),
+                ("bytes_reserved", ctypes.c_uint64), ("min_free_fraction", ctypes.c_double)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("frame_width", ctypes.c_uint32), ("frame_height", ctypes.c_uint32),
+                ("frame_period_us", ctypes.c_uint32), ("cameras", ctypes.POINTER(Camera)),
+                ("num_cameras", ctypes.c_uint32), ("allow_gc", ctypes.c_uint32), ("reserve_fraction", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("frames", ctypes.c_uint64),
+                ("frames_skipped", ctypes.c_uint64), ("gcs", ctypes.c_uint64), ("grows", ctypes.c_uint64),
+                ("deferred", ctypes.c_uint64), ("wall_s", ctypes.c_double), ("edit_busy_s", ctypes.c_double),
+                ("trace_ms_avg", ctypes.c_double), ("last_frame_root", ctypes.c_uint32),
+                ("max_batch_seen", ctypes.c_uint32)]
+
+
+REQUEST_DTYPE = np.dtype([("status", "<i4"), ("applied", "<u4"), ("root_after", "<u4"), ("batch", "<u4"),
+                          ("submit_s", "<f8"), ("admit_s", "<f8"), ("done_s", "<f8")])
+
 HIT_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("material", "<u4"),
                       ("t", "<f4"), ("px", "<f4"), ("py", "<f4"), ("pz", "<f4")])
 
@@ -75,8 +101,10 @@ EXPORTED = [
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_terrain_heights", "svdag_root",
-    "svdag_edit", "svdag_edit_batch", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
-    "svdag_trace_primary", "svdag_export_canonical", "svdag_export_tables",
+    "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
+    "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
+    "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
+    "svdag_export_tables",
 ]
 
 _lib = None
@@ -250,20 +278,40 @@ class MyGPUHashDAG:
         check(lib().svdag_edit(self.h, ctypes.byref(d), ctypes.byref(st)))
         return st
 
-    def edit_batch(self, edits):
-        """edits: list of (kind, material, center, radius[, src, dst]); returns (stats list, gpu ms)."""
-        n = len(edits)
-        descs = (EditDesc * n)()
+    @staticmethod
+    def _descs(edits):
+        descs = (EditDesc * len(edits))()
         for i, e in enumerate(edits):
             kind, mat, c, r = e[:4]
             src = e[4] if len(e) > 4 else (0, 0, 0)
             dst = e[5] if len(e) > 5 else (0, 0, 0)
             descs[i] = EditDesc(kind, mat, (ctypes.c_float * 3)(*c), float(r), (ctypes.c_float * 3)(*src),
                                 (ctypes.c_float * 3)(*dst))
+        return descs
+
+    @staticmethod
+    def _camera(position, rotation, fov, bmin, bmax):
+        return Camera((ctypes.c_double * 3)(*position), (ctypes.c_double * 9)(*rotation), float(fov),
+                      (ctypes.c_double * 3)(*bmin), (ctypes.c_double * 3)(*bmax))
+
+    def edit_batch(self, edits):
+        """edits: list of (kind, material, center, radius[, src, dst]); returns (stats list, gpu ms)."""
+        n = len(edits)
+        descs = self._descs(edits)
         stats = (EditStats * n)()
         ms = ctypes.c_float(0)
         check(lib().svdag_edit_batch(self.h, descs, n, stats, ctypes.byref(ms)))
         return list(stats), ms.value
+
+    def autotune(self, sample):
+        """Pick lanes per item for the merge kernels from a sample batch; returns (G, {G: ms})."""
+        r = TuneResult()
+        check(lib().svdag_autotune(self.h, self._descs(sample), len(sample), ctypes.byref(r)))
+        return r.group_size, dict(zip((4, 8, 16, 32), list(r.ms)))
+
+    def set_batch_mode(self, mode):
+        """0: edits of a batch one after another; 1: fused waves (default)."""
+        check(lib().svdag_set_batch_mode(self.h, int(mode)))
 
     def undo(self):
         check(lib().svdag_undo(self.h))
@@ -287,9 +335,42 @@ class MyGPUHashDAG:
         check(lib().svdag_query_voxels(self.h, xyz.ctypes.data_as(u32p), len(xyz), out.ctypes.data_as(u32p)))
         return out
 
+    def memory_usage(self):
+        m = Memory()
+        check(lib().svdag_memory_usage(self.h, ctypes.byref(m)))
+        return m
+
+    def serve_replay(self, edits, arrivals, max_batch=0, frame=(0, 0), cameras=(), frame_period_us=0,
+                     allow_gc=True, reserve_fraction=0.0):
+        """Open-loop serving run: edits[i] arrives at arrivals[i] seconds.
+        cameras: list of (position, rotation, fov, bmin, bmax) cycled per frame.
+        Returns (REQUEST_DTYPE array, ServerStats)."""
+        n = len(edits)
+        descs = self._descs(edits)
+        cams = (Camera * max(len(cameras), 1))()
+        for i, c in enumerate(cameras):
+            cams[i] = self._camera(*c)
+        cfg = ServerConfig(int(max_batch), int(frame[0]), int(frame[1]), int(frame_period_us),
+                           ctypes.cast(cams, ctypes.POINTER(Camera)), len(cameras), int(bool(allow_gc)),
+                           float(reserve_fraction))
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        assert arr.shape == (n,)
+        out = np.zeros(n, REQUEST_DTYPE)
+        st = ServerStats()
+        check(lib().svdag_serve_replay(self.h, ctypes.byref(cfg), descs,
+                                       arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                       out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
+        return out, st
+
+    def last_frame(self, w, h):
+        """Hits and root of the last frame of the last serving run."""
+        out = np.empty(w * h, HIT_DTYPE)
+        root = ctypes.c_uint32(0)
+        check(lib().svdag_server_last_frame(self.h, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(root)))
+        return out, root.value
+
     def trace(self, position, rotation, fov, bmin, bmax, w, h):
-        cam = Camera((ctypes.c_double * 3)(*position), (ctypes.c_double * 9)(*rotation), float(fov),
-                     (ctypes.c_double * 3)(*bmin), (ctypes.c_double * 3)(*bmax))
+        cam = self._camera(position, rotation, fov, bmin, bmax)
         out = np.empty(w * h, HIT_DTYPE)
         ms = ctypes.c_float(0)
         check(lib().svdag_trace_primary(self.h, ctypes.byref(cam), w, h, out.ctypes.data_as(ctypes.c_void_p),

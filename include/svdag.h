@@ -169,6 +169,30 @@ int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* edit, svdag_edit_stats* st
  * my_gpu_hash_dag_edits.cu:526-626).  stats: n entries or NULL. */
 int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* edits, uint32_t n, svdag_edit_stats* stats, float* gpu_ms);
 
+/* How svdag_edit_batch schedules a batch (no reference counterpart; the
+ * result is the same in every mode):
+ *   SVDAG_BATCH_SEQUENTIAL  each edit's kernels one after another;
+ *   SVDAG_BATCH_FUSED       (default) runs of sphere / box / paint edits whose
+ *                           regions do not collide share one edit-SVO build
+ *                           and one merge, and a single-wave kernel replays
+ *                           the top levels edit by edit.  Copy edits run on
+ *                           their own. */
+enum { SVDAG_BATCH_SEQUENTIAL = 0, SVDAG_BATCH_FUSED = 1 };
+int svdag_set_batch_mode(svdag_ctx* ctx, uint32_t mode);
+
+/* Autotuner (no reference counterpart; the reference fixes warp-cooperative
+ * probing at compile time).  Times a sample batch under each lanes-per-item
+ * choice for the merge kernels (4, 8, 16, 32) and keeps the fastest for the
+ * context.  The sample runs once untimed first so every candidate sees the
+ * same tables; the root and the history are restored after each run, so the
+ * DAG's contents do not change (items the sample inserted stay until the
+ * next garbage collection). */
+typedef struct {
+    uint32_t group_size;       /* chosen lanes per item */
+    float ms[4];               /* best device ms of the sample for 4, 8, 16, 32 */
+} svdag_tune_result;
+int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_result* out);
+
 /* reference: MyGPUHashDAGUndoRedo::undo / redo (my_gpu_base_dag.h:35-64). */
 int svdag_undo(svdag_ctx* ctx);
 int svdag_redo(svdag_ctx* ctx);
@@ -211,6 +235,82 @@ typedef struct {
 } svdag_hit;
 int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* camera, uint32_t width, uint32_t height,
                         svdag_hit* out_hits, float* gpu_ms);
+
+/* Slab-arena usage over all tables of the DAG (no reference counterpart;
+ * the reference reports memory_used_by_slabs, gpu_hash_table_base.cu:187-206).
+ * min_free_fraction: the smallest free / capacity ratio over the tables. */
+typedef struct {
+    uint64_t slab_capacity;
+    uint64_t slabs_used;
+    uint64_t slabs_free;
+    uint64_t bytes_reserved;
+    double min_free_fraction;
+} svdag_memory;
+int svdag_memory_usage(svdag_ctx* ctx, svdag_memory* out);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous batching (no reference counterpart; the reference applies one
+ * edit per Engine::edit call, engine.h:104-163).  A server thread takes the
+ * requests that are waiting, admits a FIFO prefix against slab-arena
+ * headroom and applies it as one batch (svdag_edit_batch semantics: the
+ * result equals applying the requests one by one in submission order), while
+ * requests that arrive meanwhile queue for the next batch.  A frame thread
+ * traces the latest published root on its own stream every frame period,
+ * overlapping the edit batches.  When headroom runs out the server collects
+ * garbage (clearing the undo history) and, if that is not enough, grows the
+ * arenas.  While a server runs, other calls on the context stay legal; they
+ * are serialised with the server's batches. */
+typedef struct {
+    uint32_t max_batch;            /* most requests in one batch (0 = 256) */
+    uint32_t frame_width;          /* per-frame primary trace; 0 = no frames */
+    uint32_t frame_height;
+    uint32_t frame_period_us;      /* 0 = 16667 */
+    const svdag_camera* cameras;   /* frame f uses cameras[f % num_cameras] (copied at start) */
+    uint32_t num_cameras;
+    uint32_t allow_gc;             /* collect garbage when headroom runs out */
+    double reserve_fraction;       /* admission keeps this fraction of every table free (0 = 0.02) */
+} svdag_server_config;
+
+typedef struct {
+    int32_t status;                /* SVDAG_OK or the error the batch failed with */
+    uint32_t applied;
+    uint32_t root_after;
+    uint32_t batch;                /* index of the batch the request ran in */
+    double submit_s;               /* server clock (seconds since start) */
+    double admit_s;
+    double done_s;
+} svdag_request_info;
+
+typedef struct {
+    uint64_t requests;
+    uint64_t batches;
+    uint64_t frames;
+    uint64_t frames_skipped;       /* frame due while the previous trace still ran */
+    uint64_t gcs;
+    uint64_t grows;
+    uint64_t deferred;             /* admission deferrals (a request counts once per batch it missed) */
+    double wall_s;
+    double edit_busy_s;            /* host wall time spent inside batches */
+    double trace_ms_avg;           /* device time per frame trace */
+    uint32_t last_frame_root;
+    uint32_t max_batch_seen;
+} svdag_server_stats;
+
+int svdag_server_start(svdag_ctx* ctx, const svdag_server_config* cfg);
+int svdag_server_submit(svdag_ctx* ctx, const svdag_edit_desc* edit, uint64_t* ticket);
+/* Blocks until the request is done. */
+int svdag_server_wait(svdag_ctx* ctx, uint64_t ticket, svdag_request_info* out);
+/* Drains the queue, stops both threads and reports. */
+int svdag_server_stop(svdag_ctx* ctx, svdag_server_stats* stats);
+/* Hit buffer (frame_width * frame_height records) and root of the last
+ * traced frame; valid after svdag_server_stop until the next start. */
+int svdag_server_last_frame(svdag_ctx* ctx, svdag_hit* out, uint32_t* root);
+/* Open-loop driver: starts a server, submits edits[i] at arrival_s[i]
+ * (seconds after start, non-decreasing), waits for all, stops.  out: n
+ * entries.  The timing loop runs in native code so the arrival process is
+ * not disturbed by the caller's runtime. */
+int svdag_serve_replay(svdag_ctx* ctx, const svdag_server_config* cfg, const svdag_edit_desc* edits,
+                       const double* arrival_s, uint32_t n, svdag_request_info* out, svdag_server_stats* stats);
 
 /* Canonical form of the reachable DAG (nodes renumbered per level by sorted
  * content, children replaced by canonical ids).  hash = FNV-1a 64 of the dump.

@@ -246,3 +246,144 @@ def test_batched_edits_equal_sequential_edits():
     assert a.history() == b.history()
     a.close()
     b.close()
+
+
+def _mixed_stream(levels, seed):
+    """Surface edits of all geometric kinds, clusters of colliding edits (the
+    fused planner splits waves there), a copy, an edit outside the volume and
+    an aligned fill that makes no leaf."""
+    tp = workloads.terrain_params(levels)
+    side = 1 << levels
+    heights = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                        0, 0, side, side)
+
+    def h(x, z):
+        return float(heights[z, x])
+
+    rng = np.random.default_rng(seed)
+    edits = workloads.surface_edits(h, 40, seed=seed, levels=levels, rmin=2.0, rmax=side / 8, near=side / 16)
+    for _ in range(8):
+        x, z = (int(v) for v in rng.integers(0, side, 2))
+        edits.insert(int(rng.integers(0, len(edits))),
+                     (svdag_py.PAINT, int(rng.integers(1, 15)), (float(x), h(x, z), float(z)),
+                      float(rng.uniform(3, side / 16))))
+    c = edits[3][2]
+    edits[5:5] = [(svdag_py.SPHERE_REMOVE, 0, c, 20.0), (svdag_py.BOX_ADD, 3, c, 9.0),
+                  (svdag_py.SPHERE_ADD, 5, (c[0] + 7, c[1], c[2]), 12.0), (svdag_py.BOX_REMOVE, 0, c, 4.0)]
+    src = (side * 0.3, h(int(side * 0.3), int(side * 0.3)), side * 0.3)
+    dst = (side * 0.7, h(int(side * 0.7), int(side * 0.6)) + 5.0, side * 0.6)
+    edits.insert(20, (svdag_py.COPY, 0, src, side / 32, src, dst))
+    edits.append((svdag_py.SPHERE_ADD, 2, (-5000.0, 10.0, 10.0), 40.0))
+    edits.append((svdag_py.BOX_ADD, 7, (side / 4, side / 4, side / 4), side / 4))
+    return edits
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("levels", [L10, 12])
+def test_fused_waves_match_sequential_step_by_step(levels):
+    edits = _mixed_stream(levels, seed=21 + levels)
+    a, _ = _pair(levels=levels)
+    b, o = _pair(levels=levels)
+    a.set_batch_mode(0)
+    b.set_batch_mode(1)
+    sa, _ = a.edit_batch(edits[:30])
+    sb, _ = b.edit_batch(edits[:30])
+    sa2, _ = a.edit_batch(edits[30:])
+    sb2, _ = b.edit_batch(edits[30:])
+    for k, (s1, s2) in enumerate(zip(sa + sa2, sb + sb2)):
+        assert s1.applied == s2.applied, k
+        assert list(s1.svo_items) == list(s2.svo_items), k
+    for e in edits:
+        o.edit(*e)
+    _same(b, o)
+    assert a.history() == b.history()
+    for _ in range(a.history()[0]):
+        assert a.canonical() == b.canonical()
+        a.undo()
+        b.undo()
+    assert a.canonical() == b.canonical()
+    a.close()
+    b.close()
+
+
+def _frame_camera(levels):
+    cam = workloads.config1_camera(levels)
+    rot = svdag_py.look_at(cam["position"], cam["target"])
+    return (cam["position"], rot, cam["fov"], cam["bmin"], cam["bmax"])
+
+
+@pytest.mark.gpu
+def test_serving_results_equal_alone_and_in_the_mix():
+    # Requests arrive faster than batches finish, so batches mix many of
+    # them; every request's result (applied flag and the DAG state after it)
+    # equals applying the requests one by one on their own.
+    edits = _mixed_stream(L10, seed=33)
+    rng = np.random.default_rng(3)
+    arrivals = np.cumsum(rng.exponential(1 / 3000.0, len(edits)))
+    a, o = _pair()
+    b, _ = _pair()
+    cam = _frame_camera(L10)
+    info, st = a.serve_replay(edits, arrivals, max_batch=16, frame=(256, 256), cameras=[cam], frame_period_us=1000)
+    assert (info["status"] == 0).all()
+    assert st.requests == len(edits) and st.batches >= 2 and st.max_batch_seen > 1
+    assert st.frames > 0
+    assert (info["done_s"] >= info["submit_s"]).all()
+    alone = [b.edit(*e).applied for e in edits]
+    assert list(info["applied"]) == alone
+    for e in edits:
+        o.edit(*e)
+    _same(a, o)
+    assert a.history() == b.history()
+    roots = [a.root]
+    for _ in range(a.history()[0]):
+        assert a.canonical() == b.canonical()
+        a.undo()
+        b.undo()
+        roots.append(a.root)
+    assert a.canonical() == b.canonical()
+    # The last frame traced a published state: tracing that state again
+    # gives the same hit buffer.
+    hits, froot = a.last_frame(256, 256)
+    assert froot in roots
+    for _ in range(len(roots)):
+        if a.root == froot:
+            break
+        a.redo()
+    assert a.root == froot
+    again, _ = a.trace(*cam, 256, 256)
+    assert np.array_equal(hits.view(np.uint32), again.view(np.uint32))
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_serving_admission_collects_and_grows_under_a_small_arena():
+    # Arenas far too small for the stream: admission has to collect garbage
+    # and grow the arenas, and the result still equals the oracle's.
+    a, o = _pair(expected_items=20000, arena_slabs=64)
+    edits = _mixed_stream(L10, seed=41)
+    arrivals = np.arange(len(edits)) * 2e-4
+    info, st = a.serve_replay(edits, arrivals, max_batch=8, allow_gc=True, reserve_fraction=0.05)
+    assert (info["status"] == 0).all()
+    assert st.gcs + st.grows > 0
+    for e in edits:
+        o.edit(*e)
+    _same(a, o)
+    m = a.memory_usage()
+    assert m.slabs_used + m.slabs_free == m.slab_capacity
+    a.close()
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_the_dag_and_later_edits_exact():
+    a, o = _pair()
+    edits = _mixed_stream(L10, seed=55)
+    before = a.canonical()
+    g, ms = a.autotune(edits[:16])
+    assert g in (4, 8, 16, 32) and all(v > 0 for v in ms.values())
+    assert a.canonical() == before and a.history() == (0, 0)
+    a.edit_batch(edits)
+    for e in edits:
+        o.edit(*e)
+    _same(a, o)
+    a.close()
