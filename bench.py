@@ -156,6 +156,109 @@ def slab_hash_bench(args):
     }
 
 
+def editing_bench(args):
+    """Depth-D editing on the fBm terrain (base 2^(D-1), amplitude 6000 * 2^D / 32768, 8 octaves).
+
+    plain:   the first `plain_edits` edits of the batched-editing stream, one
+             svdag_edit call each (the reference's per-edit form); per-edit
+             wall latency, voxel edits/s.  The DAG is then rolled back and
+             collected so the batched leg starts from the same state.
+    batched: the whole stream (256 edits: 40% sphere-add, 40% sphere-remove,
+             20% box fill, r log-uniform in [8, 512], centres within +-1024 of
+             the surface) as 16-edit fused batches, each followed by a garbage
+             collection; voxel edits/s over the wall time of all batches and
+             collections; an edit's latency is its batch's wall time.
+    serving: open-loop Poisson arrivals at each rate (the same mix plus a 15%
+             sphere paint) for `serve_seconds`, with a primary trace of the
+             frame size every 16.7 ms; p50/p99 of submit -> root visible.
+    voxel edits/s counts, per edit, the voxels whose unit box meets the tool's
+    overlap predicate (exact per-row interval counting)."""
+    import svdag_py
+    import workloads
+
+    D = args.edit_levels
+    tp = workloads.terrain_params(D)
+    scale = 4 ** (D - 10)
+    t0 = time.perf_counter()
+    g = svdag_py.MyGPUHashDAG(D, 4, expected_items=int(6 * scale * 60000),
+                              workspace_items=min(1 << 28, int(2 * scale * 400000)))
+    g.build_terrain(**tp)
+    gc0 = g.garbage_collect()
+    build_s = time.perf_counter() - t0
+    log(f"[bench] depth {D} terrain: {build_s:.1f} s, {gc0.items_live} live items")
+
+    def h(x, z):
+        return int(g.terrain_heights(x, z, 1, 1)[0, 0])
+
+    edits = workloads.config3_edits(h, D, n=256, seed=6)
+    counts = np.array([workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in edits], np.float64)
+
+    # plain per-edit form
+    n_plain = min(args.plain_edits, len(edits))
+    lat = []
+    t1 = time.perf_counter()
+    for e in edits[:n_plain]:
+        s0 = time.perf_counter()
+        g.edit(*e)
+        lat.append(time.perf_counter() - s0)
+    plain_s = time.perf_counter() - t1
+    for _ in range(g.history()[0]):
+        g.undo()
+    g.garbage_collect()
+    plain = {"edits": n_plain, "voxel_edits_per_s": float(counts[:n_plain].sum() / plain_s),
+             "edits_per_s": n_plain / plain_s,
+             "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+             "latency_ms_p99": float(np.percentile(lat, 99) * 1e3)}
+    log(f"[bench] plain: {plain}")
+
+    # batched fused form, 16 edits per batch, GC after each batch
+    g.set_batch_mode(1)
+    tune_g, tune_ms = g.autotune(edits[:16])
+    g.garbage_collect()
+    blat = []
+    gpu_ms = 0.0
+    t2 = time.perf_counter()
+    for b in range(0, len(edits), 16):
+        s0 = time.perf_counter()
+        _, ms = g.edit_batch(edits[b:b + 16])
+        g.garbage_collect()
+        gpu_ms += ms
+        blat += [time.perf_counter() - s0] * len(edits[b:b + 16])
+    batch_s = time.perf_counter() - t2
+    batched = {"edits": len(edits), "batch": 16, "voxel_edits_per_s": float(counts.sum() / batch_s),
+               "edits_per_s": len(edits) / batch_s, "edit_gpu_ms_total": round(gpu_ms, 3),
+               "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
+               "latency_ms_p99": float(np.percentile(blat, 99) * 1e3),
+               "autotuned_lanes_per_item": tune_g,
+               "autotune_ms": {str(k): round(v, 3) for k, v in tune_ms.items()}}
+    batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
+    log(f"[bench] batched: {batched}")
+
+    serving = []
+    cams = []
+    for c in workloads.camera_path(D, 64):
+        cams.append((c[0], svdag_py.look_at(c[0], c[1]), c[2], c[3], c[4]))
+    for rate in args.serve_rates:
+        reqs, arr = workloads.config4_requests(h, D, rate, args.serve_seconds, seed=7)
+        vox = np.array([workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in reqs], np.float64)
+        info, st = g.serve_replay(reqs, arr, max_batch=64, frame=(1920, 1080), cameras=cams, allow_gc=True)
+        latency = info["done_s"] - info["submit_s"]
+        span = float(info["done_s"].max() - info["submit_s"].min())
+        row = {"rate_per_s": rate, "requests": len(reqs), "ok": int((info["status"] == 0).sum()),
+               "latency_ms_p50": float(np.percentile(latency, 50) * 1e3),
+               "latency_ms_p99": float(np.percentile(latency, 99) * 1e3),
+               "throughput_req_per_s": len(reqs) / span, "voxel_edits_per_s": float(vox.sum() / span),
+               "batches": st.batches, "max_batch": st.max_batch_seen, "frames": st.frames,
+               "frames_skipped": st.frames_skipped, "trace_ms_avg_1920x1080": round(st.trace_ms_avg, 3),
+               "gcs": st.gcs, "grows": st.grows}
+        log(f"[bench] serving: {row}")
+        serving.append(row)
+        g.garbage_collect()
+    g.close()
+    return {"levels": D, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
+            "plain": plain, "batched": batched, "serving": serving}
+
+
 def cpu_baseline(items, finds, buckets, args):
     import oracle_py
 
@@ -198,14 +301,22 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="skip the variant sweep and use this one")
     ap.add_argument("--pmc-mode", action="store_true", help="calibration copy + find launches only (for rocprofv3 --pmc)")
+    ap.add_argument("--edit-levels", type=int, default=15, help="depth of the editing / serving workloads")
+    ap.add_argument("--plain-edits", type=int, default=64)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 1000.0, 2000.0])
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--no-editing", action="store_true", help="slab-hash line only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "
               "(see the sharding notes in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     out = slab_hash_bench(args)
-    if out is not None:
-        print(json.dumps(out), flush=True)
+    if out is None:
+        return
+    if not args.no_editing:
+        out["editing"] = editing_bench(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -151,3 +151,47 @@ def config3_edits(height_at, levels, n=256, seed=6):
     sphere-remove, 20% axis-aligned box fill; radius log-uniform in [8, 512];
     centre (x, h(x,z) + U[-1024, 1024], z), x, z ~ U[0, 2^D)."""
     return surface_edits(height_at, n, seed, levels, rmin=8.0, rmax=512.0, kinds=(0.4, 0.4, 0.2), near=1024.0)
+
+
+def config4_requests(height_at, levels, rate, duration_s, seed=7):
+    """Serving workload: Poisson arrivals at `rate` per second for
+    `duration_s`; the radius / centre mix of the batched-editing workload
+    (log-uniform r in [8, 512], centre within +-1024 of the surface) over
+    sphere-add 35%, sphere-remove 35%, box fill 15% and a sphere paint 15%
+    that sets a 4-bit material id.  Returns (edits, arrival seconds)."""
+    rng = np.random.default_rng(seed)
+    side = 1 << levels
+    arrivals = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration_s:
+            break
+        arrivals.append(t)
+    edits = []
+    for _ in arrivals:
+        k = int(rng.choice(4, p=(0.35, 0.35, 0.15, 0.15)))
+        kind = (SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, PAINT)[k]
+        r = float(np.exp(rng.uniform(np.log(8.0), np.log(512.0))))
+        x, z = rng.integers(0, side, 2)
+        y = float(height_at(int(x), int(z))) + float(rng.uniform(-1024.0, 1024.0))
+        y = min(max(y, 0.0), side - 1.0)
+        mat = 0 if kind == SPHERE_REMOVE else int(rng.integers(1, 16))
+        edits.append((kind, mat, (float(x), y, float(z)), r))
+    return edits, np.asarray(arrivals, np.float64)
+
+
+def camera_path(levels, n, seed=8):
+    """Seeded fly-over: n cameras on a circle above the terrain looking at
+    points near the centre, as (position, target, fov, bmin, bmax)."""
+    rng = np.random.default_rng(seed)
+    side = float(1 << levels)
+    out = []
+    phase = float(rng.uniform(0, 2 * np.pi))
+    for i in range(n):
+        a = phase + 2 * np.pi * i / max(n, 1)
+        pos = (0.5 * side + 0.6 * side * np.cos(a), 0.9 * side, 0.5 * side + 0.6 * side * np.sin(a))
+        tgt = (0.5 * side + float(rng.uniform(-0.05, 0.05)) * side, 0.45 * side,
+               0.5 * side + float(rng.uniform(-0.05, 0.05)) * side)
+        out.append((pos, tgt, 60.0, (0.0, 0.0, 0.0), (side, side, side)))
+    return out
