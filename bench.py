@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "voxel edits/s & p99 edit latency (depth 15); slab-hash find Mops/s vs HBM"
 HBM_PEAK_GBS = 8000.0
-VARIANTS = [0, 4, 8, 16, 32, 204, 404, 208, 408]
+VARIANTS = [0, 4, 8, 16, 32, 204, 404, 208, 408, 1004, 1008, 1204, 1208]
 
 
 def log(msg):
@@ -39,16 +39,22 @@ def log(msg):
 def variant_name(v):
     if v == 0:
         return "per-lane"
+    if v >= 1000:
+        n = max(1, (v % 1000) // 100)
+        return (f"XCD-binned (bin pass + find pass, each XCD probes its own bucket range), wave64 group of "
+                f"{v % 100} lanes, {n} finds in flight per group")
     if v >= 100:
         return f"wave64 group of {v % 100} lanes, {v // 100} finds in flight per group"
     return f"wave64 group of {v} lanes"
 
 
-def read_traffic(path):
-    """HBM bytes per find launch from a committed rocprofv3 --pmc summary, if any."""
+def read_traffic(path, variant):
+    """HBM bytes per find launch of `variant` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a 256 MiB copy), if any."""
+    key = f"traffic_bytes_per_launch[{variant}]:"
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -121,7 +127,7 @@ def slab_hash_bench(args):
         cpu = cpu_baseline(items, finds, buckets, args)
 
     hs.close()
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "slab_hash_find_traffic.txt"))
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "slab_hash_find_traffic.txt"), best)
     return {
         "metric": METRIC,
         "value": round(value, 2),

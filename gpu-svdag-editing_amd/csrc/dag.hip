@@ -21,6 +21,15 @@
 
 namespace svdag {
 
+// Full-item "counts" only gate discovery on the level above (nonzero or
+// not), so they are flags: a relaxed load first keeps thousands of full
+// items from serialising on one atomic.
+__device__ inline void set_flag(uint32_t* p)
+{
+    if (ld_relaxed(p) == 0u)
+        atomicOr(p, 1u);
+}
+
 __device__ inline uint32_t dedup_cap(uint32_t count, uint32_t maxCap)
 {
     uint32_t cap = 64;
@@ -106,16 +115,32 @@ __global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy)
 // One work item per 8 lanes, one lane per child (create_edit_svo.cu:167-303).
 // kMulti: items of several edits share the level; item w uses eds[editOf[w]]
 // and its children inherit the edit.
+//
+// Child slots are reserved once per block and iteration: waves count their
+// subdivided children with a ballot, the block sums them in LDS and one lane
+// does the single atomicAdd, so the level counter is not a hot spot.  The
+// per-edit counts of a fused wave go through an LDS histogram (the edits of
+// a wave are at most 256 consecutive indices, so e & 255 is a unique key)
+// flushed once per block.
 template <bool kMulti>
 __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, const EditorDev* eds, uint32_t L)
 {
+    __shared__ uint32_t waveTotal[4];
+    __shared__ uint32_t blockBase;
+    __shared__ uint32_t hist[kMulti ? 256 : 1];
+    __shared__ uint32_t histEdit[kMulti ? 256 : 1];
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = threadIdx.x >> 6;
     const uint32_t c = lane & 7u;
     const uint32_t gbase = lane & ~7u;
     const LevelDesc me = ws.lvl[L];
     const uint32_t childStart = me.start + me.count;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         ws.lvl[L + 1].start = childStart;
+    if (kMulti) {
+        hist[threadIdx.x] = 0u;
+        __syncthreads();
+    }
     const uint32_t childLevel = L + 1;
     const uint32_t childDepth = dag.levels - childLevel;
     const bool leafChildren = childLevel == dag.leafLevel;
@@ -123,11 +148,13 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
     // After an overflow the count may run past the array; the edit is
     // discarded then, but no lane may read or write beyond capWork.
     const uint32_t count = me.start >= ws.capWork ? 0u : min(me.count, ws.capWork - me.start);
-    for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / 8u; i < count; i += gridDim.x * groupsPerBlock) {
-        const uint32_t w = me.start + i;
-        const uint32_t e = kMulti ? ws.editOf[w] : 0u;
+    for (uint32_t i0 = blockIdx.x * groupsPerBlock; i0 < count; i0 += gridDim.x * groupsPerBlock) {
+        const uint32_t i = i0 + threadIdx.x / 8u;
+        const bool active = i < count;
+        const uint32_t w = me.start + (active ? i : 0u);
+        const uint32_t e = (kMulti && active) ? ws.editOf[w] : 0u;
         const EditorDev& ed = eds[e];
-        const uint4 item = ws.work[w];
+        const uint4 item = active ? ws.work[w] : make_uint4(0u, 0u, 0u, kSentinel);
         uint32_t childHandle = kSentinel;
         if (item.w != kSentinel) {
             const uint32_t* node = dag_node_ptr(dag, item.w);
@@ -145,23 +172,37 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
         else if (!leafChildren)
             st = header_full(dag_node_ptr(dag, childHandle)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
         uint32_t fillMat = 0;
-        Op op;
+        Op op = OP_KEEP;
         CopyState cs;
-        if (ed.kind == EK_COPY)
-            op = decide_copy(ed, dag, cx, cy, cz, childDepth, ws.state[w], cs);
-        else
-            op = decide(ed, cx, cy, cz, childDepth, st, fillMat);
-        const bool sub = op == OP_SUBDIVIDE;
-        const uint32_t subMask = group_ballot<8>(sub, gbase);
-        uint32_t base = 0;
-        if (c == 0 && subMask) {
-            base = atomicAdd(&ws.lvl[L + 1].count, (uint32_t)__builtin_popcount(subMask));
-            if (kMulti)
-                atomicAdd(&ws.edCount[e * kMaxLevels + L + 1], (uint32_t)__builtin_popcount(subMask));
+        if (active) {
+            if (ed.kind == EK_COPY)
+                op = decide_copy(ed, dag, cx, cy, cz, childDepth, ws.state[w], cs);
+            else
+                op = decide(ed, cx, cy, cz, childDepth, st, fillMat);
         }
-        base = group_bcast<8>(base, 0);
+        const bool sub = active && op == OP_SUBDIVIDE;
+        const unsigned long long subBal = __ballot(sub);
+        const uint32_t subMask = (uint32_t)(subBal >> gbase) & 0xFFu;
+        // Block-wide reservation of the children.
+        if (lane == 0)
+            waveTotal[wv] = (uint32_t)__builtin_popcountll(subBal);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t k = 0; k < blockDim.x / 64u; ++k)
+                tot += waveTotal[k];
+            blockBase = tot ? atomicAdd(&ws.lvl[L + 1].count, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t base = blockBase;
+        for (uint32_t k = 0; k < wv; ++k)
+            base += waveTotal[k];
+        if (kMulti && c == 0 && subMask) {
+            atomicAdd(&hist[e & 255u], (uint32_t)__builtin_popcount(subMask));
+            histEdit[e & 255u] = e;
+        }
         if (sub) {
-            const uint32_t local = base + __builtin_popcount(subMask & ((1u << c) - 1u));
+            const uint32_t local = base + (uint32_t)__builtin_popcountll(subBal & ((1ull << lane) - 1ull));
             const uint64_t g = (uint64_t)childStart + local;
             if (g < ws.capWork && (!leafChildren || local < ws.capLeaf)) {
                 ws.work[g] = make_uint4(cx, cy, cz, childHandle);
@@ -178,21 +219,29 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
         } else if (op == OP_FILL) {
             childHandle = dag.fullyFilled[childLevel * dag.numMaterials + fillMat];
         }
-        const bool out = childHandle != kSentinel;
+        const bool out = active && childHandle != kSentinel;
         const uint32_t cmask = group_ballot<8>(out, gbase);
-        uint32_t* rec = ws.nodes + (size_t)w * kNodeStride;
-        const uint32_t n = __builtin_popcount(cmask);
-        if (out)
-            rec[1 + __builtin_popcount(cmask & ((1u << c) - 1u))] = childHandle;
-        if (c >= n)
-            rec[1 + c] = 0u;
-        if (c == 0) {
-            rec[0] = cmask ? header_make(cmask, subMask, dag.M) : 0u;
-            rec[9] = 0u;
+        if (active) {
+            uint32_t* rec = ws.nodes + (size_t)w * kNodeStride;
+            const uint32_t n = __builtin_popcount(cmask);
+            if (out)
+                rec[1 + __builtin_popcount(cmask & ((1u << c) - 1u))] = childHandle;
+            if (c >= n)
+                rec[1 + c] = 0u;
+            if (c == 0) {
+                rec[0] = cmask ? header_make(cmask, subMask, dag.M) : 0u;
+                rec[9] = 0u;
+            }
         }
+        __syncthreads(); // waveTotal / blockBase are rewritten next iteration
+    }
+    if (kMulti) {
+        __syncthreads();
+        const uint32_t v = hist[threadIdx.x];
+        if (v)
+            atomicAdd(&ws.edCount[histEdit[threadIdx.x] * kMaxLevels + L + 1], v);
     }
 }
-
 // One leaf per wave64, one lane per voxel (create_edit_svo.cu:320-426).
 template <bool kMulti>
 __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, const EditorDev* eds)
@@ -277,7 +326,7 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
         for (uint32_t i = tid; i < me.count; i += stride) {
             uint32_t m;
             if (leaf_is_full(ws.leaves + (size_t)i * ws.leafStride, dag.M, m))
-                atomicAdd(kMulti ? &ws.edFull[ws.editOf[me.start + i] * kMaxLevels + L] : &ws.lvl[L].fullCount, 1u);
+                set_flag(kMulti ? &ws.edFull[ws.editOf[me.start + i] * kMaxLevels + L] : &ws.lvl[L].fullCount);
         }
         return;
     }
@@ -329,7 +378,7 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
         for (uint32_t j = 0; j < 8; ++j)
             rec[1 + j] = j < n ? out[j] : 0u;
         if (full)
-            atomicAdd(kMulti ? &ws.edFull[e * kMaxLevels + L] : &ws.lvl[L].fullCount, 1u);
+            set_flag(kMulti ? &ws.edFull[e * kMaxLevels + L] : &ws.lvl[L].fullCount);
     }
 }
 
@@ -1039,6 +1088,11 @@ int svdag_destroy(svdag_ctx* ctx)
     if (ctx->server)
         (void)svdag_server_stop(ctx, nullptr);
     (void)hipStreamSynchronize(ctx->stream);
+    for (uint32_t* p : ctx->gcMasks)
+        if (p)
+            (void)hipFree(p);
+    if (ctx->dGcMasks)
+        (void)hipFree(ctx->dGcMasks);
     ctx->ts.release();
     ws_release(ctx->ws);
     for (void* p : { (void*)ctx->dFullyFilled, (void*)ctx->dRoot, (void*)ctx->hmin, (void*)ctx->hmax,

@@ -13,7 +13,7 @@ constexpr uint32_t kNodeStride = 10; // SVO node record: header + 8 children + p
 struct LevelDesc {
     uint32_t start;     // first global work index of the level
     uint32_t count;     // work items (= SVO items) of the level
-    uint32_t fullCount; // items flagged full by discovery at this level
+    uint32_t fullCount; // nonzero iff discovery flagged a full item at this level
     uint32_t pad;
 };
 
@@ -110,6 +110,11 @@ struct svdag_ctx {
     uint32_t* gcFront[2] = { nullptr, nullptr };
     uint64_t gcCap = 0;
     uint32_t* gcCounters = nullptr;
+    // Per-slot level masks of every table (kept between collections and
+    // regrown with the arenas) and their device pointer array.
+    std::vector<uint32_t*> gcMasks;
+    std::vector<uint32_t> gcMaskCap;
+    uint32_t** dGcMasks = nullptr;
     // Per-edit device slots of a batch (editors, statuses, level tables) and
     // their pinned host mirror.
     svdag::EditorDev* dEditors = nullptr;

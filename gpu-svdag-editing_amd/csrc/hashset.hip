@@ -5,6 +5,7 @@
 // (hash_tables/compact_acceleration_hash_table.{h,cu}, gpu_hash_table_base.{h,cu})
 // and the HashMapBench bulk kernels (extras/hashmap_bench/src/main.cu:124-216).
 #include "svdag_internal.h"
+#include <cmath>
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -285,6 +286,173 @@ __global__ void __launch_bounds__(256) bulk_lane_kernel(const TableDev* tables, 
     }
 }
 
+// ------------------------------------------------------------------ XCD-binned find
+// MI355X has 8 XCDs with a 4 MB L2 each, and consecutive workgroups are
+// dispatched to the XCDs round robin.  A plain find probes tag lines of the
+// whole table from every XCD, so no L2 holds them (22 MB of tag lines at
+// 2^24 items) and every probe fetches 128 bytes from the fabric.  The binned
+// find first sorts the needles by bucket range -- bucket b belongs to XCD
+// b * 8 / numBuckets -- into per-(segment, XCD) bins of [out index, hash,
+// words...] records, then runs blocks b with b % 8 == x over bin x only, so
+// each XCD's L2 sees one eighth of the tag lines.  The find itself is the
+// same table_find_group, so results are identical to the other variants.
+constexpr uint32_t kXcds = 8;
+
+struct BinDesc {
+    uint32_t offset;   // first word of the bin in the bin buffer
+    uint32_t cap;      // records the bin can hold
+};
+
+// One lane per needle; records of one (segment, XCD) key are reserved once
+// per wave (the lanes sharing the key take consecutive slots).
+template <int MW>
+__global__ void __launch_bounds__(256) bin_needles(const TableDev* tables, uint32_t minWords, const uint32_t* words,
+                                                   const uint32_t* segs, uint32_t nseg, uint32_t total,
+                                                   const BinDesc* bins, uint32_t* binCount, uint32_t* binBuf,
+                                                   uint32_t* overflow)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waveBase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t i0 = waveBase; i0 < total; i0 += gridDim.x * blockDim.x) {
+        const uint32_t idx = i0 + lane;
+        const bool valid = idx < total;
+        uint32_t key = 0xFFFFFFFFu, s = 0, outIdx = 0, h = 0;
+        uint32_t w[MW];
+        if (valid) {
+            const uint32_t sg = seg_of(segs, nseg, idx);
+            s = segs[sg * 5 + 0];
+            const uint32_t local = idx - segs[sg * 5 + 4];
+            outIdx = segs[sg * 5 + 3] + local;
+            const uint32_t* p = words + segs[sg * 5 + 2] + (size_t)local * s;
+#pragma unroll
+            for (int j = 0; j < MW; ++j) {
+                w[j] = (uint32_t)j < s ? p[j] : 0u;
+                if ((uint32_t)j < s)
+                    h ^= hash_part(w[j], (uint32_t)j);
+            }
+            const uint32_t nb = tables[s - minWords].numBuckets;
+            const uint32_t x = (uint32_t)(((uint64_t)(h % nb) * kXcds) / nb);
+            key = sg * kXcds + x;
+        }
+        unsigned long long pending = __ballot(valid);
+        while (pending) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1u;
+            const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
+            const unsigned long long same = __ballot(key == lk) & pending;
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(&binCount[lk], (uint32_t)__builtin_popcountll(same));
+            base = (uint32_t)__shfl((int)base, (int)leader, 64);
+            if ((same >> lane) & 1ull) {
+                const uint32_t pos = base + (uint32_t)__builtin_popcountll(same & ((1ull << lane) - 1ull));
+                const BinDesc b = bins[lk];
+                if (pos < b.cap) {
+                    uint32_t* r = binBuf + b.offset + (size_t)pos * (s + 2);
+                    r[0] = outIdx;
+                    r[1] = h;
+#pragma unroll
+                    for (int j = 0; j < MW; ++j)
+                        if ((uint32_t)j < s)
+                            r[2 + j] = w[j];
+                } else {
+                    atomicOr(overflow, 1u);
+                }
+            }
+            pending &= ~same;
+        }
+    }
+}
+
+// Blocks b serve XCD b % 8; N records per group in flight.
+template <int G, int N, int MW>
+__global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint32_t minWords, const uint32_t* segs,
+                                                   uint32_t nseg, const BinDesc* bins, const uint32_t* binCount,
+                                                   const uint32_t* binBuf, uint32_t* out)
+{
+    constexpr int W = 32 / G;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (G - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
+    const uint32_t groupsPerBlock = blockDim.x / G;
+    const uint32_t x = blockIdx.x % kXcds;
+    const uint32_t bx = blockIdx.x / kXcds;
+    const uint32_t blocksPerXcd = gridDim.x / kXcds;
+    for (uint32_t sg = 0; sg < nseg; ++sg) {
+        const uint32_t s = segs[sg * 5 + 0];
+        const uint32_t key = sg * kXcds + x;
+        const uint32_t cnt = min(binCount[key], bins[key].cap);
+        const uint32_t* rec0 = binBuf + bins[key].offset;
+        const TableDev t = tables[s - minWords];
+        const uint32_t stride = blocksPerXcd * groupsPerBlock * N;
+        for (uint32_t base = (bx * groupsPerBlock + threadIdx.x / G) * N; base < cnt; base += stride) {
+            Needle<G, MW> n[N];
+            uint32_t h[N], slab[N], outIdx[N], tag4[N];
+            bool valid[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const uint32_t e = base + k;
+                valid[k] = e < cnt;
+                const uint32_t* r = rec0 + (size_t)(valid[k] ? e : base) * (s + 2);
+                outIdx[k] = r[0];
+                h[k] = r[1];
+                needle_load(n[k], r + 2, s, gl);
+                tag4[k] = make_tag(h[k]) * 0x01010101u;
+                slab[k] = h[k] % t.numBuckets;
+            }
+            uint32_t lines[N][W];
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+                tagline_load<G>(lines[k], t.tags + (size_t)slab[k] * kTagWords, gl);
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                uint32_t r = kNotFound;
+                uint32_t sl = slab[k];
+                bool first = true;
+                while (true) {
+                    uint32_t wv[W];
+                    if (first) {
+#pragma unroll
+                        for (int i = 0; i < W; ++i)
+                            wv[i] = lines[k][i];
+                    } else {
+                        tagline_load<G>(wv, t.tags + (size_t)sl * kTagWords, gl);
+                    }
+                    first = false;
+                    uint32_t cand = 0;
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        if (gl * W + i != kNextWord)
+                            cand |= compact_bytes(zero_bytes(wv[i] ^ tag4[k])) << (4 * i);
+                    const uint32_t next = group_bcast<G>(wv[W - 1], G - 1);
+                    bool done = false;
+                    while (true) {
+                        const uint32_t bal = group_ballot<G>(cand != 0, gbase);
+                        if (!bal)
+                            break;
+                        const uint32_t leader = __ffs(bal) - 1u;
+                        const uint32_t mySlot = gl * W * 4 + (cand ? (__ffs(cand) - 1u) : 0u);
+                        const uint32_t slot = group_bcast<G>(mySlot, leader);
+                        if (gl == leader)
+                            cand &= cand - 1u;
+                        const uint32_t* item = t.items + ((size_t)sl * kSlots + slot) * s;
+                        const bool eq = needle_equal_lane(n[k], item, s, gl);
+                        if (group_ballot<G>(!eq, gbase) == 0) {
+                            r = (sl << kSlotBits) | slot;
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (done || next == kNoSlab)
+                        break;
+                    sl = next;
+                }
+                if (gl == 0 && valid[k])
+                    out[outIdx[k]] = r;
+            }
+        }
+    }
+}
+
 // Copy the words of n items (pointers already validated on the host).
 __global__ void gather_items(TableDev t, const uint32_t* ptrs, uint32_t n, uint32_t* out)
 {
@@ -295,11 +463,106 @@ __global__ void gather_items(TableDev t, const uint32_t* ptrs, uint32_t n, uint3
     }
 }
 
-int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* segs, uint32_t nseg, uint32_t* out,
-                int variant, bool insert, hipStream_t stream, uint32_t* segBuf)
+void BinScratch::release()
 {
-    if (nseg == 0)
+    for (void* p : { (void*)buf, (void*)count, desc, (void*)overflow })
+        if (p)
+            (void)hipFree(p);
+    *this = BinScratch {};
+}
+
+static int launch_binned(const TableSet& ts, const uint32_t* words, const svdag_segment* segs, uint32_t nseg,
+                         const uint32_t* segBuf, uint32_t n, uint32_t* out, int variant, hipStream_t stream,
+                         BinScratch* bs, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (!bs) {
+        set_error("binned find needs bin scratch");
+        return SVDAG_E_INVALID;
+    }
+    // Per-(segment, XCD) capacity: mean + 8 standard deviations + 256.
+    std::vector<BinDesc> desc((size_t)nseg * kXcds);
+    uint64_t words_ = 0;
+    uint32_t maxS = 0;
+    for (uint32_t sg = 0; sg < nseg; ++sg) {
+        const double mean = (double)segs[sg].count / kXcds;
+        const uint32_t cap = (uint32_t)(mean + 8.0 * std::sqrt(mean) + 256.0);
+        maxS = std::max(maxS, segs[sg].item_words);
+        for (uint32_t x = 0; x < kXcds; ++x) {
+            desc[sg * kXcds + x] = BinDesc { (uint32_t)words_, cap };
+            words_ += (uint64_t)cap * (segs[sg].item_words + 2);
+        }
+    }
+    if (words_ >= (1ull << 32)) {
+        set_error("binned find: too many needle words");
+        return SVDAG_E_INVALID;
+    }
+    if (!bs->count) {
+        SVDAG_HIP(hipMalloc(&bs->count, 64 * kXcds * 4));
+        SVDAG_HIP(hipMalloc(&bs->desc, 64 * kXcds * sizeof(BinDesc)));
+        SVDAG_HIP(hipMalloc(&bs->overflow, 4));
+    }
+    if (bs->bufWords < words_) {
+        if (bs->buf)
+            (void)hipFree(bs->buf);
+        bs->buf = nullptr;
+        bs->bufWords = 0;
+        SVDAG_HIP(hipMalloc(&bs->buf, words_ * 4));
+        bs->bufWords = words_;
+    }
+    SVDAG_HIP(hipMemcpyAsync(bs->desc, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, stream));
+    SVDAG_HIP(hipMemsetAsync(bs->count, 0, (size_t)nseg * kXcds * 4, stream));
+    SVDAG_HIP(hipMemsetAsync(bs->overflow, 0, 4, stream));
+    const BinDesc* dDesc = reinterpret_cast<const BinDesc*>(bs->desc);
+    const bool narrow = maxS <= 10;
+    const uint32_t gridBin = grid_for(n, 256, 8192);
+    if (ev0)
+        SVDAG_HIP(hipEventRecord(ev0, stream));
+    if (narrow)
+        bin_needles<10><<<gridBin, 256, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, dDesc,
+                                                     bs->count, bs->buf, bs->overflow);
+    else
+        bin_needles<18><<<gridBin, 256, 0, stream>>>(ts.devTables, ts.minWords, words, segBuf, nseg, n, dDesc,
+                                                     bs->count, bs->buf, bs->overflow);
+    const uint32_t grid = kXcds * 512;
+#define SVDAG_BINNED_CASE(V, G, N)                                                                                   \
+    case V:                                                                                                          \
+        if (narrow)                                                                                                  \
+            find_binned<G, N, 10><<<grid, 256, 0, stream>>>(ts.devTables, ts.minWords, segBuf, nseg, dDesc,          \
+                                                            bs->count, bs->buf, out);                               \
+        else                                                                                                         \
+            find_binned<G, N, 18><<<grid, 256, 0, stream>>>(ts.devTables, ts.minWords, segBuf, nseg, dDesc,          \
+                                                            bs->count, bs->buf, out);                               \
+        break;
+    switch (variant) {
+        SVDAG_BINNED_CASE(1004, 4, 1)
+        SVDAG_BINNED_CASE(1008, 8, 1)
+        SVDAG_BINNED_CASE(1204, 4, 2)
+        SVDAG_BINNED_CASE(1208, 8, 2)
+    default:
+        set_error("unknown binned variant");
+        return SVDAG_E_INVALID;
+    }
+#undef SVDAG_BINNED_CASE
+    if (ev1)
+        SVDAG_HIP(hipEventRecord(ev1, stream));
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* segs, uint32_t nseg, uint32_t* out,
+                int variant, bool insert, hipStream_t stream, uint32_t* segBuf, BinScratch* bins, hipEvent_t ev0,
+                hipEvent_t ev1)
+{
+    // Nothing to launch: the events still bracket an empty interval.
+    auto empty = [&]() -> int {
+        if (ev0)
+            SVDAG_HIP(hipEventRecord(ev0, stream));
+        if (ev1)
+            SVDAG_HIP(hipEventRecord(ev1, stream));
         return SVDAG_OK;
+    };
+    if (nseg == 0)
+        return empty();
     if (nseg > 64) {
         set_error("too many segments (max 64)");
         return SVDAG_E_INVALID;
@@ -325,8 +588,17 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
     SVDAG_HIP(hipMemcpyAsync(segBuf, host, nseg * 5 * 4, hipMemcpyHostToDevice, stream));
     const uint32_t n = (uint32_t)total;
     if (n == 0)
-        return SVDAG_OK;
+        return empty();
+    if (variant >= 1000) {
+        if (insert) {
+            set_error("binned variants (1004, 1008, 1204, 1208) are find only");
+            return SVDAG_E_INVALID;
+        }
+        return launch_binned(ts, words, segs, nseg, segBuf, n, out, variant, stream, bins, ev0, ev1);
+    }
     const uint32_t block = 256;
+    if (ev0)
+        SVDAG_HIP(hipEventRecord(ev0, stream));
     // Items of at most 10 words need fewer needle registers per lane.
     const bool narrow = ts.maxWords <= 10;
     switch (variant) {
@@ -375,9 +647,12 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
         SVDAG_MULTI_CASE(408, 8, 4)
 #undef SVDAG_MULTI_CASE
     default:
-        set_error("variant must be 0, 4, 8, 16, 32 (lanes per item) or 204, 404, 208, 408 (N*100 + G, find only)");
+        set_error("variant must be 0, 4, 8, 16, 32 (lanes per item), 204, 404, 208, 408 (N*100 + G, find only) "
+                  "or 1004, 1008, 1204, 1208 (XCD-binned, find only)");
         return SVDAG_E_INVALID;
     }
+    if (ev1)
+        SVDAG_HIP(hipEventRecord(ev1, stream));
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
@@ -391,6 +666,7 @@ struct svdag_hashset {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t* segBuf = nullptr;
+    BinScratch bins;
 };
 
 extern "C" {
@@ -497,6 +773,7 @@ int svdag_hashset_destroy(svdag_hashset* hs)
         return SVDAG_OK;
     (void)hipStreamSynchronize(hs->stream);
     hs->ts.release();
+    hs->bins.release();
     (void)hipFree(hs->segBuf);
     (void)hipEventDestroy(hs->ev0);
     (void)hipEventDestroy(hs->ev1);
@@ -510,14 +787,19 @@ static int hashset_bulk(svdag_hashset* hs, const uint32_t* words, const svdag_se
 {
     if (!hs || !words || !segs || !out)
         return SVDAG_E_INVALID;
-    if (ms)
-        SVDAG_HIP(hipEventRecord(hs->ev0, hs->stream));
-    SVDAG_TRY(launch_bulk(hs->ts, words, segs, nseg, out, variant, insert, hs->stream, hs->segBuf));
-    if (ms)
-        SVDAG_HIP(hipEventRecord(hs->ev1, hs->stream));
+    SVDAG_TRY(launch_bulk(hs->ts, words, segs, nseg, out, variant, insert, hs->stream, hs->segBuf, &hs->bins,
+                          ms ? hs->ev0 : nullptr, ms ? hs->ev1 : nullptr));
     SVDAG_HIP(hipStreamSynchronize(hs->stream));
     if (ms)
         SVDAG_HIP(hipEventElapsedTime(ms, hs->ev0, hs->ev1));
+    if (variant >= 1000 && !insert) {
+        // A bin that overflowed (far beyond 8 standard deviations) leaves
+        // records unprocessed: redo the whole find with the plain group kernel.
+        uint32_t ovf = 0;
+        SVDAG_HIP(hipMemcpy(&ovf, hs->bins.overflow, 4, hipMemcpyDeviceToHost));
+        if (ovf)
+            return hashset_bulk(hs, words, segs, nseg, out, variant % 100, ms, insert);
+    }
     if (insert) {
         for (auto& t : hs->ts.tables) {
             AllocState a {};

@@ -50,9 +50,23 @@ struct TableSet {
     void release();
 };
 
+// Scratch of the XCD-binned find (hashset.hip): bin records, per-(segment,
+// XCD) counters and descriptors, and an overflow flag.
+struct BinScratch {
+    uint32_t* buf = nullptr;
+    size_t bufWords = 0;
+    uint32_t* count = nullptr;    // [64 segments][8]
+    void* desc = nullptr;         // BinDesc[64][8]
+    uint32_t* overflow = nullptr;
+    void release();
+};
+
 // Launch helpers implemented in hashset.hip.
+// ev0 / ev1 (optional) are recorded right before the first and after the
+// last kernel of the operation, after every descriptor upload.
 int launch_bulk(const TableSet& ts, const uint32_t* words, const svdag_segment* segs, uint32_t nseg,
-                uint32_t* out, int variant, bool insert, hipStream_t stream, uint32_t* segBuf);
+                uint32_t* out, int variant, bool insert, hipStream_t stream, uint32_t* segBuf, BinScratch* bins,
+                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 inline uint32_t grid_for(uint64_t work, uint32_t perBlock, uint32_t cap = 8192)
 {

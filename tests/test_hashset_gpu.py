@@ -106,3 +106,39 @@ def test_empty_batch_and_oom_is_reported():
         hs.add_dev(wdev, segs, odev, 16)
     assert e.value.code == -2
     hs.close()
+
+
+FIND_VARIANTS = [0, 4, 8, 16, 32, 204, 404, 208, 408, 1004, 1008, 1204, 1208]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("buckets_per_size", [None, 3])
+def test_find_variants_return_identical_pointers(buckets_per_size):
+    # All find variants (lane, group, multi-needle, XCD-binned) run the same
+    # probe over the same table: the result arrays are identical, including
+    # multi-slab chains (3 buckets per size).
+    items = workloads.slab_hash_items(1 << 16)
+    finds = workloads.slab_hash_finds(items, 1 << 16)
+    if buckets_per_size is None:
+        hs, _ = _build(items)
+    else:
+        sizes = list(range(2, 10))
+        hs = svdag_py.HashSet(2, 9, [buckets_per_size] * len(sizes),
+                              [len(items.get(s, [])) * 2 for s in sizes])
+    groups = sorted(items.items())
+    words, segs, n = svdag_py.make_segments(groups)
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    hs.add_dev(wdev, segs, odev, 16)
+    fwords, fsegs, fn = svdag_py.make_segments([(s, finds[s]) for s, _ in groups])
+    fdev = svdag_py.DeviceBuffer.from_array(fwords)
+    rdev = svdag_py.DeviceBuffer(4 * fn)
+    ref = None
+    for v in FIND_VARIANTS:
+        hs.find_dev(fdev, fsegs, rdev, v)
+        r = rdev.to_array(np.uint32, fn)
+        if ref is None:
+            ref = r
+            assert 0.4 < float(np.mean(r != svdag_py.NOT_FOUND)) < 0.6
+        assert np.array_equal(r, ref), v
+    hs.close()
