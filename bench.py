@@ -165,18 +165,20 @@ def slab_hash_bench(args):
 def editing_bench(args):
     """Depth-D editing on the fBm terrain (base 2^(D-1), amplitude 6000 * 2^D / 32768, 8 octaves).
 
-    plain:   the first `plain_edits` edits of the batched-editing stream, one
-             svdag_edit call each (the reference's per-edit form); per-edit
-             wall latency, voxel edits/s.  The DAG is then rolled back and
-             collected so the batched leg starts from the same state.
-    batched: the whole stream (256 edits: 40% sphere-add, 40% sphere-remove,
-             20% box fill, r log-uniform in [8, 512], centres within +-1024 of
-             the surface) as 16-edit fused batches, each followed by a garbage
-             collection; voxel edits/s over the wall time of all batches and
-             collections; an edit's latency is its batch's wall time.
-    serving: open-loop Poisson arrivals at each rate (the same mix plus a 15%
-             sphere paint) for `serve_seconds`, with a primary trace of the
-             frame size every 16.7 ms; p50/p99 of submit -> root visible.
+    Every leg starts from a freshly built terrain (0.2 s at depth 15).
+    plain:    the 256-edit batched-editing stream (40% sphere-add, 40%
+              sphere-remove, 20% box fill, r log-uniform in [8, 512], centres
+              within +-1024 of the surface), one svdag_edit call per edit (the
+              reference's per-edit form); per-edit wall latency.
+    batched:  the same stream as 16 fused batches of 16, followed by one
+              garbage collection, all inside the timed region; an edit's
+              latency is its batch's wall time.  `gc_each_batch` repeats it
+              with a collection after every batch.
+    serving:  open-loop Poisson arrivals at each rate (the same mix plus a 15%
+              sphere paint) for `serve_seconds`, with a 1920x1080 primary trace
+              every 16.7 ms; p50/p99 of submit -> root visible.  The
+              saturation row submits its requests all at once (offered load far
+              above capacity) and measures the sustained service rate.
     voxel edits/s counts, per edit, the voxels whose unit box meets the tool's
     overlap predicate (exact per-row interval counting)."""
     import svdag_py
@@ -184,73 +186,94 @@ def editing_bench(args):
 
     D = args.edit_levels
     tp = workloads.terrain_params(D)
-    scale = 4 ** (D - 10)
-    t0 = time.perf_counter()
-    g = svdag_py.MyGPUHashDAG(D, 4, expected_items=int(6 * scale * 60000),
-                              workspace_items=min(1 << 28, int(2 * scale * 400000)))
-    g.build_terrain(**tp)
-    gc0 = g.garbage_collect()
-    build_s = time.perf_counter() - t0
-    log(f"[bench] depth {D} terrain: {build_s:.1f} s, {gc0.items_live} live items")
+    expected = int(64e6 * 4 ** (D - 15)) if D >= 12 else 0
+
+    def fresh():
+        t0 = time.perf_counter()
+        g = svdag_py.MyGPUHashDAG(D, 4, expected_items=expected,
+                                  workspace_items=min(1 << 28, int(2 * 4 ** max(D - 10, 0) * 400000)))
+        g.build_terrain(**tp)
+        st = g.garbage_collect()
+        return g, time.perf_counter() - t0, st
+
+    g, build_s, gc0 = fresh()
+    log(f"[bench] depth {D} terrain: {build_s:.2f} s, {gc0.items_live} live items")
 
     def h(x, z):
         return int(g.terrain_heights(x, z, 1, 1)[0, 0])
 
     edits = workloads.config3_edits(h, D, n=256, seed=6)
     counts = np.array([workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in edits], np.float64)
-
-    # plain per-edit form
-    n_plain = min(args.plain_edits, len(edits))
-    lat = []
-    t1 = time.perf_counter()
-    for e in edits[:n_plain]:
-        s0 = time.perf_counter()
-        g.edit(*e)
-        lat.append(time.perf_counter() - s0)
-    plain_s = time.perf_counter() - t1
-    for _ in range(g.history()[0]):
-        g.undo()
-    g.garbage_collect()
-    plain = {"edits": n_plain, "voxel_edits_per_s": float(counts[:n_plain].sum() / plain_s),
-             "edits_per_s": n_plain / plain_s,
-             "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
-             "latency_ms_p99": float(np.percentile(lat, 99) * 1e3)}
-    log(f"[bench] plain: {plain}")
-
-    # batched fused form, 16 edits per batch, GC after each batch
-    g.set_batch_mode(1)
-    tune_g, tune_ms = g.autotune(edits[:16])
-    g.garbage_collect()
-    blat = []
-    gpu_ms = 0.0
-    t2 = time.perf_counter()
-    for b in range(0, len(edits), 16):
-        s0 = time.perf_counter()
-        _, ms = g.edit_batch(edits[b:b + 16])
-        g.garbage_collect()
-        gpu_ms += ms
-        blat += [time.perf_counter() - s0] * len(edits[b:b + 16])
-    batch_s = time.perf_counter() - t2
-    batched = {"edits": len(edits), "batch": 16, "voxel_edits_per_s": float(counts.sum() / batch_s),
-               "edits_per_s": len(edits) / batch_s, "edit_gpu_ms_total": round(gpu_ms, 3),
-               "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
-               "latency_ms_p99": float(np.percentile(blat, 99) * 1e3),
-               "autotuned_lanes_per_item": tune_g,
-               "autotune_ms": {str(k): round(v, 3) for k, v in tune_ms.items()}}
-    batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
-    log(f"[bench] batched: {batched}")
-
-    serving = []
+    creq = {}
     cams = []
     for c in workloads.camera_path(D, 64):
         cams.append((c[0], svdag_py.look_at(c[0], c[1]), c[2], c[3], c[4]))
     for rate in args.serve_rates:
-        reqs, arr = workloads.config4_requests(h, D, rate, args.serve_seconds, seed=7)
+        creq[rate] = workloads.config4_requests(h, D, rate, args.serve_seconds, seed=7)
+    sat_reqs, _ = workloads.config4_requests(h, D, 2000.0, 1.0, seed=9)
+    g.close()
+
+    # plain per-edit form
+    g, _, _ = fresh()
+    lat = []
+    t1 = time.perf_counter()
+    for e in edits:
+        s0 = time.perf_counter()
+        g.edit(*e)
+        lat.append(time.perf_counter() - s0)
+    plain_s = time.perf_counter() - t1
+    plain = {"edits": len(edits), "voxel_edits_per_s": float(counts.sum() / plain_s),
+             "edits_per_s": len(edits) / plain_s,
+             "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+             "latency_ms_p99": float(np.percentile(lat, 99) * 1e3)}
+    log(f"[bench] plain: {plain}")
+    g.close()
+
+    def batched_leg(gc_each):
+        g, _, _ = fresh()
+        g.set_batch_mode(1)
+        tune = g.autotune(edits[:16])
+        g.garbage_collect()
+        blat = []
+        gpu_ms = 0.0
+        gc_ms = 0.0
+        t2 = time.perf_counter()
+        for b in range(0, len(edits), 16):
+            s0 = time.perf_counter()
+            _, ms = g.edit_batch(edits[b:b + 16])
+            gpu_ms += ms
+            if gc_each:
+                gc_ms += g.garbage_collect().gpu_ms
+            blat += [time.perf_counter() - s0] * len(edits[b:b + 16])
+        if not gc_each:
+            gc_ms += g.garbage_collect().gpu_ms
+        dt = time.perf_counter() - t2
+        g.close()
+        return {"voxel_edits_per_s": float(counts.sum() / dt), "edits_per_s": len(edits) / dt,
+                "edit_gpu_ms_total": round(gpu_ms, 3), "gc_gpu_ms_total": round(gc_ms, 3),
+                "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
+                "latency_ms_p99": float(np.percentile(blat, 99) * 1e3)}, tune
+
+    batched, (tune_g, tune_ms) = batched_leg(False)
+    batched.update({"edits": len(edits), "batch": 16, "gc": "once after the stream",
+                    "autotuned_lanes_per_item": tune_g,
+                    "autotune_ms": {str(k): round(v, 3) for k, v in tune_ms.items()}})
+    batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
+    each, _ = batched_leg(True)
+    each["speedup_vs_plain"] = each["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
+    batched["gc_each_batch"] = each
+    log(f"[bench] batched: {batched}")
+
+    serving = []
+    g, _, _ = fresh()
+    g.autotune(edits[:16])
+
+    def serve_row(name, reqs, arr, max_batch):
         vox = np.array([workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in reqs], np.float64)
-        info, st = g.serve_replay(reqs, arr, max_batch=64, frame=(1920, 1080), cameras=cams, allow_gc=True)
+        info, st = g.serve_replay(reqs, arr, max_batch=max_batch, frame=(1920, 1080), cameras=cams, allow_gc=True)
         latency = info["done_s"] - info["submit_s"]
         span = float(info["done_s"].max() - info["submit_s"].min())
-        row = {"rate_per_s": rate, "requests": len(reqs), "ok": int((info["status"] == 0).sum()),
+        row = {"rate_per_s": name, "requests": len(reqs), "ok": int((info["status"] == 0).sum()),
                "latency_ms_p50": float(np.percentile(latency, 50) * 1e3),
                "latency_ms_p99": float(np.percentile(latency, 99) * 1e3),
                "throughput_req_per_s": len(reqs) / span, "voxel_edits_per_s": float(vox.sum() / span),
@@ -260,9 +283,46 @@ def editing_bench(args):
         log(f"[bench] serving: {row}")
         serving.append(row)
         g.garbage_collect()
+        return row
+
+    for rate in args.serve_rates:
+        reqs, arr = creq[rate]
+        serve_row(rate, reqs, arr, 64)
+    sat = serve_row("saturation", sat_reqs, np.zeros(len(sat_reqs)), 64)
     g.close()
     return {"levels": D, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
-            "plain": plain, "batched": batched, "serving": serving}
+            "plain": plain, "batched": batched, "serving": serving,
+            "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"]}
+
+
+def cpu_edit_baseline(args):
+    """The oracle's per-edit path (a single-threaded restatement of the same
+    edit pipeline) on a bounded sample: the batched-editing stream generated
+    at depth 12, applied one edit at a time until `cpu_seconds` have passed."""
+    import oracle_py
+    import workloads
+
+    D = 12
+    tp = workloads.terrain_params(D)
+    o = oracle_py.OracleDag(D, 4)
+    o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"])
+    side = 1 << D
+    hts = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                    0, 0, side, side)
+    edits = workloads.config3_edits(lambda x, z: int(hts[z, x]), D, n=256, seed=6)
+    vox, n = 0, 0
+    t0 = time.perf_counter()
+    for e in edits:
+        o.edit(*e)
+        vox += workloads.voxel_edit_count(e[0], e[2], e[3], D)
+        n += 1
+        if time.perf_counter() - t0 > args.cpu_seconds:
+            break
+    dt = time.perf_counter() - t0
+    log(f"[bench] cpu edit baseline: {n} edits in {dt:.1f} s")
+    return {"value": round(vox / dt / 1e9, 3), "unit": "G voxel edits/s", "cores": 1, "kind": "port",
+            "sample": f"{n} edits of the batched-editing stream generated at depth 12 (same mix and radii), "
+                      f"oracle restatement of the GPU edit path, one edit at a time"}
 
 
 def cpu_baseline(items, finds, buckets, args):
@@ -322,6 +382,8 @@ def main():
         return
     if not args.no_editing:
         out["editing"] = editing_bench(args)
+        if not args.no_cpu_baseline:
+            out["editing"]["cpu_baseline"] = cpu_edit_baseline(args)
     print(json.dumps(out), flush=True)
 
 

@@ -417,9 +417,15 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
         uint32_t p = mix32(h) & capMask;
         uint32_t rep = w;
         while (true) {
+            // Equal items share a slot: read first, so duplicates (common
+            // surface patterns occur thousands of times) resolve without
+            // queueing on one atomic; only an empty slot is claimed by CAS.
             unsigned long long v = 0;
-            if (gl == 0)
-                v = atomicCAS(&ws.dedup[p], 0ull, key);
+            if (gl == 0) {
+                v = ld_relaxed64(&ws.dedup[p]);
+                if (v == 0ull)
+                    v = atomicCAS(&ws.dedup[p], 0ull, key);
+            }
             const uint32_t vlo = group_bcast<G>((uint32_t)v, 0);
             const uint32_t vhi = group_bcast<G>((uint32_t)(v >> 32), 0);
             if (vlo == 0u && vhi == 0u)

@@ -303,63 +303,81 @@ struct BinDesc {
     uint32_t cap;      // records the bin can hold
 };
 
-// One lane per needle; records of one (segment, XCD) key are reserved once
-// per wave (the lanes sharing the key take consecutive slots).
+// Bin counters sit 128 bytes apart so the few dozen of them do not share
+// L2 channels.
+constexpr uint32_t kBinPad = 32;
+
+// Four needles per thread per round; the block counts its needles per
+// (segment, XCD) key in LDS, reserves each key's run with one global atomic,
+// and writes the records.
 template <int MW>
 __global__ void __launch_bounds__(256) bin_needles(const TableDev* tables, uint32_t minWords, const uint32_t* words,
                                                    const uint32_t* segs, uint32_t nseg, uint32_t total,
                                                    const BinDesc* bins, uint32_t* binCount, uint32_t* binBuf,
                                                    uint32_t* overflow)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waveBase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint32_t i0 = waveBase; i0 < total; i0 += gridDim.x * blockDim.x) {
-        const uint32_t idx = i0 + lane;
-        const bool valid = idx < total;
-        uint32_t key = 0xFFFFFFFFu, s = 0, outIdx = 0, h = 0;
-        uint32_t w[MW];
-        if (valid) {
-            const uint32_t sg = seg_of(segs, nseg, idx);
-            s = segs[sg * 5 + 0];
-            const uint32_t local = idx - segs[sg * 5 + 4];
-            outIdx = segs[sg * 5 + 3] + local;
-            const uint32_t* p = words + segs[sg * 5 + 2] + (size_t)local * s;
+    constexpr uint32_t kPer = 4;
+    __shared__ uint32_t lcount[64 * kXcds];
+    __shared__ uint32_t lbase[64 * kXcds];
+    const uint32_t nkeys = nseg * kXcds;
+    const uint32_t chunk = blockDim.x * kPer;
+    for (uint32_t c0 = blockIdx.x * chunk; c0 < total; c0 += gridDim.x * chunk) {
+        for (uint32_t k = threadIdx.x; k < nkeys; k += blockDim.x)
+            lcount[k] = 0u;
+        __syncthreads();
+        uint32_t key[kPer], local[kPer], sz[kPer], outIdx[kPer], h[kPer];
+        uint32_t w[kPer][MW];
 #pragma unroll
-            for (int j = 0; j < MW; ++j) {
-                w[j] = (uint32_t)j < s ? p[j] : 0u;
-                if ((uint32_t)j < s)
-                    h ^= hash_part(w[j], (uint32_t)j);
-            }
-            const uint32_t nb = tables[s - minWords].numBuckets;
-            const uint32_t x = (uint32_t)(((uint64_t)(h % nb) * kXcds) / nb);
-            key = sg * kXcds + x;
-        }
-        unsigned long long pending = __ballot(valid);
-        while (pending) {
-            const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1u;
-            const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
-            const unsigned long long same = __ballot(key == lk) & pending;
-            uint32_t base = 0;
-            if (lane == leader)
-                base = atomicAdd(&binCount[lk], (uint32_t)__builtin_popcountll(same));
-            base = (uint32_t)__shfl((int)base, (int)leader, 64);
-            if ((same >> lane) & 1ull) {
-                const uint32_t pos = base + (uint32_t)__builtin_popcountll(same & ((1ull << lane) - 1ull));
-                const BinDesc b = bins[lk];
-                if (pos < b.cap) {
-                    uint32_t* r = binBuf + b.offset + (size_t)pos * (s + 2);
-                    r[0] = outIdx;
-                    r[1] = h;
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t idx = c0 + q * blockDim.x + threadIdx.x;
+            key[q] = 0xFFFFFFFFu;
+            sz[q] = 0;
+            h[q] = 0;
+            outIdx[q] = 0;
+            if (idx < total) {
+                const uint32_t sg = seg_of(segs, nseg, idx);
+                const uint32_t s = segs[sg * 5 + 0];
+                const uint32_t lidx = idx - segs[sg * 5 + 4];
+                const uint32_t* p = words + segs[sg * 5 + 2] + (size_t)lidx * s;
+                uint32_t hh = 0;
 #pragma unroll
-                    for (int j = 0; j < MW; ++j)
-                        if ((uint32_t)j < s)
-                            r[2 + j] = w[j];
-                } else {
-                    atomicOr(overflow, 1u);
+                for (int j = 0; j < MW; ++j) {
+                    w[q][j] = (uint32_t)j < s ? p[j] : 0u;
+                    if ((uint32_t)j < s)
+                        hh ^= hash_part(w[q][j], (uint32_t)j);
                 }
+                const uint32_t nb = tables[s - minWords].numBuckets;
+                const uint32_t x = (uint32_t)(((uint64_t)(hh % nb) * kXcds) / nb);
+                key[q] = sg * kXcds + x;
+                sz[q] = s;
+                h[q] = hh;
+                outIdx[q] = segs[sg * 5 + 3] + lidx;
+                local[q] = atomicAdd(&lcount[key[q]], 1u);
             }
-            pending &= ~same;
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < nkeys; k += blockDim.x)
+            lbase[k] = lcount[k] ? atomicAdd(&binCount[k * kBinPad], lcount[k]) : 0u;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            if (key[q] == 0xFFFFFFFFu)
+                continue;
+            const uint32_t pos = lbase[key[q]] + local[q];
+            const BinDesc b = bins[key[q]];
+            if (pos < b.cap) {
+                uint32_t* r = binBuf + b.offset + (size_t)pos * (sz[q] + 2);
+                r[0] = outIdx[q];
+                r[1] = h[q];
+#pragma unroll
+                for (int j = 0; j < MW; ++j)
+                    if ((uint32_t)j < sz[q])
+                        r[2 + j] = w[q][j];
+            } else {
+                atomicOr(overflow, 1u);
+            }
+        }
+        __syncthreads(); // lcount is cleared next round
     }
 }
 
@@ -380,7 +398,7 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
     for (uint32_t sg = 0; sg < nseg; ++sg) {
         const uint32_t s = segs[sg * 5 + 0];
         const uint32_t key = sg * kXcds + x;
-        const uint32_t cnt = min(binCount[key], bins[key].cap);
+        const uint32_t cnt = min(binCount[key * kBinPad], bins[key].cap);
         const uint32_t* rec0 = binBuf + bins[key].offset;
         const TableDev t = tables[s - minWords];
         const uint32_t stride = blocksPerXcd * groupsPerBlock * N;
@@ -497,7 +515,7 @@ static int launch_binned(const TableSet& ts, const uint32_t* words, const svdag_
         return SVDAG_E_INVALID;
     }
     if (!bs->count) {
-        SVDAG_HIP(hipMalloc(&bs->count, 64 * kXcds * 4));
+        SVDAG_HIP(hipMalloc(&bs->count, 64 * kXcds * kBinPad * 4));
         SVDAG_HIP(hipMalloc(&bs->desc, 64 * kXcds * sizeof(BinDesc)));
         SVDAG_HIP(hipMalloc(&bs->overflow, 4));
     }
@@ -510,11 +528,11 @@ static int launch_binned(const TableSet& ts, const uint32_t* words, const svdag_
         bs->bufWords = words_;
     }
     SVDAG_HIP(hipMemcpyAsync(bs->desc, desc.data(), desc.size() * sizeof(BinDesc), hipMemcpyHostToDevice, stream));
-    SVDAG_HIP(hipMemsetAsync(bs->count, 0, (size_t)nseg * kXcds * 4, stream));
+    SVDAG_HIP(hipMemsetAsync(bs->count, 0, (size_t)nseg * kXcds * kBinPad * 4, stream));
     SVDAG_HIP(hipMemsetAsync(bs->overflow, 0, 4, stream));
     const BinDesc* dDesc = reinterpret_cast<const BinDesc*>(bs->desc);
     const bool narrow = maxS <= 10;
-    const uint32_t gridBin = grid_for(n, 256, 8192);
+    const uint32_t gridBin = grid_for(n, 256 * 4, 4096);
     if (ev0)
         SVDAG_HIP(hipEventRecord(ev0, stream));
     if (narrow)

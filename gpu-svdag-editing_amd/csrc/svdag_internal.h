@@ -55,7 +55,7 @@ struct TableSet {
 struct BinScratch {
     uint32_t* buf = nullptr;
     size_t bufWords = 0;
-    uint32_t* count = nullptr;    // [64 segments][8]
+    uint32_t* count = nullptr;    // [64 segments][8], 128 bytes apart
     void* desc = nullptr;         // BinDesc[64][8]
     uint32_t* overflow = nullptr;
     void release();
