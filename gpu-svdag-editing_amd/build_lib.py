@@ -60,15 +60,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # The C++ facade demo links the library like any C++ caller would.
-    demo_src = os.path.join(HERE, "host", "facade_demo.cpp")
-    demo = os.path.join(LIBDIR, "facade_demo")
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
-           demo_src, "-o", demo + ".tmp", f"-L{LIBDIR}", "-lsvdag", f"-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"facade demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(demo + ".tmp", demo)
+    # The C++ facade programs link the library like any C++ caller would.
+    for name in ("facade_demo", "replay_demo"):
+        demo_src = os.path.join(HERE, "host", name + ".cpp")
+        demo = os.path.join(LIBDIR, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
+               demo_src, "-o", demo + ".tmp", f"-L{LIBDIR}", "-lsvdag", f"-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(demo + ".tmp", demo)
     if verbose:
         print(f"built {LIB}")
     return LIB
