@@ -7,6 +7,7 @@
 // size, words...], followed by the root's canonical id.  Allocation order and
 // handle values therefore do not affect it.  hash = FNV-1a 64 of the dump words.
 #include "dag_internal.h"
+#include <cstdio>
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -166,11 +167,273 @@ static void canonicalize(const HostDag& h, uint32_t root, const std::vector<std:
     dump.push_back(root == kSentinel ? kSentinel : idBelow.at(root));
 }
 
+static int canonical_dump(svdag_ctx* ctx, std::vector<uint32_t>& dump, std::vector<uint32_t>& cnt)
+{
+    HostDag h;
+    SVDAG_TRY(copy_dag_to_host(ctx, h));
+    std::vector<std::vector<Rec>> levels;
+    SVDAG_TRY(reachable(h, { ctx->root }, levels));
+    canonicalize(h, ctx->root, levels, dump, cnt);
+    return SVDAG_OK;
+}
+
+// Find-or-insert one level's items (unique among themselves) into the DAG's
+// tables with the bulk kernels; fills the handles in input order.
+static int insert_level(svdag_ctx* ctx, const std::vector<std::vector<uint32_t>>& items, bool leaf,
+                        std::vector<uint32_t>& handles)
+{
+    const size_t n = items.size();
+    handles.assign(n, kSentinel);
+    if (n == 0)
+        return SVDAG_OK;
+    uint32_t* segBuf = nullptr;
+    SVDAG_HIP(hipMalloc(&segBuf, 64 * 5 * 4));
+    struct Free {
+        std::vector<void*> p;
+        ~Free()
+        {
+            for (void* q : p)
+                if (q)
+                    (void)hipFree(q);
+        }
+    } fr { { segBuf } };
+    // Items grouped by size into segments; `order` maps segment rows back.
+    auto run = [&](const std::vector<size_t>& which, bool insert, std::vector<uint32_t>& ptrs) -> int {
+        std::vector<std::vector<size_t>> bySize(kMaxItemWords + 1);
+        for (size_t i : which)
+            bySize[items[i].size()].push_back(i);
+        std::vector<uint32_t> words;
+        std::vector<svdag_segment> segs;
+        std::vector<size_t> order;
+        for (uint32_t sz = 2; sz <= kMaxItemWords; ++sz) {
+            if (bySize[sz].empty())
+                continue;
+            segs.push_back({ sz, (uint32_t)bySize[sz].size(), (uint32_t)words.size(), (uint32_t)order.size() });
+            for (size_t i : bySize[sz]) {
+                words.insert(words.end(), items[i].begin(), items[i].end());
+                order.push_back(i);
+            }
+        }
+        ptrs.assign(items.size(), kNotFound);
+        if (order.empty())
+            return SVDAG_OK;
+        uint32_t *dw = nullptr, *dout = nullptr;
+        SVDAG_HIP(hipMalloc(&dw, words.size() * 4));
+        fr.p.push_back(dw);
+        SVDAG_HIP(hipMalloc(&dout, order.size() * 4));
+        fr.p.push_back(dout);
+        SVDAG_HIP(hipMemcpy(dw, words.data(), words.size() * 4, hipMemcpyHostToDevice));
+        SVDAG_TRY(launch_bulk(ctx->ts, dw, segs.data(), (uint32_t)segs.size(), dout, 16, insert, ctx->stream, segBuf,
+                              nullptr));
+        std::vector<uint32_t> out(order.size());
+        SVDAG_HIP(hipMemcpyAsync(out.data(), dout, out.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+        SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t k = 0; k < order.size(); ++k)
+            ptrs[order[k]] = out[k];
+        return SVDAG_OK;
+    };
+    std::vector<size_t> all(n);
+    for (size_t i = 0; i < n; ++i)
+        all[i] = i;
+    std::vector<uint32_t> found;
+    SVDAG_TRY(run(all, false, found));
+    std::vector<size_t> missing;
+    for (size_t i = 0; i < n; ++i)
+        if (found[i] == kNotFound)
+            missing.push_back(i);
+    for (int attempt = 0; attempt < 8 && !missing.empty(); ++attempt) {
+        std::vector<uint32_t> ins;
+        SVDAG_TRY(run(missing, true, ins));
+        bool oom = false;
+        for (auto& t : ctx->ts.tables) {
+            AllocState a {};
+            SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
+            oom |= a.oom != 0;
+        }
+        if (!oom) {
+            for (size_t i : missing)
+                found[i] = ins[i];
+            missing.clear();
+            break;
+        }
+        // Some inserts failed: grow the arenas and find-or-insert the rest
+        // again (the ones that landed are found this time).
+        {
+            FrameQuiesce q(ctx);
+            for (auto& t : ctx->ts.tables)
+                SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, 1ull << 25), ctx->stream));
+            SVDAG_TRY(ctx_refresh_tables(ctx));
+        }
+        std::vector<uint32_t> again;
+        SVDAG_TRY(run(missing, false, again));
+        std::vector<size_t> still;
+        for (size_t i : missing) {
+            if (again[i] != kNotFound)
+                found[i] = again[i];
+            else
+                still.push_back(i);
+        }
+        missing.swap(still);
+    }
+    if (!missing.empty()) {
+        set_error("import: slab arenas could not hold the DAG");
+        return SVDAG_E_OOM;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t sz = (uint32_t)items[i].size();
+        handles[i] = leaf ? leaf_handle(ctx->dag, found[i], sz) : node_handle(found[i], sz);
+    }
+    return SVDAG_OK;
+}
+
+static int import_dump(svdag_ctx* ctx, const uint32_t* d, size_t len)
+{
+    const DagDev& dag = ctx->dag;
+    size_t p = 0;
+    auto need = [&](size_t k) { return p + k <= len; };
+    std::vector<uint32_t> below; // handles of the level below, by canonical id
+    for (int L = (int)dag.leafLevel; L >= 0; --L) {
+        if (!need(2) || d[p] != (uint32_t)L) {
+            set_error("import: level " + std::to_string(L) + " missing or out of order");
+            return SVDAG_E_INVALID;
+        }
+        const uint32_t count = d[p + 1];
+        p += 2;
+        const bool leaf = (uint32_t)L == dag.leafLevel;
+        std::vector<std::vector<uint32_t>> items;
+        items.reserve(count);
+        for (uint32_t i = 0; i < count; ++i) {
+            if (!need(1)) {
+                set_error("import: truncated");
+                return SVDAG_E_INVALID;
+            }
+            const uint32_t sz = d[p++];
+            if (sz < 2 || sz > kMaxItemWords || !need(sz)) {
+                set_error("import: bad item size");
+                return SVDAG_E_INVALID;
+            }
+            std::vector<uint32_t> w(d + p, d + p + sz);
+            p += sz;
+            if (leaf) {
+                if (sz != leaf_size_from_words(w[0], w[1], dag.M) || (w[0] == 0 && w[1] == 0)) {
+                    set_error("import: leaf size does not match its mask");
+                    return SVDAG_E_INVALID;
+                }
+            } else {
+                if ((w[0] & 0xFFu) == 0 || sz != node_size_from_header(w[0]) || header_payload(w[0], dag.M) != 0) {
+                    set_error("import: node header does not match its size");
+                    return SVDAG_E_INVALID;
+                }
+                for (uint32_t j = 1; j < sz; ++j) {
+                    if (w[j] >= below.size()) {
+                        set_error("import: child id out of range");
+                        return SVDAG_E_INVALID;
+                    }
+                    w[j] = below[w[j]];
+                }
+            }
+            items.push_back(std::move(w));
+        }
+        std::vector<uint32_t> handles;
+        SVDAG_TRY(insert_level(ctx, items, leaf, handles));
+        below.swap(handles);
+    }
+    if (!need(1)) {
+        set_error("import: root id missing");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t rid = d[p++];
+    if (p != len || (rid != kSentinel && rid >= below.size())) {
+        set_error("import: bad root id or trailing words");
+        return SVDAG_E_INVALID;
+    }
+    ctx->root = rid == kSentinel ? kSentinel : below[rid];
+    SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->undo.clear();
+    ctx->redo.clear();
+    return SVDAG_OK;
+}
+
+constexpr uint32_t kFileMagic = 0x47445653u; // "SVDG"
+constexpr uint32_t kFileVersion = 1u;
+
 } // namespace svdag
 
 using namespace svdag;
 
 extern "C" {
+
+int svdag_import_canonical(svdag_ctx* ctx, const uint32_t* buf, size_t len_words)
+{
+    if (!ctx || (!buf && len_words))
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return import_dump(ctx, buf, len_words);
+}
+
+int svdag_save(svdag_ctx* ctx, const char* path)
+{
+    if (!ctx || !path)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::vector<uint32_t> dump, cnt;
+    SVDAG_TRY(canonical_dump(ctx, dump, cnt));
+    const uint64_t hash = fnv1a(dump);
+    const uint32_t head[6] = { kFileMagic, kFileVersion, ctx->dag.levels, ctx->dag.M, (uint32_t)(dump.size() & 0xFFFFFFFFu),
+                               (uint32_t)(dump.size() >> 32) };
+    FILE* f = std::fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("svdag_save: cannot open ") + path);
+        return SVDAG_E_INVALID;
+    }
+    bool ok = std::fwrite(head, 4, 6, f) == 6 && std::fwrite(dump.data(), 4, dump.size(), f) == dump.size()
+        && std::fwrite(&hash, 8, 1, f) == 1;
+    ok &= std::fclose(f) == 0;
+    if (!ok) {
+        set_error(std::string("svdag_save: write failed: ") + path);
+        return SVDAG_E_INVALID;
+    }
+    return SVDAG_OK;
+}
+
+int svdag_load(svdag_ctx* ctx, const char* path)
+{
+    if (!ctx || !path)
+        return SVDAG_E_INVALID;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("svdag_load: cannot open ") + path);
+        return SVDAG_E_INVALID;
+    }
+    uint32_t head[6] = {};
+    std::vector<uint32_t> dump;
+    uint64_t hash = 0;
+    bool ok = std::fread(head, 4, 6, f) == 6 && head[0] == kFileMagic && head[1] == kFileVersion;
+    const uint64_t n = ok ? ((uint64_t)head[5] << 32 | head[4]) : 0;
+    if (ok && n < (1ull << 34)) {
+        dump.resize(n);
+        ok = std::fread(dump.data(), 4, n, f) == n && std::fread(&hash, 8, 1, f) == 1;
+    } else {
+        ok = false;
+    }
+    std::fclose(f);
+    if (!ok) {
+        set_error(std::string("svdag_load: not a DAG file of this format: ") + path);
+        return SVDAG_E_INVALID;
+    }
+    if (head[2] != ctx->dag.levels || head[3] != ctx->dag.M) {
+        set_error("svdag_load: the file's depth / material bits differ from the context's");
+        return SVDAG_E_INVALID;
+    }
+    if (fnv1a(dump) != hash) {
+        set_error("svdag_load: checksum mismatch");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return import_dump(ctx, dump.data(), dump.size());
+}
+
 
 int svdag_export_tables(svdag_ctx* ctx, uint32_t* buf, size_t* len)
 {
@@ -206,12 +469,8 @@ int svdag_export_canonical(svdag_ctx* ctx, uint64_t* hash, uint32_t* buf, size_t
     if (!ctx || !hash)
         return SVDAG_E_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    HostDag h;
-    SVDAG_TRY(copy_dag_to_host(ctx, h));
-    std::vector<std::vector<Rec>> levels;
-    SVDAG_TRY(reachable(h, { ctx->root }, levels));
     std::vector<uint32_t> dump, cnt;
-    canonicalize(h, ctx->root, levels, dump, cnt);
+    SVDAG_TRY(canonical_dump(ctx, dump, cnt));
     *hash = fnv1a(dump);
     if (counts)
         for (uint32_t L = 0; L < 32; ++L)

@@ -104,7 +104,7 @@ EXPORTED = [
     "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
-    "svdag_export_tables",
+    "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load",
 ]
 
 _lib = None
@@ -388,6 +388,16 @@ class MyGPUHashDAG:
         check(lib().svdag_export_canonical(self.h, ctypes.byref(hsh), buf.ctypes.data_as(u32p), ctypes.byref(n),
                                            counts))
         return hsh.value, list(counts), buf
+
+    def import_canonical(self, dump):
+        d = np.ascontiguousarray(dump, np.uint32)
+        check(lib().svdag_import_canonical(self.h, d.ctypes.data_as(u32p), ctypes.c_size_t(d.size)))
+
+    def save(self, path):
+        check(lib().svdag_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        check(lib().svdag_load(self.h, os.fsencode(path)))
 
     def export_records(self):
         n = ctypes.c_size_t(0)

@@ -328,6 +328,20 @@ int svdag_export_canonical(svdag_ctx* ctx, uint64_t* out_hash, uint32_t* buf, si
  * Fails with SVDAG_E_INVALID if a reachable handle points at a free slot. */
 int svdag_export_tables(svdag_ctx* ctx, uint32_t* buf, size_t* len_words);
 
+/* Rebuild a DAG from a canonical dump (svdag_export_canonical's format): items
+ * are found or inserted level by level, bottom up, and the root is set; the
+ * undo/redo history is cleared.  The dump must come from a context with the
+ * same depth and material bits.  Checkpoint / resume (no reference
+ * counterpart for this format; the reference's MyGPUHashDAG::writeTo /
+ * readFrom, my_gpu_hash_dag.cu:157-177, dump raw slab memory of its own
+ * table layout). */
+int svdag_import_canonical(svdag_ctx* ctx, const uint32_t* buf, size_t len_words);
+/* File form: header {"SVDG", version 1, levels, material bits, dump length},
+ * the canonical dump, then its FNV-1a 64 checksum.  svdag_load checks all of
+ * them before importing. */
+int svdag_save(svdag_ctx* ctx, const char* path);
+int svdag_load(svdag_ctx* ctx, const char* path);
+
 #ifdef __cplusplus
 }
 #endif
