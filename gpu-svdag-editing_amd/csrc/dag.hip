@@ -653,6 +653,7 @@ static void ws_release(WsDev& ws)
 
 static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
 {
+    ++ctx->bufferGeneration; // captured wave graphs hold the old pointers
     WsDev old = ctx->ws;
     const bool hadState = old.state != nullptr;
     ws_release(ctx->ws);
@@ -726,11 +727,10 @@ uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
 }
 
 template <bool kMulti>
-static void launch_merge_level(svdag_ctx* ctx, const WsDev& ws, uint32_t L, uint64_t items)
+static void launch_merge_level(svdag_ctx* ctx, const WsDev& ws, uint32_t L, uint64_t items, uint32_t kCap = 8192)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
-    const uint32_t kCap = 8192;
     merge_prepare<kMulti><<<grid_for(2 * items, 256, kCap), 256, 0, st>>>(ws, dag, L);
     const uint32_t G = ctx->groupSize;
     const uint32_t gridG = grid_for(items, 256 / G, kCap);
@@ -779,18 +779,18 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
     return SVDAG_OK;
 }
 
-int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems)
+int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems,
+                        uint32_t kCap)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
-    const uint32_t kCap = 8192;
     for (uint32_t L = S; L < dag.leafLevel; ++L)
         svo_inner_level<true><<<grid_for(std::min<uint64_t>(levelItems[L], ws.capWork), 32, kCap), 256, 0, st>>>(
             ws, dag, eds, L);
     svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf), 4, kCap), 256, 0, st>>>(
         ws, dag, eds);
     for (int L = (int)dag.leafLevel; L >= (int)S; --L)
-        launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork));
+        launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork), kCap);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
@@ -800,6 +800,7 @@ static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
 {
     if (n <= ctx->batchCap)
         return SVDAG_OK;
+    ++ctx->bufferGeneration;
     for (void* p : { (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
                      (void*)ctx->dWaveIdx })
@@ -852,7 +853,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         if (needState && !ctx->ws.state)
             SVDAG_TRY(ws_alloc(ctx, ctx->ws.capWork, true));
         uint32_t S = 0;
-        if (ctx->batchMode == 1 && n - first > 1) {
+        if (ctx->batchMode >= SVDAG_BATCH_FUSED && n - first > 1) {
             S = plan_batch(ctx, eds, first, n, steps, alone);
         } else {
             steps.clear();
@@ -1094,6 +1095,7 @@ int svdag_destroy(svdag_ctx* ctx)
     if (ctx->server)
         (void)svdag_server_stop(ctx, nullptr);
     (void)hipStreamSynchronize(ctx->stream);
+    release_wave_graphs(ctx);
     for (uint32_t* p : ctx->gcMasks)
         if (p)
             (void)hipFree(p);
@@ -1291,7 +1293,7 @@ int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, sv
 
 int svdag_set_batch_mode(svdag_ctx* ctx, uint32_t mode)
 {
-    if (!ctx || mode > SVDAG_BATCH_FUSED) {
+    if (!ctx || mode > SVDAG_BATCH_FUSED_GRAPH) {
         set_error("svdag_set_batch_mode: bad arguments");
         return SVDAG_E_INVALID;
     }

@@ -130,7 +130,17 @@ struct svdag_ctx {
     uint2* dWaves = nullptr;
     uint32_t* dWaveIdx = nullptr;
     void* hFused = nullptr;     // pinned: SeedDesc[batchCap] then uint2[batchCap]
-    uint32_t batchMode = 1;     // 0: edits one after another, 1: fused waves
+    uint32_t batchMode = 1;     // 0: edits one after another, 1: fused waves, 2: fused waves as HIP graphs
+    // Captured wave graphs, keyed by (split level, lanes per item, buffer
+    // generation); the generation changes whenever a buffer a graph points
+    // at is reallocated.
+    struct WaveGraph {
+        uint32_t S, groupSize;
+        uint64_t generation;
+        hipGraphExec_t exec;
+    };
+    std::vector<WaveGraph> waveGraphs;
+    uint64_t bufferGeneration = 0;
     // Tracer output buffer.
     void* dHits = nullptr;
     size_t hitCap = 0;
@@ -176,7 +186,8 @@ uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L);
 int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, const WsDev& ws);
 // Levels S .. leaf of a fused wave: SVO build and merge with per-item editors
 // (dag.hip).  levelItems[L] bounds the items of level L (grid sizing only).
-int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems);
+int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems,
+                        uint32_t gridCap = 8192);
 
 // Fused multi-edit waves (fused.hip).
 struct BatchStep {
@@ -189,4 +200,5 @@ struct BatchStep {
 uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32_t n, std::vector<BatchStep>& steps,
                     const std::vector<uint8_t>& alone);
 int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S);
+void release_wave_graphs(svdag_ctx* ctx);
 }

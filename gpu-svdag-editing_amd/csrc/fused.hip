@@ -461,7 +461,10 @@ uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32
     return (uint32_t)S;
 }
 
-int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S)
+// The wave's launch sequence.  levelItems == nullptr: fixed grids (every
+// kernel walks its device-side count with a grid-stride loop), as captured
+// into a graph that serves any wave of the same split level.
+static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, uint32_t seedLanes)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
@@ -470,18 +473,75 @@ int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, ui
     ws.edFull = ctx->dEdFull;
     const FusedDev f { ctx->dWaves, ctx->dWaveIdx, ctx->dSeeds, ctx->dSeedWork, S };
     wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
-    const uint32_t lanes = 8u * (step.end - step.first);
-    fused_seed<<<(lanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
-    uint64_t levelItems[kMaxLevels] {};
-    for (uint32_t k = step.first; k < step.end; ++k) {
-        const Bounds b = edit_bounds(ctx, hEds[k]);
-        for (uint32_t L = S; L <= dag.leafLevel; ++L)
-            levelItems[L] += level_bound(ctx, b, L);
+    fused_seed<<<(seedLanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
+    if (levelItems) {
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems));
+    } else {
+        uint64_t fixed[kMaxLevels];
+        for (uint32_t L = 0; L < kMaxLevels; ++L)
+            fixed[L] = ~0ull;
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, fixed, 2048));
     }
-    SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems));
     fused_spine<<<1, 64, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
+}
+
+void release_wave_graphs(svdag_ctx* ctx)
+{
+    for (auto& g : ctx->waveGraphs)
+        (void)hipGraphExecDestroy(g.exec);
+    ctx->waveGraphs.clear();
+}
+
+static int wave_graph(svdag_ctx* ctx, uint32_t S, hipGraphExec_t& out)
+{
+    for (auto it = ctx->waveGraphs.begin(); it != ctx->waveGraphs.end();) {
+        if (it->generation != ctx->bufferGeneration) {
+            (void)hipGraphExecDestroy(it->exec);
+            it = ctx->waveGraphs.erase(it);
+            continue;
+        }
+        if (it->S == S && it->groupSize == ctx->groupSize) {
+            out = it->exec;
+            return SVDAG_OK;
+        }
+        ++it;
+    }
+    hipGraph_t graph = nullptr;
+    SVDAG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_wave(ctx, S, nullptr, 8u * kMaxWave);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != SVDAG_OK) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    SVDAG_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SVDAG_HIP(ie);
+    ctx->waveGraphs.push_back({ S, ctx->groupSize, ctx->bufferGeneration, exec });
+    out = exec;
+    return SVDAG_OK;
+}
+
+int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S)
+{
+    if (ctx->batchMode == SVDAG_BATCH_FUSED_GRAPH) {
+        hipGraphExec_t exec = nullptr;
+        SVDAG_TRY(wave_graph(ctx, S, exec));
+        SVDAG_HIP(hipGraphLaunch(exec, ctx->stream));
+        return SVDAG_OK;
+    }
+    uint64_t levelItems[kMaxLevels] {};
+    for (uint32_t k = step.first; k < step.end; ++k) {
+        const Bounds b = edit_bounds(ctx, hEds[k]);
+        for (uint32_t L = S; L <= ctx->dag.leafLevel; ++L)
+            levelItems[L] += level_bound(ctx, b, L);
+    }
+    return enqueue_wave(ctx, S, levelItems, 8u * (step.end - step.first));
 }
 
 } // namespace svdag

@@ -310,7 +310,8 @@ class MyGPUHashDAG:
         return r.group_size, dict(zip((4, 8, 16, 32), list(r.ms)))
 
     def set_batch_mode(self, mode):
-        """0: edits of a batch one after another; 1: fused waves (default)."""
+        """0: edits of a batch one after another; 1: fused waves (default);
+        2: fused waves replayed from captured HIP graphs."""
         check(lib().svdag_set_batch_mode(self.h, int(mode)))
 
     def undo(self):

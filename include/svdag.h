@@ -176,8 +176,11 @@ int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* edits, uint32_t n, s
  *                           regions do not collide share one edit-SVO build
  *                           and one merge, and a single-wave kernel replays
  *                           the top levels edit by edit.  Copy edits run on
- *                           their own. */
-enum { SVDAG_BATCH_SEQUENTIAL = 0, SVDAG_BATCH_FUSED = 1 };
+ *                           their own;
+ *   SVDAG_BATCH_FUSED_GRAPH the fused form with each wave's launch sequence
+ *                           captured once as a HIP graph (per split level and
+ *                           lanes per item) and replayed. */
+enum { SVDAG_BATCH_SEQUENTIAL = 0, SVDAG_BATCH_FUSED = 1, SVDAG_BATCH_FUSED_GRAPH = 2 };
 int svdag_set_batch_mode(svdag_ctx* ctx, uint32_t mode);
 
 /* Autotuner (no reference counterpart; the reference fixes warp-cooperative

@@ -387,3 +387,47 @@ def test_autotune_keeps_the_dag_and_later_edits_exact():
         o.edit(*e)
     _same(a, o)
     a.close()
+
+
+@pytest.mark.gpu
+def test_graph_waves_replay_exactly_across_batches():
+    # Fused waves replayed from captured HIP graphs: several batches reuse one
+    # graph per split level, with eager work (single edits, a collection that
+    # clears the history) between the replays; every batch's states match the
+    # sequential form edit by edit.
+    edits = _mixed_stream(L10, seed=61)
+    a, _ = _pair()
+    b, o = _pair()
+    a.set_batch_mode(0)
+    b.set_batch_mode(2)
+    chunks = [edits[0:12], edits[12:24], edits[24:36], edits[36:]]
+    for i, chunk in enumerate(chunks):
+        sa, _ = a.edit_batch(chunk)
+        sb, _ = b.edit_batch(chunk)
+        for s1, s2 in zip(sa, sb):
+            assert s1.applied == s2.applied
+            assert list(s1.svo_items) == list(s2.svo_items)
+        assert b.history() == a.history()
+        for _ in range(min(len(chunk), a.history()[0])):
+            assert a.canonical() == b.canonical()
+            a.undo()
+            b.undo()
+        for _ in range(a.history()[1]):
+            a.redo()
+            b.redo()
+        assert a.canonical() == b.canonical()
+        # eager work between replays
+        extra = (svdag_py.SPHERE_ADD, 3, (300.0 + 40 * i, 560.0, 420.0), 25.0)
+        a.edit(*extra)
+        b.edit(*extra)
+        if i == 1:
+            a.garbage_collect()
+            b.garbage_collect()
+    for i, chunk in enumerate(chunks):
+        for e in chunk:
+            o.edit(*e)
+        o.edit(svdag_py.SPHERE_ADD, 3, (300.0 + 40 * i, 560.0, 420.0), 25.0)
+    _same(b, o)
+    assert a.canonical() == b.canonical()
+    a.close()
+    b.close()
