@@ -30,7 +30,7 @@ struct HostDag {
         const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
         if (s < 2 || s - 2 >= items.size() || slab >= fresh[s - 2] || slot >= kSlots)
             return nullptr;
-        const uint32_t tag = (tags[s - 2][(size_t)slab * kTagWords + slot / 4] >> (8 * (slot % 4))) & 0xFFu;
+        const uint32_t tag = (tags[s - 2][(size_t)slab * kTagWords + slot_word(slot)] >> (8 * (slot % 4))) & 0xFFu;
         if (!tag)
             return nullptr;
         return items[s - 2].data() + ((size_t)slab * kSlots + slot) * s;

@@ -125,16 +125,19 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint32_t* const
     }
 }
 
-// Thirty-two lanes per bucket chain, two chains per wave: lane w holds tag
-// word w of the slab (one coalesced 128-byte read), clears the tags of
-// unmarked items, and the chain link (word 31) is broadcast to walk on.
-// Wholly unmarked overflow slabs are unlinked and freed.  The loop over
+// Thirty-two lanes per bucket chain, two chains per wave: lane w holds word w
+// of the slab's tag line (one coalesced 128-byte read), clears the tags of
+// unmarked items, and the chain link (word 15 of either half) is broadcast
+// to walk on.  Overflow slabs with no marked item in either half are
+// unlinked (both next copies of the predecessor) and freed.  The loop over
 // chains is wave-uniform so the cross-lane operations see every lane.
 __global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t w = lane & 31u;
     const uint32_t half = lane >> 5;
+    const uint32_t hf = w / kHalfWords, j = w % kHalfWords;
+    const bool tagWord = j != kHalfNext;
     const uint32_t chainsPerBlock = blockDim.x / 32u;
     const uint32_t stride = gridDim.x * chainsPerBlock;
     uint32_t freedItems = 0, freedSlabs = 0, live = 0;
@@ -147,8 +150,9 @@ __global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
             if (slab != kNoSlab) {
                 uint32_t* line = t.tags + (size_t)slab * kTagWords;
                 v = line[w];
-                if (w < kNextWord) {
-                    const uint32_t a = (t.active[(size_t)slab * 4 + (w >> 3)] >> ((w * 4u) & 31u)) & 0xFu;
+                if (tagWord) {
+                    const uint32_t first = slot_of(hf, j, 0);
+                    const uint32_t a = (t.active[(size_t)slab * 4 + (first >> 5)] >> (first & 31u)) & 0xFu;
                     for (uint32_t byte = 0; byte < 4; ++byte) {
                         const uint32_t tag = (v >> (8 * byte)) & 0xFFu;
                         if (!tag)
@@ -165,11 +169,12 @@ __global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
                 }
             }
             const uint32_t next = (uint32_t)__shfl((int)v, (int)(half * 32u + kNextWord), 64);
-            const unsigned long long kept = __ballot(w < kNextWord && keep != 0u);
+            const unsigned long long kept = __ballot(tagWord && keep != 0u);
             const bool anyKept = ((kept >> (half * 32u)) & 0xFFFFFFFFull) != 0ull;
             if (slab != kNoSlab) {
                 if (slab != b0 && !anyKept) {
                     if (w == 0) {
+                        t.tags[(size_t)prev * kTagWords + kHalfNext] = next;
                         t.tags[(size_t)prev * kTagWords + kNextWord] = next;
                         slab_free(t, slab);
                         ++freedSlabs;

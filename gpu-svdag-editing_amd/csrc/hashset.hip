@@ -20,7 +20,7 @@ __global__ void table_init_heads(TableDev t)
 {
     const size_t n = (size_t)t.numBuckets * kTagWords;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        t.tags[i] = ((i & (kTagWords - 1)) == kNextWord) ? kNoSlab : 0u;
+        t.tags[i] = ((i % kHalfWords) == kHalfNext) ? kNoSlab : 0u;
         if ((i & (kTagWords - 1)) < 4)
             t.active[(i / kTagWords) * 4 + (i & 3)] = 0u;
     }
@@ -187,7 +187,6 @@ __global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* ta
                                                               const uint32_t* words, const uint32_t* segs, uint32_t nseg,
                                                               uint32_t total, uint32_t* out)
 {
-    constexpr int W = 32 / G;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gl = lane & (G - 1);
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
@@ -195,7 +194,7 @@ __global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* ta
     const uint32_t stride = gridDim.x * groupsPerBlock * N;
     for (uint32_t base = (blockIdx.x * groupsPerBlock + threadIdx.x / G) * N; base < total; base += stride) {
         Needle<G, MW> n[N];
-        uint32_t s[N], h[N], slab[N], outIdx[N], tag4[N];
+        uint32_t s[N], h[N], hf[N], slab[N], outIdx[N];
         const TableDev* tp[N];
         bool valid[N];
 #pragma unroll
@@ -212,57 +211,17 @@ __global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* ta
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             h[k] = needle_hash(n[k], s[k], gl);
-            tag4[k] = make_tag(h[k]) * 0x01010101u;
+            hf[k] = half_of(h[k]);
             slab[k] = h[k] % tp[k]->numBuckets;
         }
-        uint32_t words_[N][W];
+        HalfLine<G> lines[N];
 #pragma unroll
         for (int k = 0; k < N; ++k)
-            tagline_load<G>(words_[k], tp[k]->tags + (size_t)slab[k] * kTagWords, gl);
+            halfline_load<G>(lines[k], tp[k]->tags + (size_t)slab[k] * kTagWords + hf[k] * kHalfWords, gl);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            uint32_t r = kNotFound;
-            const TableDev t = *tp[k];
-            uint32_t sl = slab[k];
-            bool first = true;
-            while (true) {
-                uint32_t wv[W];
-                if (first) {
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        wv[i] = words_[k][i];
-                } else {
-                    tagline_load<G>(wv, t.tags + (size_t)sl * kTagWords, gl);
-                }
-                first = false;
-                uint32_t cand = 0;
-#pragma unroll
-                for (int i = 0; i < W; ++i)
-                    if (gl * W + i != kNextWord)
-                        cand |= compact_bytes(zero_bytes(wv[i] ^ tag4[k])) << (4 * i);
-                const uint32_t next = group_bcast<G>(wv[W - 1], G - 1);
-                bool done = false;
-                while (true) {
-                    const uint32_t bal = group_ballot<G>(cand != 0, gbase);
-                    if (!bal)
-                        break;
-                    const uint32_t leader = __ffs(bal) - 1u;
-                    const uint32_t mySlot = gl * W * 4 + (cand ? (__ffs(cand) - 1u) : 0u);
-                    const uint32_t slot = group_bcast<G>(mySlot, leader);
-                    if (gl == leader)
-                        cand &= cand - 1u;
-                    const uint32_t* item = t.items + ((size_t)sl * kSlots + slot) * s[k];
-                    const bool eq = needle_equal_lane(n[k], item, s[k], gl);
-                    if (group_ballot<G>(!eq, gbase) == 0) {
-                        r = (sl << kSlotBits) | slot;
-                        done = true;
-                        break;
-                    }
-                }
-                if (done || next == kNoSlab)
-                    break;
-                sl = next;
-            }
+            const uint32_t r = probe_chain<G, MW>(*tp[k], n[k], s[k], make_tag(h[k]) * 0x01010101u, hf[k], slab[k],
+                                                  lines[k], gl, gbase);
             if (gl == 0 && valid[k])
                 out[outIdx[k]] = r;
         }
@@ -387,7 +346,6 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
                                                    uint32_t nseg, const BinDesc* bins, const uint32_t* binCount,
                                                    const uint32_t* binBuf, uint32_t* out)
 {
-    constexpr int W = 32 / G;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gl = lane & (G - 1);
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
@@ -404,7 +362,7 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
         const uint32_t stride = blocksPerXcd * groupsPerBlock * N;
         for (uint32_t base = (bx * groupsPerBlock + threadIdx.x / G) * N; base < cnt; base += stride) {
             Needle<G, MW> n[N];
-            uint32_t h[N], slab[N], outIdx[N], tag4[N];
+            uint32_t h[N], hf[N], slab[N], outIdx[N];
             bool valid[N];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
@@ -414,56 +372,17 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
                 outIdx[k] = r[0];
                 h[k] = r[1];
                 needle_load(n[k], r + 2, s, gl);
-                tag4[k] = make_tag(h[k]) * 0x01010101u;
+                hf[k] = half_of(h[k]);
                 slab[k] = h[k] % t.numBuckets;
             }
-            uint32_t lines[N][W];
+            HalfLine<G> lines[N];
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                tagline_load<G>(lines[k], t.tags + (size_t)slab[k] * kTagWords, gl);
+                halfline_load<G>(lines[k], t.tags + (size_t)slab[k] * kTagWords + hf[k] * kHalfWords, gl);
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                uint32_t r = kNotFound;
-                uint32_t sl = slab[k];
-                bool first = true;
-                while (true) {
-                    uint32_t wv[W];
-                    if (first) {
-#pragma unroll
-                        for (int i = 0; i < W; ++i)
-                            wv[i] = lines[k][i];
-                    } else {
-                        tagline_load<G>(wv, t.tags + (size_t)sl * kTagWords, gl);
-                    }
-                    first = false;
-                    uint32_t cand = 0;
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        if (gl * W + i != kNextWord)
-                            cand |= compact_bytes(zero_bytes(wv[i] ^ tag4[k])) << (4 * i);
-                    const uint32_t next = group_bcast<G>(wv[W - 1], G - 1);
-                    bool done = false;
-                    while (true) {
-                        const uint32_t bal = group_ballot<G>(cand != 0, gbase);
-                        if (!bal)
-                            break;
-                        const uint32_t leader = __ffs(bal) - 1u;
-                        const uint32_t mySlot = gl * W * 4 + (cand ? (__ffs(cand) - 1u) : 0u);
-                        const uint32_t slot = group_bcast<G>(mySlot, leader);
-                        if (gl == leader)
-                            cand &= cand - 1u;
-                        const uint32_t* item = t.items + ((size_t)sl * kSlots + slot) * s;
-                        const bool eq = needle_equal_lane(n[k], item, s, gl);
-                        if (group_ballot<G>(!eq, gbase) == 0) {
-                            r = (sl << kSlotBits) | slot;
-                            done = true;
-                            break;
-                        }
-                    }
-                    if (done || next == kNoSlab)
-                        break;
-                    sl = next;
-                }
+                const uint32_t r = probe_chain<G, MW>(t, n[k], s, make_tag(h[k]) * 0x01010101u, hf[k], slab[k],
+                                                      lines[k], gl, gbase);
                 if (gl == 0 && valid[k])
                     out[outIdx[k]] = r;
             }
@@ -767,10 +686,11 @@ int svdag_hashset_create(uint32_t minWords, uint32_t maxWords, const uint32_t* b
     for (uint32_t s = minWords; s <= maxWords; ++s) {
         const uint32_t nb = std::max(1u, buckets[s - minWords]);
         const uint64_t res = reserve ? reserve[s - minWords] : 0;
-        // Overflow reserve: every reserved item may need a slab of its own at
-        // worst, but a chain only overflows past 124 items, so size the
-        // reserve at one slab per 123 items plus one eighth of the buckets.
-        uint64_t cap = (uint64_t)nb + res / 123 + nb / 8 + 64;
+        // Overflow reserve: a chain grows when the half an item hashes to is
+        // full (60 slots), so equal-hash items need a slab per 59 at worst;
+        // at 96 items per bucket about one half in twenty overflows, so a
+        // quarter of the bucket count covers the tail.
+        uint64_t cap = (uint64_t)nb + res / (kHalfSlots - 1) + nb / 4 + 64;
         cap = std::min<uint64_t>(cap, (1ull << (32 - kSlotBits)));
         int r = table_alloc(hs->ts.tables[s - minWords], s, nb, (uint32_t)cap, hs->stream);
         if (r != SVDAG_OK) {

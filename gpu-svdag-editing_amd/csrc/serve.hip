@@ -249,7 +249,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                     items += s.svo_items[L];
             if (read_memory(ctx, after) == SVDAG_OK && items > 0 && after.slabs_used >= usedBefore) {
                 const double r = (double)(after.slabs_used - usedBefore) / (double)items;
-                srv->slabsPerItem = std::max(1.0 / 124.0, 0.5 * srv->slabsPerItem + 0.5 * r);
+                srv->slabsPerItem = std::max(1.0 / (double)kSlots, 0.5 * srv->slabsPerItem + 0.5 * r);
             }
         }
         srv->published.store(ctx->root);

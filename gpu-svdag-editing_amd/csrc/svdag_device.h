@@ -11,21 +11,25 @@
 // share one table (my_gpu_hash_dag.h:173, 240).
 //
 // Table layout (own design for wave64, not the reference's 32-slot slab):
-//   slab  = one 128-byte tag line (31 words x 4 one-byte tags = 124 slots,
-//           word 31 = next slab id) kept in a dense tag array, plus a separate
-//           item array [slab][124][itemWords].  Head slabs 0..numBuckets-1 are
-//           the buckets; overflow slabs come from the slab allocator.
-//   The dense tag array (128 B per bucket) is what every probe touches first;
-//   keeping it apart from the item words keeps it cache resident.
+//   slab  = one 128-byte tag line in a dense tag array plus a separate item
+//           array [slab][120][itemWords].  The line is two 64-byte halves;
+//           half h holds 15 words x 4 one-byte tags (slots 60h .. 60h+59) and
+//           a copy of the next slab id in its word 15.  An item lives in the
+//           half its hash selects, so a probe reads 64 bytes per slab, not
+//           128.  Head slabs 0..numBuckets-1 are the buckets; overflow slabs
+//           come from the slab allocator and serve both halves.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace svdag {
 
-constexpr uint32_t kSlots = 124;        // item slots per slab
+constexpr uint32_t kSlots = 120;        // item slots per slab (two halves of 60)
+constexpr uint32_t kHalfSlots = 60;
 constexpr uint32_t kTagWords = 32;      // words per tag line (128 B)
-constexpr uint32_t kNextWord = 31;      // tag-line word that holds the next slab id
+constexpr uint32_t kHalfWords = 16;     // words per half line (64 B)
+constexpr uint32_t kHalfNext = 15;      // word of a half that holds the next slab id
+constexpr uint32_t kNextWord = 31;      // the second copy (half 1); both are kept equal
 constexpr uint32_t kSlotBits = 7;       // pointer = slab << 7 | slot
 constexpr uint32_t kNoSlab = 0xFFFFFFFFu;
 constexpr uint32_t kNotFound = 0xFFFFFFFFu;
@@ -47,6 +51,22 @@ __host__ __device__ inline uint32_t make_tag(uint32_t h)
 {
     const uint32_t t = (h * 0x9E3779B1u) >> 24;
     return t ? t : 1u;
+}
+// Half of the tag line an item lives in (independent of the bucket, h %
+// numBuckets, and of the tag, the top byte of h * 0x9E3779B1).
+__host__ __device__ inline uint32_t half_of(uint32_t h)
+{
+    return (h * 0x85EBCA6Bu) >> 31;
+}
+// Slot of tag byte b of word j (0..14) of half hf.
+__host__ __device__ inline uint32_t slot_of(uint32_t hf, uint32_t j, uint32_t b)
+{
+    return hf * kHalfSlots + j * 4u + b;
+}
+// Tag-line word and byte of a slot.
+__host__ __device__ inline uint32_t slot_word(uint32_t slot)
+{
+    return (slot / kHalfSlots) * kHalfWords + (slot % kHalfSlots) / 4u;
 }
 __host__ __device__ inline uint32_t mix32(uint32_t h)
 {
@@ -149,20 +169,23 @@ __device__ inline void slab_free(const TableDev& t, uint32_t id)
     }
 }
 
-// The chain of `bucket` was seen full with head successor `observed`.  The
-// bucket's grow flag (taken by CAS, never waited on) admits one grower at a
-// time: it allocates a slab and links it right after the head, which cannot
-// race because only flag holders move a head's successor during inserts.
-// Everyone else rescans.  Returns the slab to continue with: the new slab,
-// the head (rescan), or kNoSlab when the arena is exhausted.
-__device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32_t observed)
+// The chain of `bucket` was seen full with head successor `observed` (read
+// from the head's half-`hf` next word).  The bucket's grow flag (taken by
+// CAS, never waited on) admits one grower at a time: it allocates a slab,
+// clears both halves, and links it right after the head by writing both
+// next copies; only flag holders move a head's successor, so the two copies
+// differ only while a grower holds the flag.  Everyone else rescans.
+// Returns the slab to continue with: the new slab, the head (rescan), or
+// kNoSlab when the arena is exhausted.
+__device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32_t observed, uint32_t hf)
 {
-    uint32_t* headNext = t.tags + (size_t)bucket * kTagWords + kNextWord;
-    if (ld_relaxed(headNext) != observed)
+    uint32_t* head = t.tags + (size_t)bucket * kTagWords;
+    uint32_t* mine = head + hf * kHalfWords + kHalfNext;
+    if (ld_relaxed(mine) != observed)
         return bucket;
     if (atomicCAS(&t.grow[bucket], 0u, 1u) != 0u)
         return bucket;
-    if (ld_relaxed(headNext) != observed) {
+    if (ld_relaxed(mine) != observed || ld_relaxed(head + kHalfNext) != ld_relaxed(head + kNextWord)) {
         atomicExch(&t.grow[bucket], 0u);
         return bucket;
     }
@@ -172,13 +195,13 @@ __device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32
         return kNoSlab;
     }
     uint32_t* nl = t.tags + (size_t)id * kTagWords;
-    for (uint32_t i = 0; i < kNextWord; ++i)
-        st_relaxed(nl + i, 0u);
-    st_relaxed(nl + kNextWord, observed);
+    for (uint32_t i = 0; i < kTagWords; ++i)
+        st_relaxed(nl + i, (i % kHalfWords) == kHalfNext ? observed : 0u);
     for (uint32_t i = 0; i < 4; ++i)
         t.active[(size_t)id * 4 + i] = 0u;
     __threadfence();
-    atomicExch(headNext, id);
+    atomicExch(head + kHalfNext, id);
+    atomicExch(head + kNextWord, id);
     __threadfence();
     atomicExch(&t.grow[bucket], 0u);
     return id;
@@ -255,57 +278,87 @@ __device__ inline bool needle_equal_lane(const Needle<G, MW>& n, const uint32_t*
     return eq;
 }
 
-// Load this lane's W = 32/G words of a tag line.
+// A group's view of one half line: lane gl holds words gl*HW .. gl*HW+HW-1
+// of the 16 (groups wider than 16 lanes leave the upper lanes empty).
 template <int G>
-__device__ inline void tagline_load(uint32_t (&words)[32 / G], const uint32_t* line, uint32_t gl)
+struct HalfLine {
+    static constexpr int HW = G >= 16 ? 1 : 16 / G;
+    uint32_t w[HW];
+};
+
+template <int G>
+__device__ inline void halfline_load(HalfLine<G>& L, const uint32_t* half, uint32_t gl)
 {
-    constexpr int W = 32 / G;
-    const uint32_t* p = line + gl * W;
-    if constexpr (W == 1) {
-        words[0] = p[0];
-    } else if constexpr (W == 2) {
+    constexpr int HW = HalfLine<G>::HW;
+    if (gl * HW >= kHalfWords) {
+#pragma unroll
+        for (int i = 0; i < HW; ++i)
+            L.w[i] = 0u;
+        return;
+    }
+    const uint32_t* p = half + gl * HW;
+    if constexpr (HW == 1) {
+        L.w[0] = p[0];
+    } else if constexpr (HW == 2) {
         const uint2 v = *reinterpret_cast<const uint2*>(p);
-        words[0] = v.x;
-        words[1] = v.y;
+        L.w[0] = v.x;
+        L.w[1] = v.y;
     } else {
 #pragma unroll
-        for (int q = 0; q < W / 4; ++q) {
+        for (int q = 0; q < HW / 4; ++q) {
             const uint4 v = reinterpret_cast<const uint4*>(p)[q];
-            words[4 * q + 0] = v.x;
-            words[4 * q + 1] = v.y;
-            words[4 * q + 2] = v.z;
-            words[4 * q + 3] = v.w;
+            L.w[4 * q + 0] = v.x;
+            L.w[4 * q + 1] = v.y;
+            L.w[4 * q + 2] = v.z;
+            L.w[4 * q + 3] = v.w;
         }
     }
 }
 
-// Group-cooperative find (G in {4, 8, 16, 32}): the tag line is read once by
-// the whole group (W words per lane), candidate slots are resolved one at a
-// time with the group comparing the candidate's words in parallel.
-template <int G, int MW>
-__device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
-                                            uint32_t gl, uint32_t gbase)
+// The half line's next slab id (word 15), broadcast to the group.
+template <int G>
+__device__ inline uint32_t halfline_next(const HalfLine<G>& L)
 {
-    static_assert(G >= 4 && G <= 32, "group size");
-    constexpr int W = 32 / G;
-    const uint32_t tag4 = make_tag(h) * 0x01010101u;
-    uint32_t slab = h % t.numBuckets;
-    while (true) {
-        uint32_t words[W];
-        tagline_load<G>(words, t.tags + (size_t)slab * kTagWords, gl);
-        uint32_t cand = 0;
+    constexpr int HW = HalfLine<G>::HW;
+    return group_bcast<G>(L.w[HW - 1], kHalfNext / HW);
+}
+
+// Candidate bits (4 per word, word i of this lane at bits 4i..4i+3) of the
+// slots whose tag equals tag4 (tag replicated into 4 bytes).
+template <int G>
+__device__ inline uint32_t halfline_match(const HalfLine<G>& L, uint32_t tag4, uint32_t gl)
+{
+    constexpr int HW = HalfLine<G>::HW;
+    uint32_t cand = 0;
 #pragma unroll
-        for (int i = 0; i < W; ++i) {
-            if (gl * W + i != kNextWord)
-                cand |= compact_bytes(zero_bytes(words[i] ^ tag4)) << (4 * i);
-        }
-        const uint32_t next = group_bcast<G>(words[W - 1], G - 1);
+    for (int i = 0; i < HW; ++i) {
+        const uint32_t j = gl * HW + i;
+        if (j < kHalfNext)
+            cand |= compact_bytes(zero_bytes(L.w[i] ^ tag4)) << (4 * i);
+    }
+    return cand;
+}
+
+// Walk a chain in half hf starting at `slab` whose half line is already in
+// L; compares candidates one at a time with the whole group.
+template <int G, int MW>
+__device__ inline uint32_t probe_chain(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t tag4, uint32_t hf,
+                                       uint32_t slab, HalfLine<G> L, uint32_t gl, uint32_t gbase)
+{
+    constexpr int HW = HalfLine<G>::HW;
+    while (true) {
+        uint32_t cand = halfline_match<G>(L, tag4, gl);
+        const uint32_t next = halfline_next<G>(L);
         while (true) {
             const uint32_t bal = group_ballot<G>(cand != 0, gbase);
             if (!bal)
                 break;
             const uint32_t leader = __ffs(bal) - 1u;
-            const uint32_t mySlot = gl * W * 4 + (cand ? (__ffs(cand) - 1u) : 0u);
+            uint32_t mySlot = 0;
+            if (cand) {
+                const uint32_t bit = __ffs(cand) - 1u;
+                mySlot = slot_of(hf, gl * HW + (bit >> 2), bit & 3u);
+            }
             const uint32_t slot = group_bcast<G>(mySlot, leader);
             if (gl == leader)
                 cand &= cand - 1u;
@@ -317,35 +370,52 @@ __device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, M
         if (next == kNoSlab)
             return kNotFound;
         slab = next;
+        halfline_load<G>(L, t.tags + (size_t)slab * kTagWords + hf * kHalfWords, gl);
     }
 }
 
-// Per-lane find: one lane walks the whole 128-byte tag line with 16-byte loads.
+// Group-cooperative find (G in {4, 8, 16, 32}): one 64-byte half line per
+// slab, read once by the group.
+template <int G, int MW>
+__device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
+                                            uint32_t gl, uint32_t gbase)
+{
+    static_assert(G >= 4 && G <= 32, "group size");
+    const uint32_t hf = half_of(h);
+    const uint32_t slab = h % t.numBuckets;
+    HalfLine<G> L;
+    halfline_load<G>(L, t.tags + (size_t)slab * kTagWords + hf * kHalfWords, gl);
+    return probe_chain<G, MW>(t, n, s, make_tag(h) * 0x01010101u, hf, slab, L, gl, gbase);
+}
+
+// Per-lane find: one lane reads the 64-byte half line with 16-byte loads.
 __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* needle, uint32_t s, uint32_t h)
 {
     const uint32_t tag4 = make_tag(h) * 0x01010101u;
+    const uint32_t hf = half_of(h);
     uint32_t slab = h % t.numBuckets;
     while (true) {
-        const uint4* line = reinterpret_cast<const uint4*>(t.tags + (size_t)slab * kTagWords);
+        const uint4* line = reinterpret_cast<const uint4*>(t.tags + (size_t)slab * kTagWords + hf * kHalfWords);
         uint32_t next = kNoSlab;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < 4; ++q) {
             const uint4 v = line[q];
             const uint32_t ws[4] = { v.x, v.y, v.z, v.w };
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (q * 4 + i == (int)kNextWord) {
+                const uint32_t j = (uint32_t)(q * 4 + i);
+                if (j == kHalfNext) {
                     next = ws[i];
                     continue;
                 }
                 uint32_t c = compact_bytes(zero_bytes(ws[i] ^ tag4));
                 while (c) {
-                    const uint32_t slot = (uint32_t)(q * 4 + i) * 4 + (__ffs(c) - 1u);
+                    const uint32_t slot = slot_of(hf, j, __ffs(c) - 1u);
                     c &= c - 1u;
                     const uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
                     bool eq = true;
-                    for (uint32_t j = 0; j < s && eq; ++j)
-                        eq = item[j] == needle[j];
+                    for (uint32_t k = 0; k < s && eq; ++k)
+                        eq = item[k] == needle[k];
                     if (eq)
                         return (slab << kSlotBits) | slot;
                 }
@@ -358,31 +428,40 @@ __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* ne
 }
 
 // Group-cooperative insert of an item the caller knows to be absent (or, for
-// bulk multiset inserts, allowed to be present).  Claims a slot with a CAS on
-// its tag word; a full chain grows through chain_grow.
+// bulk multiset inserts, allowed to be present).  Claims a slot of the
+// item's half with a CAS on its tag word; a chain whose halves `hf` are all
+// full grows through chain_grow.
 template <int G, int MW>
 __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
                                               uint32_t gl, uint32_t gbase)
 {
     static_assert(G >= 4 && G <= 32, "group size");
-    constexpr int W = 32 / G;
+    constexpr int HW = HalfLine<G>::HW;
     const uint32_t tag = make_tag(h);
+    const uint32_t hf = half_of(h);
     const uint32_t bucket = h % t.numBuckets;
     uint32_t slab = bucket;
     uint32_t headNext = kNoSlab;
     while (true) {
-        uint32_t* line = t.tags + (size_t)slab * kTagWords;
-        uint32_t words[W];
+        uint32_t* half = t.tags + (size_t)slab * kTagWords + hf * kHalfWords;
+        HalfLine<G> L;
+        if (gl * HW < kHalfWords) {
 #pragma unroll
-        for (int i = 0; i < W; ++i)
-            words[i] = ld_relaxed(line + gl * W + i);
+            for (int i = 0; i < HW; ++i)
+                L.w[i] = ld_relaxed(half + gl * HW + i);
+        } else {
+#pragma unroll
+            for (int i = 0; i < HW; ++i)
+                L.w[i] = 0u;
+        }
         uint32_t emp = 0;
 #pragma unroll
-        for (int i = 0; i < W; ++i) {
-            if (gl * W + i != kNextWord)
-                emp |= compact_bytes(zero_bytes(words[i])) << (4 * i);
+        for (int i = 0; i < HW; ++i) {
+            const uint32_t j = gl * HW + i;
+            if (j < kHalfNext)
+                emp |= compact_bytes(zero_bytes(L.w[i])) << (4 * i);
         }
-        const uint32_t next = group_bcast<G>(words[W - 1], G - 1);
+        const uint32_t next = halfline_next<G>(L);
         const uint32_t bal = group_ballot<G>(emp != 0, gbase);
         if (bal) {
             const uint32_t leader = __ffs(bal) - 1u;
@@ -392,12 +471,12 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G,
                 const uint32_t i = bit >> 2, b = bit & 3u;
                 uint32_t old = 0;
 #pragma unroll
-                for (int q = 0; q < W; ++q)
+                for (int q = 0; q < HW; ++q)
                     if ((uint32_t)q == i)
-                        old = words[q];
-                const uint32_t r = atomicCAS(line + gl * W + i, old, old | (tag << (8u * b)));
+                        old = L.w[q];
+                const uint32_t r = atomicCAS(half + gl * HW + i, old, old | (tag << (8u * b)));
                 ok = (r == old) ? 1u : 0u;
-                slot = (gl * W + i) * 4u + b;
+                slot = slot_of(hf, gl * HW + i, b);
             }
             ok = group_bcast<G>(ok, leader);
             slot = group_bcast<G>(slot, leader);
@@ -423,7 +502,7 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G,
         const unsigned long long growers = __ballot(gl == 0);
         uint32_t grown = bucket;
         if (gl == 0 && (uint32_t)(__ffsll((long long)growers) - 1) == __lane_id())
-            grown = chain_grow(t, bucket, headNext);
+            grown = chain_grow(t, bucket, headNext, hf);
         slab = group_bcast<G>(grown, 0);
         if (slab == kNoSlab)
             return kNotFound;
@@ -434,24 +513,25 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G,
 __device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* item, uint32_t s, uint32_t h)
 {
     const uint32_t tag = make_tag(h);
+    const uint32_t hf = half_of(h);
     const uint32_t bucket = h % t.numBuckets;
     uint32_t slab = bucket;
     uint32_t headNext = kNoSlab;
     while (true) {
-        uint32_t* line = t.tags + (size_t)slab * kTagWords;
+        uint32_t* half = t.tags + (size_t)slab * kTagWords + hf * kHalfWords;
         bool retry = false;
-        for (uint32_t w = 0; w < kNextWord; ++w) {
-            const uint32_t v = ld_relaxed(line + w);
+        for (uint32_t w = 0; w < kHalfNext; ++w) {
+            const uint32_t v = ld_relaxed(half + w);
             uint32_t e = compact_bytes(zero_bytes(v));
             if (!e)
                 continue;
             const uint32_t b = __ffs(e) - 1u;
-            const uint32_t r = atomicCAS(line + w, v, v | (tag << (8u * b)));
+            const uint32_t r = atomicCAS(half + w, v, v | (tag << (8u * b)));
             if (r != v) {
                 retry = true;
                 break;
             }
-            const uint32_t slot = w * 4u + b;
+            const uint32_t slot = slot_of(hf, w, b);
             uint32_t* dst = t.items + ((size_t)slab * kSlots + slot) * s;
             for (uint32_t j = 0; j < s; ++j)
                 dst[j] = item[j];
@@ -459,7 +539,7 @@ __device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* 
         }
         if (retry)
             continue;
-        const uint32_t next = ld_relaxed(line + kNextWord);
+        const uint32_t next = ld_relaxed(half + kHalfNext);
         if (slab == bucket)
             headNext = next;
         if (next != kNoSlab) {
@@ -470,7 +550,7 @@ __device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* 
         const unsigned long long growers = __ballot(1);
         slab = bucket;
         if ((uint32_t)(__ffsll((long long)growers) - 1) == __lane_id())
-            slab = chain_grow(t, bucket, headNext);
+            slab = chain_grow(t, bucket, headNext, hf);
         if (slab == kNoSlab)
             return kNotFound;
     }
