@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
         }
         const uint32_t imask = header_payload(hdr, dag.M) & 0xFFu;
         uint32_t out[8];
-        uint32_t n = 0, newMask = 0, k = 0;
+        uint32_t n = 0, newMask = 0, k = 0, newChild = 0;
         for (uint32_t c = 0; c < 8; ++c) {
             if (!((mask >> c) & 1u))
                 continue;
@@ -355,6 +355,8 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
             if ((imask >> c) & 1u) {
                 const uint32_t r = ws.rep[childStart + h];
                 h = (r == kSentinel) ? kSentinel : ws.handle[r];
+                if (r != kSentinel)
+                    newChild |= ws.created[r];
             }
             if (h != kSentinel) {
                 out[n++] = h;
@@ -377,6 +379,7 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
         rec[0] = full ? header_make_full(0xFFu, fm, dag.M) : header_make(newMask, 0u, dag.M);
         for (uint32_t j = 0; j < 8; ++j)
             rec[1 + j] = j < n ? out[j] : 0u;
+        rec[9] = newChild; // outside the item's words: equal items agree on it
         if (full)
             set_flag(kMulti ? &ws.edFull[e * kMaxLevels + L] : &ws.lvl[L].fullCount);
     }
@@ -445,9 +448,14 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
         if (gl == 0)
             ws.rep[w] = rep;
         if (rep == w) {
-            const uint32_t r = table_find_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
-            if (gl == 0)
+            // A node with a child this merge inserted is new: no find.
+            uint32_t r = kNotFound;
+            if (isLeaf || rec[9] == 0u)
+                r = table_find_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
+            if (gl == 0) {
                 ws.handle[w] = (r == kNotFound) ? kNotFound : (isLeaf ? leaf_handle(dag, r, s) : node_handle(r, s));
+                ws.created[w] = 0;
+            }
         }
     }
 }
@@ -483,6 +491,7 @@ __global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32
                 ws.handle[w] = kSentinel;
             } else {
                 ws.handle[w] = isLeaf ? leaf_handle(dag, r, s) : node_handle(r, s);
+                ws.created[w] = 1;
             }
         }
     }
@@ -645,7 +654,8 @@ static uint32_t bit_width(uint32_t v)
 static void ws_release(WsDev& ws)
 {
     for (void* p : { (void*)ws.work, (void*)ws.state, (void*)ws.nodes, (void*)ws.leaves, (void*)ws.rep,
-                     (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status, (void*)ws.editOf })
+                     (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status, (void*)ws.editOf,
+                     (void*)ws.created })
         if (p)
             (void)hipFree(p);
     ws = WsDev {};
@@ -676,6 +686,7 @@ static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
     SVDAG_HIP(hipMalloc(&ws.rep, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.handle, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.editOf, (size_t)capWork * 4));
+    SVDAG_HIP(hipMalloc(&ws.created, (size_t)capWork));
     SVDAG_HIP(hipMalloc(&ws.dedup, (size_t)ws.dedupCap * 8));
     SVDAG_HIP(hipMalloc(&ws.lvl, kMaxLevels * sizeof(LevelDesc)));
     SVDAG_HIP(hipMalloc(&ws.status, sizeof(EditStatus)));

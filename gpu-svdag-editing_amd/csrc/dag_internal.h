@@ -45,6 +45,9 @@ struct WsDev {
     uint32_t* editOf;            // [capWork]
     uint32_t* edCount;           // [batch][kMaxLevels]
     uint32_t* edFull;            // [batch][kMaxLevels]
+    // 1 where the merge inserted representative w (0 where it found it); a
+    // parent with such a child cannot be in the DAG yet.
+    uint8_t* created;            // [capWork]
 };
 
 // Level-S cell range of an edit in a fused wave: cells lo .. lo + n - 1 per

@@ -111,6 +111,8 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
     __shared__ uint4 sItem[kMaxLevels][kSpineItems];
     __shared__ uint32_t sRec[kMaxLevels][kSpineItems][kNodeStride];
     __shared__ uint32_t sHandle[kMaxLevels][kSpineItems];
+    __shared__ uint32_t sCreated[kMaxLevels][kSpineItems]; // 1: the spine inserted it
+    __shared__ uint32_t sHasNew[kSpineItems];               // item has a created child
     __shared__ uint32_t sCnt[kMaxLevels];
     const uint32_t lane = threadIdx.x;
     const uint2 wv = f.waves[*f.waveIdx];
@@ -255,7 +257,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                 if (mask) {
                     const uint32_t imask = header_payload(hdr, dag.M) & 0xFFu;
                     uint32_t outH[8];
-                    uint32_t nOut = 0, newMask = 0, q = 0;
+                    uint32_t nOut = 0, newMask = 0, q = 0, newChild = 0;
                     for (uint32_t c = 0; c < 8; ++c) {
                         if (!((mask >> c) & 1u))
                             continue;
@@ -264,7 +266,10 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                             if (childSeed) {
                                 const uint32_t r = ws.rep[h];
                                 h = (r == kSentinel) ? kSentinel : ws.handle[r];
+                                if (r != kSentinel)
+                                    newChild |= ws.created[r];
                             } else {
+                                newChild |= sCreated[l + 1][h];
                                 h = sHandle[l + 1][h];
                             }
                         }
@@ -288,6 +293,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                     rec[0] = full ? header_make_full(0xFFu, fm, dag.M) : header_make(newMask, 0u, dag.M);
                     for (uint32_t j = 0; j < 8; ++j)
                         rec[1 + j] = j < nOut ? outH[j] : 0u;
+                    sHasNew[lane] = newChild;
                 }
             }
             prevFull = (uint32_t)__builtin_popcountll(__ballot(full));
@@ -308,26 +314,35 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                     }
                 }
             }
-            uint32_t handle = kSentinel;
+            uint32_t handle = kSentinel, created = 0;
             if (i8 < n && !empty && repIdx == i8) {
                 const uint32_t* rec = sRec[l][i8];
                 const uint32_t s = node_size_from_header(rec[0]);
                 Needle<8> nd;
                 needle_load(nd, rec, s, c8);
                 const uint32_t h = needle_hash(nd, s, c8);
-                uint32_t r = table_find_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
-                if (r == kNotFound)
+                // A node with a child inserted by this merge is new: no find.
+                uint32_t r = kNotFound;
+                if (sHasNew[i8] == 0u)
+                    r = table_find_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
+                if (r == kNotFound) {
                     r = table_insert_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
+                    created = 1u;
+                }
                 if (r == kNotFound)
                     oom = true;
                 else
                     handle = node_handle(r, s);
             }
-            if (c8 == 0 && i8 < n)
+            if (c8 == 0 && i8 < n) {
                 sHandle[l][i8] = handle;
+                sCreated[l][i8] = created;
+            }
             __syncthreads();
-            if (c8 == 0 && i8 < n && !empty && repIdx != i8)
+            if (c8 == 0 && i8 < n && !empty && repIdx != i8) {
                 sHandle[l][i8] = sHandle[l][repIdx];
+                sCreated[l][i8] = sCreated[l][repIdx];
+            }
             __syncthreads();
             if (__ballot(oom) != 0)
                 break;
