@@ -290,8 +290,8 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         uint32_t* rec = ws.leaves + (size_t)i * ws.leafStride;
+        uint32_t v = 0;
         if (lane < ws.leafStride) {
-            uint32_t v;
             if (lane == 0)
                 v = (uint32_t)mask;
             else if (lane == 1)
@@ -299,6 +299,24 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
             else
                 v = (lane - 2 < 16) ? lds[wv][lane - 2] : 0u;
             rec[lane] = v;
+        }
+        // A leaf the edit left as it was resolves to its old handle: no dedup
+        // and no DAG find (created = 2 marks it for the merge).
+        bool unchanged = false;
+        if (item.w != kSentinel && mask != 0ull) {
+            const uint32_t newSize = leaf_size_from_words((uint32_t)mask, (uint32_t)(mask >> 32), dag.M);
+            if (leaf_handle_size(dag, item.w) == newSize) {
+                const uint32_t* old = dag_leaf_ptr(dag, item.w);
+                const bool eq = lane >= newSize || old[lane] == v;
+                unchanged = __ballot(!eq) == 0ull;
+            }
+        }
+        if (lane == 0) {
+            ws.created[w] = unchanged ? 2u : 0u;
+            if (unchanged) {
+                ws.rep[w] = w;
+                ws.handle[w] = item.w;
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -356,7 +374,7 @@ __global__ void __launch_bounds__(256) merge_prepare(WsDev ws, DagDev dag, uint3
                 const uint32_t r = ws.rep[childStart + h];
                 h = (r == kSentinel) ? kSentinel : ws.handle[r];
                 if (r != kSentinel)
-                    newChild |= ws.created[r];
+                    newChild |= ws.created[r] & 1u; // 1: inserted by this merge (2: unchanged leaf)
             }
             if (h != kSentinel) {
                 out[n++] = h;
@@ -404,6 +422,8 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
     const uint32_t groupsPerBlock = blockDim.x / G;
     for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / G; i < me.count; i += gridDim.x * groupsPerBlock) {
         const uint32_t w = me.start + i;
+        if (isLeaf && ws.created[w] == 2u)
+            continue; // unchanged leaf: rep and handle were set by svo_leaf_level
         const uint32_t* rec = isLeaf ? ws.leaves + (size_t)i * ws.leafStride : ws.nodes + (size_t)w * kNodeStride;
         const uint32_t w0 = rec[0], w1 = rec[1];
         const bool empty = isLeaf ? (w0 == 0u && w1 == 0u) : ((w0 & 0xFFu) == 0u);

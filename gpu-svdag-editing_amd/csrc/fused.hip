@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                                 const uint32_t r = ws.rep[h];
                                 h = (r == kSentinel) ? kSentinel : ws.handle[r];
                                 if (r != kSentinel)
-                                    newChild |= ws.created[r];
+                                    newChild |= ws.created[r] & 1u;
                             } else {
                                 newChild |= sCreated[l + 1][h];
                                 h = sHandle[l + 1][h];
