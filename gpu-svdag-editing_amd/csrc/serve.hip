@@ -51,7 +51,7 @@ struct Server {
     std::thread batchThread, frameThread;
     svdag_server_stats stats {};
     std::atomic<uint32_t> published { kSentinel };
-    double slabsPerItem = 1.0 / 16.0;
+    std::atomic<double> slabsPerItem { 1.0 / 16.0 }; // written by the batch thread, read by submitters
     // Frame state (frame thread only, plus frameMu for launches).
     hipEvent_t fStart = nullptr, fEnd = nullptr;
     void* dFrame = nullptr;
@@ -59,7 +59,7 @@ struct Server {
     double traceMsSum = 0.0;
     uint64_t traceMsCount = 0;
     uint32_t lastFrameRoot = kSentinel;
-    int error = SVDAG_OK;
+    std::atomic<int> error { SVDAG_OK };
 };
 
 static double seconds_since(const Clock::time_point& t0)
@@ -249,7 +249,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                     items += s.svo_items[L];
             if (read_memory(ctx, after) == SVDAG_OK && items > 0 && after.slabs_used >= usedBefore) {
                 const double r = (double)(after.slabs_used - usedBefore) / (double)items;
-                srv->slabsPerItem = std::max(1.0 / (double)kSlots, 0.5 * srv->slabsPerItem + 0.5 * r);
+                srv->slabsPerItem.store(std::max(1.0 / (double)kSlots, 0.5 * srv->slabsPerItem.load() + 0.5 * r));
             }
         }
         srv->published.store(ctx->root);
@@ -358,7 +358,7 @@ int svdag_server_submit(svdag_ctx* ctx, const svdag_edit_desc* edit, uint64_t* t
     Server* srv = ctx->server;
     Request r {};
     SVDAG_TRY(make_editor(ctx, *edit, r.ed));
-    r.estSlabs = estimate_slabs(ctx, r.ed, srv->slabsPerItem);
+    r.estSlabs = estimate_slabs(ctx, r.ed, srv->slabsPerItem.load());
     {
         std::lock_guard<std::mutex> lk(srv->qMu);
         if (srv->stopping) {
@@ -421,7 +421,7 @@ int svdag_server_stop(svdag_ctx* ctx, svdag_server_stats* stats)
     srv->stats.last_frame_root = srv->lastFrameRoot;
     if (stats)
         *stats = srv->stats;
-    const int err = srv->error;
+    const int err = srv->error.load();
     // Keep the last frame for svdag_server_last_frame.
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (ctx->dLastFrame)

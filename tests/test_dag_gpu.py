@@ -15,7 +15,8 @@ def _pair(levels=L10, material_bits=4, group_size=0, terrain=True, **kw):
     g = svdag_py.MyGPUHashDAG(levels, material_bits, group_size=group_size, **kw)
     o = oracle_py.OracleDag(levels, material_bits)
     if terrain:
-        tp = workloads.terrain_params(levels)
+        tp = dict(workloads.terrain_params(levels))
+        tp["material"] = min(tp["material"], (1 << material_bits) - 1)
         g.build_terrain(**tp)
         o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"])
     return g, o
@@ -431,3 +432,34 @@ def test_graph_waves_replay_exactly_across_batches():
     assert a.canonical() == b.canonical()
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("material_bits", [0, 8])
+def test_other_material_widths_match_oracle(material_bits):
+    # 8 material bits: leaves of up to 18 words (the wide needle kernels);
+    # 0 bits: geometry only, one leaf size.  Edits of every kind, fused
+    # batches, undo, and a traced frame.
+    levels = L10
+    g, o = _pair(material_bits=material_bits)
+    nm = 1 << material_bits
+    edits = _mixed_stream(levels, seed=90 + material_bits)
+    edits = [(k, (m * 37) % nm if k != svdag_py.SPHERE_REMOVE else 0, *rest) for k, m, *rest in edits]
+    g.edit_batch(edits[:30])
+    for e in edits[30:]:
+        g.edit(*e)
+    for e in edits:
+        o.edit(*e)
+    _same(g, o)
+    g.undo()
+    g.undo()
+    g.redo()
+    cam = workloads.config1_camera(levels)
+    rot = svdag_py.look_at(cam["position"], cam["target"])
+    gh, _ = g.trace(cam["position"], rot, cam["fov"], cam["bmin"], cam["bmax"], 128, 128)
+    g.redo()
+    _same(g, o)
+    oh = o.trace(cam["position"], rot, cam["fov"], cam["bmin"], cam["bmax"], 128, 128)
+    gh2, _ = g.trace(cam["position"], rot, cam["fov"], cam["bmin"], cam["bmax"], 128, 128)
+    assert np.array_equal(gh2.view(np.uint32), oh.view(np.uint32))
+    g.close()
