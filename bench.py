@@ -3,7 +3,7 @@
 
 Headline line: the slab-hash microbenchmark -- 2^24 DAG interior-node items
 (2..9 words, child mask uniform over the 255 non-empty masks, child words
-uniform u32) bulk-inserted into per-size tables at 96 items per bucket, then
+uniform u32) bulk-inserted into per-size tables whose head lines are 75% full, then
 2^24 finds with a 50% hit rate.  One step = one find pass over the 2^24
 needles already resident in HBM.  The wave64-group and per-lane variants are
 all timed during warm-up; the fastest is used for the timed steps.
@@ -69,7 +69,7 @@ def slab_hash_bench(args):
     log(f"[bench] generating {n_items} items / finds")
     items = workloads.slab_hash_items(n_items)
     finds = workloads.slab_hash_finds(items, n_items, hit_rate=0.5)
-    buckets = workloads.buckets_for(items, 96)
+    buckets = workloads.buckets_for(items, 0.75)
     sizes = list(range(2, 10))
     groups = sorted(items.items())
     fgroups = [(s, finds[s]) for s, _ in groups]
@@ -143,7 +143,7 @@ def slab_hash_bench(args):
         "data": "synthetic",
         "config": {
             "workload": f"slab-hash find: 2^{args.log2_items} DAG node items of 2-9 u32 in per-size tables "
-                        f"at 96 items/bucket, 2^{args.log2_items} finds, 50% hits",
+                        f"with head lines 75% full, 2^{args.log2_items} finds, 50% hits",
             "variant": variant_name(best),
             "find_kernel_ms_by_variant": {str(k): round(v, 4) for k, v in per_variant.items()},
             "bulk_insert_ms": round(add_ms, 3),

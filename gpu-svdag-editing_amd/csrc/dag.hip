@@ -965,7 +965,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
                 AllocState a {};
                 SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
                 if (a.oom)
-                    SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, 1ull << 25), ctx->stream));
+                    SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit), ctx->stream));
             }
             SVDAG_TRY(ctx_refresh_tables(ctx));
         } else {
@@ -1068,7 +1068,7 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
         return SVDAG_E_NO_DEVICE;
     }
     if (cfg.load_factor == 0)
-        cfg.load_factor = 96;
+        cfg.load_factor = 75;
     if (cfg.expected_items == 0)
         cfg.expected_items = 1ull << 20;
     if (cfg.workspace_items == 0)
@@ -1094,13 +1094,23 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
     d.numTables = maxWords - 2 + 1;
     const uint32_t indexBits = std::min(32u - kNodeSizeBits, 32u - d.leafSizeBits);
     const uint64_t slabLimit = 1ull << (indexBits - kSlotBits);
+    ctx->slabLimit = slabLimit;
 
-    const uint64_t perTable = std::max<uint64_t>(64, cfg.expected_items / (d.numTables * (uint64_t)cfg.load_factor));
-    const uint32_t buckets = (uint32_t)std::min<uint64_t>(perTable, slabLimit / 2);
+    // Buckets per table: the table's share of expected_items fills its head
+    // lines to load_factor percent.  Half of the items are spread evenly over
+    // the tables; the other half follow the node sizes of a surface voxel DAG
+    // (a depth-14 fBm terrain: 2..9 words in thousandths below), where nodes
+    // with all eight children are the largest group by far.
+    static constexpr uint32_t kNodeShare[8] = { 14, 56, 42, 110, 72, 156, 145, 405 };
     ctx->ts.minWords = 2;
     ctx->ts.maxWords = maxWords;
     ctx->ts.tables.resize(d.numTables);
     for (uint32_t i = 0; i < d.numTables; ++i) {
+        const double frac = 0.5 / d.numTables + (i < 8 ? 0.5e-3 * kNodeShare[i] : 0.0);
+        const uint64_t share = (uint64_t)(frac * (double)cfg.expected_items);
+        const uint64_t perLine = (uint64_t)line_slots(2 + i) * cfg.load_factor;
+        const uint64_t want = std::max<uint64_t>(64, (share * 100 + perLine - 1) / perLine);
+        const uint32_t buckets = (uint32_t)std::min<uint64_t>(want, slabLimit / 2);
         uint64_t cap = buckets + (cfg.arena_slabs ? cfg.arena_slabs : buckets / 2 + 4096);
         cap = std::min<uint64_t>(cap, slabLimit);
         SVDAG_TRY(table_alloc(ctx->ts.tables[i], 2 + i, buckets, (uint32_t)cap, ctx->stream));

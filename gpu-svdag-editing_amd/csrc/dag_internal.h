@@ -102,6 +102,7 @@ struct svdag_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<uint32_t> undo, redo;
     uint32_t groupSize = 16;
+    uint64_t slabLimit = 0; // most slabs a table may hold: slab ids must fit the handles' index bits
     // Host-visible status block (pinned): EditStatus followed by the level table.
     svdag::EditStatus* hStatus = nullptr;
     svdag::LevelDesc* hLevels = nullptr;

@@ -18,9 +18,8 @@ namespace svdag {
 
 struct HostDag {
     uint32_t levels = 0, leafLevel = 0, M = 0, minLeaf = 0, leafSizeBits = 0;
-    std::vector<std::vector<uint32_t>> items; // per table: slots [fresh][kSlots][words]
-    std::vector<std::vector<uint32_t>> tags;  // per table: tag lines [fresh][32]
-    std::vector<uint32_t> fresh;
+    std::vector<std::vector<uint32_t>> lines; // per table: [fresh][32] (tags, items, next)
+    std::vector<uint32_t> fresh, slots, tagWords;
     std::vector<uint32_t> fullyFilledLevel0;  // level-0 fully-filled node per material
 
     // Words of a stored item; nullptr if the pointer is out of range or its
@@ -28,12 +27,12 @@ struct HostDag {
     const uint32_t* item(uint32_t s, uint32_t ptr) const
     {
         const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
-        if (s < 2 || s - 2 >= items.size() || slab >= fresh[s - 2] || slot >= kSlots)
+        if (s < 2 || s - 2 >= lines.size() || slab >= fresh[s - 2] || slot >= slots[s - 2])
             return nullptr;
-        const uint32_t tag = (tags[s - 2][(size_t)slab * kTagWords + slot_word(slot)] >> (8 * (slot % 4))) & 0xFFu;
-        if (!tag)
+        const uint32_t* line = lines[s - 2].data() + (size_t)slab * kLineWords;
+        if (!((line[slot / 4] >> (8 * (slot % 4))) & 0xFFu))
             return nullptr;
-        return items[s - 2].data() + ((size_t)slab * kSlots + slot) * s;
+        return line + tagWords[s - 2] + slot * s;
     }
 };
 
@@ -45,9 +44,10 @@ static int copy_dag_to_host(svdag_ctx* ctx, HostDag& h)
     h.M = d.M;
     h.minLeaf = d.minLeafWords;
     h.leafSizeBits = d.leafSizeBits;
-    h.items.resize(ctx->ts.tables.size());
-    h.tags.resize(ctx->ts.tables.size());
+    h.lines.resize(ctx->ts.tables.size());
     h.fresh.resize(ctx->ts.tables.size());
+    h.slots.resize(ctx->ts.tables.size());
+    h.tagWords.resize(ctx->ts.tables.size());
     h.fullyFilledLevel0.resize(d.numMaterials);
     SVDAG_HIP(hipMemcpyAsync(h.fullyFilledLevel0.data(), ctx->dFullyFilled, d.numMaterials * 4, hipMemcpyDeviceToHost,
                              ctx->stream));
@@ -57,11 +57,10 @@ static int copy_dag_to_host(svdag_ctx* ctx, HostDag& h)
         SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
         const uint32_t fresh = std::min(a.fresh, t.slabCap);
         h.fresh[i] = fresh;
-        h.items[i].resize((size_t)fresh * kSlots * t.itemWords);
-        h.tags[i].resize((size_t)fresh * kTagWords);
-        SVDAG_HIP(hipMemcpyAsync(h.items[i].data(), t.dev.items, h.items[i].size() * 4, hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        SVDAG_HIP(hipMemcpyAsync(h.tags[i].data(), t.dev.tags, h.tags[i].size() * 4, hipMemcpyDeviceToHost,
+        h.slots[i] = t.dev.slots;
+        h.tagWords[i] = t.dev.tagWords;
+        h.lines[i].resize((size_t)fresh * kLineWords);
+        SVDAG_HIP(hipMemcpyAsync(h.lines[i].data(), t.dev.lines, h.lines[i].size() * 4, hipMemcpyDeviceToHost,
                                  ctx->stream));
     }
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));
@@ -261,7 +260,7 @@ static int insert_level(svdag_ctx* ctx, const std::vector<std::vector<uint32_t>>
         {
             FrameQuiesce q(ctx);
             for (auto& t : ctx->ts.tables)
-                SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, 1ull << 25), ctx->stream));
+                SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit), ctx->stream));
             SVDAG_TRY(ctx_refresh_tables(ctx));
         }
         std::vector<uint32_t> again;

@@ -31,10 +31,14 @@ struct GcCounters {
 __global__ void gc_clear_active(TableDev t, uint32_t* levelMask)
 {
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
-    const uint64_t n = (uint64_t)fresh * kSlots;
+    // A pop past the free stack's end leaves its count below zero; the
+    // sweep's pushes start from an empty stack.
+    if (blockIdx.x == 0 && threadIdx.x == 0 && t.alloc->freeCount < 0)
+        t.alloc->freeCount = 0;
+    const uint64_t n = (uint64_t)fresh * kMaxSlots;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         levelMask[i] = 0u;
-        if (i < (uint64_t)fresh * 4)
+        if (i < fresh)
             t.active[i] = 0u;
     }
 }
@@ -49,7 +53,7 @@ __device__ inline bool gc_claim(const DagDev& dag, uint32_t* const* levelMask, u
     const uint32_t s = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
     const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h);
     const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
-    uint32_t* m = &levelMask[s - 2][(size_t)slab * kSlots + slot];
+    uint32_t* m = &levelMask[s - 2][(size_t)slab * kMaxSlots + slot];
     const uint32_t bit = 1u << L;
     if (ld_relaxed(m) & bit)
         return false;
@@ -57,7 +61,7 @@ __device__ inline bool gc_claim(const DagDev& dag, uint32_t* const* levelMask, u
     if (seen & bit)
         return false;
     if (!seen)
-        atomicOr(&dag.tables[s - 2].active[(size_t)slab * 4 + (slot >> 5)], 1u << (slot & 31u));
+        atomicOr(&dag.tables[s - 2].active[slab], 1u << slot);
     return true;
 }
 
@@ -126,18 +130,17 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint32_t* const
 }
 
 // Thirty-two lanes per bucket chain, two chains per wave: lane w holds word w
-// of the slab's tag line (one coalesced 128-byte read), clears the tags of
-// unmarked items, and the chain link (word 15 of either half) is broadcast
-// to walk on.  Overflow slabs with no marked item in either half are
-// unlinked (both next copies of the predecessor) and freed.  The loop over
-// chains is wave-uniform so the cross-lane operations see every lane.
+// of the slab's line (one coalesced 128-byte read); the tag-word lanes clear
+// the tags of unmarked items, and the chain link (word 31) is broadcast to
+// walk on.  Overflow slabs with no marked item are unlinked and freed.  The
+// loop over chains is wave-uniform so the cross-lane operations see every
+// lane.
 __global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t w = lane & 31u;
     const uint32_t half = lane >> 5;
-    const uint32_t hf = w / kHalfWords, j = w % kHalfWords;
-    const bool tagWord = j != kHalfNext;
+    const bool tagWord = w < t.tagWords;
     const uint32_t chainsPerBlock = blockDim.x / 32u;
     const uint32_t stride = gridDim.x * chainsPerBlock;
     uint32_t freedItems = 0, freedSlabs = 0, live = 0;
@@ -148,14 +151,13 @@ __global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
         while (__ballot(slab != kNoSlab) != 0ull) {
             uint32_t v = 0, keep = 0;
             if (slab != kNoSlab) {
-                uint32_t* line = t.tags + (size_t)slab * kTagWords;
+                uint32_t* line = t.lines + (size_t)slab * kLineWords;
                 v = line[w];
                 if (tagWord) {
-                    const uint32_t first = slot_of(hf, j, 0);
-                    const uint32_t a = (t.active[(size_t)slab * 4 + (first >> 5)] >> (first & 31u)) & 0xFu;
+                    const uint32_t a = t.active[slab] >> (4u * w);
                     for (uint32_t byte = 0; byte < 4; ++byte) {
                         const uint32_t tag = (v >> (8 * byte)) & 0xFFu;
-                        if (!tag)
+                        if (!tag || 4u * w + byte >= t.slots)
                             continue;
                         if ((a >> byte) & 1u) {
                             keep |= tag << (8 * byte);
@@ -171,17 +173,16 @@ __global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
             const uint32_t next = (uint32_t)__shfl((int)v, (int)(half * 32u + kNextWord), 64);
             const unsigned long long kept = __ballot(tagWord && keep != 0u);
             const bool anyKept = ((kept >> (half * 32u)) & 0xFFFFFFFFull) != 0ull;
+            const bool unlink = slab != kNoSlab && slab != b0 && !anyKept;
+            if (unlink && w == 0) {
+                t.lines[(size_t)prev * kLineWords + kNextWord] = next;
+                t.lines[(size_t)slab * kLineWords + kNextWord] = kNoSlab; // tags were cleared above
+                ++freedSlabs;
+            }
+            wave_slab_free(t, unlink && w == 0, slab);
             if (slab != kNoSlab) {
-                if (slab != b0 && !anyKept) {
-                    if (w == 0) {
-                        t.tags[(size_t)prev * kTagWords + kHalfNext] = next;
-                        t.tags[(size_t)prev * kTagWords + kNextWord] = next;
-                        slab_free(t, slab);
-                        ++freedSlabs;
-                    }
-                } else {
+                if (!unlink)
                     prev = slab;
-                }
                 slab = next;
             }
         }
@@ -218,7 +219,7 @@ static int gc_masks(svdag_ctx* ctx)
             (void)hipFree(ctx->gcMasks[i]);
         ctx->gcMasks[i] = nullptr;
         ctx->gcMaskCap[i] = 0;
-        SVDAG_HIP(hipMalloc(&ctx->gcMasks[i], (size_t)cap * kSlots * 4));
+        SVDAG_HIP(hipMalloc(&ctx->gcMasks[i], (size_t)cap * kMaxSlots * 4));
         ctx->gcMaskCap[i] = cap;
         changed = true;
     }
@@ -253,7 +254,7 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
         SVDAG_HIP(hipEventRecord(ctx->ev0, st));
         SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
         for (size_t i = 0; i < nt; ++i)
-            gc_clear_active<<<grid_for((uint64_t)ctx->ts.tables[i].slabCap * kSlots, 256, 8192), 256, 0, st>>>(
+            gc_clear_active<<<grid_for((uint64_t)ctx->ts.tables[i].slabCap * kMaxSlots, 256, 8192), 256, 0, st>>>(
                 ctx->ts.tables[i].dev, ctx->gcMasks[i]);
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dGcMasks, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L < dag.leafLevel; ++L)

@@ -15,14 +15,20 @@ namespace svdag {
 static thread_local std::string g_lastError;
 void set_error(const std::string& msg) { g_lastError = msg; }
 
-// Head slabs: empty tag line, no successor, no active bits.
-__global__ void table_init_heads(TableDev t)
+// Head slabs: no tags, no successor, no active bits.
+// Lines [first, last) become empty: tag words 0, next kNoSlab, no GC marks.
+__global__ void table_init_lines(TableDev t, uint32_t first, uint32_t last)
 {
-    const size_t n = (size_t)t.numBuckets * kTagWords;
+    const size_t n = (size_t)(last - first) * kLineWords;
+    uint32_t* base = t.lines + (size_t)first * kLineWords;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        t.tags[i] = ((i % kHalfWords) == kHalfNext) ? kNoSlab : 0u;
-        if ((i & (kTagWords - 1)) < 4)
-            t.active[(i / kTagWords) * 4 + (i & 3)] = 0u;
+        const uint32_t w = (uint32_t)(i % kLineWords);
+        if (w == kNextWord)
+            base[i] = kNoSlab;
+        else if (w < t.tagWords)
+            base[i] = 0u;
+        if (w == 0)
+            t.active[first + i / kLineWords] = 0u;
     }
 }
 
@@ -43,16 +49,15 @@ int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t 
     d.itemWords = itemWords;
     d.numBuckets = numBuckets;
     d.slabCap = slabCap;
-    SVDAG_HIP(hipMalloc(&d.tags, (size_t)slabCap * kTagWords * 4));
-    SVDAG_HIP(hipMalloc(&d.items, (size_t)slabCap * kSlots * itemWords * 4));
-    SVDAG_HIP(hipMalloc(&d.active, (size_t)slabCap * 4 * 4));
+    d.slots = line_slots(itemWords);
+    d.tagWords = line_tag_words(d.slots);
+    SVDAG_HIP(hipMalloc(&d.lines, (size_t)slabCap * kLineWords * 4));
+    SVDAG_HIP(hipMalloc(&d.active, (size_t)slabCap * 4));
     SVDAG_HIP(hipMalloc(&d.freeList, (size_t)slabCap * 4));
-    SVDAG_HIP(hipMalloc(&d.grow, (size_t)numBuckets * 4));
-    SVDAG_HIP(hipMemsetAsync(d.grow, 0, (size_t)numBuckets * 4, stream));
     SVDAG_HIP(hipMalloc(&d.alloc, sizeof(AllocState)));
-    AllocState a { 0xFFFFFFFFull, numBuckets, 0u };
+    AllocState a { numBuckets, 0, 0u, 0u };
     SVDAG_HIP(hipMemcpyAsync(d.alloc, &a, sizeof(a), hipMemcpyHostToDevice, stream));
-    table_init_heads<<<grid_for((uint64_t)numBuckets * kTagWords, 256, 4096), 256, 0, stream>>>(d);
+    table_init_lines<<<grid_for((uint64_t)slabCap * kLineWords, 256, 8192), 256, 0, stream>>>(d, 0u, slabCap);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
@@ -60,11 +65,9 @@ int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t 
 void table_free(HostTable& t)
 {
     TableDev& d = t.dev;
-    if (d.tags) (void)hipFree(d.tags);
-    if (d.items) (void)hipFree(d.items);
+    if (d.lines) (void)hipFree(d.lines);
     if (d.active) (void)hipFree(d.active);
     if (d.freeList) (void)hipFree(d.freeList);
-    if (d.grow) (void)hipFree(d.grow);
     if (d.alloc) (void)hipFree(d.alloc);
     d = TableDev {};
     t = HostTable {};
@@ -82,26 +85,24 @@ int table_grow(HostTable& t, uint32_t newCap, hipStream_t stream)
     }
     TableDev& d = t.dev;
     const uint32_t old = t.slabCap;
-    uint32_t *tags = nullptr, *items = nullptr, *active = nullptr, *freeList = nullptr;
-    SVDAG_HIP(hipMalloc(&tags, (size_t)newCap * kTagWords * 4));
-    SVDAG_HIP(hipMalloc(&items, (size_t)newCap * kSlots * t.itemWords * 4));
-    SVDAG_HIP(hipMalloc(&active, (size_t)newCap * 4 * 4));
+    uint32_t *lines = nullptr, *active = nullptr, *freeList = nullptr;
+    SVDAG_HIP(hipMalloc(&lines, (size_t)newCap * kLineWords * 4));
+    SVDAG_HIP(hipMalloc(&active, (size_t)newCap * 4));
     SVDAG_HIP(hipMalloc(&freeList, (size_t)newCap * 4));
-    SVDAG_HIP(hipMemcpyAsync(tags, d.tags, (size_t)old * kTagWords * 4, hipMemcpyDeviceToDevice, stream));
-    SVDAG_HIP(hipMemcpyAsync(items, d.items, (size_t)old * kSlots * t.itemWords * 4, hipMemcpyDeviceToDevice, stream));
-    SVDAG_HIP(hipMemcpyAsync(active, d.active, (size_t)old * 4 * 4, hipMemcpyDeviceToDevice, stream));
+    SVDAG_HIP(hipMemcpyAsync(lines, d.lines, (size_t)old * kLineWords * 4, hipMemcpyDeviceToDevice, stream));
+    SVDAG_HIP(hipMemcpyAsync(active, d.active, (size_t)old * 4, hipMemcpyDeviceToDevice, stream));
     SVDAG_HIP(hipMemcpyAsync(freeList, d.freeList, (size_t)old * 4, hipMemcpyDeviceToDevice, stream));
     SVDAG_HIP(hipStreamSynchronize(stream));
-    (void)hipFree(d.tags);
-    (void)hipFree(d.items);
+    (void)hipFree(d.lines);
     (void)hipFree(d.active);
     (void)hipFree(d.freeList);
-    d.tags = tags;
-    d.items = items;
+    d.lines = lines;
     d.active = active;
     d.freeList = freeList;
     d.slabCap = newCap;
     t.slabCap = newCap;
+    table_init_lines<<<grid_for((uint64_t)(newCap - old) * kLineWords, 256, 8192), 256, 0, stream>>>(d, old, newCap);
+    SVDAG_HIP(hipGetLastError());
     // The OOM flag and the over-counted fresh watermark are reset so that the
     // next allocation continues from the first never-used slab.
     AllocState a {};
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* ta
     const uint32_t stride = gridDim.x * groupsPerBlock * N;
     for (uint32_t base = (blockIdx.x * groupsPerBlock + threadIdx.x / G) * N; base < total; base += stride) {
         Needle<G, MW> n[N];
-        uint32_t s[N], h[N], hf[N], slab[N], outIdx[N];
+        uint32_t s[N], h[N], slab[N], outIdx[N];
         const TableDev* tp[N];
         bool valid[N];
 #pragma unroll
@@ -211,17 +212,16 @@ __global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* ta
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             h[k] = needle_hash(n[k], s[k], gl);
-            hf[k] = half_of(h[k]);
             slab[k] = h[k] % tp[k]->numBuckets;
         }
-        HalfLine<G> lines[N];
+        LineHead lines[N];
 #pragma unroll
         for (int k = 0; k < N; ++k)
-            halfline_load<G>(lines[k], tp[k]->tags + (size_t)slab[k] * kTagWords + hf[k] * kHalfWords, gl);
+            line_head_load(lines[k], tp[k]->lines + (size_t)slab[k] * kLineWords);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            const uint32_t r = probe_chain<G, MW>(*tp[k], n[k], s[k], make_tag(h[k]) * 0x01010101u, hf[k], slab[k],
-                                                  lines[k], gl, gbase);
+            const uint32_t r = probe_chain<G, MW>(*tp[k], n[k], s[k], make_tag(h[k]) * 0x01010101u, slab[k], lines[k],
+                                                  gl, gbase);
             if (gl == 0 && valid[k])
                 out[outIdx[k]] = r;
         }
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
         const uint32_t stride = blocksPerXcd * groupsPerBlock * N;
         for (uint32_t base = (bx * groupsPerBlock + threadIdx.x / G) * N; base < cnt; base += stride) {
             Needle<G, MW> n[N];
-            uint32_t h[N], hf[N], slab[N], outIdx[N];
+            uint32_t h[N], slab[N], outIdx[N];
             bool valid[N];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
@@ -372,17 +372,16 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
                 outIdx[k] = r[0];
                 h[k] = r[1];
                 needle_load(n[k], r + 2, s, gl);
-                hf[k] = half_of(h[k]);
                 slab[k] = h[k] % t.numBuckets;
             }
-            HalfLine<G> lines[N];
+            LineHead lines[N];
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                halfline_load<G>(lines[k], t.tags + (size_t)slab[k] * kTagWords + hf[k] * kHalfWords, gl);
+                line_head_load(lines[k], t.lines + (size_t)slab[k] * kLineWords);
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                const uint32_t r = probe_chain<G, MW>(t, n[k], s, make_tag(h[k]) * 0x01010101u, hf[k], slab[k],
-                                                      lines[k], gl, gbase);
+                const uint32_t r = probe_chain<G, MW>(t, n[k], s, make_tag(h[k]) * 0x01010101u, slab[k], lines[k], gl,
+                                                      gbase);
                 if (gl == 0 && valid[k])
                     out[outIdx[k]] = r;
             }
@@ -686,11 +685,11 @@ int svdag_hashset_create(uint32_t minWords, uint32_t maxWords, const uint32_t* b
     for (uint32_t s = minWords; s <= maxWords; ++s) {
         const uint32_t nb = std::max(1u, buckets[s - minWords]);
         const uint64_t res = reserve ? reserve[s - minWords] : 0;
-        // Overflow reserve: a chain grows when the half an item hashes to is
-        // full (60 slots), so equal-hash items need a slab per 59 at worst;
-        // at 96 items per bucket about one half in twenty overflows, so a
-        // quarter of the bucket count covers the tail.
-        uint64_t cap = (uint64_t)nb + res / (kHalfSlots - 1) + nb / 4 + 64;
+        // Overflow reserve: equal-hash items (bulk multiset inserts) need a
+        // line per `slots` at worst; in a table sized for 75% of its slots
+        // about a fifth of the buckets overflow into a second line.
+        const uint32_t slots = line_slots(s);
+        uint64_t cap = (uint64_t)nb + res / slots + nb / 2 + 64;
         cap = std::min<uint64_t>(cap, (1ull << (32 - kSlotBits)));
         int r = table_alloc(hs->ts.tables[s - minWords], s, nb, (uint32_t)cap, hs->stream);
         if (r != SVDAG_OK) {
@@ -772,7 +771,7 @@ int svdag_hashset_read(svdag_hashset* hs, uint32_t s, const uint32_t* ptrs, uint
     const HostTable& t = hs->ts.tables[s - hs->ts.minWords];
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t slab = ptrs[i] >> kSlotBits, slot = ptrs[i] & ((1u << kSlotBits) - 1u);
-        if (ptrs[i] == kNotFound || slab >= t.slabCap || slot >= kSlots) {
+        if (ptrs[i] == kNotFound || slab >= t.slabCap || slot >= t.dev.slots) {
             set_error("svdag_hashset_read: invalid pointer");
             return SVDAG_E_INVALID;
         }

@@ -51,7 +51,7 @@ struct Server {
     std::thread batchThread, frameThread;
     svdag_server_stats stats {};
     std::atomic<uint32_t> published { kSentinel };
-    std::atomic<double> slabsPerItem { 1.0 / 16.0 }; // written by the batch thread, read by submitters
+    std::atomic<double> slabsPerItem { 1.0 / 4.0 }; // written by the batch thread, read by submitters
     // Frame state (frame thread only, plus frameMu for launches).
     hipEvent_t fStart = nullptr, fEnd = nullptr;
     void* dFrame = nullptr;
@@ -85,11 +85,11 @@ static int read_memory(svdag_ctx* ctx, svdag_memory& m)
         AllocState a {};
         SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
         const uint64_t fresh = std::min<uint64_t>(a.fresh, t.slabCap);
-        const uint64_t freeSlabs = (t.slabCap - fresh) + a.freeCount;
+        const uint64_t freeSlabs = (t.slabCap - fresh) + (uint64_t)std::max(a.freeCount, 0);
         m.slab_capacity += t.slabCap;
         m.slabs_free += freeSlabs;
         m.slabs_used += t.slabCap - freeSlabs;
-        m.bytes_reserved += (uint64_t)t.slabCap * (kTagWords + (uint64_t)kSlots * t.itemWords + 5) * 4;
+        m.bytes_reserved += (uint64_t)t.slabCap * (kLineWords + 2) * 4; // lines, active, free links
         m.min_free_fraction = std::min(m.min_free_fraction, (double)freeSlabs / (double)t.slabCap);
     }
     return SVDAG_OK;
@@ -107,7 +107,7 @@ static double headroom(svdag_ctx* ctx, double reserveFraction, svdag_memory& m)
         if (table_read_alloc(t, a, ctx->stream) != SVDAG_OK)
             return 0.0;
         const uint64_t fresh = std::min<uint64_t>(a.fresh, t.slabCap);
-        const uint64_t freeSlabs = (t.slabCap - fresh) + a.freeCount;
+        const uint64_t freeSlabs = (t.slabCap - fresh) + (uint64_t)std::max(a.freeCount, 0);
         m.slab_capacity += t.slabCap;
         m.slabs_free += freeSlabs;
         m.slabs_used += t.slabCap - freeSlabs;
@@ -206,7 +206,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
             FrameQuiesce q(ctx);
             bool grew = false;
             for (auto& t : ctx->ts.tables) {
-                const uint64_t target = std::min<uint64_t>((uint64_t)t.slabCap * 2, 1ull << 25);
+                const uint64_t target = std::min<uint64_t>((uint64_t)t.slabCap * 2, ctx->slabLimit);
                 if (target > t.slabCap && table_grow(t, (uint32_t)target, ctx->stream) == SVDAG_OK)
                     grew = true;
             }
@@ -249,7 +249,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                     items += s.svo_items[L];
             if (read_memory(ctx, after) == SVDAG_OK && items > 0 && after.slabs_used >= usedBefore) {
                 const double r = (double)(after.slabs_used - usedBefore) / (double)items;
-                srv->slabsPerItem.store(std::max(1.0 / (double)kSlots, 0.5 * srv->slabsPerItem.load() + 0.5 * r));
+                srv->slabsPerItem.store(std::max(1.0 / (double)kMaxSlots, 0.5 * srv->slabsPerItem.load() + 0.5 * r));
             }
         }
         srv->published.store(ctx->root);

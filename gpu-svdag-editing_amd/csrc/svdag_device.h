@@ -10,31 +10,44 @@
 // Tables are indexed by item size - 2, nodes and leaves of equal word count
 // share one table (my_gpu_hash_dag.h:173, 240).
 //
-// Table layout (own design for wave64, not the reference's 32-slot slab):
-//   slab  = one 128-byte tag line in a dense tag array plus a separate item
-//           array [slab][120][itemWords].  The line is two 64-byte halves;
-//           half h holds 15 words x 4 one-byte tags (slots 60h .. 60h+59) and
-//           a copy of the next slab id in its word 15.  An item lives in the
-//           half its hash selects, so a probe reads 64 bytes per slab, not
-//           128.  Head slabs 0..numBuckets-1 are the buckets; overflow slabs
-//           come from the slab allocator and serve both halves.
+// Table layout (own design, not the reference's 32-slot slab): one table per
+// item size s; a slab is one 128-byte line holding everything a probe needs:
+//   words [0, tagWords)                 one-byte tags (0 = empty), one per slot
+//   words [tagWords, tagWords + slots*s) the items themselves
+//   word 31                             next slab id of the bucket's chain
+// where slots is the most items of s words that fit beside their tags (13 at
+// s = 2, 5 at s = 5, 3 at s = 9, 1 at s >= 16).  A probe fetches one line,
+// and the compare of a candidate reads the same line, which the fetch left
+// in cache.  Head slabs 0..numBuckets-1 are the buckets; overflow slabs come
+// from the slab allocator.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace svdag {
 
-constexpr uint32_t kSlots = 120;        // item slots per slab (two halves of 60)
-constexpr uint32_t kHalfSlots = 60;
-constexpr uint32_t kTagWords = 32;      // words per tag line (128 B)
-constexpr uint32_t kHalfWords = 16;     // words per half line (64 B)
-constexpr uint32_t kHalfNext = 15;      // word of a half that holds the next slab id
-constexpr uint32_t kNextWord = 31;      // the second copy (half 1); both are kept equal
-constexpr uint32_t kSlotBits = 7;       // pointer = slab << 7 | slot
+constexpr uint32_t kLineWords = 32;     // words per line (128 B)
+constexpr uint32_t kNextWord = 31;      // line word that holds the next slab id
+constexpr uint32_t kSlotBits = 4;       // pointer = slab << 4 | slot
+constexpr uint32_t kMaxSlots = 1u << kSlotBits; // bound on slots per line (13 at most)
 constexpr uint32_t kNoSlab = 0xFFFFFFFFu;
+constexpr uint32_t kPendingSlab = 0xFFFFFFFEu; // tail link while its successor is being allocated
 constexpr uint32_t kNotFound = 0xFFFFFFFFu;
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;
 constexpr uint32_t kMaxItemWords = 18;  // 2 + 64*8/32 (8 material bits)
+
+// Items of s words per line: the largest k with k*s + ceil(k/4) <= 31.
+__host__ __device__ inline uint32_t line_slots(uint32_t s)
+{
+    uint32_t k = 1;
+    while ((k + 1) * s + (k + 4) / 4 <= kNextWord)
+        ++k;
+    return k;
+}
+__host__ __device__ inline uint32_t line_tag_words(uint32_t slots)
+{
+    return (slots + 3) / 4;
+}
 
 // Bucket hash: SlabHash-style xor of per-word modular hashes (utils.h:386-403).
 constexpr uint32_t kHashX = 2717583853u;
@@ -51,22 +64,6 @@ __host__ __device__ inline uint32_t make_tag(uint32_t h)
 {
     const uint32_t t = (h * 0x9E3779B1u) >> 24;
     return t ? t : 1u;
-}
-// Half of the tag line an item lives in (independent of the bucket, h %
-// numBuckets, and of the tag, the top byte of h * 0x9E3779B1).
-__host__ __device__ inline uint32_t half_of(uint32_t h)
-{
-    return (h * 0x85EBCA6Bu) >> 31;
-}
-// Slot of tag byte b of word j (0..14) of half hf.
-__host__ __device__ inline uint32_t slot_of(uint32_t hf, uint32_t j, uint32_t b)
-{
-    return hf * kHalfSlots + j * 4u + b;
-}
-// Tag-line word and byte of a slot.
-__host__ __device__ inline uint32_t slot_word(uint32_t slot)
-{
-    return (slot / kHalfSlots) * kHalfWords + (slot % kHalfSlots) / 4u;
 }
 __host__ __device__ inline uint32_t mix32(uint32_t h)
 {
@@ -90,28 +87,33 @@ __host__ __device__ inline uint32_t compact_bytes(uint32_t z)
 }
 
 // ---------------------------------------------------------------- allocator
-// Slab ids come from a lock-free free stack (64-bit head = ABA tag << 32 |
-// slab id, links in freeList[id]) and, when it is empty, from the never-used
-// tail of the pre-reserved arena (bump counter `fresh`).
+// Slab ids come from a free stack (ids in freeList[0, freeCount)) and, when
+// it is empty, from the never-used tail of the pre-reserved arena (bump
+// counter `fresh`).  Pops (chain growth during inserts) and pushes (the GC
+// sweep) never run in the same kernel, so both are plain counter moves,
+// aggregated to one atomic per wave.  A pop may drive freeCount below zero
+// (the lanes past the stack's end take fresh ids instead); the GC clamps it
+// back to zero before it pushes.
 struct AllocState {
-    unsigned long long freeHead; // tag << 32 | id; id == kNoSlab when empty
-    uint32_t fresh;              // next never-used slab id (starts at numBuckets)
-    uint32_t oom;                // set when an allocation fails
-    uint32_t freeCount;          // slabs on the free stack (for admission control)
+    uint32_t fresh;     // next never-used slab id (starts at numBuckets)
+    int32_t freeCount;  // ids on the free stack; <= 0 when empty
+    uint32_t oom;       // set when an allocation fails
     uint32_t pad;
 };
 
+// Every line that is not on a chain (the arena's never-used tail and the
+// free stack) is empty: tag words 0 and next = kNoSlab.
 struct TableDev {
-    uint32_t* tags;      // [slabCap][32]
-    uint32_t* items;     // [slabCap][kSlots][itemWords]
-    uint32_t* active;    // [slabCap][4] GC mark bits
-    uint32_t* freeList;  // [slabCap] free-stack links
-    uint32_t* grow;      // [numBuckets] chain-growth flags
+    uint32_t* lines;     // [slabCap][32]: tags, items, next
+    uint32_t* active;    // [slabCap] GC mark bits, bit = slot
+    uint32_t* freeList;  // [slabCap] free stack of slab ids
     AllocState* alloc;
     uint32_t itemWords;
     uint32_t numBuckets;
     uint32_t slabCap;
-    uint32_t pad;
+    uint32_t slots;      // items per line
+    uint32_t tagWords;   // tag words at the line start
+    uint32_t pad[5];
 };
 
 __device__ inline uint32_t ld_relaxed(const uint32_t* p)
@@ -127,23 +129,35 @@ __device__ inline unsigned long long ld_relaxed64(const unsigned long long* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Pop a recycled slab or bump the arena; never waits on another thread.
-__device__ inline uint32_t slab_alloc(const TableDev& t)
+// Lanes below this one in the wave.
+__device__ inline unsigned long long lanes_below()
 {
+    const uint32_t l = __lane_id();
+    return l ? (~0ull >> (64u - l)) : 0ull;
+}
+
+// Slabs for the lanes of mask m, all of one table: one atomic on the
+// free-stack count and at most one on the bump counter.
+__device__ inline uint32_t slab_alloc_lanes(const TableDev& t, unsigned long long m)
+{
+    const uint32_t k = (uint32_t)__popcll(m);
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+    const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
     AllocState* a = t.alloc;
-    unsigned long long old = ld_relaxed64(&a->freeHead);
-    while ((uint32_t)old != kNoSlab) {
-        const uint32_t id = (uint32_t)old;
-        const uint32_t nxt = ld_relaxed(&t.freeList[id]);
-        const unsigned long long nv = (((old >> 32) + 1ull) << 32) | nxt;
-        const unsigned long long r = atomicCAS(&a->freeHead, old, nv);
-        if (r == old) {
-            atomicSub(&a->freeCount, 1u);
-            return id;
-        }
-        old = r;
+    int top = 0;
+    uint32_t base = 0;
+    if (__lane_id() == leader) {
+        top = atomicSub(&a->freeCount, (int)k);
+        const uint32_t avail = top <= 0 ? 0u : ((uint32_t)top < k ? (uint32_t)top : k);
+        if (avail < k)
+            base = atomicAdd(&a->fresh, k - avail);
     }
-    const uint32_t id = atomicAdd(&a->fresh, 1u);
+    top = __shfl(top, (int)leader, 64);
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    const uint32_t avail = top <= 0 ? 0u : ((uint32_t)top < k ? (uint32_t)top : k);
+    if (rank < avail)
+        return ld_relaxed(&t.freeList[(uint32_t)top - 1u - rank]);
+    const uint32_t id = base + (rank - avail);
     if (id >= t.slabCap) {
         atomicOr(&a->oom, 1u);
         return kNoSlab;
@@ -151,59 +165,64 @@ __device__ inline uint32_t slab_alloc(const TableDev& t)
     return id;
 }
 
-// Push a slab nobody references any more.
-__device__ inline void slab_free(const TableDev& t, uint32_t id)
+// A slab for every active lane that passes need = true, from that lane's
+// table.  Lanes of one wave may insert into different tables (items of
+// different sizes), so requests are served one table at a time, each with
+// one aggregated allocation.  Lanes passing false get kNoSlab; kNoSlab with
+// oom set when the arena is exhausted.
+__device__ inline uint32_t wave_slab_alloc(const TableDev& t, bool need)
 {
-    AllocState* a = t.alloc;
-    unsigned long long old = ld_relaxed64(&a->freeHead);
-    while (true) {
-        st_relaxed(&t.freeList[id], (uint32_t)old);
-        __threadfence();
-        const unsigned long long nv = (((old >> 32) + 1ull) << 32) | id;
-        const unsigned long long r = atomicCAS(&a->freeHead, old, nv);
-        if (r == old) {
-            atomicAdd(&a->freeCount, 1u);
-            return;
-        }
-        old = r;
+    unsigned long long pending = __ballot(need);
+    const uint64_t mine = (uint64_t)(uintptr_t)t.alloc;
+    uint32_t id = kNoSlab;
+    while (pending != 0ull) {
+        const int first = __ffsll((long long)pending) - 1;
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)mine, first, 64);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(mine >> 32), first, 64);
+        const bool take = ((pending >> __lane_id()) & 1ull) != 0ull && mine == (((uint64_t)hi << 32) | lo);
+        const unsigned long long m = __ballot(take);
+        pending &= ~m;
+        if (take)
+            id = slab_alloc_lanes(t, m);
     }
+    return id;
 }
 
-// The chain of `bucket` was seen full with head successor `observed` (read
-// from the head's half-`hf` next word).  The bucket's grow flag (taken by
-// CAS, never waited on) admits one grower at a time: it allocates a slab,
-// clears both halves, and links it right after the head by writing both
-// next copies; only flag holders move a head's successor, so the two copies
-// differ only while a grower holds the flag.  Everyone else rescans.
-// Returns the slab to continue with: the new slab, the head (rescan), or
-// kNoSlab when the arena is exhausted.
-__device__ inline uint32_t chain_grow(const TableDev& t, uint32_t bucket, uint32_t observed, uint32_t hf)
+// Push the slabs of the active lanes passing give = true (one atomic per
+// wave).  Every calling lane must use the same table t (the GC sweep runs
+// one table per launch); only the sweep frees, after freeCount was clamped
+// to >= 0.
+__device__ inline void wave_slab_free(const TableDev& t, bool give, uint32_t id)
 {
-    uint32_t* head = t.tags + (size_t)bucket * kTagWords;
-    uint32_t* mine = head + hf * kHalfWords + kHalfNext;
-    if (ld_relaxed(mine) != observed)
-        return bucket;
-    if (atomicCAS(&t.grow[bucket], 0u, 1u) != 0u)
-        return bucket;
-    if (ld_relaxed(mine) != observed || ld_relaxed(head + kHalfNext) != ld_relaxed(head + kNextWord)) {
-        atomicExch(&t.grow[bucket], 0u);
-        return bucket;
-    }
-    const uint32_t id = slab_alloc(t);
-    if (id == kNoSlab) {
-        atomicExch(&t.grow[bucket], 0u);
-        return kNoSlab;
-    }
-    uint32_t* nl = t.tags + (size_t)id * kTagWords;
-    for (uint32_t i = 0; i < kTagWords; ++i)
-        st_relaxed(nl + i, (i % kHalfWords) == kHalfNext ? observed : 0u);
-    for (uint32_t i = 0; i < 4; ++i)
-        t.active[(size_t)id * 4 + i] = 0u;
-    __threadfence();
-    atomicExch(head + kHalfNext, id);
-    atomicExch(head + kNextWord, id);
-    __threadfence();
-    atomicExch(&t.grow[bucket], 0u);
+    const unsigned long long m = __ballot(give);
+    if (!give)
+        return;
+    const uint32_t k = (uint32_t)__popcll(m);
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+    int top = 0;
+    if (__lane_id() == leader)
+        top = atomicAdd(&t.alloc->freeCount, (int)k);
+    top = __shfl(top, (int)leader, 64);
+    t.freeList[(uint32_t)top + (uint32_t)__popcll(m & lanes_below())] = id;
+}
+
+// A chain was seen full up to its tail line `tail`, whose link is kNoSlab.
+// Each lane with try_ = true claims the link (CAS kNoSlab -> kPendingSlab);
+// the claimers of a wave get their slabs from one aggregated allocation and
+// publish them with one atomic store each.  The new line is appended at the
+// tail and was already empty (tags 0, next kNoSlab) before this kernel, so
+// a reader that follows the link sees a valid line without any fence.
+// Returns the new slab, `tail` to re-read it (the claim was lost), or
+// kNoSlab when the arena is exhausted (the link is restored).  Lanes with
+// try_ = false only take part in the wave's allocation and get kNoSlab.
+__device__ inline uint32_t chain_grow(const TableDev& t, uint32_t tail, bool try_)
+{
+    uint32_t* link = t.lines + (size_t)tail * kLineWords + kNextWord;
+    const bool mine = try_ && atomicCAS(link, kNoSlab, kPendingSlab) == kNoSlab;
+    const uint32_t id = wave_slab_alloc(t, mine);
+    if (!mine)
+        return try_ ? tail : kNoSlab;
+    st_relaxed(link, id);
     return id;
 }
 
@@ -278,232 +297,168 @@ __device__ inline bool needle_equal_lane(const Needle<G, MW>& n, const uint32_t*
     return eq;
 }
 
-// A group's view of one half line: lane gl holds words gl*HW .. gl*HW+HW-1
-// of the 16 (groups wider than 16 lanes leave the upper lanes empty).
-template <int G>
-struct HalfLine {
-    static constexpr int HW = G >= 16 ? 1 : 16 / G;
-    uint32_t w[HW];
+// The tag words of a line (at most 4) and its next slab id.  One 16-byte
+// load: the line is 128-byte aligned and tagWords <= 4.
+struct LineHead {
+    uint32_t tag[4];
+    uint32_t next;
 };
-
-template <int G>
-__device__ inline void halfline_load(HalfLine<G>& L, const uint32_t* half, uint32_t gl)
+__device__ inline void line_head_load(LineHead& L, const uint32_t* line)
 {
-    constexpr int HW = HalfLine<G>::HW;
-    if (gl * HW >= kHalfWords) {
-#pragma unroll
-        for (int i = 0; i < HW; ++i)
-            L.w[i] = 0u;
-        return;
-    }
-    const uint32_t* p = half + gl * HW;
-    if constexpr (HW == 1) {
-        L.w[0] = p[0];
-    } else if constexpr (HW == 2) {
-        const uint2 v = *reinterpret_cast<const uint2*>(p);
-        L.w[0] = v.x;
-        L.w[1] = v.y;
-    } else {
-#pragma unroll
-        for (int q = 0; q < HW / 4; ++q) {
-            const uint4 v = reinterpret_cast<const uint4*>(p)[q];
-            L.w[4 * q + 0] = v.x;
-            L.w[4 * q + 1] = v.y;
-            L.w[4 * q + 2] = v.z;
-            L.w[4 * q + 3] = v.w;
-        }
-    }
+    const uint4 v = *reinterpret_cast<const uint4*>(line);
+    L.tag[0] = v.x;
+    L.tag[1] = v.y;
+    L.tag[2] = v.z;
+    L.tag[3] = v.w;
+    L.next = line[kNextWord];
 }
-
-// The half line's next slab id (word 15), broadcast to the group.
-template <int G>
-__device__ inline uint32_t halfline_next(const HalfLine<G>& L)
+// Bit 4j+b set where tag byte b of tag word j equals tag4's byte and the
+// slot exists.
+__device__ inline uint32_t line_match(const TableDev& t, const LineHead& L, uint32_t tag4)
 {
-    constexpr int HW = HalfLine<G>::HW;
-    return group_bcast<G>(L.w[HW - 1], kHalfNext / HW);
-}
-
-// Candidate bits (4 per word, word i of this lane at bits 4i..4i+3) of the
-// slots whose tag equals tag4 (tag replicated into 4 bytes).
-template <int G>
-__device__ inline uint32_t halfline_match(const HalfLine<G>& L, uint32_t tag4, uint32_t gl)
-{
-    constexpr int HW = HalfLine<G>::HW;
     uint32_t cand = 0;
 #pragma unroll
-    for (int i = 0; i < HW; ++i) {
-        const uint32_t j = gl * HW + i;
-        if (j < kHalfNext)
-            cand |= compact_bytes(zero_bytes(L.w[i] ^ tag4)) << (4 * i);
-    }
-    return cand;
+    for (uint32_t j = 0; j < 4; ++j)
+        if (j < t.tagWords)
+            cand |= compact_bytes(zero_bytes(L.tag[j] ^ tag4)) << (4 * j);
+    return cand & ((1u << t.slots) - 1u);
+}
+__device__ inline uint32_t line_empty(const TableDev& t, const uint32_t* tagWordsOfLine, uint32_t& empties)
+{
+    empties = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j)
+        if (j < t.tagWords)
+            empties |= compact_bytes(zero_bytes(tagWordsOfLine[j])) << (4 * j);
+    empties &= (1u << t.slots) - 1u;
+    return empties;
+}
+__host__ __device__ inline const uint32_t* line_item(const TableDev& t, uint32_t slab, uint32_t slot)
+{
+    return t.lines + (size_t)slab * kLineWords + t.tagWords + slot * t.itemWords;
+}
+__host__ __device__ inline const uint32_t* table_item_ptr(const TableDev& t, uint32_t ptr)
+{
+    return line_item(t, ptr >> kSlotBits, ptr & ((1u << kSlotBits) - 1u));
 }
 
-// Walk a chain in half hf starting at `slab` whose half line is already in
-// L; compares candidates one at a time with the whole group.
+// Walk a chain from `slab` whose line head is already in L; the group
+// compares candidates one at a time (every lane holds the same line head).
 template <int G, int MW>
-__device__ inline uint32_t probe_chain(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t tag4, uint32_t hf,
-                                       uint32_t slab, HalfLine<G> L, uint32_t gl, uint32_t gbase)
+__device__ inline uint32_t probe_chain(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t tag4,
+                                       uint32_t slab, LineHead L, uint32_t gl, uint32_t gbase)
 {
-    constexpr int HW = HalfLine<G>::HW;
     while (true) {
-        uint32_t cand = halfline_match<G>(L, tag4, gl);
-        const uint32_t next = halfline_next<G>(L);
-        while (true) {
-            const uint32_t bal = group_ballot<G>(cand != 0, gbase);
-            if (!bal)
-                break;
-            const uint32_t leader = __ffs(bal) - 1u;
-            uint32_t mySlot = 0;
-            if (cand) {
-                const uint32_t bit = __ffs(cand) - 1u;
-                mySlot = slot_of(hf, gl * HW + (bit >> 2), bit & 3u);
-            }
-            const uint32_t slot = group_bcast<G>(mySlot, leader);
-            if (gl == leader)
-                cand &= cand - 1u;
-            const uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
-            const bool eq = needle_equal_lane<G>(n, item, s, gl);
+        uint32_t cand = line_match(t, L, tag4);
+        while (cand) {
+            const uint32_t slot = __ffs(cand) - 1u;
+            cand &= cand - 1u;
+            const bool eq = needle_equal_lane<G>(n, line_item(t, slab, slot), s, gl);
             if (group_ballot<G>(!eq, gbase) == 0)
                 return (slab << kSlotBits) | slot;
         }
-        if (next == kNoSlab)
+        if (L.next >= kPendingSlab)
             return kNotFound;
-        slab = next;
-        halfline_load<G>(L, t.tags + (size_t)slab * kTagWords + hf * kHalfWords, gl);
+        slab = L.next;
+        line_head_load(L, t.lines + (size_t)slab * kLineWords);
     }
 }
 
-// Group-cooperative find (G in {4, 8, 16, 32}): one 64-byte half line per
-// slab, read once by the group.
+// Group-cooperative find (G in {4, 8, 16, 32}).
 template <int G, int MW>
 __device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
                                             uint32_t gl, uint32_t gbase)
 {
     static_assert(G >= 4 && G <= 32, "group size");
-    const uint32_t hf = half_of(h);
     const uint32_t slab = h % t.numBuckets;
-    HalfLine<G> L;
-    halfline_load<G>(L, t.tags + (size_t)slab * kTagWords + hf * kHalfWords, gl);
-    return probe_chain<G, MW>(t, n, s, make_tag(h) * 0x01010101u, hf, slab, L, gl, gbase);
+    LineHead L;
+    line_head_load(L, t.lines + (size_t)slab * kLineWords);
+    return probe_chain<G, MW>(t, n, s, make_tag(h) * 0x01010101u, slab, L, gl, gbase);
 }
 
-// Per-lane find: one lane reads the 64-byte half line with 16-byte loads.
+// Per-lane find.
 __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* needle, uint32_t s, uint32_t h)
 {
     const uint32_t tag4 = make_tag(h) * 0x01010101u;
-    const uint32_t hf = half_of(h);
     uint32_t slab = h % t.numBuckets;
     while (true) {
-        const uint4* line = reinterpret_cast<const uint4*>(t.tags + (size_t)slab * kTagWords + hf * kHalfWords);
-        uint32_t next = kNoSlab;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint4 v = line[q];
-            const uint32_t ws[4] = { v.x, v.y, v.z, v.w };
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t j = (uint32_t)(q * 4 + i);
-                if (j == kHalfNext) {
-                    next = ws[i];
-                    continue;
-                }
-                uint32_t c = compact_bytes(zero_bytes(ws[i] ^ tag4));
-                while (c) {
-                    const uint32_t slot = slot_of(hf, j, __ffs(c) - 1u);
-                    c &= c - 1u;
-                    const uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
-                    bool eq = true;
-                    for (uint32_t k = 0; k < s && eq; ++k)
-                        eq = item[k] == needle[k];
-                    if (eq)
-                        return (slab << kSlotBits) | slot;
-                }
-            }
+        LineHead L;
+        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+        uint32_t cand = line_match(t, L, tag4);
+        while (cand) {
+            const uint32_t slot = __ffs(cand) - 1u;
+            cand &= cand - 1u;
+            const uint32_t* item = line_item(t, slab, slot);
+            bool eq = true;
+            for (uint32_t k = 0; k < s && eq; ++k)
+                eq = item[k] == needle[k];
+            if (eq)
+                return (slab << kSlotBits) | slot;
         }
-        if (next == kNoSlab)
+        if (L.next >= kPendingSlab)
             return kNotFound;
-        slab = next;
+        slab = L.next;
     }
 }
 
 // Group-cooperative insert of an item the caller knows to be absent (or, for
-// bulk multiset inserts, allowed to be present).  Claims a slot of the
-// item's half with a CAS on its tag word; a chain whose halves `hf` are all
-// full grows through chain_grow.
+// bulk multiset inserts, allowed to be present).  The group's first lane
+// claims an empty slot with a CAS on its tag word and the group writes the
+// item; a chain that is full up to its tail grows through chain_grow.
 template <int G, int MW>
 __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
                                               uint32_t gl, uint32_t gbase)
 {
     static_assert(G >= 4 && G <= 32, "group size");
-    constexpr int HW = HalfLine<G>::HW;
     const uint32_t tag = make_tag(h);
-    const uint32_t hf = half_of(h);
     const uint32_t bucket = h % t.numBuckets;
     uint32_t slab = bucket;
-    uint32_t headNext = kNoSlab;
     while (true) {
-        uint32_t* half = t.tags + (size_t)slab * kTagWords + hf * kHalfWords;
-        HalfLine<G> L;
-        if (gl * HW < kHalfWords) {
-#pragma unroll
-            for (int i = 0; i < HW; ++i)
-                L.w[i] = ld_relaxed(half + gl * HW + i);
-        } else {
-#pragma unroll
-            for (int i = 0; i < HW; ++i)
-                L.w[i] = 0u;
-        }
-        uint32_t emp = 0;
-#pragma unroll
-        for (int i = 0; i < HW; ++i) {
-            const uint32_t j = gl * HW + i;
-            if (j < kHalfNext)
-                emp |= compact_bytes(zero_bytes(L.w[i])) << (4 * i);
-        }
-        const uint32_t next = halfline_next<G>(L);
-        const uint32_t bal = group_ballot<G>(emp != 0, gbase);
-        if (bal) {
-            const uint32_t leader = __ffs(bal) - 1u;
-            uint32_t ok = 0, slot = 0;
-            if (gl == leader) {
+        uint32_t* line = t.lines + (size_t)slab * kLineWords;
+        uint32_t ok = 0, slot = 0, next = 0, full = 0;
+        if (gl == 0) {
+            uint32_t tw[4];
+            for (uint32_t j = 0; j < 4; ++j)
+                tw[j] = j < t.tagWords ? ld_relaxed(line + j) : 0u;
+            next = ld_relaxed(line + kNextWord);
+            uint32_t emp;
+            line_empty(t, tw, emp);
+            if (emp) {
                 const uint32_t bit = __ffs(emp) - 1u;
-                const uint32_t i = bit >> 2, b = bit & 3u;
+                const uint32_t j = bit >> 2, b = bit & 3u;
                 uint32_t old = 0;
-#pragma unroll
-                for (int q = 0; q < HW; ++q)
-                    if ((uint32_t)q == i)
-                        old = L.w[q];
-                const uint32_t r = atomicCAS(half + gl * HW + i, old, old | (tag << (8u * b)));
-                ok = (r == old) ? 1u : 0u;
-                slot = slot_of(hf, gl * HW + i, b);
+                for (uint32_t q = 0; q < 4; ++q)
+                    if (q == j)
+                        old = tw[q];
+                ok = atomicCAS(line + j, old, old | (tag << (8u * b))) == old ? 1u : 0u;
+                slot = bit;
+            } else {
+                full = 1u;
             }
-            ok = group_bcast<G>(ok, leader);
-            slot = group_bcast<G>(slot, leader);
-            if (ok) {
-                uint32_t* item = t.items + ((size_t)slab * kSlots + slot) * s;
-#pragma unroll
-                for (int k = 0; k < Needle<G, MW>::K; ++k) {
-                    const uint32_t j = gl + (uint32_t)k * G;
-                    if (j < s)
-                        item[j] = n.w[k];
-                }
-                return (slab << kSlotBits) | slot;
-            }
-            continue;
         }
-        if (slab == bucket)
-            headNext = next;
+        ok = group_bcast<G>(ok, 0);
+        full = group_bcast<G>(full, 0);
+        slot = group_bcast<G>(slot, 0);
+        next = group_bcast<G>(next, 0);
+        if (ok) {
+            uint32_t* item = line + t.tagWords + slot * s;
+#pragma unroll
+            for (int k = 0; k < Needle<G, MW>::K; ++k) {
+                const uint32_t j = gl + (uint32_t)k * G;
+                if (j < s)
+                    item[j] = n.w[k];
+            }
+            return (slab << kSlotBits) | slot;
+        }
+        if (!full || next == kPendingSlab)
+            continue; // lost the CAS, or a successor is being linked: re-read the line
         if (next != kNoSlab) {
             slab = next;
             continue;
         }
-        // One group per wave grows a chain per round; the others rescan.
-        const unsigned long long growers = __ballot(gl == 0);
-        uint32_t grown = bucket;
-        if (gl == 0 && (uint32_t)(__ffsll((long long)growers) - 1) == __lane_id())
-            grown = chain_grow(t, bucket, headNext, hf);
-        slab = group_bcast<G>(grown, 0);
+        // Every group at a full tail competes for its link; the winners'
+        // slabs are allocated together.
+        slab = group_bcast<G>(chain_grow(t, slab, gl == 0), 0);
         if (slab == kNoSlab)
             return kNotFound;
     }
@@ -513,53 +468,40 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G,
 __device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* item, uint32_t s, uint32_t h)
 {
     const uint32_t tag = make_tag(h);
-    const uint32_t hf = half_of(h);
     const uint32_t bucket = h % t.numBuckets;
     uint32_t slab = bucket;
-    uint32_t headNext = kNoSlab;
     while (true) {
-        uint32_t* half = t.tags + (size_t)slab * kTagWords + hf * kHalfWords;
-        bool retry = false;
-        for (uint32_t w = 0; w < kHalfNext; ++w) {
-            const uint32_t v = ld_relaxed(half + w);
-            uint32_t e = compact_bytes(zero_bytes(v));
-            if (!e)
+        uint32_t* line = t.lines + (size_t)slab * kLineWords;
+        uint32_t tw[4];
+        for (uint32_t j = 0; j < 4; ++j)
+            tw[j] = j < t.tagWords ? ld_relaxed(line + j) : 0u;
+        uint32_t emp;
+        line_empty(t, tw, emp);
+        if (emp) {
+            const uint32_t bit = __ffs(emp) - 1u;
+            const uint32_t j = bit >> 2, b = bit & 3u;
+            uint32_t old = 0;
+            for (uint32_t q = 0; q < 4; ++q)
+                if (q == j)
+                    old = tw[q];
+            if (atomicCAS(line + j, old, old | (tag << (8u * b))) != old)
                 continue;
-            const uint32_t b = __ffs(e) - 1u;
-            const uint32_t r = atomicCAS(half + w, v, v | (tag << (8u * b)));
-            if (r != v) {
-                retry = true;
-                break;
-            }
-            const uint32_t slot = slot_of(hf, w, b);
-            uint32_t* dst = t.items + ((size_t)slab * kSlots + slot) * s;
-            for (uint32_t j = 0; j < s; ++j)
-                dst[j] = item[j];
-            return (slab << kSlotBits) | slot;
+            uint32_t* dst = line + t.tagWords + bit * s;
+            for (uint32_t k = 0; k < s; ++k)
+                dst[k] = item[k];
+            return (slab << kSlotBits) | bit;
         }
-        if (retry)
-            continue;
-        const uint32_t next = ld_relaxed(half + kHalfNext);
-        if (slab == bucket)
-            headNext = next;
+        const uint32_t next = ld_relaxed(line + kNextWord);
+        if (next == kPendingSlab)
+            continue; // a successor is being linked: re-read the line
         if (next != kNoSlab) {
             slab = next;
             continue;
         }
-        // One lane per wave grows a chain per round; the others rescan.
-        const unsigned long long growers = __ballot(1);
-        slab = bucket;
-        if ((uint32_t)(__ffsll((long long)growers) - 1) == __lane_id())
-            slab = chain_grow(t, bucket, headNext, hf);
+        slab = chain_grow(t, slab, true);
         if (slab == kNoSlab)
             return kNotFound;
     }
-}
-
-__host__ __device__ inline const uint32_t* table_item_ptr(const TableDev& t, uint32_t ptr)
-{
-    const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
-    return t.items + ((size_t)slab * kSlots + slot) * t.itemWords;
 }
 
 // ---------------------------------------------------------------- DAG encoding

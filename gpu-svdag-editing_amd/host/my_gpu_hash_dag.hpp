@@ -53,7 +53,7 @@ class MyGPUHashDAG {
 public:
     static constexpr uint32_t invalid_handle = 0xFFFFFFFFu;
 
-    explicit MyGPUHashDAG(uint32_t levels, uint32_t materialBits = 4, uint32_t loadFactor = 96)
+    explicit MyGPUHashDAG(uint32_t levels, uint32_t materialBits = 4, uint32_t loadFactor = 75)
     {
         svdag_config cfg {};
         cfg.levels = levels;

@@ -248,7 +248,7 @@ class MyGPUHashDAG:
     """Editable DAG on the GPU (reference: MyGPUHashDAG<GPU_Malloc> with
     MyGPUHashDAGUndoRedo, editMyHashDag and DAGTracer::resolve_paths)."""
 
-    def __init__(self, levels, material_bits=4, load_factor=96, group_size=0, expected_items=0,
+    def __init__(self, levels, material_bits=4, load_factor=75, group_size=0, expected_items=0,
                  arena_slabs=0, workspace_items=0):
         cfg = Config(levels, material_bits, load_factor, group_size, expected_items, arena_slabs, workspace_items)
         self.h = ctypes.c_void_p()

@@ -55,9 +55,19 @@ def slab_hash_finds(items, n_finds, hit_rate=0.5, hit_seed=4, miss_seed=5):
     return out
 
 
-def buckets_for(items, load_factor=96):
-    """Bucket count per size, ceil(n_s / load_factor) (reference TARGET_LOAD_FACTOR)."""
-    return {s: max(1, -(-len(a) // load_factor)) for s, a in items.items()}
+def line_slots(s):
+    """Items of s words per 128-byte table line: the largest k with
+    k*s + ceil(k/4) <= 31 (tags, items and the chain link share the line)."""
+    k = 1
+    while (k + 1) * s + (k + 4) // 4 <= 31:
+        k += 1
+    return k
+
+
+def buckets_for(items, fill=0.75):
+    """Bucket count per size: the items fill `fill` of the head lines' slots,
+    ceil(n_s / (line_slots(s) * fill))."""
+    return {s: max(1, int(np.ceil(len(a) / (line_slots(s) * fill)))) for s, a in items.items()}
 
 
 # ---------------------------------------------------------------- editing workloads

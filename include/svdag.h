@@ -95,7 +95,8 @@ typedef struct svdag_ctx svdag_ctx;
 typedef struct {
     uint32_t levels;           /* D: the voxel grid is 2^D per side (reference MAX_LEVELS) */
     uint32_t material_bits;    /* 0, 4 or 8 (reference HASH_DAG_MATERIAL_BITS) */
-    uint32_t load_factor;      /* items per bucket when sizing tables (reference TARGET_LOAD_FACTOR=96) */
+    uint32_t load_factor;      /* percent of each bucket's head-line slots filled at expected_items (0 = 75;
+                                  replaces the reference's TARGET_LOAD_FACTOR=96 items per 32-slot bucket) */
     uint32_t group_size;       /* lanes per item in DAG find/insert: 0 = default, 1, 4, 8, 16, 32 */
     uint64_t expected_items;   /* expected DAG items, sizes the bucket count (0 = 1<<20) */
     uint64_t arena_slabs;      /* overflow slab reserve per table (0 = buckets/2 + 4096) */

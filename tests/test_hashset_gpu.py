@@ -10,9 +10,9 @@ import workloads
 VARIANTS = [0, 4, 8, 16, 32]
 
 
-def _build(items, load_factor=96, reserve_scale=1):
+def _build(items, fill=0.75, reserve_scale=1):
     sizes = list(range(2, 10))
-    b = workloads.buckets_for(items, load_factor)
+    b = workloads.buckets_for(items, fill)
     hs = svdag_py.HashSet(2, 9, [b.get(s, 1) for s in sizes],
                           [reserve_scale * len(items.get(s, [])) for s in sizes])
     return hs, b
@@ -95,8 +95,8 @@ def test_empty_batch_and_oom_is_reported():
     hs = svdag_py.HashSet(2, 2, [1], [0])
     usage = hs.usage(2)
     assert usage[0] == 1 and usage[2] == 1
-    # 1 head slab + a reserve of 64 overflow slabs: 124 * 65 items fit, more do not.
-    cap_items = 124 * usage[1]
+    # 1 head line + a reserve of 64 overflow lines of 13 two-word items: 13 * 65 fit, more do not.
+    cap_items = workloads.line_slots(2) * usage[1]
     items = np.stack([np.full(cap_items + 500, 1, np.uint32),
                       np.arange(cap_items + 500, dtype=np.uint32)], axis=1)
     words, segs, n = svdag_py.make_segments([(2, items)])
