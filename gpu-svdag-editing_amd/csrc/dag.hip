@@ -469,12 +469,13 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
             ws.rep[w] = rep;
         if (rep == w) {
             // A node with a child this merge inserted is new: no find.
-            uint32_t r = kNotFound;
+            uint32_t r = kNotFound, hint = kNoSlab;
             if (isLeaf || rec[9] == 0u)
-                r = table_find_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
+                r = table_find_group_hint<G>(dag.tables[s - 2], n, s, h, gl, gbase, hint);
             if (gl == 0) {
                 ws.handle[w] = (r == kNotFound) ? kNotFound : (isLeaf ? leaf_handle(dag, r, s) : node_handle(r, s));
                 ws.created[w] = 0;
+                ws.hint[w] = hint;
             }
         }
     }
@@ -504,7 +505,7 @@ __global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32
         Needle<G> n;
         needle_load(n, rec, s, gl);
         const uint32_t h = needle_hash(n, s, gl);
-        const uint32_t r = table_insert_group<G>(dag.tables[s - 2], n, s, h, gl, gbase);
+        const uint32_t r = table_insert_group<G>(dag.tables[s - 2], n, s, h, gl, gbase, ws.hint[w]);
         if (gl == 0) {
             if (r == kNotFound) {
                 atomicOr(&ws.status->oom, 1u);
@@ -675,7 +676,7 @@ static void ws_release(WsDev& ws)
 {
     for (void* p : { (void*)ws.work, (void*)ws.state, (void*)ws.nodes, (void*)ws.leaves, (void*)ws.rep,
                      (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status, (void*)ws.editOf,
-                     (void*)ws.created })
+                     (void*)ws.created, (void*)ws.hint })
         if (p)
             (void)hipFree(p);
     ws = WsDev {};
@@ -707,6 +708,7 @@ static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
     SVDAG_HIP(hipMalloc(&ws.handle, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.editOf, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.created, (size_t)capWork));
+    SVDAG_HIP(hipMalloc(&ws.hint, (size_t)capWork * 4));
     SVDAG_HIP(hipMalloc(&ws.dedup, (size_t)ws.dedupCap * 8));
     SVDAG_HIP(hipMalloc(&ws.lvl, kMaxLevels * sizeof(LevelDesc)));
     SVDAG_HIP(hipMalloc(&ws.status, sizeof(EditStatus)));

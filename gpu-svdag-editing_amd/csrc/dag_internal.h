@@ -48,6 +48,9 @@ struct WsDev {
     // 1 where the merge inserted representative w (0 where it found it); a
     // parent with such a child cannot be in the DAG yet.
     uint8_t* created;            // [capWork]
+    // Where the insert of a representative the DAG find missed starts: the
+    // first line of its chain the find saw with a free slot (kNoSlab: head).
+    uint32_t* hint;              // [capWork]
 };
 
 // Level-S cell range of an edit in a fused wave: cells lo .. lo + n - 1 per

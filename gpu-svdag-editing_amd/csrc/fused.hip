@@ -322,11 +322,11 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                 needle_load(nd, rec, s, c8);
                 const uint32_t h = needle_hash(nd, s, c8);
                 // A node with a child inserted by this merge is new: no find.
-                uint32_t r = kNotFound;
+                uint32_t r = kNotFound, hint = kNoSlab;
                 if (sHasNew[i8] == 0u)
-                    r = table_find_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
+                    r = table_find_group_hint<8>(dag.tables[s - 2], nd, s, h, c8, gbase, hint);
                 if (r == kNotFound) {
-                    r = table_insert_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase);
+                    r = table_insert_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase, hint);
                     created = 1u;
                 }
                 if (r == kNotFound)

@@ -12,12 +12,13 @@
 // go through FrameQuiesce.
 //
 // Admission.  A request's demand is estimated from its bounds: an upper
-// bound on the edit-SVO items of its surface, 6 (span/4)^2 * 4/3, times the
-// measured slabs-per-item ratio of earlier batches.  Headroom is the
-// smallest free fraction over the tables times their capacity, minus a
-// reserve.  When the head request does not fit, the batch thread collects
-// garbage (if allowed) and then grows the arenas; the edit pipeline's own
-// grow-and-retry remains the last line.
+// bound on the edit-SVO items of its surface, 6 (span/4)^2 * 4/3.  Each
+// table converts items to slabs with its own measured rate (slabs the table
+// gained per edit-SVO item over earlier batches), and a prefix of requests
+// is admitted while every table's demand fits its headroom (free slabs
+// minus a reserve fraction of its capacity).  When the head request does
+// not fit, the batch thread collects garbage (if allowed) and then grows the
+// arenas; the edit pipeline's own grow-and-retry remains the last line.
 #include "dag_internal.h"
 #include <algorithm>
 #include <atomic>
@@ -33,7 +34,7 @@ using Clock = std::chrono::steady_clock;
 struct Request {
     uint64_t ticket;
     EditorDev ed;
-    double estSlabs;
+    double estItems;
 };
 
 struct Server {
@@ -51,7 +52,7 @@ struct Server {
     std::thread batchThread, frameThread;
     svdag_server_stats stats {};
     std::atomic<uint32_t> published { kSentinel };
-    std::atomic<double> slabsPerItem { 1.0 / 4.0 }; // written by the batch thread, read by submitters
+    std::vector<double> slabsPerItem; // per table (batch thread only)
     // Frame state (frame thread only, plus frameMu for launches).
     hipEvent_t fStart = nullptr, fEnd = nullptr;
     void* dFrame = nullptr;
@@ -67,14 +68,13 @@ static double seconds_since(const Clock::time_point& t0)
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-static double estimate_slabs(const svdag_ctx* ctx, const EditorDev& ed, double slabsPerItem)
+static double estimate_items(const svdag_ctx* ctx, const EditorDev& ed)
 {
     const Bounds b = edit_bounds(ctx, ed);
     double span = 0.0;
     for (int a = 0; a < 3; ++a)
         span = std::max(span, (double)(b.hi[a] - b.lo[a] + 1));
-    const double items = 6.0 * (span / 4.0) * (span / 4.0) * 4.0 / 3.0 + 64.0;
-    return items * slabsPerItem;
+    return 6.0 * (span / 4.0) * (span / 4.0) * 4.0 / 3.0 + 64.0;
 }
 
 static int read_memory(svdag_ctx* ctx, svdag_memory& m)
@@ -95,26 +95,35 @@ static int read_memory(svdag_ctx* ctx, svdag_memory& m)
     return SVDAG_OK;
 }
 
-// Headroom in slabs of the tightest table: min over tables of
-// free - reserve * capacity.
-static double headroom(svdag_ctx* ctx, double reserveFraction, svdag_memory& m)
+// Per table: slabs in use and headroom (free - reserve * capacity).
+struct TableRoom {
+    std::vector<uint64_t> used;
+    std::vector<double> room;
+};
+static int headroom(svdag_ctx* ctx, double reserveFraction, TableRoom& r)
 {
-    m = svdag_memory {};
-    m.min_free_fraction = 1.0;
-    double room = 1e300;
-    for (const HostTable& t : ctx->ts.tables) {
+    const size_t nt = ctx->ts.tables.size();
+    r.used.assign(nt, 0);
+    r.room.assign(nt, 0.0);
+    for (size_t i = 0; i < nt; ++i) {
+        const HostTable& t = ctx->ts.tables[i];
         AllocState a {};
-        if (table_read_alloc(t, a, ctx->stream) != SVDAG_OK)
-            return 0.0;
+        SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
         const uint64_t fresh = std::min<uint64_t>(a.fresh, t.slabCap);
         const uint64_t freeSlabs = (t.slabCap - fresh) + (uint64_t)std::max(a.freeCount, 0);
-        m.slab_capacity += t.slabCap;
-        m.slabs_free += freeSlabs;
-        m.slabs_used += t.slabCap - freeSlabs;
-        m.min_free_fraction = std::min(m.min_free_fraction, (double)freeSlabs / (double)t.slabCap);
-        room = std::min(room, (double)freeSlabs - reserveFraction * (double)t.slabCap);
+        r.used[i] = t.slabCap - freeSlabs;
+        r.room[i] = (double)freeSlabs - reserveFraction * (double)t.slabCap;
     }
-    return room;
+    return SVDAG_OK;
+}
+
+// Whether `items` edit-SVO items fit every table's headroom.
+static bool fits(const Server* srv, const TableRoom& r, double items)
+{
+    for (size_t i = 0; i < r.room.size(); ++i)
+        if (items * srv->slabsPerItem[i] > r.room[i])
+            return false;
+    return true;
 }
 
 static void frame_loop(svdag_ctx* ctx, Server* srv)
@@ -189,35 +198,44 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
             }
         }
         std::unique_lock<std::mutex> ctxLock(ctx->mu);
-        // Admission against the tightest table's headroom.
-        svdag_memory m {};
-        double room = headroom(ctx, reserve, m);
-        double need = batch.front().estSlabs;
-        if (need > room && srv->cfg.allow_gc) {
+        // Admission against every table's headroom.
+        TableRoom room;
+        if (srv->slabsPerItem.size() != ctx->ts.tables.size()) {
+            // Before any measurement: one slot per item in every table.
+            srv->slabsPerItem.resize(ctx->ts.tables.size());
+            for (size_t i = 0; i < ctx->ts.tables.size(); ++i)
+                srv->slabsPerItem[i] = 1.0 / (double)line_slots(ctx->ts.tables[i].itemWords);
+        }
+        (void)headroom(ctx, reserve, room);
+        const double need = batch.front().estItems;
+        if (!fits(srv, room, need) && srv->cfg.allow_gc) {
             ctx->undo.clear();
             ctx->redo.clear();
             svdag_gc_stats gs {};
             FrameQuiesce q(ctx);
             if (gc_run(ctx, &gs) == SVDAG_OK)
                 srv->stats.gcs++;
-            room = headroom(ctx, reserve, m);
+            (void)headroom(ctx, reserve, room);
         }
-        if (need > room) {
+        if (!fits(srv, room, need)) {
             FrameQuiesce q(ctx);
             bool grew = false;
-            for (auto& t : ctx->ts.tables) {
+            for (size_t i = 0; i < ctx->ts.tables.size(); ++i) {
+                HostTable& t = ctx->ts.tables[i];
+                if (need * srv->slabsPerItem[i] <= room.room[i])
+                    continue;
                 const uint64_t target = std::min<uint64_t>((uint64_t)t.slabCap * 2, ctx->slabLimit);
                 if (target > t.slabCap && table_grow(t, (uint32_t)target, ctx->stream) == SVDAG_OK)
                     grew = true;
             }
             if (grew && ctx_refresh_tables(ctx) == SVDAG_OK)
                 srv->stats.grows++;
-            room = headroom(ctx, reserve, m);
+            (void)headroom(ctx, reserve, room);
         }
         size_t admit = 1; // the head request always runs (the pipeline grows on OOM)
         double sum = need;
-        while (admit < batch.size() && sum + batch[admit].estSlabs <= room) {
-            sum += batch[admit].estSlabs;
+        while (admit < batch.size() && fits(srv, room, sum + batch[admit].estItems)) {
+            sum += batch[admit].estItems;
             ++admit;
         }
         if (admit < batch.size()) {
@@ -232,7 +250,6 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
         st.assign(batch.size(), svdag_edit_stats {});
         for (size_t i = 0; i < batch.size(); ++i)
             eds[i] = batch[i].ed;
-        const uint64_t usedBefore = m.slabs_used;
         const auto b0 = Clock::now();
         const int rc = run_edits(ctx, eds.data(), (uint32_t)eds.size(), st.data(), nullptr);
         srv->stats.edit_busy_s += std::chrono::duration<double>(Clock::now() - b0).count();
@@ -242,14 +259,19 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                     ctx->redo.clear();
                     ctx->undo.push_back(st[i].root_before);
                 }
-            svdag_memory after {};
             uint64_t items = 0;
             for (const auto& s : st)
                 for (uint32_t L = 0; L < 32; ++L)
                     items += s.svo_items[L];
-            if (read_memory(ctx, after) == SVDAG_OK && items > 0 && after.slabs_used >= usedBefore) {
-                const double r = (double)(after.slabs_used - usedBefore) / (double)items;
-                srv->slabsPerItem.store(std::max(1.0 / (double)kMaxSlots, 0.5 * srv->slabsPerItem.load() + 0.5 * r));
+            TableRoom after;
+            if (items > 0 && headroom(ctx, reserve, after) == SVDAG_OK) {
+                // Slabs each table gained per item; a table that gained none
+                // keeps a small floor so a sudden demand still counts.
+                for (size_t i = 0; i < after.used.size() && i < room.used.size(); ++i) {
+                    const double gained = after.used[i] >= room.used[i] ? (double)(after.used[i] - room.used[i]) : 0.0;
+                    const double r = gained / (double)items;
+                    srv->slabsPerItem[i] = std::max(1e-3, 0.5 * srv->slabsPerItem[i] + 0.5 * r);
+                }
             }
         }
         srv->published.store(ctx->root);
@@ -358,7 +380,7 @@ int svdag_server_submit(svdag_ctx* ctx, const svdag_edit_desc* edit, uint64_t* t
     Server* srv = ctx->server;
     Request r {};
     SVDAG_TRY(make_editor(ctx, *edit, r.ed));
-    r.estSlabs = estimate_slabs(ctx, r.ed, srv->slabsPerItem.load());
+    r.estItems = estimate_items(ctx, r.ed);
     {
         std::lock_guard<std::mutex> lk(srv->qMu);
         if (srv->stopping) {

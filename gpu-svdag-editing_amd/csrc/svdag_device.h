@@ -376,6 +376,44 @@ __device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, M
     return probe_chain<G, MW>(t, n, s, make_tag(h) * 0x01010101u, slab, L, gl, gbase);
 }
 
+// Group find that also reports where an insert of the needle should start:
+// the first line of the chain whose tags showed a free slot, else the tail.
+// The hint stays on the bucket's chain (chains only grow at the tail, and
+// lines leave a chain only in a collection), so an insert that starts there
+// and walks on finds every free slot it would have found from the head
+// after that line -- it only skips lines the find saw full.
+template <int G, int MW>
+__device__ inline uint32_t table_find_group_hint(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
+                                                 uint32_t gl, uint32_t gbase, uint32_t& hint)
+{
+    static_assert(G >= 4 && G <= 32, "group size");
+    const uint32_t tag4 = make_tag(h) * 0x01010101u;
+    uint32_t slab = h % t.numBuckets;
+    hint = kNoSlab;
+    LineHead L;
+    line_head_load(L, t.lines + (size_t)slab * kLineWords);
+    while (true) {
+        uint32_t cand = line_match(t, L, tag4);
+        while (cand) {
+            const uint32_t slot = __ffs(cand) - 1u;
+            cand &= cand - 1u;
+            const bool eq = needle_equal_lane<G>(n, line_item(t, slab, slot), s, gl);
+            if (group_ballot<G>(!eq, gbase) == 0)
+                return (slab << kSlotBits) | slot;
+        }
+        uint32_t emp;
+        if (hint == kNoSlab && line_empty(t, L.tag, emp))
+            hint = slab;
+        if (L.next >= kPendingSlab) {
+            if (hint == kNoSlab)
+                hint = slab;
+            return kNotFound;
+        }
+        slab = L.next;
+        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+    }
+}
+
 // Per-lane find.
 __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* needle, uint32_t s, uint32_t h)
 {
@@ -405,14 +443,15 @@ __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* ne
 // bulk multiset inserts, allowed to be present).  The group's first lane
 // claims an empty slot with a CAS on its tag word and the group writes the
 // item; a chain that is full up to its tail grows through chain_grow.
+// `start` is a line of the bucket's chain to begin at (a find's hint), or
+// kNoSlab for the head.
 template <int G, int MW>
 __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
-                                              uint32_t gl, uint32_t gbase)
+                                              uint32_t gl, uint32_t gbase, uint32_t start = kNoSlab)
 {
     static_assert(G >= 4 && G <= 32, "group size");
     const uint32_t tag = make_tag(h);
-    const uint32_t bucket = h % t.numBuckets;
-    uint32_t slab = bucket;
+    uint32_t slab = start == kNoSlab ? h % t.numBuckets : start;
     while (true) {
         uint32_t* line = t.lines + (size_t)slab * kLineWords;
         uint32_t ok = 0, slot = 0, next = 0, full = 0;
