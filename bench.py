@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "voxel edits/s & p99 edit latency (depth 15); slab-hash find Mops/s vs HBM"
 HBM_PEAK_GBS = 8000.0
-VARIANTS = [0, 4, 8, 16, 32, 204, 404, 208, 408, 1004, 1008, 1204, 1208]
+VARIANTS = [0, 1, 2, 3, 4, 8, 16, 32, 204, 404, 208, 408, 1004, 1008, 1204, 1208]
 
 
 def log(msg):
@@ -39,6 +39,8 @@ def log(msg):
 def variant_name(v):
     if v == 0:
         return "per-lane"
+    if v in (1, 2, 3):
+        return f"per-lane, blocks assigned to item-size segments, {1 << (v - 1)} finds in flight per lane"
     if v >= 1000:
         n = max(1, (v % 1000) // 100)
         return (f"XCD-binned (bin pass + find pass, each XCD probes its own bucket range), wave64 group of "
@@ -122,6 +124,24 @@ def slab_hash_bench(args):
     avg_kernel_ms = float(np.mean(kernel_ms))
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
 
+    # Practical ceilings on this GPU: a 4 GiB stream read, and one random
+    # 128-byte line read per find over a buffer the size of the tables.
+    probe = None
+    if not args.no_probe:
+        table_bytes = sum(hs.usage(s)[1] for s in sizes if s in items) * 128
+        stream_bytes = 4 << 30
+        stream_ms = svdag_py.hbm_probe(svdag_py.PROBE_STREAM, stream_bytes)
+        line_ms = svdag_py.hbm_probe(svdag_py.PROBE_RANDOM_LINES, table_bytes, fn)
+        probe = {
+            "stream_read_GBs": round(stream_bytes / (stream_ms * 1e-3) / 1e9, 1),
+            "random_line_buffer_MB": round(table_bytes / 2**20, 1),
+            "random_lines": fn,
+            "random_lines_ms": round(line_ms, 4),
+            "random_line_GBs": round(fn * 128 / (line_ms * 1e-3) / 1e9, 1),
+            "find_kernel_over_random_lines": round(avg_kernel_ms / line_ms, 3),
+        }
+        log(f"[bench] probes: {probe}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(items, finds, buckets, args)
@@ -159,6 +179,7 @@ def slab_hash_bench(args):
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "hbm_probe": probe,
     }
 
 
@@ -372,6 +393,7 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 200.0, 1000.0, 2000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-editing", action="store_true", help="slab-hash line only")
+    ap.add_argument("--no-probe", action="store_true", help="skip the stream / random-line memory probes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "

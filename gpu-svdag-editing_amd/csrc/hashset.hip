@@ -389,6 +389,97 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
     }
 }
 
+// ------------------------------------------------------------------ segment-uniform per-lane find
+// Blocks are assigned to segments on the host, so a block's item size and
+// table are uniform: the table descriptor comes in through scalar loads and
+// the segment lookup costs no memory access per lane.  Each lane carries NPL
+// finds: the needle hashes and line loads of all NPL are issued before any
+// candidate is compared, so a lane keeps NPL random line reads in flight.
+constexpr uint32_t kMaxLaneSegs = 32;
+struct LaneSegs {
+    uint32_t n;
+    uint32_t minWords;
+    uint32_t s[kMaxLaneSegs];
+    uint32_t count[kMaxLaneSegs];
+    uint32_t wordOff[kMaxLaneSegs];
+    uint32_t outOff[kMaxLaneSegs];
+    uint32_t blockStart[kMaxLaneSegs + 1];
+};
+
+// Chain walk of a per-lane find whose first line head is already in L and
+// whose needle words are in registers: a candidate's words are loaded all
+// at once (no early exit), so a compare costs one round trip to the line
+// the probe just fetched.
+template <int MW>
+__device__ inline uint32_t probe_lane(const TableDev& t, const uint32_t (&nw)[MW], uint32_t s, uint32_t tag4,
+                                      uint32_t slab, LineHead L)
+{
+    while (true) {
+        uint32_t cand = line_match(t, L, tag4);
+        while (cand) {
+            const uint32_t slot = __ffs(cand) - 1u;
+            cand &= cand - 1u;
+            const uint32_t* item = line_item(t, slab, slot);
+            uint32_t w[MW];
+#pragma unroll
+            for (int k = 0; k < MW; ++k)
+                w[k] = (uint32_t)k < s ? item[k] : 0u;
+            bool eq = true;
+#pragma unroll
+            for (int k = 0; k < MW; ++k)
+                eq &= w[k] == nw[k];
+            if (eq)
+                return (slab << kSlotBits) | slot;
+        }
+        if (L.next >= kPendingSlab)
+            return kNotFound;
+        slab = L.next;
+        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+    }
+}
+
+template <int NPL, int MW>
+__global__ void __launch_bounds__(256) find_lanes_seg(const TableDev* tables, const uint32_t* words, LaneSegs a,
+                                                      uint32_t* out)
+{
+    uint32_t sg = 0;
+    while (sg + 1 < a.n && blockIdx.x >= a.blockStart[sg + 1])
+        ++sg;
+    const uint32_t s = a.s[sg];
+    const TableDev t = tables[s - a.minWords];
+    const uint32_t count = a.count[sg];
+    const uint32_t first = (blockIdx.x - a.blockStart[sg]) * (256u * NPL) + threadIdx.x;
+    const uint32_t* base = words + a.wordOff[sg];
+    uint32_t nw[NPL][MW];
+    uint32_t h[NPL], slab[NPL];
+    LineHead L[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        const uint32_t idx = first + (uint32_t)k * 256u;
+        const uint32_t* nd = base + (size_t)(idx < count ? idx : 0u) * s;
+#pragma unroll
+        for (int j = 0; j < MW; ++j)
+            nw[k][j] = (uint32_t)j < s ? nd[j] : 0u;
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < MW; ++j)
+            if ((uint32_t)j < s)
+                x ^= hash_part(nw[k][j], (uint32_t)j);
+        h[k] = x;
+        slab[k] = x % t.numBuckets;
+    }
+#pragma unroll
+    for (int k = 0; k < NPL; ++k)
+        line_head_load(L[k], t.lines + (size_t)slab[k] * kLineWords);
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        const uint32_t idx = first + (uint32_t)k * 256u;
+        const uint32_t r = probe_lane<MW>(t, nw[k], s, make_tag(h[k]) * 0x01010101u, slab[k], L[k]);
+        if (idx < count)
+            out[a.outOff[sg] + idx] = r;
+    }
+}
+
 // Copy the words of n items (pointers already validated on the host).
 __global__ void gather_items(TableDev t, const uint32_t* ptrs, uint32_t n, uint32_t* out)
 {
@@ -533,6 +624,50 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
         return launch_binned(ts, words, segs, nseg, segBuf, n, out, variant, stream, bins, ev0, ev1);
     }
     const uint32_t block = 256;
+    if (variant >= 1 && variant <= 3) {
+        if (insert) {
+            set_error("segment-uniform lane variants (1, 2, 3) are find only");
+            return SVDAG_E_INVALID;
+        }
+        if (nseg > kMaxLaneSegs) {
+            set_error("segment-uniform lane variants take at most 32 segments");
+            return SVDAG_E_INVALID;
+        }
+        const int npl = variant == 1 ? 1 : (variant == 2 ? 2 : 4);
+        LaneSegs a {};
+        a.n = nseg;
+        a.minWords = ts.minWords;
+        uint64_t blocks = 0;
+        for (uint32_t i = 0; i < nseg; ++i) {
+            a.s[i] = segs[i].item_words;
+            a.count[i] = segs[i].count;
+            a.wordOff[i] = segs[i].word_offset;
+            a.outOff[i] = segs[i].out_offset;
+            a.blockStart[i] = (uint32_t)blocks;
+            blocks += ((uint64_t)segs[i].count + 256u * npl - 1) / (256u * npl);
+        }
+        a.blockStart[nseg] = (uint32_t)blocks;
+        if (ev0)
+            SVDAG_HIP(hipEventRecord(ev0, stream));
+        const bool narrowItems = ts.maxWords <= 10;
+#define SVDAG_LANE_SEG(NPL)                                                                                  \
+        if (narrowItems)                                                                                     \
+            find_lanes_seg<NPL, 10><<<(uint32_t)blocks, block, 0, stream>>>(ts.devTables, words, a, out);    \
+        else                                                                                                 \
+            find_lanes_seg<NPL, 18><<<(uint32_t)blocks, block, 0, stream>>>(ts.devTables, words, a, out);
+        if (npl == 1) {
+            SVDAG_LANE_SEG(1)
+        } else if (npl == 2) {
+            SVDAG_LANE_SEG(2)
+        } else {
+            SVDAG_LANE_SEG(4)
+        }
+#undef SVDAG_LANE_SEG
+        if (ev1)
+            SVDAG_HIP(hipEventRecord(ev1, stream));
+        SVDAG_HIP(hipGetLastError());
+        return SVDAG_OK;
+    }
     if (ev0)
         SVDAG_HIP(hipEventRecord(ev0, stream));
     // Items of at most 10 words need fewer needle registers per lane.
@@ -583,7 +718,8 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
         SVDAG_MULTI_CASE(408, 8, 4)
 #undef SVDAG_MULTI_CASE
     default:
-        set_error("variant must be 0, 4, 8, 16, 32 (lanes per item), 204, 404, 208, 408 (N*100 + G, find only) "
+        set_error("variant must be 0, 4, 8, 16, 32 (lanes per item), 1, 2, 3 (lanes, 1/2/4 finds each, find only), "
+                  "204, 404, 208, 408 (N*100 + G, find only) "
                   "or 1004, 1008, 1204, 1208 (XCD-binned, find only)");
         return SVDAG_E_INVALID;
     }
@@ -596,6 +732,18 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
 } // namespace svdag
 
 using namespace svdag;
+
+namespace svdag {
+int require_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device visible");
+        return SVDAG_E_NO_DEVICE;
+    }
+    return SVDAG_OK;
+}
+} // namespace svdag
 
 struct svdag_hashset {
     TableSet ts;
@@ -618,16 +766,6 @@ int svdag_device_count(int* out)
     if (hipGetDeviceCount(&n) != hipSuccess)
         n = 0;
     *out = n;
-    return SVDAG_OK;
-}
-
-static int require_device()
-{
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        set_error("no HIP device visible");
-        return SVDAG_E_NO_DEVICE;
-    }
     return SVDAG_OK;
 }
 

@@ -27,6 +27,8 @@ namespace svdag {
     } while (0)
 
 void set_error(const std::string& msg);
+// SVDAG_E_NO_DEVICE (with the message set) when no HIP device is visible.
+int require_device();
 
 // One slab hash table: device storage plus the host copy of its descriptor.
 struct HostTable {

@@ -104,7 +104,7 @@ EXPORTED = [
     "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
-    "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load",
+    "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
 ]
 
 _lib = None
@@ -131,6 +131,18 @@ def device_count():
     n = ctypes.c_int(0)
     check(lib().svdag_device_count(ctypes.byref(n)))
     return n.value
+
+
+PROBE_STREAM, PROBE_RANDOM_LINES = 0, 1
+
+
+def hbm_probe(kind, nbytes, reads=0):
+    """Best kernel ms of a stream read of nbytes (PROBE_STREAM) or of `reads`
+    random 128-byte line reads in an nbytes buffer (PROBE_RANDOM_LINES)."""
+    ms = ctypes.c_float(0.0)
+    check(lib().svdag_hbm_probe(ctypes.c_uint32(kind), ctypes.c_uint64(nbytes), ctypes.c_uint64(reads),
+                                ctypes.byref(ms)))
+    return ms.value
 
 
 class DeviceBuffer:

@@ -45,6 +45,16 @@ int svdag_memcpy_d2h(void* dst, const void* src_dev, size_t bytes);
 int svdag_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes);
 int svdag_device_synchronize(void);
 
+/* ------------------------------------------------------------------ memory-system probes
+ * No reference counterpart: the practical ceilings the slab-hash find is
+ * measured against.  SVDAG_PROBE_STREAM reads `bytes` sequentially with
+ * 16-byte loads; SVDAG_PROBE_RANDOM_LINES reads `reads` random 128-byte lines
+ * of a `bytes` buffer, 20 bytes of each (the words a find probe loads).
+ * *ms is the best of five launches (HIP events on the probe's own stream).    */
+#define SVDAG_PROBE_STREAM 0u
+#define SVDAG_PROBE_RANDOM_LINES 1u
+int svdag_hbm_probe(uint32_t kind, uint64_t bytes, uint64_t reads, float* ms);
+
 /* ------------------------------------------------------------------ slab hash tables
  * A set of slab hash tables, one per item size in [min_words, max_words].
  * Stands in for CompactAccelerationHashTable + GpuHashTableBase
@@ -62,8 +72,11 @@ typedef struct {
 
 /* Variant of a bulk operation: 0 = one lane per item (cf. add/find),
  * 4/8/16/32 = that many lanes of a wave64 per item (cf. addAsWarp/findAsWarp),
+ * 1/2/3 = find only, one lane per item with blocks assigned to segments and
+ * 1/2/4 finds per lane in flight (at most 32 segments),
  * N*100 + G (204, 404, 208, 408) = find only, G lanes per group with N
- * needles per group in flight. */
+ * needles per group in flight,
+ * 1004/1008/1204/1208 = find only, needles binned by XCD first. */
 enum { SVDAG_LANE = 0 };
 
 /* reference: CompactAccelerationHashTable::allocate(numBuckets, numElements, itemSize)

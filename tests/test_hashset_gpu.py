@@ -108,7 +108,7 @@ def test_empty_batch_and_oom_is_reported():
     hs.close()
 
 
-FIND_VARIANTS = [0, 4, 8, 16, 32, 204, 404, 208, 408, 1004, 1008, 1204, 1208]
+FIND_VARIANTS = [0, 1, 2, 3, 4, 8, 16, 32, 204, 404, 208, 408, 1004, 1008, 1204, 1208]
 
 
 @pytest.mark.gpu
@@ -141,4 +141,43 @@ def test_find_variants_return_identical_pointers(buckets_per_size):
             ref = r
             assert 0.4 < float(np.mean(r != svdag_py.NOT_FOUND)) < 0.6
         assert np.array_equal(r, ref), v
+    hs.close()
+
+
+@pytest.mark.gpu
+def test_memory_probes_run():
+    # Stream and random-line probes (bench.py's practical ceilings) return a
+    # positive kernel time; bad arguments are rejected.
+    assert svdag_py.hbm_probe(svdag_py.PROBE_STREAM, 64 << 20) > 0.0
+    assert svdag_py.hbm_probe(svdag_py.PROBE_RANDOM_LINES, 64 << 20, 1 << 20) > 0.0
+    with pytest.raises(svdag_py.SvdagError):
+        svdag_py.hbm_probe(7, 64 << 20)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_lane_segment_variants_ragged_and_empty_segments(variant):
+    # Blocks are assigned to segments on the host: an empty segment, a
+    # one-item segment and a ragged one must still give the per-lane result.
+    rng = np.random.default_rng(11)
+    items = {3: rng.integers(0, 1 << 32, size=(1000, 3), dtype=np.uint64).astype(np.uint32),
+             5: rng.integers(0, 1 << 32, size=(777, 5), dtype=np.uint64).astype(np.uint32)}
+    items[3][:, 0] = (items[3][:, 0] & ~np.uint32(0xFF)) | np.uint32(3)
+    items[5][:, 0] = (items[5][:, 0] & ~np.uint32(0xFF)) | np.uint32(15)
+    hs = svdag_py.HashSet(2, 9, [7] * 8, [2000] * 8)
+    words, segs, n = svdag_py.make_segments(sorted(items.items()))
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    hs.add_dev(wdev, segs, odev, 16)
+    probes = [(4, np.zeros((0, 4), np.uint32)), (3, items[3][:1]), (5, np.concatenate([items[5][::3], items[5][:5] ^ 1]))]
+    fwords, fsegs, fn = svdag_py.make_segments(probes)
+    fdev = svdag_py.DeviceBuffer.from_array(fwords)
+    ref_dev = svdag_py.DeviceBuffer(4 * fn)
+    rdev = svdag_py.DeviceBuffer(4 * fn)
+    hs.find_dev(fdev, fsegs, ref_dev, 0)
+    hs.find_dev(fdev, fsegs, rdev, variant)
+    ref = ref_dev.to_array(np.uint32, fn)
+    assert np.array_equal(rdev.to_array(np.uint32, fn), ref)
+    assert (ref[:1 + len(items[5][::3])] != svdag_py.NOT_FOUND).all()
+    assert (ref[1 + len(items[5][::3]):] == svdag_py.NOT_FOUND).all()
     hs.close()
