@@ -885,8 +885,11 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         }
         if (needState && !ctx->ws.state)
             SVDAG_TRY(ws_alloc(ctx, ctx->ws.capWork, true));
+        // In the fused modes a single edit also runs as a wave: its levels
+        // above the split run in the one-wave spine kernel instead of four
+        // launches per level.
         uint32_t S = 0;
-        if (ctx->batchMode >= SVDAG_BATCH_FUSED && n - first > 1) {
+        if (ctx->batchMode >= SVDAG_BATCH_FUSED) {
             S = plan_batch(ctx, eds, first, n, steps, alone);
         } else {
             steps.clear();
