@@ -22,11 +22,11 @@
 namespace svdag {
 
 // Full-item "counts" only gate discovery on the level above (nonzero or
-// not), so they are flags: a relaxed load first keeps thousands of full
+// not), so they are flags: a cached load first keeps thousands of full
 // items from serialising on one atomic.
 __device__ inline void set_flag(uint32_t* p)
 {
-    if (ld_relaxed(p) == 0u)
+    if (ld_filter(p) == 0u)
         atomicOr(p, 1u);
 }
 
@@ -167,9 +167,11 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
         const uint32_t cz = (item.z << 1) | (c & 1u);
         NState st = NS_PARTIAL;
         uint32_t stMat = 0;
+        // Only the paint tool's decision reads Filled vs Partial; the others
+        // skip the child header read.
         if (childHandle == kSentinel)
             st = NS_EMPTY;
-        else if (!leafChildren)
+        else if (!leafChildren && ed.kind == EK_PAINT)
             st = header_full(dag_node_ptr(dag, childHandle)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
         uint32_t fillMat = 0;
         Op op = OP_KEEP;
@@ -445,7 +447,7 @@ __global__ void __launch_bounds__(256) merge_dedup_find(WsDev ws, DagDev dag, ui
             // queueing on one atomic; only an empty slot is claimed by CAS.
             unsigned long long v = 0;
             if (gl == 0) {
-                v = ld_relaxed64(&ws.dedup[p]);
+                v = ld_filter64(&ws.dedup[p]);
                 if (v == 0ull)
                     v = atomicCAS(&ws.dedup[p], 0ull, key);
             }

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) fused_seed(WsDev ws, DagDev dag, const Ed
     uint32_t stMat = 0;
     if (h == kSentinel)
         st = NS_EMPTY;
-    else if (S != dag.leafLevel)
+    else if (S != dag.leafLevel && eds[k].kind == EK_PAINT) // only paint reads Filled vs Partial
         st = header_full(dag_node_ptr(dag, h)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
     uint32_t fillMat = 0;
     if (decide(eds[k], cx, cy, cz, dag.levels - S, st, fillMat) != OP_SUBDIVIDE)
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
             uint32_t stMat = 0;
             if (childHandle == kSentinel)
                 st = NS_EMPTY;
-            else if (!leafChildren)
+            else if (!leafChildren && ed.kind == EK_PAINT) // only paint reads Filled vs Partial
                 st = header_full(dag_node_ptr(dag, childHandle)[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
             uint32_t fillMat = 0;
             const Op op = active ? decide(ed, cx, cy, cz, dag.levels - childLevel, st, fillMat) : OP_KEEP;

@@ -8,14 +8,16 @@
 // Mark: one frontier per level holding each (item, level) pair once.  Every
 // slot has a 32-bit mask of the levels it was reached at; a parent claims a
 // child for level L + 1 with the atomicOr that sets bit L + 1 (after a
-// relaxed load shows it unset), and only the claimer appends it -- this
+// cached load shows it unset), and only the claimer appends it -- this
 // replaces the reference's per-level radix sort + unique.  Deduplication must be per
 // level: one item can occur at several levels, and the same child handle
 // decodes to a node at one depth and to a leaf (another table) next to the
 // leaf level.
-// Sweep (always on; the reference sweeps only under CAPTURE_MEMORY_STATS_SLOW):
-// tags of unmarked items are cleared and wholly unmarked overflow slabs are
-// unlinked and pushed on the slab free stack.
+// Sweep (always on; the reference sweeps only under CAPTURE_MEMORY_STATS_SLOW),
+// in two passes: a flat pass over every line clears the tags of unmarked
+// items and marks the buckets that have overflow lines; a pass over those
+// chains unlinks the overflow lines left without items and pushes them on
+// the slab free stack.
 #include "dag_internal.h"
 #include <algorithm>
 
@@ -28,7 +30,7 @@ struct GcCounters {
     unsigned long long itemsFreed, slabsFreed, itemsLive;
 };
 
-__global__ void gc_clear_active(TableDev t, uint32_t* levelMask)
+__global__ void gc_clear_masks(TableDev t, uint32_t* levelMask)
 {
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
     // A pop past the free stack's end leaves its count below zero; the
@@ -38,16 +40,15 @@ __global__ void gc_clear_active(TableDev t, uint32_t* levelMask)
     const uint64_t n = (uint64_t)fresh * kMaxSlots;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         levelMask[i] = 0u;
-        if (i < fresh)
-            t.active[i] = 0u;
     }
 }
 
 // Claim (item, level): true for exactly one caller, the first to reach the
-// item at that level.  A relaxed load filters the common case of an item
-// already claimed (DAG children are shared by many parents), so only first
-// visits pay for the atomic.  The first claim at any level also sets the
-// item's active bit.
+// item at that level.  A load served by the caches filters the common case
+// of an item already claimed (DAG children are shared by many parents, and
+// the hot ones stay cached), so only first visits pay for the atomic; mask
+// bits are only ever set during the mark, so a stale load merely sends the
+// caller to the atomic.  An item is live iff its mask is nonzero.
 __device__ inline bool gc_claim(const DagDev& dag, uint32_t* const* levelMask, uint32_t h, uint32_t L, bool leaf)
 {
     const uint32_t s = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
@@ -55,14 +56,9 @@ __device__ inline bool gc_claim(const DagDev& dag, uint32_t* const* levelMask, u
     const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
     uint32_t* m = &levelMask[s - 2][(size_t)slab * kMaxSlots + slot];
     const uint32_t bit = 1u << L;
-    if (ld_relaxed(m) & bit)
+    if (ld_filter(m) & bit)
         return false;
-    const uint32_t seen = atomicOr(m, bit);
-    if (seen & bit)
-        return false;
-    if (!seen)
-        atomicOr(&dag.tables[s - 2].active[slab], 1u << slot);
-    return true;
+    return (atomicOr(m, bit) & bit) == 0u;
 }
 
 __global__ void gc_seed(DagDev dag, uint32_t* const* levelMask, const uint32_t* root, uint32_t* front,
@@ -82,17 +78,51 @@ __global__ void gc_seed(DagDev dag, uint32_t* const* levelMask, const uint32_t* 
 
 // Expand the claimed nodes of level L: claim each child at L + 1 and append
 // the claimed interior children to the next frontier (leaves have nothing to
-// expand).  Appends are reserved once per wave.
+// expand).  Appends are staged in LDS and flushed with one atomic per
+// kGcStage entries, so the frontier counter sees a few thousand atomics per
+// collection instead of one per wave.
+constexpr uint32_t kGcStage = 4096;
+
+__device__ inline void gc_flush_stage(uint32_t* sStage, uint32_t& sFill, uint32_t& sBase, uint32_t* out,
+                                      uint32_t outCap, uint32_t* counter, uint32_t* overflow)
+{
+    __syncthreads();
+    const uint32_t fill = sFill;
+    if (fill == 0u)
+        return; // uniform: every thread read the same sFill after the barrier
+    if (threadIdx.x == 0)
+        sBase = atomicAdd(counter, fill);
+    __syncthreads();
+    const uint32_t base = sBase;
+    if ((uint64_t)base + fill > outCap) {
+        if (threadIdx.x == 0)
+            atomicOr(overflow, 1u);
+    } else {
+        for (uint32_t j = threadIdx.x; j < fill; j += blockDim.x)
+            out[base + j] = sStage[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        sFill = 0u;
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint32_t* const* levelMask, uint32_t L,
                                                      const uint32_t* in, uint32_t* out, uint32_t outCap,
                                                      GcCounters* ctr)
 {
+    __shared__ uint32_t sStage[kGcStage];
+    __shared__ uint32_t sFill, sBase;
     const uint32_t n = min(ctr->count[L], outCap); // both frontier buffers hold outCap entries
     const bool childLeaf = L + 1 == dag.leafLevel;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waveBase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint32_t i0 = waveBase; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + lane;
+    if (threadIdx.x == 0)
+        sFill = 0u;
+    __syncthreads();
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        // One pass adds at most 8 entries per thread.
+        if (sFill > kGcStage - 8u * blockDim.x)
+            gc_flush_stage(sStage, sFill, sBase, out, outCap, &ctr->count[L + 1], &ctr->overflow);
+        const uint32_t i = i0 + threadIdx.x;
         uint32_t kept[8];
         uint32_t k = 0;
         if (i < n) {
@@ -104,103 +134,116 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint32_t* const
                     kept[k++] = ch;
             }
         }
-        uint32_t incl = k;
-#pragma unroll
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t v = (uint32_t)__shfl_up((int)incl, d, 64);
-            if (lane >= d)
-                incl += v;
+        if (k) {
+            const uint32_t pos = atomicAdd(&sFill, k);
+            for (uint32_t j = 0; j < k; ++j)
+                sStage[pos + j] = kept[j];
         }
-        const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
-        if (!total)
-            continue;
-        uint32_t base = 0;
-        if (lane == 63)
-            base = atomicAdd(&ctr->count[L + 1], total);
-        base = (uint32_t)__shfl((int)base, 63, 64);
-        if ((uint64_t)base + total > outCap) {
-            if (lane == 0)
-                atomicOr(&ctr->overflow, 1u);
-            continue;
-        }
-        const uint32_t off = base + incl - k;
-        for (uint32_t j = 0; j < k; ++j)
-            out[off + j] = kept[j];
+        __syncthreads();
+    }
+    gc_flush_stage(sStage, sFill, sBase, out, outCap, &ctr->count[L + 1], &ctr->overflow);
+}
+
+// Sums of per-thread counters over the block, one atomic per block (the
+// counters are single words, so per-wave atomics would serialise).
+__device__ inline void gc_block_add(uint32_t v, unsigned long long* dst)
+{
+    __shared__ uint32_t sPart[4];
+    for (int d = 32; d >= 1; d >>= 1)
+        v += (uint32_t)__shfl_xor((int)v, d, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0)
+        sPart[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w)
+            sum += sPart[w];
+        if (sum)
+            atomicAdd(dst, (unsigned long long)sum);
     }
 }
 
-// Thirty-two lanes per bucket chain, two chains per wave: lane w holds word w
-// of the slab's line (one coalesced 128-byte read); the tag-word lanes clear
-// the tags of unmarked items, and the chain link (word 31) is broadcast to
-// walk on.  Overflow slabs with no marked item are unlinked and freed.  The
-// loop over chains is wave-uniform so the cross-lane operations see every
-// lane.
-__global__ void __launch_bounds__(256) gc_sweep(TableDev t, GcCounters* ctr)
+// Sweep, pass 1: every line handed out so far, one lane per line (a flat,
+// streaming pass).  Tags of items whose level mask stayed zero are cleared,
+// and hasChain[b] records whether bucket b has overflow lines (if the byte
+// map is given).  Lines on the free stack have no tags, so they pass
+// through unchanged.  blockDim.x = 256.
+__global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint32_t* levelMask, uint8_t* hasChain,
+                                                      GcCounters* ctr)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t w = lane & 31u;
-    const uint32_t half = lane >> 5;
-    const bool tagWord = w < t.tagWords;
-    const uint32_t chainsPerBlock = blockDim.x / 32u;
-    const uint32_t stride = gridDim.x * chainsPerBlock;
-    uint32_t freedItems = 0, freedSlabs = 0, live = 0;
-    for (uint32_t c0 = blockIdx.x * chainsPerBlock + ((threadIdx.x >> 6) << 1); c0 < t.numBuckets; c0 += stride) {
-        const uint32_t b0 = c0 + half;
-        uint32_t slab = b0 < t.numBuckets ? b0 : kNoSlab;
-        uint32_t prev = kNoSlab;
-        while (__ballot(slab != kNoSlab) != 0ull) {
-            uint32_t v = 0, keep = 0;
-            if (slab != kNoSlab) {
-                uint32_t* line = t.lines + (size_t)slab * kLineWords;
-                v = line[w];
-                if (tagWord) {
-                    const uint32_t a = t.active[slab] >> (4u * w);
-                    for (uint32_t byte = 0; byte < 4; ++byte) {
-                        const uint32_t tag = (v >> (8 * byte)) & 0xFFu;
-                        if (!tag || 4u * w + byte >= t.slots)
-                            continue;
-                        if ((a >> byte) & 1u) {
-                            keep |= tag << (8 * byte);
-                            ++live;
-                        } else {
-                            ++freedItems;
-                        }
-                    }
-                    if (keep != v)
-                        line[w] = keep;
+    const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
+    uint32_t freedItems = 0, live = 0;
+    for (uint32_t slab = blockIdx.x * blockDim.x + threadIdx.x; slab < fresh; slab += gridDim.x * blockDim.x) {
+        uint32_t* line = t.lines + (size_t)slab * kLineWords;
+        const uint4 v = *reinterpret_cast<const uint4*>(line);
+        const uint32_t tw[4] = { v.x, v.y, v.z, v.w };
+        const uint32_t* masks = levelMask + (size_t)slab * kMaxSlots;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if (j >= t.tagWords)
+                break;
+            uint32_t keep = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b) {
+                const uint32_t slot = 4u * j + b;
+                const uint32_t tag = (tw[j] >> (8u * b)) & 0xFFu;
+                if (!tag || slot >= t.slots)
+                    continue;
+                if (masks[slot] != 0u) {
+                    keep |= tag << (8u * b);
+                    ++live;
+                } else {
+                    ++freedItems;
                 }
             }
-            const uint32_t next = (uint32_t)__shfl((int)v, (int)(half * 32u + kNextWord), 64);
-            const unsigned long long kept = __ballot(tagWord && keep != 0u);
-            const bool anyKept = ((kept >> (half * 32u)) & 0xFFFFFFFFull) != 0ull;
-            const bool unlink = slab != kNoSlab && slab != b0 && !anyKept;
-            if (unlink && w == 0) {
-                t.lines[(size_t)prev * kLineWords + kNextWord] = next;
-                t.lines[(size_t)slab * kLineWords + kNextWord] = kNoSlab; // tags were cleared above
-                ++freedSlabs;
+            if (keep != tw[j])
+                line[j] = keep;
+        }
+        if (hasChain && slab < t.numBuckets)
+            hasChain[slab] = line[kNextWord] != kNoSlab ? 1u : 0u;
+    }
+    gc_block_add(freedItems, &ctr->itemsFreed);
+    gc_block_add(live, &ctr->itemsLive);
+}
+
+// Sweep, pass 2: one lane per bucket that has overflow lines (every bucket
+// without the byte map) walks its chain and unlinks the overflow lines pass
+// 1 left without tags; they become empty lines (link kNoSlab) on the free
+// stack.  The loop is wave-uniform so the frees aggregate per wave.
+__global__ void __launch_bounds__(256) gc_sweep_chains(TableDev t, const uint8_t* hasChain, GcCounters* ctr)
+{
+    uint32_t freedSlabs = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < t.numBuckets; base += gridDim.x * blockDim.x) {
+        const uint32_t b = base + threadIdx.x;
+        uint32_t prev = kNoSlab, cur = kNoSlab;
+        if (b < t.numBuckets && (!hasChain || hasChain[b])) {
+            prev = b;
+            cur = t.lines[(size_t)b * kLineWords + kNextWord];
+        }
+        while (__ballot(cur != kNoSlab) != 0ull) {
+            bool give = false;
+            const uint32_t id = cur;
+            if (cur != kNoSlab) {
+                uint32_t* line = t.lines + (size_t)cur * kLineWords;
+                const uint4 v = *reinterpret_cast<const uint4*>(line);
+                const uint32_t next = line[kNextWord];
+                const uint32_t tags = v.x | (t.tagWords > 1 ? v.y : 0u) | (t.tagWords > 2 ? v.z : 0u)
+                                      | (t.tagWords > 3 ? v.w : 0u);
+                if (tags == 0u) {
+                    t.lines[(size_t)prev * kLineWords + kNextWord] = next;
+                    line[kNextWord] = kNoSlab;
+                    give = true;
+                    ++freedSlabs;
+                } else {
+                    prev = cur;
+                }
+                cur = next;
             }
-            wave_slab_free(t, unlink && w == 0, slab);
-            if (slab != kNoSlab) {
-                if (!unlink)
-                    prev = slab;
-                slab = next;
-            }
+            wave_slab_free(t, give, id);
         }
     }
-    // Wave sums, then one atomic per counter per wave.
-    for (int m = 32; m >= 1; m >>= 1) {
-        freedItems += (uint32_t)__shfl_xor((int)freedItems, m, 64);
-        freedSlabs += (uint32_t)__shfl_xor((int)freedSlabs, m, 64);
-        live += (uint32_t)__shfl_xor((int)live, m, 64);
-    }
-    if (lane == 0) {
-        if (freedItems)
-            atomicAdd(&ctr->itemsFreed, (unsigned long long)freedItems);
-        if (freedSlabs)
-            atomicAdd(&ctr->slabsFreed, (unsigned long long)freedSlabs);
-        if (live)
-            atomicAdd(&ctr->itemsLive, (unsigned long long)live);
-    }
+    gc_block_add(freedSlabs, &ctr->slabsFreed);
 }
 
 // Level masks for every table, kept between collections; regrown when an
@@ -254,7 +297,7 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
         SVDAG_HIP(hipEventRecord(ctx->ev0, st));
         SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
         for (size_t i = 0; i < nt; ++i)
-            gc_clear_active<<<grid_for((uint64_t)ctx->ts.tables[i].slabCap * kMaxSlots, 256, 8192), 256, 0, st>>>(
+            gc_clear_masks<<<grid_for((uint64_t)ctx->ts.tables[i].slabCap * kMaxSlots, 256, 8192), 256, 0, st>>>(
                 ctx->ts.tables[i].dev, ctx->gcMasks[i]);
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dGcMasks, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L < dag.leafLevel; ++L)
@@ -268,8 +311,14 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
             cap = std::min<uint64_t>(cap * 2, 1ull << 31);
             continue;
         }
-        for (auto& t : ctx->ts.tables)
-            gc_sweep<<<grid_for(t.numBuckets, 8, 8192), 256, 0, st>>>(t.dev, ctr);
+        // The frontiers are free after the mark; the first one holds the
+        // sweep's per-bucket chain bytes, one table at a time.
+        for (size_t i = 0; i < nt; ++i) {
+            const HostTable& t = ctx->ts.tables[i];
+            uint8_t* hasChain = t.numBuckets <= ctx->gcCap * 4 ? reinterpret_cast<uint8_t*>(ctx->gcFront[0]) : nullptr;
+            gc_sweep_lines<<<grid_for(t.slabCap, 256, 2048), 256, 0, st>>>(t.dev, ctx->gcMasks[i], hasChain, ctr);
+            gc_sweep_chains<<<grid_for(t.numBuckets, 256, 2048), 256, 0, st>>>(t.dev, hasChain, ctr);
+        }
         SVDAG_HIP(hipEventRecord(ctx->ev1, st));
         SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         SVDAG_HIP(hipStreamSynchronize(st));

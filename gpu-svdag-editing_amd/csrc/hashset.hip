@@ -16,7 +16,7 @@ static thread_local std::string g_lastError;
 void set_error(const std::string& msg) { g_lastError = msg; }
 
 // Head slabs: no tags, no successor, no active bits.
-// Lines [first, last) become empty: tag words 0, next kNoSlab, no GC marks.
+// Lines [first, last) become empty: tag words 0, next kNoSlab.
 __global__ void table_init_lines(TableDev t, uint32_t first, uint32_t last)
 {
     const size_t n = (size_t)(last - first) * kLineWords;
@@ -27,8 +27,6 @@ __global__ void table_init_lines(TableDev t, uint32_t first, uint32_t last)
             base[i] = kNoSlab;
         else if (w < t.tagWords)
             base[i] = 0u;
-        if (w == 0)
-            t.active[first + i / kLineWords] = 0u;
     }
 }
 
@@ -52,7 +50,6 @@ int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t 
     d.slots = line_slots(itemWords);
     d.tagWords = line_tag_words(d.slots);
     SVDAG_HIP(hipMalloc(&d.lines, (size_t)slabCap * kLineWords * 4));
-    SVDAG_HIP(hipMalloc(&d.active, (size_t)slabCap * 4));
     SVDAG_HIP(hipMalloc(&d.freeList, (size_t)slabCap * 4));
     SVDAG_HIP(hipMalloc(&d.alloc, sizeof(AllocState)));
     AllocState a { numBuckets, 0, 0u, 0u };
@@ -66,7 +63,6 @@ void table_free(HostTable& t)
 {
     TableDev& d = t.dev;
     if (d.lines) (void)hipFree(d.lines);
-    if (d.active) (void)hipFree(d.active);
     if (d.freeList) (void)hipFree(d.freeList);
     if (d.alloc) (void)hipFree(d.alloc);
     d = TableDev {};
@@ -85,19 +81,15 @@ int table_grow(HostTable& t, uint32_t newCap, hipStream_t stream)
     }
     TableDev& d = t.dev;
     const uint32_t old = t.slabCap;
-    uint32_t *lines = nullptr, *active = nullptr, *freeList = nullptr;
+    uint32_t *lines = nullptr, *freeList = nullptr;
     SVDAG_HIP(hipMalloc(&lines, (size_t)newCap * kLineWords * 4));
-    SVDAG_HIP(hipMalloc(&active, (size_t)newCap * 4));
     SVDAG_HIP(hipMalloc(&freeList, (size_t)newCap * 4));
     SVDAG_HIP(hipMemcpyAsync(lines, d.lines, (size_t)old * kLineWords * 4, hipMemcpyDeviceToDevice, stream));
-    SVDAG_HIP(hipMemcpyAsync(active, d.active, (size_t)old * 4, hipMemcpyDeviceToDevice, stream));
     SVDAG_HIP(hipMemcpyAsync(freeList, d.freeList, (size_t)old * 4, hipMemcpyDeviceToDevice, stream));
     SVDAG_HIP(hipStreamSynchronize(stream));
     (void)hipFree(d.lines);
-    (void)hipFree(d.active);
     (void)hipFree(d.freeList);
     d.lines = lines;
-    d.active = active;
     d.freeList = freeList;
     d.slabCap = newCap;
     t.slabCap = newCap;

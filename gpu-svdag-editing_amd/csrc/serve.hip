@@ -89,7 +89,7 @@ static int read_memory(svdag_ctx* ctx, svdag_memory& m)
         m.slab_capacity += t.slabCap;
         m.slabs_free += freeSlabs;
         m.slabs_used += t.slabCap - freeSlabs;
-        m.bytes_reserved += (uint64_t)t.slabCap * (kLineWords + 2) * 4; // lines, active, free links
+        m.bytes_reserved += (uint64_t)t.slabCap * (kLineWords + 1) * 4; // lines and free links
         m.min_free_fraction = std::min(m.min_free_fraction, (double)freeSlabs / (double)t.slabCap);
     }
     return SVDAG_OK;

@@ -105,7 +105,6 @@ struct AllocState {
 // free stack) is empty: tag words 0 and next = kNoSlab.
 struct TableDev {
     uint32_t* lines;     // [slabCap][32]: tags, items, next
-    uint32_t* active;    // [slabCap] GC mark bits, bit = slot
     uint32_t* freeList;  // [slabCap] free stack of slab ids
     AllocState* alloc;
     uint32_t itemWords;
@@ -113,7 +112,7 @@ struct TableDev {
     uint32_t slabCap;
     uint32_t slots;      // items per line
     uint32_t tagWords;   // tag words at the line start
-    uint32_t pad[5];
+    uint32_t pad[7];
 };
 
 __device__ inline uint32_t ld_relaxed(const uint32_t* p)
@@ -127,6 +126,19 @@ __device__ inline void st_relaxed(uint32_t* p, uint32_t v)
 __device__ inline unsigned long long ld_relaxed64(const unsigned long long* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Loads that may be served by the CU's caches and so may return an older
+// value.  Only for words that change monotonically within a kernel (bits
+// set, a slot going from empty to claimed), as a filter in front of the
+// agent-scope atomic that decides: a stale value only sends the caller to
+// that atomic.
+__device__ inline uint32_t ld_filter(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline unsigned long long ld_filter64(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Lanes below this one in the wave.
