@@ -232,6 +232,18 @@ def editing_bench(args):
     for rate in args.serve_rates:
         creq[rate] = workloads.config4_requests(h, D, rate, args.serve_seconds, seed=7)
     sat_reqs, _ = workloads.config4_requests(h, D, 2000.0, 1.0, seed=9)
+    # Frame passes on the fresh terrain at 1920x1080: primary paths, then
+    # shadows and ambient occlusion (3 samples, rays of 2^(D-10) voxels), the
+    # reference's defaults (engine.h:202-213).
+    shading = None
+    if not args.no_shading:
+        pos, rot, fov, bmin, bmax = cams[0]
+        tr = [g.trace(pos, rot, fov, bmin, bmax, 1920, 1080)[1] for _ in range(3)]
+        sh = [g.trace_shadows(1.0)[1] for _ in range(3)]
+        ao = [g.trace_ambient_occlusion(3, 1.0, float(1 << (D - 10)), 123 + i)[1] for i in range(3)]
+        shading = {"width": 1920, "height": 1080, "primary_ms": round(min(tr), 3), "shadows_ms": round(min(sh), 3),
+                   "ambient_occlusion_3spp_ms": round(min(ao), 3)}
+        log(f"[bench] shading: {shading}")
     g.close()
 
     # plain per-edit form
@@ -312,7 +324,7 @@ def editing_bench(args):
     sat = serve_row("saturation", sat_reqs, np.zeros(len(sat_reqs)), 64)
     g.close()
     return {"levels": D, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
-            "plain": plain, "batched": batched, "serving": serving,
+            "plain": plain, "batched": batched, "serving": serving, "shading": shading,
             "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"]}
 
 
@@ -394,6 +406,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-editing", action="store_true", help="slab-hash line only")
     ap.add_argument("--no-probe", action="store_true", help="skip the stream / random-line memory probes")
+    ap.add_argument("--no-shading", action="store_true", help="skip the shadow / occlusion timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "

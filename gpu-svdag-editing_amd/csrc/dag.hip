@@ -1155,7 +1155,7 @@ int svdag_destroy(svdag_ctx* ctx)
                      (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits,
                      (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
-                     (void*)ctx->dWaveIdx, ctx->dLastFrame })
+                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade })
         if (p)
             (void)hipFree(p);
     for (void* p : { ctx->hBatch, ctx->hFused })

@@ -151,6 +151,12 @@ struct svdag_ctx {
     // Tracer output buffer.
     void* dHits = nullptr;
     size_t hitCap = 0;
+    // Surface codes of the last primary trace (0: no surface, else
+    // 1 + 2 * axis + (normal points along +axis)), its size and root, for
+    // the shadow and ambient-occlusion passes.
+    uint8_t* dSurf = nullptr;
+    float* dShade = nullptr; // per-pixel output of the shadow / occlusion passes (hitCap floats)
+    uint32_t traceW = 0, traceH = 0, traceRoot = 0xFFFFFFFFu;
     std::mutex mu;
     // Serving (serve.hip).  While a server runs, frame traces read the tables
     // on frameStream; whoever reallocates or sweeps tables first takes
@@ -175,8 +181,9 @@ struct FrameQuiesce {
 };
 int ctx_refresh_tables(svdag_ctx* ctx);
 int gc_run(svdag_ctx* ctx, svdag_gc_stats* st);
+// surfDev (optional) receives one surface code per pixel (see dSurf).
 int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
-                 svdag_hit* outDev);
+                 svdag_hit* outDev, uint8_t* surfDev = nullptr);
 // Run edits [0, n) in order on ctx->stream (one sync at the end), with the
 // overflow / OOM retry; used by svdag_edit, svdag_edit_batch and the server.
 int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats* stats, float* totalMs);

@@ -276,6 +276,9 @@ static int gc_masks(svdag_ctx* ctx)
 
 int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
 {
+    // A collection may free the nodes of the last traced root, so the shadow
+    // and occlusion passes need a new primary trace afterwards.
+    ctx->traceW = ctx->traceH = 0;
     const hipStream_t st = ctx->stream;
     const DagDev& dag = ctx->dag;
     uint64_t cap = std::max<uint64_t>(ctx->gcCap, 1ull << 22);

@@ -98,7 +98,8 @@ __device__ inline uint32_t intersection_mask(uint32_t levels, uint32_t level, ui
 
 constexpr uint32_t kTraceBlock = 256;
 
-__global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t root, TraceParams p, svdag_hit* out)
+__global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t root, TraceParams p, svdag_hit* out,
+                                                             uint8_t* surf)
 {
     __shared__ uint2 stack[20][kTraceBlock]; // levels <= 20
     const uint32_t tid = threadIdx.x;
@@ -213,7 +214,243 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
     h.px = ray.o[0] + t * ray.d[0];
     h.py = ray.o[1] + t * ray.d[1];
     h.pz = ray.o[2] + t * ray.d[2];
-    out[(size_t)(p.height - 1 - py) * p.width + px] = h;
+    const size_t at = (size_t)(p.height - 1 - py) * p.width + px;
+    out[at] = h;
+    // The surface normal points against the ray on the entry axis; a null
+    // path (0,0,0) is no surface (Path::is_null, path.h:61-64).
+    if (surf)
+        surf[at] = (x | y | z) == 0u ? 0u : (uint8_t)(1u + 2u * (uint32_t)axis + (ray.d[axis] < 0.0f ? 1u : 0u));
+}
+
+// ------------------------------------------------------------------ sampling
+// PCG32: XSH-RR output over a 64-bit LCG (O'Neill 2014), seeded as the
+// reference seeds it (utils.h:45-82): the stream value is the increment, the
+// state starts at PCG's default initializer and two outputs are dropped.
+struct Pcg32 {
+    uint64_t state, inc;
+    __device__ explicit Pcg32(uint64_t stream) : state(0x853c49e6748fea9bull), inc(stream)
+    {
+        next();
+        next();
+    }
+    __device__ uint32_t next()
+    {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ull + (inc | 1ull);
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    // float(u32) / float(UINT32_MAX); the divisor rounds to 2^32.
+    __device__ float next_float() { return (float)next() / 4294967296.0f; }
+};
+
+// sin and cos of 2*pi*u for u in [0, 1].  u is split into the nearest
+// quarter turn k and a remainder f of at most an eighth of a turn; sin and
+// cos of f*pi/2 come from their Taylor series to degree 9 and 10 (remainder
+// below 2e-9 on [-pi/4, pi/4]), then rotate by k quarter turns.  This is our
+// own formulation; the oracle evaluates the same float operations, so the
+// sampled directions agree bit for bit.
+__device__ inline void sincos_turns(float u, float& sn, float& cs)
+{
+    const float q = 4.0f * u;
+    const float k = floorf(q + 0.5f);
+    const float x = (q - k) * 1.57079632679489662f;
+    const float x2 = x * x;
+    const float sp = x * (1.0f + x2 * (-0.166666667f + x2 * (8.33333333e-3f + x2 * (-1.98412698e-4f + x2 * 2.75573192e-6f))));
+    const float cp = 1.0f + x2 * (-0.5f + x2 * (4.16666667e-2f + x2 * (-1.38888889e-3f + x2 * (2.48015873e-5f + x2 * -2.75573192e-7f))));
+    switch ((int)k & 3) {
+    case 0:
+        sn = sp;
+        cs = cp;
+        break;
+    case 1:
+        sn = cp;
+        cs = -sp;
+        break;
+    case 2:
+        sn = -sp;
+        cs = -cp;
+        break;
+    default:
+        sn = -cp;
+        cs = sp;
+        break;
+    }
+}
+
+// cosineSampleHemisphere (tracer.cu:645-659): r = sqrt(u.x), theta = 2*pi*u.y.
+__device__ inline void cosine_hemisphere(float ux, float uy, float out[3])
+{
+    const float r = sqrtf(ux);
+    float sn, cs;
+    sincos_turns(uy, sn, cs);
+    const float zz = 1.0f - ux;
+    out[0] = r * cs;
+    out[1] = r * sn;
+    out[2] = sqrtf(0.0f < zz ? zz : 0.0f);
+}
+
+// ------------------------------------------------------------------ shadows and ambient occlusion
+// reference: Tracer::trace_shadows / trace_ambient_occlusion (tracer.cu:613-691),
+// intersect_ray_node_out_of_order (274-351), cosineSampleHemisphere (645-659),
+// SurfaceInteraction::transformDirectionToWorld (tracer.h:106-109) and
+// Utils::PCG_RNG (utils.h:45-82).  Both passes read the surface of the last
+// primary trace: the hit position and the entry-face normal.
+
+// Any set voxel along the ray: children are taken highest visit bit first
+// and the first voxel reached ends the walk.  stk is this lane's column of
+// the block's [level][lane] LDS stack.
+__device__ inline bool occluded(const DagDev& dag, uint32_t root, const Ray& ray, uint2 (*stk)[kTraceBlock],
+                                uint32_t tid)
+{
+    if (root == kSentinel)
+        return false;
+    const uint32_t levels = dag.levels, leafLevel = dag.leafLevel;
+    uint32_t level = 0, x = 0, y = 0, z = 0;
+    uint32_t cIndex = root;
+    uint32_t cChild = dag_node_ptr(dag, root)[0] & 0xFFu;
+    uint32_t cVisit = cChild & intersection_mask<true>(levels, 0, 0, 0, 0, ray);
+    uint64_t leafBits = 0;
+    for (;;) {
+        uint32_t newLevel = level;
+        while (newLevel > 0 && !cVisit) {
+            --newLevel;
+            const uint2 e = stk[newLevel][tid];
+            cIndex = e.x;
+            cChild = e.y & 0xFFu;
+            cVisit = (e.y >> 8) & 0xFFu;
+        }
+        if (newLevel == 0 && !cVisit)
+            return false;
+        const uint32_t up = level - newLevel;
+        x >>= up;
+        y >>= up;
+        z >>= up;
+        level = newLevel;
+        const uint32_t nc = 31u - (uint32_t)__clz((int)cVisit);
+        cVisit &= ~(1u << nc);
+        x = (x << 1) | ((nc >> 2) & 1u);
+        y = (y << 1) | ((nc >> 1) & 1u);
+        z = (z << 1) | (nc & 1u);
+        stk[level][tid] = make_uint2(cIndex, cChild | (cVisit << 8));
+        ++level;
+        if (level == levels)
+            return true;
+        if (level < leafLevel) {
+            const uint32_t* node = dag_node_ptr(dag, cIndex);
+            cIndex = node[1 + __builtin_popcount(cChild & ((1u << nc) - 1u))];
+            cChild = dag_node_ptr(dag, cIndex)[0] & 0xFFu;
+        } else if (level == leafLevel) {
+            const uint32_t* node = dag_node_ptr(dag, cIndex);
+            const uint32_t* leaf = dag_leaf_ptr(dag, node[1 + __builtin_popcount(cChild & ((1u << nc) - 1u))]);
+            leafBits = (uint64_t)leaf[0] | ((uint64_t)leaf[1] << 32);
+            uint32_t fm = 0;
+            for (uint32_t b = 0; b < 8; ++b)
+                fm |= ((leafBits >> (8 * b)) & 0xFFull) ? (1u << b) : 0u;
+            cChild = fm;
+        } else {
+            cChild = (uint32_t)(leafBits >> (8 * nc)) & 0xFFu;
+        }
+        cVisit = cChild & intersection_mask<false>(levels, level, x, y, z, ray);
+    }
+}
+
+__device__ inline void make_ray(Ray& r, const float o[3], const float d[3], float tmax)
+{
+    for (int k = 0; k < 3; ++k) {
+        r.o[k] = o[k];
+        r.d[k] = d[k];
+        r.inv[k] = 1.0f / d[k];
+    }
+    r.tmin = 0.0f;
+    r.tmax = tmax;
+}
+
+// Normal, dpdu and dpdv of a surface code (createSurfaceInteraction,
+// tracer.cu:353-389).
+__device__ inline void surface_frame(uint32_t code, float n[3], float du[3], float dv[3])
+{
+    const uint32_t axis = (code - 1u) >> 1;
+    for (int k = 0; k < 3; ++k)
+        n[k] = du[k] = dv[k] = 0.0f;
+    n[axis] = ((code - 1u) & 1u) ? 1.0f : -1.0f;
+    if (axis == 0) {
+        du[2] = 1.0f;
+        dv[1] = 1.0f;
+    } else if (axis == 1) {
+        du[0] = 1.0f;
+        dv[2] = 1.0f;
+    } else {
+        du[0] = 1.0f;
+        dv[1] = 1.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) shadow_kernel(DagDev dag, uint32_t root, const svdag_hit* hits,
+                                                             const uint8_t* surf, uint32_t n, float bias, float3 sun,
+                                                             float* out)
+{
+    __shared__ uint2 stk[20][kTraceBlock];
+    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t code = surf[i];
+    if (code == 0u) {
+        out[i] = 0.0f;
+        return;
+    }
+    const svdag_hit h = hits[i];
+    const float L[3] = { sun.x, sun.y, sun.z };
+    const float p[3] = { h.px, h.py, h.pz };
+    float o[3];
+    for (int k = 0; k < 3; ++k)
+        o[k] = p[k] + bias * L[k];
+    Ray r;
+    make_ray(r, o, L, FLT_MAX);
+    float nrm[3], du[3], dv[3];
+    surface_frame(code, nrm, du, dv);
+    float ndl = (nrm[0] * L[0] + nrm[1] * L[1]) + nrm[2] * L[2];
+    ndl = ndl < 0.0f ? 0.0f : ndl; // std::max(NdotL, 0.0f)
+    out[i] = occluded(dag, root, r, stk, threadIdx.x) ? 0.0f : ndl;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) ao_kernel(DagDev dag, uint32_t root, const svdag_hit* hits,
+                                                         const uint8_t* surf, uint32_t n, uint32_t numSamples,
+                                                         float bias, float rayLength, uint64_t seed, float* out)
+{
+    __shared__ uint2 stk[20][kTraceBlock];
+    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t code = surf[i];
+    if (code == 0u) {
+        out[i] = 0.0f;
+        return;
+    }
+    const svdag_hit h = hits[i];
+    const float p[3] = { h.px, h.py, h.pz };
+    float nrm[3], du[3], dv[3];
+    surface_frame(code, nrm, du, dv);
+    float o[3];
+    for (int k = 0; k < 3; ++k)
+        o[k] = p[k] + nrm[k] * bias;
+    // The sample stream is keyed by the pixel's index in the stored image.
+    Pcg32 rng((uint64_t)i * (seed > 1ull ? seed : 1ull));
+    uint32_t hitsUnoccluded = 0;
+    for (uint32_t s = 0; s < numSamples; ++s) {
+        const float ux = rng.next_float(), uy = rng.next_float();
+        float local[3];
+        cosine_hemisphere(ux, uy, local);
+        float d[3];
+        for (int k = 0; k < 3; ++k)
+            d[k] = (local[0] * du[k] + local[1] * dv[k]) + local[2] * nrm[k];
+        Ray r;
+        make_ray(r, o, d, rayLength);
+        if (!occluded(dag, root, r, stk, threadIdx.x))
+            ++hitsUnoccluded;
+    }
+    out[i] = (float)hitsUnoccluded / (float)numSamples;
 }
 
 // get_trace_params (dag_tracer.cu:59-101), double precision.
@@ -256,11 +493,11 @@ static TraceParams trace_params(const svdag_camera& c, uint32_t levels, uint32_t
 // Enqueue a trace of the DAG rooted at `root` (a value, so concurrent edits
 // that publish new roots cannot change what this frame sees).
 int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
-                 svdag_hit* outDev)
+                 svdag_hit* outDev, uint8_t* surfDev)
 {
     const TraceParams p = trace_params(cam, ctx->dag.levels, W, H);
     const uint32_t tiles = ((W + 15) / 16) * ((H + 15) / 16);
-    trace_kernel<<<tiles, kTraceBlock, 0, stream>>>(ctx->dag, root, p, outDev);
+    trace_kernel<<<tiles, kTraceBlock, 0, stream>>>(ctx->dag, root, p, outDev, surfDev);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
@@ -281,16 +518,84 @@ extern "C" int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* cam, uint
     if (ctx->hitCap < n) {
         if (ctx->dHits)
             (void)hipFree(ctx->dHits);
+        if (ctx->dSurf)
+            (void)hipFree(ctx->dSurf);
+        if (ctx->dShade)
+            (void)hipFree(ctx->dShade);
         ctx->dHits = nullptr;
+        ctx->dSurf = nullptr;
+        ctx->dShade = nullptr;
+        ctx->hitCap = 0;
         SVDAG_HIP(hipMalloc(&ctx->dHits, n * sizeof(svdag_hit)));
+        SVDAG_HIP(hipMalloc(&ctx->dSurf, n));
+        SVDAG_HIP(hipMalloc(&ctx->dShade, n * sizeof(float)));
         ctx->hitCap = n;
     }
+    ctx->traceW = ctx->traceH = 0;
     SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    SVDAG_TRY(launch_trace(ctx, *cam, W, H, ctx->root, ctx->stream, (svdag_hit*)ctx->dHits));
+    SVDAG_TRY(launch_trace(ctx, *cam, W, H, ctx->root, ctx->stream, (svdag_hit*)ctx->dHits, ctx->dSurf));
     SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     SVDAG_HIP(hipMemcpyAsync(out, ctx->dHits, n * sizeof(svdag_hit), hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->traceW = W;
+    ctx->traceH = H;
+    ctx->traceRoot = ctx->root;
+    if (ms)
+        SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return SVDAG_OK;
+}
+
+// Sun direction of the reference (tracer.cu:600-603): normalize(0.3f, 1, 0.5f),
+// here normalised in double and rounded once.
+static float3 sun_direction()
+{
+    const double v[3] = { (double)0.3f, 1.0, 0.5 };
+    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    return make_float3((float)(v[0] / len), (float)(v[1] / len), (float)(v[2] / len));
+}
+
+static int shade_pass(svdag_ctx* ctx, int kind, uint32_t numSamples, float bias, float rayLength, uint64_t seed,
+                      float* out, float* ms)
+{
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->traceW == 0 || !ctx->dShade) {
+        set_error("shadows / ambient occlusion need a primary trace first (svdag_trace_primary)");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t n = ctx->traceW * ctx->traceH;
+    const uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (kind == 0)
+        shadow_kernel<<<grid, kTraceBlock, 0, ctx->stream>>>(ctx->dag, ctx->traceRoot, (const svdag_hit*)ctx->dHits,
+                                                             ctx->dSurf, n, bias, sun_direction(), ctx->dShade);
+    else
+        ao_kernel<<<grid, kTraceBlock, 0, ctx->stream>>>(ctx->dag, ctx->traceRoot, (const svdag_hit*)ctx->dHits,
+                                                         ctx->dSurf, n, numSamples, bias, rayLength, seed, ctx->dShade);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    SVDAG_HIP(hipMemcpyAsync(out, ctx->dShade, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));
     if (ms)
         SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
     return SVDAG_OK;
+}
+
+extern "C" int svdag_trace_shadows(svdag_ctx* ctx, float shadow_bias, float* out, float* ms)
+{
+    if (!ctx || !out || !std::isfinite(shadow_bias)) {
+        set_error("svdag_trace_shadows: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    return shade_pass(ctx, 0, 0, shadow_bias, 0.0f, 0, out, ms);
+}
+
+extern "C" int svdag_trace_ambient_occlusion(svdag_ctx* ctx, uint32_t num_samples, float shadow_bias,
+                                             float ao_ray_length, uint64_t seed, float* out, float* ms)
+{
+    if (!ctx || !out || num_samples == 0 || num_samples > 4096 || !std::isfinite(shadow_bias)
+        || !(ao_ray_length > 0.0f)) {
+        set_error("svdag_trace_ambient_occlusion: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    return shade_pass(ctx, 1, num_samples, shadow_bias, ao_ray_length, seed, out, ms);
 }

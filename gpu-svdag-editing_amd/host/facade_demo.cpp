@@ -50,7 +50,16 @@ int main(int argc, char** argv)
         uint32_t hits = 0;
         for (const auto& h : tracer.hits())
             hits += h.material != 0xFFFFFFFFu;
-        std::printf("%016llx %u\n", (unsigned long long)dag.canonical_hash(), hits);
+        // Shadows and ambient occlusion of the traced surface, as Engine
+        // resolves them after the paths (engine.cpp:743-750).
+        tracer.resolve_ambient_occlusion(dag, 3, 1.0f, 32.0f);
+        tracer.resolve_shadows(dag, 1.0f);
+        uint32_t lit = 0, open = 0;
+        for (float v : tracer.sunLight())
+            lit += v > 0.0f;
+        for (float v : tracer.ambientOcclusion())
+            open += v > 0.0f;
+        std::printf("%016llx %u %u %u\n", (unsigned long long)dag.canonical_hash(), hits, lit, open);
         return 0;
     } catch (const Error& e) {
         std::fprintf(stderr, "%s\n", e.what());

@@ -252,9 +252,33 @@ public:
     }
     const std::vector<svdag_hit>& hits() const { return hits_; }
 
+    // DAGTracer::resolve_shadows (dag_tracer.cu:198-221) over the surface of
+    // the last resolve_paths; sun visibility * max(N.L, 0) lands in sunLight().
+    float resolve_shadows(const MyGPUHashDAG& dag, float shadowBias)
+    {
+        sun_.resize(hits_.size());
+        float ms = 0.0f;
+        check(svdag_trace_shadows(dag.context(), shadowBias, sun_.data(), &ms));
+        return ms;
+    }
+    // DAGTracer::resolve_ambient_occlusion (dag_tracer.cu:163-187); the
+    // occlusion seed advances per call as the reference's does.
+    float resolve_ambient_occlusion(const MyGPUHashDAG& dag, uint32_t numSamples, float shadowBias, float aoRayLength)
+    {
+        ao_.resize(hits_.size());
+        float ms = 0.0f;
+        check(svdag_trace_ambient_occlusion(dag.context(), numSamples, shadowBias, aoRayLength, aoSeed_++, ao_.data(),
+                                            &ms));
+        return ms;
+    }
+    const std::vector<float>& sunLight() const { return sun_; }
+    const std::vector<float>& ambientOcclusion() const { return ao_; }
+
 private:
     uint32_t width_, height_;
     std::vector<svdag_hit> hits_;
+    std::vector<float> sun_, ao_;
+    uint64_t aoSeed_ = 123; // the reference starts its occlusion seed at 123 (dag_tracer.h:66)
 };
 
 } // namespace svdag

@@ -105,6 +105,7 @@ EXPORTED = [
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
+    "svdag_trace_shadows", "svdag_trace_ambient_occlusion",
 ]
 
 _lib = None
@@ -382,12 +383,33 @@ class MyGPUHashDAG:
         check(lib().svdag_server_last_frame(self.h, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(root)))
         return out, root.value
 
+    def _last_pixels(self):
+        return getattr(self, "_trace_pixels", 0)
+
     def trace(self, position, rotation, fov, bmin, bmax, w, h):
+        self._trace_pixels = w * h
         cam = self._camera(position, rotation, fov, bmin, bmax)
         out = np.empty(w * h, HIT_DTYPE)
         ms = ctypes.c_float(0)
         check(lib().svdag_trace_primary(self.h, ctypes.byref(cam), w, h, out.ctypes.data_as(ctypes.c_void_p),
                                         ctypes.byref(ms)))
+        return out, ms.value
+
+    def trace_shadows(self, shadow_bias=1.0):
+        """Sun visibility * max(N.L, 0) per pixel of the last trace(), in its layout."""
+        out = np.empty(self._last_pixels(), np.float32)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_trace_shadows(self.h, ctypes.c_float(shadow_bias), out.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(ms)))
+        return out, ms.value
+
+    def trace_ambient_occlusion(self, num_samples=3, shadow_bias=1.0, ao_ray_length=32.0, seed=1):
+        """Unoccluded fraction of num_samples cosine hemisphere rays per pixel of the last trace()."""
+        out = np.empty(self._last_pixels(), np.float32)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_trace_ambient_occlusion(self.h, ctypes.c_uint32(num_samples), ctypes.c_float(shadow_bias),
+                                                  ctypes.c_float(ao_ray_length), ctypes.c_uint64(seed),
+                                                  out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ms)))
         return out, ms.value
 
     def canonical(self, dump=False):

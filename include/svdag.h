@@ -253,6 +253,26 @@ typedef struct {
 int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* camera, uint32_t width, uint32_t height,
                         svdag_hit* out_hits, float* gpu_ms);
 
+/* Shadows and ambient occlusion of the surface the last svdag_trace_primary
+ * call on this context found (same size, same root; nothing else traced in
+ * between).  Output: width*height floats in the hit buffer's layout; pixels
+ * without a surface (a miss, or the null path (0,0,0) as Path::is_null
+ * treats it) get 0.
+ * svdag_trace_shadows: sun visibility times max(N.L, 0), L = the reference's
+ *   sun direction normalize(0.3, 1, 0.5); the shadow ray starts at
+ *   position + shadow_bias * L.
+ *   reference: DAGTracer::resolve_shadows (dag_tracer.cu:198-221) ->
+ *   Tracer::trace_shadows (tracer.cu:613-633).
+ * svdag_trace_ambient_occlusion: the fraction of num_samples cosine-weighted
+ *   hemisphere rays (length ao_ray_length, from position + normal *
+ *   shadow_bias) that reach no voxel; the samples of stored pixel i come from
+ *   PCG32 stream i * max(seed, 1).
+ *   reference: DAGTracer::resolve_ambient_occlusion (dag_tracer.cu:163-187) ->
+ *   Tracer::trace_ambient_occlusion (tracer.cu:661-691).                      */
+int svdag_trace_shadows(svdag_ctx* ctx, float shadow_bias, float* out_sun, float* gpu_ms);
+int svdag_trace_ambient_occlusion(svdag_ctx* ctx, uint32_t num_samples, float shadow_bias, float ao_ray_length,
+                                  uint64_t seed, float* out_ao, float* gpu_ms);
+
 /* Slab-arena usage over all tables of the DAG (no reference counterpart;
  * the reference reports memory_used_by_slabs, gpu_hash_table_base.cu:187-206).
  * min_free_fraction: the smallest free / capacity ratio over the tables. */

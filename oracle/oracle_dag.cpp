@@ -17,6 +17,8 @@
 //                       and the no-leaf skip at 651
 //   get_value           dag_utils.h:146-182
 //   tracer              dag_tracer.cu:59-101, tracer.cu:45-273, 353-411
+//   shadows / occlusion tracer.cu:274-351 (any-hit walk), 600-691,
+//                       tracer.h:106-109, PCG_RNG utils.h:45-82
 //   GC mark             my_gpu_hash_dag.cu:364-458 (roots: root + level-0
 //                       fully-filled node of every material)
 // Float rules: every float/double operation is written out in the
@@ -835,7 +837,8 @@ struct Hit {
     float t, px, py, pz;
 };
 
-Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const double* ddx, const double* ddy, uint32_t px, uint32_t py)
+Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const double* ddx, const double* ddy, uint32_t px, uint32_t py,
+                uint32_t* surfCode = nullptr)
 {
     double a[3];
     for (int k = 0; k < 3; ++k)
@@ -919,7 +922,166 @@ Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const dou
     }
     const int axis = dist[0] > dist[1] ? (dist[0] > dist[2] ? 0 : 2) : (dist[1] > dist[2] ? 1 : 2);
     const float t = dist[axis];
+    // createSurfaceInteraction: the normal points against the ray on the
+    // entry axis; the null path (0,0,0) is no surface (path.h:61-64).
+    if (surfCode)
+        *surfCode = (path[0] | path[1] | path[2]) == 0 ? 0u : 1u + 2u * (uint32_t)axis + (r.d[axis] < 0.0f ? 1u : 0u);
     return { path[0], path[1], path[2], material, t, r.o[0] + t * r.d[0], r.o[1] + t * r.d[1], r.o[2] + t * r.d[2] };
+}
+
+// intersect_ray_node_out_of_order (tracer.cu:274-351): any set voxel along
+// the ray; children are visited highest visit bit first.
+bool occluded_ray(const Dag& d, const Ray& r)
+{
+    if (d.root == SENT)
+        return false;
+    struct Entry {
+        uint32_t index, childMask, visitMask;
+    };
+    std::vector<Entry> stack(d.levels);
+    uint32_t path[3] = { 0, 0, 0 };
+    Entry cache { d.root, d.get(d.root)[0] & 0xFFu, 0 };
+    cache.visitMask = cache.childMask & isect_mask(true, d.levels, 0, path, r);
+    uint32_t level = 0;
+    uint64_t leafBits = 0;
+    for (;;) {
+        uint32_t nl = level;
+        while (nl > 0 && !cache.visitMask)
+            cache = stack[--nl];
+        if (nl == 0 && !cache.visitMask)
+            return false;
+        for (int k = 0; k < 3; ++k)
+            path[k] >>= (level - nl);
+        level = nl;
+        uint32_t c = 7;
+        while (!(cache.visitMask & (1u << c)))
+            --c; // 31 - clz(visitMask)
+        cache.visitMask &= ~(1u << c);
+        path[0] = (path[0] << 1) | ((c >> 2) & 1u);
+        path[1] = (path[1] << 1) | ((c >> 1) & 1u);
+        path[2] = (path[2] << 1) | (c & 1u);
+        stack[level] = cache;
+        ++level;
+        if (level == d.levels)
+            return true;
+        if (level < d.leafLevel) {
+            const uint32_t* n = d.get(cache.index);
+            cache.index = n[1 + popc(cache.childMask & ((1u << c) - 1u))];
+            cache.childMask = d.get(cache.index)[0] & 0xFFu;
+        } else if (level == d.leafLevel) {
+            const uint32_t* n = d.get(cache.index);
+            const uint32_t* leaf = d.get(n[1 + popc(cache.childMask & ((1u << c) - 1u))]);
+            leafBits = (uint64_t)leaf[0] | ((uint64_t)leaf[1] << 32);
+            uint32_t m = 0;
+            for (int b = 0; b < 8; ++b)
+                if ((leafBits >> (8 * b)) & 0xFF)
+                    m |= 1u << b;
+            cache.childMask = m;
+        } else {
+            const uint32_t word = (c & 4) ? (uint32_t)(leafBits >> 32) : (uint32_t)leafBits;
+            cache.childMask = (word >> ((c & 3) * 8)) & 0xFFu;
+        }
+        cache.visitMask = cache.childMask & isect_mask(false, d.levels, level, path, r);
+    }
+}
+
+Ray make_ray(const float* o, const float* dir, float tmax)
+{
+    Ray r;
+    for (int k = 0; k < 3; ++k) {
+        r.o[k] = o[k];
+        r.d[k] = dir[k];
+        r.inv[k] = 1.0f / dir[k];
+    }
+    r.tmin = 0.0f;
+    r.tmax = tmax;
+    return r;
+}
+
+// PCG32 XSH-RR (O'Neill 2014) with the reference's seeding (utils.h:45-82).
+struct Pcg32 {
+    uint64_t state, inc;
+    uint32_t next()
+    {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ull + (inc | 1ull);
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+};
+Pcg32 pcg_seeded(uint64_t stream)
+{
+    Pcg32 g { 0x853c49e6748fea9bull, stream };
+    g.next();
+    g.next();
+    return g;
+}
+float pcg_float(Pcg32& g) { return (float)g.next() / 4294967296.0f; }
+
+// sin/cos of 2*pi*u: nearest quarter turn plus Taylor series (degree 9 / 10)
+// of the remainder; the same float operations as the GPU sampler.
+void sincos_turns(float u, float& sn, float& cs)
+{
+    const float q = 4.0f * u;
+    const float k = std::floor(q + 0.5f);
+    const float x = (q - k) * 1.57079632679489662f;
+    const float x2 = x * x;
+    const float sp = x * (1.0f + x2 * (-0.166666667f + x2 * (8.33333333e-3f + x2 * (-1.98412698e-4f + x2 * 2.75573192e-6f))));
+    const float cp = 1.0f + x2 * (-0.5f + x2 * (4.16666667e-2f + x2 * (-1.38888889e-3f + x2 * (2.48015873e-5f + x2 * -2.75573192e-7f))));
+    switch ((int)k & 3) {
+    case 0: sn = sp; cs = cp; break;
+    case 1: sn = cp; cs = -sp; break;
+    case 2: sn = -sp; cs = -cp; break;
+    default: sn = -cp; cs = sp; break;
+    }
+}
+
+// Normal / dpdu / dpdv of a surface code (createSurfaceInteraction).
+void surface_frame(uint32_t code, float* n, float* du, float* dv)
+{
+    const uint32_t axis = (code - 1) >> 1;
+    for (int k = 0; k < 3; ++k)
+        n[k] = du[k] = dv[k] = 0.0f;
+    n[axis] = ((code - 1) & 1) ? 1.0f : -1.0f;
+    if (axis == 0) { du[2] = 1.0f; dv[1] = 1.0f; }
+    else if (axis == 1) { du[0] = 1.0f; dv[2] = 1.0f; }
+    else { du[0] = 1.0f; dv[1] = 1.0f; }
+}
+
+// trace_ambient_occlusion's per-pixel body (tracer.cu:661-691).
+float ao_at(const Dag& d, const float* p, uint32_t code, uint32_t samples, float bias, float len, uint64_t stream)
+{
+    float n[3], du[3], dv[3];
+    surface_frame(code, n, du, dv);
+    float o[3];
+    for (int k = 0; k < 3; ++k)
+        o[k] = p[k] + n[k] * bias;
+    Pcg32 g = pcg_seeded(stream);
+    uint32_t open = 0;
+    for (uint32_t s = 0; s < samples; ++s) {
+        const float ux = pcg_float(g), uy = pcg_float(g);
+        const float rr = std::sqrt(ux);
+        float sn, cs;
+        sincos_turns(uy, sn, cs);
+        const float zz = 1.0f - ux;
+        const float local[3] = { rr * cs, rr * sn, std::sqrt(0.0f < zz ? zz : 0.0f) };
+        float dir[3];
+        for (int k = 0; k < 3; ++k)
+            dir[k] = (local[0] * du[k] + local[1] * dv[k]) + local[2] * n[k];
+        if (!occluded_ray(d, make_ray(o, dir, len)))
+            ++open;
+    }
+    return (float)open / (float)samples;
+}
+
+// The reference's sun direction normalize(0.3f, 1, 0.5f), normalised in double.
+void sun_dir(float* L)
+{
+    const double v[3] = { (double)0.3f, 1.0, 0.5 };
+    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int k = 0; k < 3; ++k)
+        L[k] = (float)(v[k] / len);
 }
 
 } // namespace
@@ -1070,6 +1232,82 @@ void ora_dag_trace(void* dp, const double* position, const double* rotation, dou
             std::memcpy(out + ((size_t)(H - 1 - py) * W + px) * 8, &h, sizeof(h));
         }
 }
+
+// Primary trace + trace_shadows + trace_ambient_occlusion for every pixel;
+// outputs in the stored (row H-1-y) layout, 0 where there is no surface.
+void ora_dag_shade(void* dp, const double* position, const double* rotation, double fovDeg, const double* bmin,
+                   const double* bmax, uint32_t W, uint32_t H, float bias, uint32_t aoSamples, float aoLength,
+                   uint64_t seed, float* outSun, float* outAo)
+{
+    const Dag& d = *static_cast<Dag*>(dp);
+    const double right[3] = { -rotation[0], -rotation[1], -rotation[2] };
+    const double up[3] = { rotation[3], rotation[4], rotation[5] };
+    const double fwd[3] = { rotation[6], rotation[7], rotation[8] };
+    const double fov = fovDeg / 2.0 * (double(M_PI) / 180.);
+    const double aspect = double(W) / double(H);
+    double cam[3], rayMin[3], ddx[3], ddy[3];
+    for (int k = 0; k < 3; ++k) {
+        const double X = right[k] * std::sin(fov) * aspect, Y = up[k] * std::sin(fov), Z = fwd[k] * std::cos(fov);
+        const double bl = position[k] + Z - Y - X, br = position[k] + Z - Y + X, tl = position[k] + Z + Y - X;
+        const double tr = -bmin[k], sc = double(1 << d.levels) / (bmax[k] - bmin[k]);
+        cam[k] = (position[k] + tr) * sc;
+        rayMin[k] = (bl + tr) * sc;
+        ddx[k] = ((br + tr) * sc - rayMin[k]) * (1.0 / W);
+        ddy[k] = ((tl + tr) * sc - rayMin[k]) * (1.0 / H);
+    }
+    float L[3];
+    sun_dir(L);
+    for (uint32_t py = 0; py < H; ++py)
+        for (uint32_t px = 0; px < W; ++px) {
+            uint32_t code = 0;
+            const Hit h = trace_pixel(d, cam, rayMin, ddx, ddy, px, py, &code);
+            const size_t i = (size_t)(H - 1 - py) * W + px;
+            if (!code) {
+                outSun[i] = 0.0f;
+                outAo[i] = 0.0f;
+                continue;
+            }
+            const float p[3] = { h.px, h.py, h.pz };
+            float n[3], du[3], dv[3];
+            surface_frame(code, n, du, dv);
+            float o[3];
+            for (int k = 0; k < 3; ++k)
+                o[k] = p[k] + bias * L[k];
+            float ndl = (n[0] * L[0] + n[1] * L[1]) + n[2] * L[2];
+            ndl = ndl < 0.0f ? 0.0f : ndl;
+            outSun[i] = occluded_ray(d, make_ray(o, L, FLT_MAX)) ? 0.0f : ndl;
+            outAo[i] = aoSamples ? ao_at(d, p, code, aoSamples, bias, aoLength, (uint64_t)i * (seed > 1 ? seed : 1))
+                                 : 0.0f;
+        }
+}
+
+// Any-hit query of one ray (for the brute-force checks).
+int ora_dag_occluded(void* dp, const float* origin, const float* dir, float tmax)
+{
+    return occluded_ray(*static_cast<Dag*>(dp), make_ray(origin, dir, tmax)) ? 1 : 0;
+}
+
+// Occlusion at an explicit surface point with a surface code.
+float ora_dag_ao_at(void* dp, const float* p, uint32_t code, uint32_t samples, float bias, float len, uint64_t stream)
+{
+    return ao_at(*static_cast<Dag*>(dp), p, code, samples, bias, len, stream);
+}
+
+// PCG32 outputs from an explicit (state, inc), and from the reference seeding.
+void ora_pcg32_raw(uint64_t state, uint64_t inc, uint32_t n, uint32_t* out)
+{
+    Pcg32 g { state, inc };
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = g.next();
+}
+void ora_pcg32_seeded(uint64_t stream, uint32_t n, uint32_t* out)
+{
+    Pcg32 g = pcg_seeded(stream);
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = g.next();
+}
+void ora_sincos_turns(float u, float* sn, float* cs) { sincos_turns(u, *sn, *cs); }
+void ora_sun_dir(float* L) { sun_dir(L); }
 
 // Encoding helpers exposed for the reference's own encoding tests.
 uint32_t ora_leaf_build(uint32_t M, const uint32_t* present, const uint32_t* materials, uint32_t* out)

@@ -109,6 +109,17 @@ def _dag_sigs(L):
     L.ora_dag_live_items.restype = ctypes.c_uint64
     L.ora_dag_live_items.argtypes = [vp]
     L.ora_dag_trace.argtypes = [vp, f64p, f64p, ctypes.c_double, f64p, f64p, ctypes.c_uint32, ctypes.c_uint32, u32p]
+    L.ora_dag_shade.argtypes = [vp, f64p, f64p, ctypes.c_double, f64p, f64p, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_float, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64, f32p, f32p]
+    L.ora_dag_occluded.restype = ctypes.c_int
+    L.ora_dag_occluded.argtypes = [vp, f32p, f32p, ctypes.c_float]
+    L.ora_dag_ao_at.restype = ctypes.c_float
+    L.ora_dag_ao_at.argtypes = [vp, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_uint64]
+    L.ora_pcg32_raw.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, u32p]
+    L.ora_pcg32_seeded.argtypes = [ctypes.c_uint64, ctypes.c_uint32, u32p]
+    L.ora_sincos_turns.argtypes = [ctypes.c_float, f32p, f32p]
+    L.ora_sun_dir.argtypes = [f32p]
     L.ora_leaf_build.restype = ctypes.c_uint32
     L.ora_leaf_build.argtypes = [ctypes.c_uint32, u32p, u32p, u32p]
     L.ora_leaf_material.argtypes = [ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
@@ -205,6 +216,21 @@ class OracleDag:
                              out.ctypes.data_as(u32p))
         return out
 
+    def shade(self, position, rotation, fov, bmin, bmax, w, h, bias=1.0, ao_samples=3, ao_length=32.0, seed=1):
+        """(sun visibility * max(N.L, 0), ambient occlusion) per pixel, stored layout."""
+        sun = np.empty(w * h, np.float32)
+        ao = np.empty(w * h, np.float32)
+        f32p = ctypes.POINTER(ctypes.c_float)
+        dlib().ora_dag_shade(self.h, _d(position), _d(rotation), float(fov), _d(bmin), _d(bmax), w, h, float(bias),
+                             ao_samples, float(ao_length), seed, sun.ctypes.data_as(f32p), ao.ctypes.data_as(f32p))
+        return sun, ao
+
+    def occluded(self, origin, direction, tmax=3.4e38):
+        return bool(dlib().ora_dag_occluded(self.h, _f3(origin), _f3(direction), float(tmax)))
+
+    def ao_at(self, point, code, samples, bias, length, stream):
+        return float(dlib().ora_dag_ao_at(self.h, _f3(point), code, samples, float(bias), float(length), stream))
+
     def __del__(self):
         try:
             dlib().ora_dag_destroy(self.h)
@@ -260,3 +286,27 @@ def build_dense(dag, vox):
     """Factory-rule DAG over a dense voxel grid vox[x, y, z] (uint8, 0xFF = empty)."""
     vox = np.ascontiguousarray(vox, dtype=np.uint8)
     dlib().ora_dag_build_dense(dag.h, vox.ctypes.data_as(ctypes.c_void_p))
+
+
+def pcg32_raw(state, inc, n):
+    out = np.empty(n, np.uint32)
+    dlib().ora_pcg32_raw(state, inc, n, out.ctypes.data_as(u32p))
+    return out
+
+
+def pcg32_seeded(stream, n):
+    out = np.empty(n, np.uint32)
+    dlib().ora_pcg32_seeded(stream, n, out.ctypes.data_as(u32p))
+    return out
+
+
+def sincos_turns(u):
+    sn, cs = ctypes.c_float(0), ctypes.c_float(0)
+    dlib().ora_sincos_turns(ctypes.c_float(u), ctypes.byref(sn), ctypes.byref(cs))
+    return sn.value, cs.value
+
+
+def sun_dir():
+    L = (ctypes.c_float * 3)()
+    dlib().ora_sun_dir(L)
+    return np.array(list(L), np.float32)

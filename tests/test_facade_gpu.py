@@ -33,4 +33,5 @@ def test_facade_demo_matches_oracle():
     o.edit(workloads.PAINT, 7, (600, 500, 600), 50)
     o.edit(workloads.COPY, 0, (300, 520, 400), 30, src=(300, 520, 400), dst=(700, 530, 200))
     assert gpu_hash == o.canonical()[0]
-    assert int(r.stdout.split()[1]) > 0
+    hits, lit, unoccluded = (int(v) for v in r.stdout.split()[1:4])
+    assert hits > 0 and 0 < lit < hits and 0 < unoccluded <= hits
