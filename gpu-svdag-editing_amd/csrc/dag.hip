@@ -271,7 +271,10 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
         const Op op = decide(ed, vx, vy, vz, 0, hasOld ? NS_FILLED : NS_EMPTY, fm);
         bool has = false;
         uint32_t mat = 0;
-        if (op == OP_FILL || op == OP_SUBDIVIDE) {
+        if (ed.kind == EK_DENSE) {
+            has = op == OP_FILL; // a set voxel of the grid, with its own material
+            mat = fm;
+        } else if (op == OP_FILL || op == OP_SUBDIVIDE) {
             if (ed.kind == EK_COPY)
                 has = copy_value(ed, dag, vx, vy, vz, ws.state[w], hasOld, oldMat, mat);
             else
@@ -639,6 +642,40 @@ __global__ void pyramid_level_kernel(const int32_t* srcMin, const int32_t* srcMa
     }
 }
 
+// Dense-grid pyramid, level 0: voxel materials (0xFF = empty) to codes;
+// a material the DAG cannot hold sets *bad.
+__global__ void dense_base_kernel(const uint8_t* in, uint64_t n, uint32_t numMaterials, uint16_t* out, uint32_t* bad)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = in[i];
+        if (v == 0xFFu) {
+            out[i] = kDenseEmpty;
+        } else {
+            if (v >= numMaterials)
+                *bad = 1u;
+            out[i] = (uint16_t)v;
+        }
+    }
+}
+
+// Level k from level k-1: a cell keeps its children's code when all eight
+// agree, else it is mixed.
+__global__ void dense_level_kernel(const uint16_t* fine, uint32_t fineSide, uint16_t* coarse)
+{
+    const uint32_t s = fineSide / 2;
+    const uint64_t n = (uint64_t)s * s * s;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t z = (uint32_t)(i % s), y = (uint32_t)((i / s) % s), x = (uint32_t)(i / ((uint64_t)s * s));
+        const uint16_t c0 = fine[((uint64_t)(2 * x) * fineSide + 2 * y) * fineSide + 2 * z];
+        bool same = true;
+        for (uint32_t c = 1; c < 8 && same; ++c) {
+            const uint32_t fx = 2 * x + ((c >> 2) & 1u), fy = 2 * y + ((c >> 1) & 1u), fz = 2 * z + (c & 1u);
+            same = fine[((uint64_t)fx * fineSide + fy) * fineSide + fz] == c0;
+        }
+        coarse[i] = same ? c0 : kDenseMixed;
+    }
+}
+
 // splitmix64-driven Fisher-Yates shuffle of 0..255, duplicated to 512 entries.
 void perlin_perm(uint64_t seed, uint32_t* perm)
 {
@@ -740,6 +777,9 @@ Bounds edit_bounds(const svdag_ctx* ctx, const EditorDev& e)
         } else if (e.kind == EK_BOX_ADD || e.kind == EK_BOX_REMOVE || e.kind == EK_COPY) {
             lo = e.bmin[a] - 2.0;
             hi = e.bmax[a] + 2.0;
+        } else if (e.kind == EK_DENSE) {
+            lo = 0.0;
+            hi = (double)((1ull << e.denseLog2) - 1);
         }
         // Clamp in double first: far-away editors must not overflow the cast.
         lo = std::min(std::max(lo, -1.0), (double)side);
@@ -1167,6 +1207,120 @@ int svdag_destroy(svdag_ctx* ctx)
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SVDAG_OK;
+}
+
+// The factory build from a voxel source: the grid's material pyramid drives
+// the edit pipeline from an empty root, as the terrain's height pyramid does.
+int svdag_build_voxels(svdag_ctx* ctx, const uint8_t* materials, uint32_t sideLog2)
+{
+    if (!ctx || !materials || sideLog2 > ctx->dag.levels || sideLog2 > 11) {
+        set_error("svdag_build_voxels: bad arguments (side_log2 must be <= levels and <= 11)");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const uint64_t side = 1ull << sideLog2;
+    const uint64_t n = side * side * side;
+    uint64_t off[kMaxLevels] {};
+    uint64_t total = 0;
+    for (uint32_t k = 0; k <= sideLog2; ++k) {
+        off[k] = total;
+        const uint64_t sk = side >> k;
+        total += sk * sk * sk;
+    }
+    uint8_t* dIn = nullptr;
+    uint16_t* dPyr = nullptr;
+    uint32_t* dBad = nullptr;
+    auto release = [&]() {
+        if (dIn)
+            (void)hipFree(dIn);
+        if (dPyr)
+            (void)hipFree(dPyr);
+        if (dBad)
+            (void)hipFree(dBad);
+    };
+    int rc = SVDAG_OK;
+    hipError_t e = hipMalloc(&dIn, n);
+    if (e == hipSuccess)
+        e = hipMalloc(&dPyr, total * sizeof(uint16_t));
+    if (e == hipSuccess)
+        e = hipMalloc(&dBad, 4);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dIn, materials, n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(dBad, 0, 4, ctx->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("svdag_build_voxels: ") + hipGetErrorString(e));
+        release();
+        return SVDAG_E_DEVICE;
+    }
+    dense_base_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, ctx->stream>>>(dIn, n, ctx->dag.numMaterials, dPyr, dBad);
+    for (uint32_t k = 1; k <= sideLog2; ++k) {
+        const uint64_t sk = side >> k;
+        dense_level_kernel<<<grid_for(sk * sk * sk, 256, 1u << 16), 256, 0, ctx->stream>>>(
+            dPyr + off[k - 1], (uint32_t)(side >> (k - 1)), dPyr + off[k]);
+    }
+    uint32_t bad = 0;
+    e = hipMemcpyAsync(&bad, dBad, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("svdag_build_voxels: ") + hipGetErrorString(e));
+        release();
+        return SVDAG_E_DEVICE;
+    }
+    if (bad) {
+        set_error("svdag_build_voxels: a voxel holds a material the DAG's material bits cannot encode");
+        release();
+        return SVDAG_E_INVALID;
+    }
+    // A grid that fills the whole volume with one material is that
+    // material's level-0 fully-filled node.  The edit pipeline would build
+    // the root from eight filled children with the discovery gate closed
+    // (nothing below the root was built), so it is set directly.
+    uint16_t top = kDenseMixed;
+    e = hipMemcpy(&top, dPyr + off[sideLog2], 2, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sideLog2 == ctx->dag.levels && top < kDenseEmpty) {
+        uint32_t root = kSentinel;
+        e = hipMemcpy(&root, ctx->dFullyFilled + top, 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(ctx->dRoot, &root, 4, hipMemcpyHostToDevice);
+        release();
+        if (e != hipSuccess) {
+            set_error(std::string("svdag_build_voxels: ") + hipGetErrorString(e));
+            return SVDAG_E_DEVICE;
+        }
+        ctx->root = root;
+        ctx->undo.clear();
+        ctx->redo.clear();
+        return SVDAG_OK;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("svdag_build_voxels: ") + hipGetErrorString(e));
+        release();
+        return SVDAG_E_DEVICE;
+    }
+    EditorDev ed {};
+    ed.kind = EK_DENSE;
+    ed.levels = ctx->dag.levels;
+    ed.dense = dPyr;
+    ed.denseLog2 = sideLog2;
+    for (uint32_t k = 0; k <= sideLog2; ++k)
+        ed.pyrOffset[k] = off[k];
+    // The grid replaces whatever the DAG held: build from the empty root.
+    ctx->root = kSentinel;
+    e = hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("svdag_build_voxels: ") + hipGetErrorString(e));
+        release();
+        return SVDAG_E_DEVICE;
+    }
+    svdag_edit_stats stats;
+    rc = run_edit(ctx, ed, &stats);
+    (void)hipStreamSynchronize(ctx->stream);
+    release();
+    ctx->undo.clear();
+    ctx->redo.clear();
+    return rc;
 }
 
 int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)

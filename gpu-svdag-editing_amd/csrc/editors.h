@@ -24,8 +24,12 @@ enum EditKind : uint32_t {
     EK_BOX_REMOVE = 3,
     EK_PAINT = 4,
     EK_COPY = 5,
-    EK_TERRAIN = 6
+    EK_TERRAIN = 6,
+    EK_DENSE = 7
 };
+// Dense-grid pyramid codes besides the materials 0..255.
+constexpr uint16_t kDenseEmpty = 0x100;
+constexpr uint16_t kDenseMixed = 0x101;
 enum Op : uint32_t { OP_SUBDIVIDE = 0, OP_FILL = 1, OP_EMPTY = 2, OP_KEEP = 3 };
 enum NState : uint32_t { NS_PARTIAL = 0, NS_FILLED = 1, NS_EMPTY = 2 };
 
@@ -47,6 +51,10 @@ struct EditorDev {
     const int32_t* hmin;
     const int32_t* hmax;
     uint64_t pyrOffset[kMaxLevels];
+    // dense grid of side 2^denseLog2 at the origin: pyramid level k holds
+    // (side >> k)^3 codes indexed (x * s + y) * s + z (pyrOffset as above)
+    const uint16_t* dense;
+    uint32_t denseLog2;
 };
 
 // One copy-editor source parent: path and handle (handle == kSentinel ends the list).
@@ -152,6 +160,26 @@ __device__ inline Op decide(const EditorDev& e, uint32_t x, uint32_t y, uint32_t
     }
     case EK_COPY:
         return box_overlap(e.bmin, e.bmax, mn, mx) ? OP_SUBDIVIDE : OP_KEEP;
+    case EK_DENSE: {
+        // A cell of one material is filled with it, an empty cell is empty,
+        // a mixed cell is subdivided.  Nodes larger than the grid hold it at
+        // their low corner (the origin node) or nothing.
+        if (depth > e.denseLog2) {
+            if ((x | y | z) != 0u)
+                return OP_EMPTY;
+            return e.dense[e.pyrOffset[e.denseLog2]] == kDenseEmpty ? OP_EMPTY : OP_SUBDIVIDE;
+        }
+        const uint64_t s = 1ull << (e.denseLog2 - depth);
+        if (x >= s || y >= s || z >= s)
+            return OP_EMPTY;
+        const uint32_t c = e.dense[e.pyrOffset[depth] + ((uint64_t)x * s + y) * s + z];
+        if (c == kDenseEmpty)
+            return OP_EMPTY;
+        if (c == kDenseMixed)
+            return OP_SUBDIVIDE;
+        fillMat = c;
+        return OP_FILL;
+    }
     case EK_TERRAIN: {
         const uint64_t width = (uint64_t)1 << (e.levels - depth);
         const uint64_t idx = e.pyrOffset[depth] + (uint64_t)z * width + x;

@@ -86,6 +86,12 @@ public:
     }
 
     void build_terrain(const svdag_terrain& t) { check(svdag_build_terrain(ctx_, &t)); }
+    // The factory build (my_gpu_hash_dag_factory.cpp:317-508) from a dense grid
+    // materials[(x * side + y) * side + z] of side 2^sideLog2, 0xFF = empty.
+    void build_from_voxels(const uint8_t* materials, uint32_t sideLog2)
+    {
+        check(svdag_build_voxels(ctx_, materials, sideLog2));
+    }
 
     // DAGUtils::get_value over many voxels: 0xFFFFFFFF marks an empty voxel.
     std::vector<uint32_t> get_values(const std::vector<std::array<uint32_t, 3>>& xyz) const

@@ -105,7 +105,7 @@ EXPORTED = [
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
-    "svdag_trace_shadows", "svdag_trace_ambient_occlusion",
+    "svdag_trace_shadows", "svdag_trace_ambient_occlusion", "svdag_build_voxels",
 ]
 
 _lib = None
@@ -268,6 +268,14 @@ class MyGPUHashDAG:
         check(lib().svdag_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.levels = levels
         self.material_bits = material_bits
+
+    def build_voxels(self, vox):
+        """Replace the DAG by a dense grid vox[x, y, z] (uint8 materials, 0xFF = empty) of side 2^k."""
+        vox = np.ascontiguousarray(vox, dtype=np.uint8)
+        side = vox.shape[0]
+        if vox.ndim != 3 or vox.shape != (side, side, side) or side & (side - 1):
+            raise ValueError("vox must be a cube of side 2^k")
+        check(lib().svdag_build_voxels(self.h, vox.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(side.bit_length() - 1)))
 
     def build_terrain(self, seed, base, amplitude, frequency, octaves, material):
         t = Terrain(seed, base, amplitude, frequency, octaves, material)

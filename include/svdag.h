@@ -138,6 +138,16 @@ int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* terrain);
 /* Heights of a rectangle [x0,x0+w) x [z0,z0+d), row-major in z (out: w*d int32). */
 int svdag_terrain_heights(svdag_ctx* ctx, uint32_t x0, uint32_t z0, uint32_t w, uint32_t d, int32_t* out);
 
+/* Build the DAG from a dense voxel grid of side 2^side_log2 (<= levels and
+ * <= 11) at the origin: materials[(x * side + y) * side + z] is the voxel's
+ * material, 0xFF = empty.  The grid replaces the DAG (history cleared); a
+ * material the material bits cannot hold is SVDAG_E_INVALID.  Every node
+ * whose voxels all hold one material is that material's fully-filled node,
+ * the rule of the reference's factory.
+ * reference: MyGPUHashDAGFactory::load_from_DAG / create_dag_cpu
+ * (my_gpu_hash_dag_factory.cpp:317-508), here from a voxel source.          */
+int svdag_build_voxels(svdag_ctx* ctx, const uint8_t* materials, uint32_t side_log2);
+
 /* reference: MyGPUHashDAG::get_first_node_index / firstNodeIndex (my_gpu_hash_dag.h:72, 290-293). */
 int svdag_root(svdag_ctx* ctx, uint32_t* out_root);
 
