@@ -878,9 +878,10 @@ static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
     ++ctx->bufferGeneration;
     for (void* p : { (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
-                     (void*)ctx->dWaveIdx })
+                     (void*)ctx->dWaveIdx, ctx->dSpine })
         if (p)
             (void)hipFree(p);
+    ctx->dSpine = nullptr;
     for (void* p : { ctx->hBatch, ctx->hFused })
         if (p)
             (void)hipHostFree(p);
@@ -892,7 +893,8 @@ static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
     SVDAG_HIP(hipMalloc(&ctx->dSeeds, (size_t)n * sizeof(SeedDesc)));
     SVDAG_HIP(hipMalloc(&ctx->dSeedWork, (size_t)n * 8 * 4));
     SVDAG_HIP(hipMalloc(&ctx->dWaves, (size_t)n * sizeof(uint2)));
-    SVDAG_HIP(hipMalloc(&ctx->dWaveIdx, 4));
+    SVDAG_HIP(hipMalloc(&ctx->dWaveIdx, 8)); // wave index, then the parallel spine's done flag
+    SVDAG_HIP(hipMalloc(&ctx->dSpine, spine_scratch_bytes()));
     SVDAG_HIP(hipHostMalloc(&ctx->hFused, (size_t)n * (sizeof(SeedDesc) + sizeof(uint2))));
     SVDAG_HIP(hipMalloc(&ctx->dEditors, (size_t)n * sizeof(EditorDev)));
     SVDAG_HIP(hipMalloc(&ctx->dStatusArr, (size_t)n * sizeof(EditStatus)));
@@ -949,7 +951,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
             SVDAG_HIP(hipMemcpyAsync(ctx->dSeeds + first, seeds + first, (n - first) * sizeof(SeedDesc),
                                      hipMemcpyHostToDevice, ctx->stream));
             SVDAG_HIP(hipMemcpyAsync(ctx->dWaves, waves, nWaves * sizeof(uint2), hipMemcpyHostToDevice, ctx->stream));
-            SVDAG_HIP(hipMemsetAsync(ctx->dWaveIdx, 0, 4, ctx->stream));
+            SVDAG_HIP(hipMemsetAsync(ctx->dWaveIdx, 0, 8, ctx->stream));
         }
         SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         for (const BatchStep& b : steps) {
@@ -1195,7 +1197,7 @@ int svdag_destroy(svdag_ctx* ctx)
                      (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits,
                      (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
-                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade })
+                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade, ctx->dSpine })
         if (p)
             (void)hipFree(p);
     for (void* p : { ctx->hBatch, ctx->hFused })

@@ -69,7 +69,12 @@ struct FusedDev {
     const SeedDesc* seeds;  // [batch]
     uint32_t* seedWork;     // [batch][8]: level-S work index of a cell, or kSentinel
     uint32_t S;             // split level: the fused part covers levels S .. leaf
+    uint32_t* spineDone;    // set by the parallel spine when it built the wave's spine
+    void* spine;            // the parallel spine's scratch (spine_scratch_bytes())
 };
+
+// Bytes of device scratch the parallel spine needs (fused.hip).
+size_t spine_scratch_bytes();
 
 // A leaf is fully filled iff all 64 voxels are set with one material
 // (my_gpu_hash_dag_factory.cpp:357-360).
@@ -156,6 +161,7 @@ struct svdag_ctx {
     // the shadow and ambient-occlusion passes.
     uint8_t* dSurf = nullptr;
     float* dShade = nullptr; // per-pixel output of the shadow / occlusion passes (hitCap floats)
+    void* dSpine = nullptr;  // parallel spine scratch, allocated with the batch slots
     uint32_t traceW = 0, traceH = 0, traceRoot = 0xFFFFFFFFu;
     std::mutex mu;
     // Serving (serve.hip).  While a server runs, frame traces read the tables

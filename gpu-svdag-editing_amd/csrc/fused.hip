@@ -104,6 +104,440 @@ __global__ void __launch_bounds__(256) fused_seed(WsDev ws, DagDev dag, const Ed
     }
 }
 
+// ------------------------------------------------------------------ parallel spine
+// The spine of a wave of sphere and box edits, built one level at a time for
+// all edits at once instead of edit after edit.
+//
+// Why it is the same DAG.  Above S these tools decide by geometry alone (a
+// node there is larger than any edit of the wave, so it is neither contained
+// nor left untouched by state), so every edit's spine SVO is the same against
+// the wave's input root as against the root the edits before it left.  The
+// node an edit k builds at spine position P takes each child c either from
+// its own level below (c subdivided by k) or from the DAG as the edits
+// before k left it: the version of c built by the latest earlier edit whose
+// spine contains c, or c's handle in the input DAG.  So level l needs only
+// level l + 1 of every edit, and all edits' nodes of one level are built,
+// deduplicated and found / inserted together: S sequential steps instead of
+// S per edit.  Paint reads node state above S, and anything unexpected (a
+// fill above S, more than eight items on a level, a missing seed) leaves the
+// wave to the sequential spine.
+constexpr uint32_t kParEdits = 64;
+constexpr uint32_t kParItems = kParEdits * kSpineItems;
+constexpr uint32_t kParThreads = 1024;
+
+struct SpineItem {
+    uint32_t x, y, z, orig; // position at its level; its handle in the wave's input DAG
+    uint32_t sub;           // children this edit subdivides
+    uint32_t ref[8];        // per subdivided child: its item one level down, or its seed work item at S
+    uint32_t handle, created;
+};
+struct SpineScratch {
+    SpineItem item[kParEdits][kMaxLevels][kSpineItems];
+    uint32_t cnt[kParEdits][kMaxLevels];
+    uint32_t full[kParEdits][kMaxLevels];
+};
+
+size_t spine_scratch_bytes()
+{
+    return sizeof(SpineScratch);
+}
+
+__device__ inline uint32_t joint_handle(const WsDev& ws, uint32_t w, uint32_t& created)
+{
+    const uint32_t r = ws.rep[w];
+    if (r == kSentinel) {
+        created = 0u;
+        return kSentinel;
+    }
+    created = ws.created[r] & 1u;
+    return ws.handle[r];
+}
+
+__global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev dag, const EditorDev* eds, FusedDev f,
+                                                               uint32_t* rootPtr, EditStatus* status,
+                                                               LevelDesc* levels)
+{
+    __shared__ uint32_t sPos[kParItems][3];
+    __shared__ uint32_t sRec[kParItems][kNodeStride];
+    __shared__ uint32_t sNew[kParItems], sRep[kParItems], sResH[kParItems], sResC[kParItems];
+    __shared__ uint32_t sRepList[kParItems], sSubCnt[kParItems];
+    __shared__ uint32_t sSub[kParItems], sChildNew[kParItems], sChildH[kParItems][8];
+    __shared__ uint32_t sHash[2 * kParItems], sPosHash[2 * kParItems], sPosOf[kParItems], sPrev[kParItems];
+    __shared__ unsigned long long sPosEdits[kParItems];
+    __shared__ uint32_t sFullCnt[kParEdits], sDropped[kParEdits];
+    __shared__ uint32_t sOk, sBad, sOom, sNumReps;
+    const uint32_t t = threadIdx.x;
+    const uint2 wv = f.waves[*f.waveIdx];
+    const uint32_t K = wv.y - wv.x, S = f.S;
+    const uint32_t leafLevel = dag.leafLevel;
+    SpineScratch& sc = *reinterpret_cast<SpineScratch*>(f.spine);
+    const EditStatus joint = *ws.status;
+    if (t == 0) {
+        uint32_t ok = K >= 1 && K <= kParEdits && S >= 1 && S <= leafLevel && !joint.overflow && !joint.oom;
+        for (uint32_t k = 0; ok && k < K; ++k) {
+            const uint32_t kind = eds[wv.x + k].kind;
+            ok = kind == EK_SPHERE_ADD || kind == EK_SPHERE_REMOVE || kind == EK_BOX_ADD || kind == EK_BOX_REMOVE;
+        }
+        sOk = ok;
+        sBad = 0u;
+        sOom = 0u;
+    }
+    __syncthreads();
+    if (!sOk)
+        return; // the sequential spine takes the wave
+    const uint32_t root0 = *rootPtr;
+    // An edit's level may end up without items (an edit off the volume
+    // subdivides nothing), so every count starts at zero.
+    for (uint32_t q = t; q < K * kMaxLevels; q += blockDim.x)
+        sc.cnt[q / kMaxLevels][q % kMaxLevels] = q % kMaxLevels == 0u ? 1u : 0u;
+    if (t < K) {
+        SpineItem& it = sc.item[t][0][0];
+        it.x = it.y = it.z = 0u;
+        it.orig = root0;
+        sDropped[t] = ws.edCount[(wv.x + t) * kMaxLevels + leafLevel] == 0u ? 1u : 0u;
+    }
+    __syncthreads();
+
+    // ---- top-down: every edit's spine items on levels 0 .. S-1.
+    const uint32_t k = t >> 3, i = t & 7u; // one (edit, item) pair per thread
+    for (uint32_t l = 0; l < S; ++l) {
+        uint32_t sub = 0, origChild[8];
+        const bool active = k < K && i < sc.cnt[k][l];
+        if (active) {
+            SpineItem& it = sc.item[k][l][i];
+            const EditorDev& ed = eds[wv.x + k];
+            uint32_t mask = 0;
+            const uint32_t* node = nullptr;
+            if (it.orig != kSentinel) {
+                node = dag_node_ptr(dag, it.orig);
+                mask = node[0] & 0xFFu;
+            }
+            for (uint32_t c = 0; c < 8; ++c) {
+                origChild[c] = ((mask >> c) & 1u) ? node[1 + __builtin_popcount(mask & ((1u << c) - 1u))] : kSentinel;
+                const uint32_t cx = (it.x << 1) | ((c >> 2) & 1u), cy = (it.y << 1) | ((c >> 1) & 1u),
+                               cz = (it.z << 1) | (c & 1u);
+                uint32_t fm = 0;
+                const Op op = decide(ed, cx, cy, cz, dag.levels - (l + 1), NS_PARTIAL, fm);
+                if (op == OP_SUBDIVIDE)
+                    sub |= 1u << c;
+                else if (op != OP_KEEP)
+                    atomicOr(&sBad, 1u);
+            }
+            it.sub = sub;
+        }
+        if (t < kParItems)
+            sSubCnt[t] = active ? (uint32_t)__builtin_popcount(sub) : 0u;
+        __syncthreads();
+        if (active) {
+            SpineItem& it = sc.item[k][l][i];
+            uint32_t n = 0;
+            for (uint32_t q = 0; q < i; ++q)
+                n += sSubCnt[k * kSpineItems + q];
+            const SeedDesc sd = f.seeds[wv.x + k];
+            for (uint32_t c = 0; c < 8; ++c) {
+                if (!((sub >> c) & 1u))
+                    continue;
+                const uint32_t cx = (it.x << 1) | ((c >> 2) & 1u), cy = (it.y << 1) | ((c >> 1) & 1u),
+                               cz = (it.z << 1) | (c & 1u);
+                if (l + 1 < S) {
+                    if (n < kSpineItems) {
+                        SpineItem& ch = sc.item[k][l + 1][n];
+                        ch.x = cx;
+                        ch.y = cy;
+                        ch.z = cz;
+                        ch.orig = origChild[c];
+                    } else {
+                        atomicOr(&sBad, 1u);
+                    }
+                    it.ref[c] = n++;
+                } else {
+                    const uint32_t dx = cx - sd.lo[0], dy = cy - sd.lo[1], dz = cz - sd.lo[2];
+                    const bool inRange = dx < sd.n[0] && dy < sd.n[1] && dz < sd.n[2];
+                    const uint32_t w = inRange ? f.seedWork[(wv.x + k) * 8 + ((dx << 2) | (dy << 1) | dz)] : kSentinel;
+                    if (w == kSentinel)
+                        atomicOr(&sBad, 1u);
+                    it.ref[c] = w;
+                }
+            }
+            if (i == 0 && l + 1 < S) {
+                uint32_t tot = 0;
+                for (uint32_t q = 0; q < kSpineItems; ++q)
+                    tot += sSubCnt[k * kSpineItems + q];
+                sc.cnt[k][l + 1] = tot < kSpineItems ? tot : kSpineItems;
+                if (tot > kSpineItems)
+                    atomicOr(&sBad, 1u);
+            }
+        }
+        __syncthreads();
+        if (sBad)
+            return; // nothing written outside the scratch: the sequential spine takes the wave
+    }
+
+    // ---- bottom-up: levels S-1 .. 0, every edit's nodes of a level at once.
+    const uint32_t lane = t & 63u, gl = t & 7u, gbase = lane & ~7u;
+    for (int l = (int)S - 1; l >= 0; --l) {
+        const bool active = k < K && i < sc.cnt[k][l] && !sDropped[k];
+        const uint32_t idx = k * kSpineItems + i;
+        const bool childLevelS = (uint32_t)l + 1 == S;
+        const bool leafChildren = (uint32_t)l + 1 == leafLevel;
+        // Stage the level in LDS: positions, and the subdivided children with
+        // their merge results.  A skipped (no-leaf) edit rebuilds nothing, so
+        // it stages no subdivided children.
+        if (t < kParItems) {
+            uint32_t subm = 0, newm = 0;
+            if (k < K && i < sc.cnt[k][l]) {
+                const SpineItem& it = sc.item[k][l][i];
+                sPos[t][0] = it.x;
+                sPos[t][1] = it.y;
+                sPos[t][2] = it.z;
+                if (!sDropped[k]) {
+                    subm = it.sub;
+                    for (uint32_t c = 0; c < 8; ++c) {
+                        if (!((subm >> c) & 1u))
+                            continue;
+                        uint32_t cr = 0;
+                        if (childLevelS) {
+                            sChildH[t][c] = joint_handle(ws, it.ref[c], cr);
+                        } else {
+                            const SpineItem& ch = sc.item[k][l + 1][it.ref[c]];
+                            sChildH[t][c] = ch.handle;
+                            cr = ch.created;
+                        }
+                        newm |= cr << c;
+                    }
+                }
+            } else {
+                sPos[t][0] = kSentinel;
+            }
+            sSub[t] = subm;
+            sChildNew[t] = newm;
+        }
+        for (uint32_t h = t; h < 2 * kParItems; h += blockDim.x) {
+            sHash[h] = 0u;
+            sPosHash[h] = 0u;
+        }
+        if (t < kParItems)
+            sPosEdits[t] = 0ull;
+        if (t < kParEdits)
+            sFullCnt[t] = 0u;
+        if (t == 0)
+            sNumReps = 0u;
+        __syncthreads();
+        // Group the level's items by position: the first item to claim a
+        // position names it, and collects the set of edits with an item there.
+        if (t < kParItems && sPos[t][0] != kSentinel) {
+            const uint32_t px = sPos[t][0], py = sPos[t][1], pz = sPos[t][2];
+            uint32_t p = mix32(px * 0x9E3779B1u ^ mix32(py * 0x85EBCA77u ^ mix32(pz))) & (2 * kParItems - 1u);
+            uint32_t owner = t;
+            while (true) {
+                uint32_t v = sPosHash[p];
+                if (v == 0u) {
+                    v = atomicCAS(&sPosHash[p], 0u, t + 1u);
+                    if (v == 0u)
+                        break;
+                }
+                if (sPos[v - 1u][0] == px && sPos[v - 1u][1] == py && sPos[v - 1u][2] == pz) {
+                    owner = v - 1u;
+                    break;
+                }
+                p = (p + 1u) & (2 * kParItems - 1u);
+            }
+            sPosOf[t] = owner;
+            atomicOr(&sPosEdits[owner], 1ull << (t / kSpineItems));
+        }
+        __syncthreads();
+        // Each item's predecessor: the same position's item in the latest
+        // earlier edit that has one.
+        if (t < kParItems) {
+            uint32_t prev = kSentinel;
+            if (sPos[t][0] != kSentinel) {
+                const uint32_t e = t / kSpineItems;
+                const unsigned long long before = sPosEdits[sPosOf[t]] & ((1ull << e) - 1ull);
+                if (before != 0ull) {
+                    const uint32_t j = 63u - (uint32_t)__builtin_clzll(before);
+                    for (uint32_t q = 0; q < kSpineItems; ++q) {
+                        const uint32_t o = j * kSpineItems + q;
+                        if (sPos[o][0] != kSentinel && sPosOf[o] == sPosOf[t]) {
+                            prev = o;
+                            break;
+                        }
+                    }
+                }
+            }
+            sPrev[t] = prev;
+        }
+        __syncthreads();
+        bool empty = true;
+        if (active) {
+            const SpineItem& it = sc.item[k][l][i];
+            uint32_t mask = 0;
+            const uint32_t* node = nullptr;
+            if (it.orig != kSentinel) {
+                node = dag_node_ptr(dag, it.orig);
+                mask = node[0] & 0xFFu;
+            }
+            const uint32_t mySub = sSub[idx];
+            uint32_t ver[8];
+            for (uint32_t c = 0; c < 8; ++c)
+                ver[c] = ((mySub >> c) & 1u) ? sChildH[idx][c] : kSentinel;
+            // A child this edit leaves alone is the version the latest earlier
+            // edit rebuilt, else the input DAG's: walk the position's
+            // predecessors until every such child has its version.
+            uint32_t left = ~mySub & 0xFFu, from = 0u;
+            for (uint32_t o = sPrev[idx]; left != 0u && o != kSentinel; o = sPrev[o]) {
+                const uint32_t hit = sSub[o] & left;
+                for (uint32_t c = 0; c < 8; ++c)
+                    if ((hit >> c) & 1u)
+                        ver[c] = sChildH[o][c];
+                left &= ~hit;
+                from |= hit;
+            }
+            from |= mySub;
+            uint32_t outH[8], nOut = 0, newMask = 0;
+            const uint32_t newChild = sChildNew[idx] != 0u ? 1u : 0u;
+            for (uint32_t c = 0; c < 8; ++c) {
+                uint32_t h = ver[c];
+                if (!((from >> c) & 1u) && ((mask >> c) & 1u))
+                    h = node[1 + __builtin_popcount(mask & ((1u << c) - 1u))];
+                if (h != kSentinel) {
+                    outH[nOut++] = h;
+                    newMask |= 1u << c;
+                }
+            }
+            // Discovery, gated on this edit's full items one level down.
+            const uint32_t prevFull = childLevelS ? ws.edFull[(wv.x + k) * kMaxLevels + S] : sc.full[k][l + 1];
+            bool full = false;
+            uint32_t fm = 0;
+            if (prevFull != 0u && newMask == 0xFFu) {
+                full = true;
+                for (uint32_t j = 0; j < 8 && full; ++j) {
+                    uint32_t m = 0;
+                    const bool fj = leafChildren ? leaf_is_full(dag_leaf_ptr(dag, outH[j]), dag.M, m)
+                                                 : header_full(dag_node_ptr(dag, outH[j])[0], dag.M, m);
+                    if (j == 0)
+                        fm = m;
+                    full = fj && m == fm;
+                }
+            }
+            empty = newMask == 0u;
+            uint32_t* rec = sRec[idx];
+            rec[0] = full ? header_make_full(0xFFu, fm, dag.M) : header_make(newMask, 0u, dag.M);
+            for (uint32_t j = 0; j < 8; ++j)
+                rec[1 + j] = j < nOut ? outH[j] : 0u;
+            rec[9] = 0u;
+            sNew[idx] = newChild;
+            if (full)
+                atomicAdd(&sFullCnt[k], 1u);
+        }
+        __syncthreads();
+        // In-level dedup over every edit's nodes: the first claimant of a
+        // content is its representative.
+        if (active && !empty) {
+            const uint32_t* rec = sRec[idx];
+            uint32_t hh = 0;
+            for (uint32_t w = 0; w < kNodeStride; ++w)
+                hh = mix32(hh ^ (rec[w] + w * 0x9E3779B9u));
+            uint32_t p = hh & (2 * kParItems - 1u);
+            uint32_t rep = idx;
+            while (true) {
+                uint32_t v = sHash[p];
+                if (v == 0u) {
+                    v = atomicCAS(&sHash[p], 0u, idx + 1u);
+                    if (v == 0u)
+                        break;
+                }
+                bool eq = true;
+                for (uint32_t w = 0; w < kNodeStride && eq; ++w)
+                    eq = sRec[v - 1u][w] == rec[w];
+                if (eq) {
+                    rep = v - 1u;
+                    break;
+                }
+                p = (p + 1u) & (2 * kParItems - 1u);
+            }
+            sRep[idx] = rep;
+            if (rep == idx)
+                sRepList[atomicAdd(&sNumReps, 1u)] = idx;
+        } else if (t < kParItems) {
+            sRep[t] = kSentinel;
+        }
+        if (t < kParEdits && t < K)
+            sc.full[t][l] = sFullCnt[t];
+        __syncthreads();
+        // DAG find / insert of the representatives, eight lanes each.
+        const uint32_t nReps = sNumReps;
+        for (uint32_t r = t >> 3; r < nReps; r += kParThreads / 8u) {
+            const uint32_t ri = sRepList[r];
+            const uint32_t* rec = sRec[ri];
+            const uint32_t s = node_size_from_header(rec[0]);
+            Needle<8> nd;
+            needle_load(nd, rec, s, gl);
+            const uint32_t hsh = needle_hash(nd, s, gl);
+            uint32_t res = kNotFound, hint = kNoSlab, created = 0u;
+            if (sNew[ri] == 0u)
+                res = table_find_group_hint<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, hint);
+            if (res == kNotFound) {
+                res = table_insert_group<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, hint);
+                created = 1u;
+            }
+            if (gl == 0) {
+                if (res == kNotFound) {
+                    sOom = 1u;
+                    sResH[ri] = kSentinel;
+                } else {
+                    sResH[ri] = node_handle(res, s);
+                }
+                sResC[ri] = created;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            SpineItem& it = sc.item[k][l][i];
+            const uint32_t rep = sRep[idx];
+            it.handle = rep == kSentinel ? kSentinel : sResH[rep];
+            it.created = rep == kSentinel ? 0u : sResC[rep];
+        }
+        __syncthreads();
+        if (sOom)
+            break;
+    }
+
+    // ---- per-edit results, in edit order.
+    if (sOom) {
+        if (t == 0) {
+            EditStatus st {};
+            st.oom = 1u;
+            st.rootBefore = st.rootAfter = root0;
+            status[wv.x] = st;
+            *f.spineDone = 1u;
+        }
+        return;
+    }
+    for (uint32_t q = t; q < K * kMaxLevels; q += blockDim.x) {
+        const uint32_t e = q / kMaxLevels, lev = q % kMaxLevels;
+        LevelDesc d {};
+        d.count = lev < S ? sc.cnt[e][lev] : ws.edCount[(wv.x + e) * kMaxLevels + lev];
+        levels[(wv.x + e) * kMaxLevels + lev] = d;
+    }
+    if (t == 0) {
+        uint32_t root = root0;
+        for (uint32_t e = 0; e < K; ++e) {
+            EditStatus st {};
+            st.done = 1u;
+            st.rootBefore = root;
+            if (!sDropped[e]) {
+                st.applied = 1u;
+                st.leafCount = ws.edCount[(wv.x + e) * kMaxLevels + leafLevel];
+                root = sc.item[e][0][0].handle;
+            }
+            st.rootAfter = root;
+            status[wv.x + e] = st;
+        }
+        *rootPtr = root;
+        *f.spineDone = 1u;
+    }
+}
+
 // One wave64 (the whole block) replays the wave's edits on levels 0 .. S-1.
 __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const EditorDev* eds, FusedDev f,
                                                   uint32_t* rootPtr, EditStatus* status, LevelDesc* levels)
@@ -115,6 +549,15 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
     __shared__ uint32_t sHasNew[kSpineItems];               // item has a created child
     __shared__ uint32_t sCnt[kMaxLevels];
     const uint32_t lane = threadIdx.x;
+    if (*f.spineDone) {
+        // The parallel spine took this wave.
+        __syncthreads();
+        if (lane == 0) {
+            *f.spineDone = 0u;
+            *f.waveIdx += 1u;
+        }
+        return;
+    }
     const uint2 wv = f.waves[*f.waveIdx];
     const uint32_t S = f.S;
     const uint32_t leafLevel = dag.leafLevel;
@@ -486,7 +929,7 @@ static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, 
     WsDev ws = ctx->ws;
     ws.edCount = ctx->dEdCount;
     ws.edFull = ctx->dEdFull;
-    const FusedDev f { ctx->dWaves, ctx->dWaveIdx, ctx->dSeeds, ctx->dSeedWork, S };
+    const FusedDev f { ctx->dWaves, ctx->dWaveIdx, ctx->dSeeds, ctx->dSeedWork, S, ctx->dWaveIdx + 1, ctx->dSpine };
     wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
     fused_seed<<<(seedLanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
     if (levelItems) {
@@ -497,6 +940,7 @@ static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, 
             fixed[L] = ~0ull;
         SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, fixed, 2048));
     }
+    fused_spine_par<<<1, kParThreads, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
     fused_spine<<<1, 64, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;

@@ -307,6 +307,49 @@ def test_fused_waves_match_sequential_step_by_step(levels):
     b.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch", [16, 64])
+def test_parallel_spine_waves_match_sequential(batch):
+    # Sphere and box streams only (no paint): their waves take the parallel
+    # spine.  Every edit's applied flag, SVO sizes and resulting DAG (walked
+    # back through undo) equal the sequential form; colliding clusters, an
+    # off-volume edit and a no-leaf fill are mixed in.
+    levels = L10
+    tp = workloads.terrain_params(levels)
+    side = 1 << levels
+    heights = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                        0, 0, side, side)
+    edits = workloads.surface_edits(lambda x, z: float(heights[z, x]), 72, seed=31 + batch, levels=levels,
+                                    rmin=2.0, rmax=side / 32, near=side / 32)
+    c = edits[3][2]
+    edits[5:5] = [(svdag_py.SPHERE_REMOVE, 0, c, 20.0), (svdag_py.BOX_ADD, 3, c, 9.0),
+                  (svdag_py.SPHERE_ADD, 5, (c[0] + 7, c[1], c[2]), 12.0)]
+    edits.insert(40, (svdag_py.SPHERE_ADD, 2, (-5000.0, 10.0, 10.0), 40.0))
+    edits.insert(41, (svdag_py.BOX_ADD, 7, (side / 4, side / 4, side / 4), side / 4))
+    a, _ = _pair(levels=levels)
+    b, o = _pair(levels=levels)
+    a.set_batch_mode(0)
+    b.set_batch_mode(1)
+    sa, sb = [], []
+    for i in range(0, len(edits), batch):
+        sa += a.edit_batch(edits[i:i + batch])[0]
+        sb += b.edit_batch(edits[i:i + batch])[0]
+    for k, (s1, s2) in enumerate(zip(sa, sb)):
+        assert s1.applied == s2.applied, k
+        assert list(s1.svo_items) == list(s2.svo_items), k
+    for e in edits:
+        o.edit(*e)
+    _same(b, o)
+    assert a.history() == b.history()
+    for _ in range(a.history()[0]):
+        assert a.canonical() == b.canonical()
+        a.undo()
+        b.undo()
+    assert a.canonical() == b.canonical()
+    a.close()
+    b.close()
+
+
 def _frame_camera(levels):
     cam = workloads.config1_camera(levels)
     rot = svdag_py.look_at(cam["position"], cam["target"])
