@@ -244,83 +244,125 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
             atomicAdd(&ws.edCount[histEdit[threadIdx.x] * kMaxLevels + L + 1], v);
     }
 }
-// One leaf per wave64, one lane per voxel (create_edit_svo.cu:320-426).
+// Four leaves per wave64: sixteen lanes per leaf, each deciding voxels
+// s, s + 16, s + 32 and s + 48 of it (create_edit_svo.cu:320-426).  A leaf's
+// inputs (work item, editor, old leaf) are a chain of dependent loads, so
+// a wave keeps four such chains in flight instead of one.
+constexpr uint32_t kLeavesPerWave = 4;
+
 template <bool kMulti>
 __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, const EditorDev* eds)
 {
-    __shared__ uint32_t lds[4][16];
+    __shared__ uint32_t lds[4][kLeavesPerWave][16];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t seg = lane >> 4, s = lane & 15u;
     const LevelDesc me = ws.lvl[dag.leafLevel];
     const uint32_t wavesPerBlock = blockDim.x / 64u;
     const uint32_t count = min(me.count, ws.capLeaf);
-    for (uint32_t i = blockIdx.x * wavesPerBlock + wv; i < count; i += gridDim.x * wavesPerBlock) {
+    for (uint32_t i0 = (blockIdx.x * wavesPerBlock + wv) * kLeavesPerWave; i0 < count;
+         i0 += gridDim.x * wavesPerBlock * kLeavesPerWave) {
+        const uint32_t i = i0 + seg;
         const uint32_t w = me.start + i;
-        if (w >= ws.capWork)
-            break;
-        const EditorDev& ed = eds[kMulti ? ws.editOf[w] : 0u];
-        const uint4 item = ws.work[w];
-        bool hasOld = false;
-        uint32_t oldMat = 0;
-        if (item.w != kSentinel)
-            hasOld = leaf_material(dag_leaf_ptr(dag, item.w), lane, dag.M, oldMat);
-        const uint32_t vx = (item.x << 2) | (((lane >> 5) & 1u) << 1) | ((lane >> 2) & 1u);
-        const uint32_t vy = (item.y << 2) | (((lane >> 4) & 1u) << 1) | ((lane >> 1) & 1u);
-        const uint32_t vz = (item.z << 2) | (((lane >> 3) & 1u) << 1) | (lane & 1u);
-        uint32_t fm;
-        const Op op = decide(ed, vx, vy, vz, 0, hasOld ? NS_FILLED : NS_EMPTY, fm);
-        bool has = false;
-        uint32_t mat = 0;
-        if (ed.kind == EK_DENSE) {
-            has = op == OP_FILL; // a set voxel of the grid, with its own material
-            mat = fm;
-        } else if (op == OP_FILL || op == OP_SUBDIVIDE) {
-            if (ed.kind == EK_COPY)
-                has = copy_value(ed, dag, vx, vy, vz, ws.state[w], hasOld, oldMat, mat);
-            else
-                has = new_value(ed, hasOld, oldMat, mat);
-        } else if (op == OP_KEEP) {
-            has = hasOld;
-            mat = oldMat;
+        const bool valid = i < count && w < ws.capWork;
+        uint32_t has4 = 0, mats[4] = { 0u, 0u, 0u, 0u };
+        uint32_t oldH = kSentinel;
+        const uint32_t* oldLeaf = nullptr;
+        if (valid) {
+            const EditorDev& ed = eds[kMulti ? ws.editOf[w] : 0u];
+            const uint4 item = ws.work[w];
+            oldH = item.w;
+            uint64_t oldBits = 0;
+            if (oldH != kSentinel) {
+                oldLeaf = dag_leaf_ptr(dag, oldH);
+                oldBits = (uint64_t)oldLeaf[0] | ((uint64_t)oldLeaf[1] << 32);
+            }
+#pragma unroll
+            for (uint32_t v = 0; v < 4; ++v) {
+                const uint32_t b = v * 16u + s;
+                const bool hasOld = (oldBits >> b) & 1ull;
+                uint32_t oldMat = 0;
+                if (hasOld && dag.M) {
+                    const uint32_t off = (uint32_t)__builtin_popcountll(oldBits & ((1ull << b) - 1ull)) * dag.M;
+                    oldMat = (oldLeaf[2 + (off >> 5)] >> (off & 31u)) & ((1u << dag.M) - 1u);
+                }
+                const uint32_t vx = (item.x << 2) | (((b >> 5) & 1u) << 1) | ((b >> 2) & 1u);
+                const uint32_t vy = (item.y << 2) | (((b >> 4) & 1u) << 1) | ((b >> 1) & 1u);
+                const uint32_t vz = (item.z << 2) | (((b >> 3) & 1u) << 1) | (b & 1u);
+                uint32_t fm;
+                const Op op = decide(ed, vx, vy, vz, 0, hasOld ? NS_FILLED : NS_EMPTY, fm);
+                bool has = false;
+                uint32_t mat = 0;
+                if (ed.kind == EK_DENSE) {
+                    has = op == OP_FILL; // a set voxel of the grid, with its own material
+                    mat = fm;
+                } else if (op == OP_FILL || op == OP_SUBDIVIDE) {
+                    if (ed.kind == EK_COPY)
+                        has = copy_value(ed, dag, vx, vy, vz, ws.state[w], hasOld, oldMat, mat);
+                    else
+                        has = new_value(ed, hasOld, oldMat, mat);
+                } else if (op == OP_KEEP) {
+                    has = hasOld;
+                    mat = oldMat;
+                }
+                has4 |= (has ? 1u : 0u) << v;
+                mats[v] = mat;
+            }
         }
-        const unsigned long long mask = __ballot(has);
-        if (lane < 16)
-            lds[wv][lane] = 0u;
+        // The leaf's 64-bit mask: voxel v * 16 + s is bit s of this leaf's
+        // sixteen lanes in the ballot of voxel slot v.
+        uint64_t mask = 0;
+#pragma unroll
+        for (uint32_t v = 0; v < 4; ++v) {
+            const uint64_t bal = __ballot((has4 >> v) & 1u);
+            mask |= ((bal >> (16u * seg)) & 0xFFFFull) << (16u * v);
+        }
+        lds[wv][seg][s] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (has && dag.M) {
-            const uint32_t off = (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull)) * dag.M;
-            atomicOr(&lds[wv][off >> 5], mat << (off & 31u));
+        if (dag.M) {
+#pragma unroll
+            for (uint32_t v = 0; v < 4; ++v) {
+                if ((has4 >> v) & 1u) {
+                    const uint32_t b = v * 16u + s;
+                    const uint32_t off = (uint32_t)__builtin_popcountll(mask & ((1ull << b) - 1ull)) * dag.M;
+                    atomicOr(&lds[wv][seg][off >> 5], mats[v] << (off & 31u));
+                }
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        uint32_t* rec = ws.leaves + (size_t)i * ws.leafStride;
-        uint32_t v = 0;
-        if (lane < ws.leafStride) {
-            if (lane == 0)
-                v = (uint32_t)mask;
-            else if (lane == 1)
-                v = (uint32_t)(mask >> 32);
-            else
-                v = (lane - 2 < 16) ? lds[wv][lane - 2] : 0u;
-            rec[lane] = v;
+        // Words s and s + 16 of the record (a record has at most 18 words).
+        uint32_t word[2] = { 0u, 0u };
+#pragma unroll
+        for (uint32_t k = 0; k < 2; ++k) {
+            const uint32_t q = s + 16u * k;
+            word[k] = q == 0u ? (uint32_t)mask : q == 1u ? (uint32_t)(mask >> 32) : (q - 2u < 16u ? lds[wv][seg][q - 2u] : 0u);
+            if (valid && q < ws.leafStride)
+                ws.leaves[(size_t)i * ws.leafStride + q] = word[k];
         }
         // A leaf the edit left as it was resolves to its old handle: no dedup
         // and no DAG find (created = 2 marks it for the merge).
-        bool unchanged = false;
-        if (item.w != kSentinel && mask != 0ull) {
+        bool cmp = false, differ = false;
+        if (valid && oldH != kSentinel && mask != 0ull) {
             const uint32_t newSize = leaf_size_from_words((uint32_t)mask, (uint32_t)(mask >> 32), dag.M);
-            if (leaf_handle_size(dag, item.w) == newSize) {
-                const uint32_t* old = dag_leaf_ptr(dag, item.w);
-                const bool eq = lane >= newSize || old[lane] == v;
-                unchanged = __ballot(!eq) == 0ull;
+            if (leaf_handle_size(dag, oldH) == newSize) {
+                cmp = true;
+#pragma unroll
+                for (uint32_t k = 0; k < 2; ++k) {
+                    const uint32_t q = s + 16u * k;
+                    if (q < newSize && oldLeaf[q] != word[k])
+                        differ = true;
+                }
             }
         }
-        if (lane == 0) {
+        const uint64_t diff = __ballot(differ);
+        const bool unchanged = cmp && ((diff >> (16u * seg)) & 0xFFFFull) == 0ull;
+        if (valid && s == 0u) {
             ws.created[w] = unchanged ? 2u : 0u;
             if (unchanged) {
                 ws.rep[w] = w;
-                ws.handle[w] = item.w;
+                ws.handle[w] = oldH;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -845,7 +887,7 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
     }
     {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, dag.leafLevel), ws.capLeaf);
-        svo_leaf_level<false><<<grid_for(items, 4, kCap), 256, 0, st>>>(ws, dag, edDev);
+        svo_leaf_level<false><<<grid_for(items, 4 * kLeavesPerWave, kCap), 256, 0, st>>>(ws, dag, edDev);
     }
     for (int L = (int)dag.leafLevel; L >= 0; --L)
         launch_merge_level<false>(ctx, ws, (uint32_t)L, std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork));
@@ -862,7 +904,8 @@ int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, u
     for (uint32_t L = S; L < dag.leafLevel; ++L)
         svo_inner_level<true><<<grid_for(std::min<uint64_t>(levelItems[L], ws.capWork), 32, kCap), 256, 0, st>>>(
             ws, dag, eds, L);
-    svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf), 4, kCap), 256, 0, st>>>(
+    svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf), 4 * kLeavesPerWave, kCap),
+                           256, 0, st>>>(
         ws, dag, eds);
     for (int L = (int)dag.leafLevel; L >= (int)S; --L)
         launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork), kCap);
