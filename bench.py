@@ -328,6 +328,41 @@ def editing_bench(args):
             "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"]}
 
 
+def editing_profile(args):
+    """The batched-editing stream alone (depth 15 terrain, 256 edits in
+    16-edit fused batches, one collection after the stream), for a kernel
+    trace of the edit waves and the collection: rocprofv3 --kernel-trace
+    --stats -- python3 bench.py --editing-profile."""
+    import svdag_py
+    import workloads
+
+    D = args.edit_levels
+    tp = workloads.terrain_params(D)
+    g = svdag_py.MyGPUHashDAG(D, 4, expected_items=int(64e6 * 4 ** (D - 15)) if D >= 12 else 0,
+                              workspace_items=min(1 << 28, int(2 * 4 ** max(D - 10, 0) * 400000)))
+    g.build_terrain(**tp)
+    g.garbage_collect()
+
+    def h(x, z):
+        return int(g.terrain_heights(x, z, 1, 1)[0, 0])
+
+    edits = workloads.config3_edits(h, D, n=256, seed=6)
+    g.set_batch_mode(1)
+    tune_g, _ = g.autotune(edits[:16])
+    g.garbage_collect()
+    rows = []
+    for b in range(0, len(edits), 16):
+        st, ms = g.edit_batch(edits[b:b + 16])
+        items = [sum(x.svo_items[l] for x in st) for l in range(D - 1)]
+        log(f"[profile] batch {b // 16}: {ms:.3f} ms, SVO items per level {items}")
+        rows.append(ms)
+    gc = g.garbage_collect()
+    log(f"[profile] collection: {gc.gpu_ms:.3f} ms, {gc.items_live} live, {gc.items_freed} freed")
+    g.close()
+    print(json.dumps({"edit_batches_ms": [round(x, 3) for x in rows], "edit_gpu_ms_total": round(sum(rows), 3),
+                      "gc_ms": round(gc.gpu_ms, 3), "lanes_per_item": tune_g}), flush=True)
+
+
 def cpu_edit_baseline(args):
     """The oracle's per-edit path (a single-threaded restatement of the same
     edit pipeline) on a bounded sample: the batched-editing stream generated
@@ -407,11 +442,16 @@ def main():
     ap.add_argument("--no-editing", action="store_true", help="slab-hash line only")
     ap.add_argument("--no-probe", action="store_true", help="skip the stream / random-line memory probes")
     ap.add_argument("--no-shading", action="store_true", help="skip the shadow / occlusion timing")
+    ap.add_argument("--editing-profile", action="store_true",
+                    help="only the batched-editing stream and its collection (for rocprofv3 --kernel-trace)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "
               "(see the sharding notes in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.editing_profile:
+        editing_profile(args)
+        return
     out = slab_hash_bench(args)
     if out is None:
         return

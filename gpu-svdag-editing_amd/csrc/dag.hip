@@ -1229,7 +1229,7 @@ int svdag_destroy(svdag_ctx* ctx)
         (void)svdag_server_stop(ctx, nullptr);
     (void)hipStreamSynchronize(ctx->stream);
     release_wave_graphs(ctx);
-    for (uint32_t* p : ctx->gcMasks)
+    for (uint8_t* p : ctx->gcMasks)
         if (p)
             (void)hipFree(p);
     if (ctx->dGcMasks)

@@ -122,11 +122,11 @@ struct svdag_ctx {
     uint32_t* gcFront[2] = { nullptr, nullptr };
     uint64_t gcCap = 0;
     uint32_t* gcCounters = nullptr;
-    // Per-slot level masks of every table (kept between collections and
+    // Per-slot visit bytes of every table (kept between collections and
     // regrown with the arenas) and their device pointer array.
-    std::vector<uint32_t*> gcMasks;
+    std::vector<uint8_t*> gcMasks;
     std::vector<uint32_t> gcMaskCap;
-    uint32_t** dGcMasks = nullptr;
+    uint8_t** dGcMasks = nullptr;
     // Per-edit device slots of a batch (editors, statuses, level tables) and
     // their pinned host mirror.
     svdag::EditorDev* dEditors = nullptr;

@@ -464,7 +464,10 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
         if (t < kParEdits && t < K)
             sc.full[t][l] = sFullCnt[t];
         __syncthreads();
-        // DAG find / insert of the representatives, eight lanes each.
+        // DAG finds of the representatives, eight lanes each, then the
+        // inserts of those not found.  Finds and inserts never run at once:
+        // an insert claims its slot's tag before writing the item, so a
+        // concurrent find could otherwise match a half-written slot.
         const uint32_t nReps = sNumReps;
         for (uint32_t r = t >> 3; r < nReps; r += kParThreads / 8u) {
             const uint32_t ri = sRepList[r];
@@ -473,13 +476,27 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
             Needle<8> nd;
             needle_load(nd, rec, s, gl);
             const uint32_t hsh = needle_hash(nd, s, gl);
-            uint32_t res = kNotFound, hint = kNoSlab, created = 0u;
+            uint32_t res = kNotFound, hint = kNoSlab;
             if (sNew[ri] == 0u)
                 res = table_find_group_hint<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, hint);
-            if (res == kNotFound) {
-                res = table_insert_group<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, hint);
-                created = 1u;
+            if (gl == 0) {
+                sResH[ri] = res == kNotFound ? hint : node_handle(res, s);
+                sResC[ri] = res == kNotFound ? 2u : 0u; // 2: to insert, from the line in sResH
             }
+        }
+        __syncthreads();
+        for (uint32_t r = t >> 3; r < nReps; r += kParThreads / 8u) {
+            const uint32_t ri = sRepList[r];
+            if (sResC[ri] != 2u)
+                continue; // uniform over the eight lanes of the item
+            const uint32_t* rec = sRec[ri];
+            const uint32_t s = node_size_from_header(rec[0]);
+            Needle<8> nd;
+            needle_load(nd, rec, s, gl);
+            const uint32_t hsh = needle_hash(nd, s, gl);
+            const uint32_t res = table_insert_group<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, sResH[ri]);
+            // Every lane has read sResH above; the group's lanes are in
+            // lockstep within the wave, so lane 0 may overwrite it.
             if (gl == 0) {
                 if (res == kNotFound) {
                     sOom = 1u;
@@ -487,7 +504,7 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
                 } else {
                     sResH[ri] = node_handle(res, s);
                 }
-                sResC[ri] = created;
+                sResC[ri] = 1u;
             }
         }
         __syncthreads();

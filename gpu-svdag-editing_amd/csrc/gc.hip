@@ -5,14 +5,19 @@
 // (gpu_hash_table_base.cu:230-374).  Roots are the current root plus the
 // level-0 fully-filled node of every material (377-380).
 //
-// Mark: one frontier per level holding each (item, level) pair once.  Every
-// slot has a 32-bit mask of the levels it was reached at; a parent claims a
-// child for level L + 1 with the atomicOr that sets bit L + 1 (after a
-// cached load shows it unset), and only the claimer appends it -- this
-// replaces the reference's per-level radix sort + unique.  Deduplication must be per
-// level: one item can occur at several levels, and the same child handle
-// decodes to a node at one depth and to a leaf (another table) next to the
-// leaf level.
+// Mark: one frontier per level, with no atomics on the items.  Every slot
+// has a visit byte holding 1 + the last level the mark reached it at; a
+// parent visiting a child at level L + 1 skips it if the byte already says
+// L + 1, else stores L + 1 and appends it -- this replaces the reference's
+// per-level radix sort + unique.  Levels run in increasing order, so the
+// byte only ever grows and a byte of L + 1 can only come from the running
+// level.  Two parents that visit one child at the same moment may both
+// append it; the child is then expanded twice, which marks nothing new, so
+// the frontier holds each (item, level) pair at least once and rarely more
+// (a plain byte store also leaves the neighbouring slots' bytes alone).
+// Visits must be per level: one item can occur at several levels, and the
+// same child handle decodes to a node at one depth and to a leaf (another
+// table) next to the leaf level.  An item is live iff its byte is nonzero.
 // Sweep (always on; the reference sweeps only under CAPTURE_MEMORY_STATS_SLOW),
 // in two passes: a flat pass over every line clears the tags of unmarked
 // items and marks the buckets that have overflow lines; a pass over those
@@ -30,55 +35,51 @@ struct GcCounters {
     unsigned long long itemsFreed, slabsFreed, itemsLive;
 };
 
-__global__ void gc_clear_masks(TableDev t, uint32_t* levelMask)
+// Clears the visit bytes of the lines handed out so far (16 bytes a line).
+__global__ void gc_clear_visits(TableDev t, uint8_t* visit)
 {
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
     // A pop past the free stack's end leaves its count below zero; the
     // sweep's pushes start from an empty stack.
     if (blockIdx.x == 0 && threadIdx.x == 0 && t.alloc->freeCount < 0)
         t.alloc->freeCount = 0;
-    const uint64_t n = (uint64_t)fresh * kMaxSlots;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        levelMask[i] = 0u;
-    }
+    uint4* v = reinterpret_cast<uint4*>(visit);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fresh; i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// Claim (item, level): true for exactly one caller, the first to reach the
-// item at that level.  A load served by the caches filters the common case
-// of an item already claimed (DAG children are shared by many parents, and
-// the hot ones stay cached), so only first visits pay for the atomic; mask
-// bits are only ever set during the mark, so a stale load merely sends the
-// caller to the atomic.  An item is live iff its mask is nonzero.
-__device__ inline bool gc_claim(const DagDev& dag, uint32_t* const* levelMask, uint32_t h, uint32_t L, bool leaf)
+// Visit (item, level): true if the caller is to expand the item at L (the
+// first visitor, or one of several simultaneous first visitors).  A stale
+// cached byte can only read as an earlier level, which also means "expand".
+__device__ inline bool gc_visit(const DagDev& dag, uint8_t* const* visit, uint32_t h, uint32_t L, bool leaf)
 {
     const uint32_t s = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
-    const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h);
-    const uint32_t slab = ptr >> kSlotBits, slot = ptr & ((1u << kSlotBits) - 1u);
-    uint32_t* m = &levelMask[s - 2][(size_t)slab * kMaxSlots + slot];
-    const uint32_t bit = 1u << L;
-    if (ld_filter(m) & bit)
+    const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h); // slab * 16 + slot
+    uint8_t* b = &visit[s - 2][ptr];
+    const uint8_t lv = (uint8_t)(L + 1u);
+    if (*b == lv)
         return false;
-    return (atomicOr(m, bit) & bit) == 0u;
+    *b = lv;
+    return true;
 }
 
-__global__ void gc_seed(DagDev dag, uint32_t* const* levelMask, const uint32_t* root, uint32_t* front,
-                        GcCounters* ctr)
+__global__ void gc_seed(DagDev dag, uint8_t* const* visit, const uint32_t* root, uint32_t* front, GcCounters* ctr)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     uint32_t n = 0;
     const bool leaf = dag.leafLevel == 0;
-    if (*root != kSentinel && gc_claim(dag, levelMask, *root, 0, leaf))
+    if (*root != kSentinel && gc_visit(dag, visit, *root, 0, leaf))
         front[n++] = *root;
     for (uint32_t m = 0; m < dag.numMaterials; ++m)
-        if (gc_claim(dag, levelMask, dag.fullyFilled[m], 0, leaf))
+        if (gc_visit(dag, visit, dag.fullyFilled[m], 0, leaf))
             front[n++] = dag.fullyFilled[m];
     ctr->count[0] = n;
 }
 
-// Expand the claimed nodes of level L: claim each child at L + 1 and append
-// the claimed interior children to the next frontier (leaves have nothing to
-// expand).  Appends are staged in LDS and flushed with one atomic per
+// Expand the frontier nodes of level L: visit each child at L + 1 and append
+// the interior children to be expanded to the next frontier (leaves have
+// nothing to expand).  Appends are staged in LDS and flushed with one atomic per
 // kGcStage entries, so the frontier counter sees a few thousand atomics per
 // collection instead of one per wave.
 constexpr uint32_t kGcStage = 4096;
@@ -107,7 +108,7 @@ __device__ inline void gc_flush_stage(uint32_t* sStage, uint32_t& sFill, uint32_
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint32_t* const* levelMask, uint32_t L,
+__global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint8_t* const* visit, uint32_t L,
                                                      const uint32_t* in, uint32_t* out, uint32_t outCap,
                                                      GcCounters* ctr)
 {
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint32_t* const
             const uint32_t cnt = __builtin_popcount(node[0] & 0xFFu);
             for (uint32_t j = 0; j < cnt; ++j) {
                 const uint32_t ch = node[1 + j];
-                if (gc_claim(dag, levelMask, ch, L + 1, childLeaf) && !childLeaf)
+                if (gc_visit(dag, visit, ch, L + 1, childLeaf) && !childLeaf)
                     kept[k++] = ch;
             }
         }
@@ -165,11 +166,11 @@ __device__ inline void gc_block_add(uint32_t v, unsigned long long* dst)
 }
 
 // Sweep, pass 1: every line handed out so far, one lane per line (a flat,
-// streaming pass).  Tags of items whose level mask stayed zero are cleared,
+// streaming pass).  Tags of items whose visit byte stayed zero are cleared,
 // and hasChain[b] records whether bucket b has overflow lines (if the byte
 // map is given).  Lines on the free stack have no tags, so they pass
 // through unchanged.  blockDim.x = 256.
-__global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint32_t* levelMask, uint8_t* hasChain,
+__global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint8_t* visit, uint8_t* hasChain,
                                                       GcCounters* ctr)
 {
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
@@ -178,7 +179,8 @@ __global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint32_t
         uint32_t* line = t.lines + (size_t)slab * kLineWords;
         const uint4 v = *reinterpret_cast<const uint4*>(line);
         const uint32_t tw[4] = { v.x, v.y, v.z, v.w };
-        const uint32_t* masks = levelMask + (size_t)slab * kMaxSlots;
+        const uint4 vb = reinterpret_cast<const uint4*>(visit)[slab];
+        const uint32_t vw[4] = { vb.x, vb.y, vb.z, vb.w };
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             if (j >= t.tagWords)
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint32_t
                 const uint32_t tag = (tw[j] >> (8u * b)) & 0xFFu;
                 if (!tag || slot >= t.slots)
                     continue;
-                if (masks[slot] != 0u) {
+                if (((vw[j] >> (8u * b)) & 0xFFu) != 0u) { // slot 4j + b
                     keep |= tag << (8u * b);
                     ++live;
                 } else {
@@ -246,8 +248,8 @@ __global__ void __launch_bounds__(256) gc_sweep_chains(TableDev t, const uint8_t
     gc_block_add(freedSlabs, &ctr->slabsFreed);
 }
 
-// Level masks for every table, kept between collections; regrown when an
-// arena has grown.
+// Visit bytes for every table (16 per line), kept between collections;
+// regrown when an arena has grown.
 static int gc_masks(svdag_ctx* ctx)
 {
     const size_t nt = ctx->ts.tables.size();
@@ -262,14 +264,14 @@ static int gc_masks(svdag_ctx* ctx)
             (void)hipFree(ctx->gcMasks[i]);
         ctx->gcMasks[i] = nullptr;
         ctx->gcMaskCap[i] = 0;
-        SVDAG_HIP(hipMalloc(&ctx->gcMasks[i], (size_t)cap * kMaxSlots * 4));
+        SVDAG_HIP(hipMalloc(&ctx->gcMasks[i], (size_t)cap * kMaxSlots));
         ctx->gcMaskCap[i] = cap;
         changed = true;
     }
     if (!ctx->dGcMasks)
-        SVDAG_HIP(hipMalloc(&ctx->dGcMasks, std::max<size_t>(nt, 1) * 32 * sizeof(uint32_t*)));
+        SVDAG_HIP(hipMalloc(&ctx->dGcMasks, std::max<size_t>(nt, 1) * 32 * sizeof(uint8_t*)));
     if (changed)
-        SVDAG_HIP(hipMemcpyAsync(ctx->dGcMasks, ctx->gcMasks.data(), nt * sizeof(uint32_t*), hipMemcpyHostToDevice,
+        SVDAG_HIP(hipMemcpyAsync(ctx->dGcMasks, ctx->gcMasks.data(), nt * sizeof(uint8_t*), hipMemcpyHostToDevice,
                                  ctx->stream));
     return SVDAG_OK;
 }
@@ -300,8 +302,8 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
         SVDAG_HIP(hipEventRecord(ctx->ev0, st));
         SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
         for (size_t i = 0; i < nt; ++i)
-            gc_clear_masks<<<grid_for((uint64_t)ctx->ts.tables[i].slabCap * kMaxSlots, 256, 8192), 256, 0, st>>>(
-                ctx->ts.tables[i].dev, ctx->gcMasks[i]);
+            gc_clear_visits<<<grid_for(ctx->ts.tables[i].slabCap, 256, 8192), 256, 0, st>>>(ctx->ts.tables[i].dev,
+                                                                                       ctx->gcMasks[i]);
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dGcMasks, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L < dag.leafLevel; ++L)
             gc_mark_level<<<4096, 256, 0, st>>>(dag, ctx->dGcMasks, L, ctx->gcFront[L & 1],
