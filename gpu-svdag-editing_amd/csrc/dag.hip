@@ -244,11 +244,16 @@ __global__ void __launch_bounds__(256) svo_inner_level(WsDev ws, DagDev dag, con
             atomicAdd(&ws.edCount[histEdit[threadIdx.x] * kMaxLevels + L + 1], v);
     }
 }
-// Four leaves per wave64: sixteen lanes per leaf, each deciding voxels
-// s, s + 16, s + 32 and s + 48 of it (create_edit_svo.cu:320-426).  A leaf's
-// inputs (work item, editor, old leaf) are a chain of dependent loads, so
-// a wave keeps four such chains in flight instead of one.
-constexpr uint32_t kLeavesPerWave = 4;
+// kLeavesPerWave leaves per wave64: 64 / kLeavesPerWave lanes per leaf, lane
+// s deciding voxels s, s + kLeafLanes, ... of it (create_edit_svo.cu:320-426).
+// A leaf's inputs (work item, editor, old leaf) are a chain of dependent
+// loads, so a wave keeps several such chains in flight instead of one.
+#ifndef SVDAG_LEAVES_PER_WAVE
+#define SVDAG_LEAVES_PER_WAVE 4
+#endif
+constexpr uint32_t kLeavesPerWave = SVDAG_LEAVES_PER_WAVE;
+constexpr uint32_t kLeafLanes = 64u / kLeavesPerWave; // lanes per leaf
+constexpr uint32_t kLeafWordsPerLane = (18u + kLeafLanes - 1u) / kLeafLanes; // a record has at most 18 words
 
 template <bool kMulti>
 __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, const EditorDev* eds)
@@ -256,7 +261,7 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
     __shared__ uint32_t lds[4][kLeavesPerWave][16];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
-    const uint32_t seg = lane >> 4, s = lane & 15u;
+    const uint32_t seg = lane / kLeafLanes, s = lane % kLeafLanes;
     const LevelDesc me = ws.lvl[dag.leafLevel];
     const uint32_t wavesPerBlock = blockDim.x / 64u;
     const uint32_t count = min(me.count, ws.capLeaf);
@@ -265,7 +270,10 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
         const uint32_t i = i0 + seg;
         const uint32_t w = me.start + i;
         const bool valid = i < count && w < ws.capWork;
-        uint32_t has4 = 0, mats[4] = { 0u, 0u, 0u, 0u };
+        uint32_t has4 = 0, mats[kLeavesPerWave];
+#pragma unroll
+        for (uint32_t v = 0; v < kLeavesPerWave; ++v)
+            mats[v] = 0u;
         uint32_t oldH = kSentinel;
         const uint32_t* oldLeaf = nullptr;
         if (valid) {
@@ -278,8 +286,8 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
                 oldBits = (uint64_t)oldLeaf[0] | ((uint64_t)oldLeaf[1] << 32);
             }
 #pragma unroll
-            for (uint32_t v = 0; v < 4; ++v) {
-                const uint32_t b = v * 16u + s;
+            for (uint32_t v = 0; v < kLeavesPerWave; ++v) {
+                const uint32_t b = v * kLeafLanes + s;
                 const bool hasOld = (oldBits >> b) & 1ull;
                 uint32_t oldMat = 0;
                 if (hasOld && dag.M) {
@@ -313,18 +321,19 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
         // sixteen lanes in the ballot of voxel slot v.
         uint64_t mask = 0;
 #pragma unroll
-        for (uint32_t v = 0; v < 4; ++v) {
+        for (uint32_t v = 0; v < kLeavesPerWave; ++v) {
             const uint64_t bal = __ballot((has4 >> v) & 1u);
-            mask |= ((bal >> (16u * seg)) & 0xFFFFull) << (16u * v);
+            mask |= ((bal >> (kLeafLanes * seg)) & ((1ull << kLeafLanes) - 1ull)) << (kLeafLanes * v);
         }
-        lds[wv][seg][s] = 0u;
+        for (uint32_t q = s; q < 16u; q += kLeafLanes)
+            lds[wv][seg][q] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (dag.M) {
 #pragma unroll
-            for (uint32_t v = 0; v < 4; ++v) {
+            for (uint32_t v = 0; v < kLeavesPerWave; ++v) {
                 if ((has4 >> v) & 1u) {
-                    const uint32_t b = v * 16u + s;
+                    const uint32_t b = v * kLeafLanes + s;
                     const uint32_t off = (uint32_t)__builtin_popcountll(mask & ((1ull << b) - 1ull)) * dag.M;
                     atomicOr(&lds[wv][seg][off >> 5], mats[v] << (off & 31u));
                 }
@@ -332,11 +341,11 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // Words s and s + 16 of the record (a record has at most 18 words).
-        uint32_t word[2] = { 0u, 0u };
+        // Words s, s + kLeafLanes, ... of the record.
+        uint32_t word[kLeafWordsPerLane];
 #pragma unroll
-        for (uint32_t k = 0; k < 2; ++k) {
-            const uint32_t q = s + 16u * k;
+        for (uint32_t k = 0; k < kLeafWordsPerLane; ++k) {
+            const uint32_t q = s + kLeafLanes * k;
             word[k] = q == 0u ? (uint32_t)mask : q == 1u ? (uint32_t)(mask >> 32) : (q - 2u < 16u ? lds[wv][seg][q - 2u] : 0u);
             if (valid && q < ws.leafStride)
                 ws.leaves[(size_t)i * ws.leafStride + q] = word[k];
@@ -349,15 +358,15 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
             if (leaf_handle_size(dag, oldH) == newSize) {
                 cmp = true;
 #pragma unroll
-                for (uint32_t k = 0; k < 2; ++k) {
-                    const uint32_t q = s + 16u * k;
+                for (uint32_t k = 0; k < kLeafWordsPerLane; ++k) {
+                    const uint32_t q = s + kLeafLanes * k;
                     if (q < newSize && oldLeaf[q] != word[k])
                         differ = true;
                 }
             }
         }
         const uint64_t diff = __ballot(differ);
-        const bool unchanged = cmp && ((diff >> (16u * seg)) & 0xFFFFull) == 0ull;
+        const bool unchanged = cmp && ((diff >> (kLeafLanes * seg)) & ((1ull << kLeafLanes) - 1ull)) == 0ull;
         if (valid && s == 0u) {
             ws.created[w] = unchanged ? 2u : 0u;
             if (unchanged) {

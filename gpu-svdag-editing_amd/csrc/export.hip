@@ -18,7 +18,7 @@ namespace svdag {
 
 struct HostDag {
     uint32_t levels = 0, leafLevel = 0, M = 0, minLeaf = 0, leafSizeBits = 0;
-    std::vector<std::vector<uint32_t>> lines; // per table: [fresh][32] (tags, items, next)
+    std::vector<std::vector<uint32_t>> lines; // per table: [fresh][32] (tags, next, items)
     std::vector<uint32_t> fresh, slots, tagWords;
     std::vector<uint32_t> fullyFilledLevel0;  // level-0 fully-filled node per material
 
@@ -32,7 +32,7 @@ struct HostDag {
         const uint32_t* line = lines[s - 2].data() + (size_t)slab * kLineWords;
         if (!((line[slot / 4] >> (8 * (slot % 4))) & 0xFFu))
             return nullptr;
-        return line + tagWords[s - 2] + slot * s;
+        return line + tagWords[s - 2] + 1u + slot * s;
     }
 };
 

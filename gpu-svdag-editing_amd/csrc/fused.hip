@@ -130,6 +130,7 @@ struct SpineItem {
     uint32_t sub;           // children this edit subdivides
     uint32_t ref[8];        // per subdivided child: its item one level down, or its seed work item at S
     uint32_t handle, created;
+    uint32_t och[8];        // its children in the wave's input DAG (kSentinel: none)
 };
 struct SpineScratch {
     SpineItem item[kParEdits][kMaxLevels][kSpineItems];
@@ -224,6 +225,8 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
                     atomicOr(&sBad, 1u);
             }
             it.sub = sub;
+            for (uint32_t c = 0; c < 8; ++c)
+                it.och[c] = origChild[c];
         }
         if (t < kParItems)
             sSubCnt[t] = active ? (uint32_t)__builtin_popcount(sub) : 0u;
@@ -370,12 +373,6 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
         bool empty = true;
         if (active) {
             const SpineItem& it = sc.item[k][l][i];
-            uint32_t mask = 0;
-            const uint32_t* node = nullptr;
-            if (it.orig != kSentinel) {
-                node = dag_node_ptr(dag, it.orig);
-                mask = node[0] & 0xFFu;
-            }
             const uint32_t mySub = sSub[idx];
             uint32_t ver[8];
             for (uint32_t c = 0; c < 8; ++c)
@@ -396,9 +393,7 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
             uint32_t outH[8], nOut = 0, newMask = 0;
             const uint32_t newChild = sChildNew[idx] != 0u ? 1u : 0u;
             for (uint32_t c = 0; c < 8; ++c) {
-                uint32_t h = ver[c];
-                if (!((from >> c) & 1u) && ((mask >> c) & 1u))
-                    h = node[1 + __builtin_popcount(mask & ((1u << c) - 1u))];
+                const uint32_t h = ((from >> c) & 1u) ? ver[c] : it.och[c];
                 if (h != kSentinel) {
                     outH[nOut++] = h;
                     newMask |= 1u << c;

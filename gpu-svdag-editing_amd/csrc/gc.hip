@@ -35,9 +35,20 @@ struct GcCounters {
     unsigned long long itemsFreed, slabsFreed, itemsLive;
 };
 
+// Every table's descriptor, visit bytes and bucket byte map, for the
+// per-table passes: one launch covers all tables, blockIdx.y picks one.
+constexpr uint32_t kGcMaxTables = 20; // item sizes 2 .. 18 words use 17
+struct GcTables {
+    TableDev t[kGcMaxTables];
+    uint8_t* visit[kGcMaxTables];
+    uint8_t* hasChain[kGcMaxTables]; // per-bucket "has overflow lines" bytes, or nullptr: walk every bucket
+};
+
 // Clears the visit bytes of the lines handed out so far (16 bytes a line).
-__global__ void gc_clear_visits(TableDev t, uint8_t* visit)
+__global__ void __launch_bounds__(256) gc_clear_visits(GcTables a)
 {
+    const TableDev& t = a.t[blockIdx.y];
+    uint8_t* visit = a.visit[blockIdx.y];
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
     // A pop past the free stack's end leaves its count below zero; the
     // sweep's pushes start from an empty stack.
@@ -51,11 +62,15 @@ __global__ void gc_clear_visits(TableDev t, uint8_t* visit)
 // Visit (item, level): true if the caller is to expand the item at L (the
 // first visitor, or one of several simultaneous first visitors).  A stale
 // cached byte can only read as an earlier level, which also means "expand".
-__device__ inline bool gc_visit(const DagDev& dag, uint8_t* const* visit, uint32_t h, uint32_t L, bool leaf)
+__device__ inline uint8_t* gc_visit_byte(const DagDev& dag, uint8_t* const* visit, uint32_t h, bool leaf)
 {
     const uint32_t s = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
     const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h); // slab * 16 + slot
-    uint8_t* b = &visit[s - 2][ptr];
+    return &visit[s - 2][ptr];
+}
+__device__ inline bool gc_visit(const DagDev& dag, uint8_t* const* visit, uint32_t h, uint32_t L, bool leaf)
+{
+    uint8_t* b = gc_visit_byte(dag, visit, h, leaf);
     const uint8_t lv = (uint8_t)(L + 1u);
     if (*b == lv)
         return false;
@@ -79,7 +94,8 @@ __global__ void gc_seed(DagDev dag, uint8_t* const* visit, const uint32_t* root,
 
 // Expand the frontier nodes of level L: visit each child at L + 1 and append
 // the interior children to be expanded to the next frontier (leaves have
-// nothing to expand).  Appends are staged in LDS and flushed with one atomic per
+// nothing to expand).  Visits use plain loads and stores, so a node's
+// child visits are issued together.  Appends are staged in LDS and flushed with one atomic per
 // kGcStage entries, so the frontier counter sees a few thousand atomics per
 // collection instead of one per wave.
 constexpr uint32_t kGcStage = 4096;
@@ -124,21 +140,38 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint8_t* const*
         if (sFill > kGcStage - 8u * blockDim.x)
             gc_flush_stage(sStage, sFill, sBase, out, outCap, &ctr->count[L + 1], &ctr->overflow);
         const uint32_t i = i0 + threadIdx.x;
-        uint32_t kept[8];
-        uint32_t k = 0;
+        uint32_t ch[8], keep = 0;
         if (i < n) {
             const uint32_t* node = dag_node_ptr(dag, in[i]);
             const uint32_t cnt = __builtin_popcount(node[0] & 0xFFu);
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t ch = node[1 + j];
-                if (gc_visit(dag, visit, ch, L + 1, childLeaf) && !childLeaf)
-                    kept[k++] = ch;
+            uint8_t* b[8];
+            uint8_t v[8];
+            // The node's children, then all their visit bytes, then the
+            // stores: three waits per node instead of two per child.
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                ch[j] = j < cnt ? node[1 + j] : 0u;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                b[j] = j < cnt ? gc_visit_byte(dag, visit, ch[j], childLeaf) : nullptr;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                v[j] = j < cnt ? *b[j] : 0u;
+            const uint8_t lv = (uint8_t)(L + 2u); // 1 + the children's level
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) {
+                if (j < cnt && v[j] != lv) {
+                    *b[j] = lv;
+                    keep |= childLeaf ? 0u : 1u << j;
+                }
             }
         }
-        if (k) {
-            const uint32_t pos = atomicAdd(&sFill, k);
-            for (uint32_t j = 0; j < k; ++j)
-                sStage[pos + j] = kept[j];
+        if (keep) {
+            uint32_t pos = atomicAdd(&sFill, (uint32_t)__builtin_popcount(keep));
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if ((keep >> j) & 1u)
+                    sStage[pos++] = ch[j];
         }
         __syncthreads();
     }
@@ -169,10 +202,15 @@ __device__ inline void gc_block_add(uint32_t v, unsigned long long* dst)
 // streaming pass).  Tags of items whose visit byte stayed zero are cleared,
 // and hasChain[b] records whether bucket b has overflow lines (if the byte
 // map is given).  Lines on the free stack have no tags, so they pass
-// through unchanged.  blockDim.x = 256.
-__global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint8_t* visit, uint8_t* hasChain,
-                                                      GcCounters* ctr)
+// through unchanged.  blockDim.x = 256; nothing runs after a frontier
+// overflow (the collection is retried with larger frontiers).
+__global__ void __launch_bounds__(256) gc_sweep_lines(GcTables a, GcCounters* ctr)
 {
+    if (ctr->overflow)
+        return;
+    const TableDev& t = a.t[blockIdx.y];
+    const uint8_t* visit = a.visit[blockIdx.y];
+    uint8_t* hasChain = a.hasChain[blockIdx.y];
     const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
     uint32_t freedItems = 0, live = 0;
     for (uint32_t slab = blockIdx.x * blockDim.x + threadIdx.x; slab < fresh; slab += gridDim.x * blockDim.x) {
@@ -203,7 +241,7 @@ __global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint8_t*
                 line[j] = keep;
         }
         if (hasChain && slab < t.numBuckets)
-            hasChain[slab] = line[kNextWord] != kNoSlab ? 1u : 0u;
+            hasChain[slab] = line[t.tagWords] != kNoSlab ? 1u : 0u;
     }
     gc_block_add(freedItems, &ctr->itemsFreed);
     gc_block_add(live, &ctr->itemsLive);
@@ -213,15 +251,19 @@ __global__ void __launch_bounds__(256) gc_sweep_lines(TableDev t, const uint8_t*
 // without the byte map) walks its chain and unlinks the overflow lines pass
 // 1 left without tags; they become empty lines (link kNoSlab) on the free
 // stack.  The loop is wave-uniform so the frees aggregate per wave.
-__global__ void __launch_bounds__(256) gc_sweep_chains(TableDev t, const uint8_t* hasChain, GcCounters* ctr)
+__global__ void __launch_bounds__(256) gc_sweep_chains(GcTables a, GcCounters* ctr)
 {
+    if (ctr->overflow)
+        return;
+    const TableDev& t = a.t[blockIdx.y];
+    const uint8_t* hasChain = a.hasChain[blockIdx.y];
     uint32_t freedSlabs = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < t.numBuckets; base += gridDim.x * blockDim.x) {
         const uint32_t b = base + threadIdx.x;
         uint32_t prev = kNoSlab, cur = kNoSlab;
         if (b < t.numBuckets && (!hasChain || hasChain[b])) {
             prev = b;
-            cur = t.lines[(size_t)b * kLineWords + kNextWord];
+            cur = t.lines[(size_t)b * kLineWords + t.tagWords];
         }
         while (__ballot(cur != kNoSlab) != 0ull) {
             bool give = false;
@@ -229,12 +271,12 @@ __global__ void __launch_bounds__(256) gc_sweep_chains(TableDev t, const uint8_t
             if (cur != kNoSlab) {
                 uint32_t* line = t.lines + (size_t)cur * kLineWords;
                 const uint4 v = *reinterpret_cast<const uint4*>(line);
-                const uint32_t next = line[kNextWord];
+                const uint32_t next = line[t.tagWords];
                 const uint32_t tags = v.x | (t.tagWords > 1 ? v.y : 0u) | (t.tagWords > 2 ? v.z : 0u)
                                       | (t.tagWords > 3 ? v.w : 0u);
                 if (tags == 0u) {
-                    t.lines[(size_t)prev * kLineWords + kNextWord] = next;
-                    line[kNextWord] = kNoSlab;
+                    t.lines[(size_t)prev * kLineWords + t.tagWords] = next;
+                    line[t.tagWords] = kNoSlab;
                     give = true;
                     ++freedSlabs;
                 } else {
@@ -289,6 +331,15 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
     GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
     SVDAG_TRY(gc_masks(ctx));
     const size_t nt = ctx->ts.tables.size();
+    if (nt > kGcMaxTables) {
+        set_error("too many item tables for the collection");
+        return SVDAG_E_INVALID;
+    }
+    uint32_t maxCap = 1, maxBuckets = 1;
+    for (size_t i = 0; i < nt; ++i) {
+        maxCap = std::max(maxCap, ctx->ts.tables[i].slabCap);
+        maxBuckets = std::max(maxBuckets, ctx->ts.tables[i].numBuckets);
+    }
     for (int attempt = 0; attempt < 6; ++attempt) {
         if (cap > ctx->gcCap) {
             for (int k = 0; k < 2; ++k) {
@@ -299,34 +350,40 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
             }
             ctx->gcCap = cap;
         }
+        // The frontiers are free after the mark; the first one holds the
+        // sweep's per-bucket chain bytes, the tables' maps one after another.
+        GcTables a {};
+        uint64_t mapUsed = 0;
+        for (size_t i = 0; i < nt; ++i) {
+            const HostTable& t = ctx->ts.tables[i];
+            a.t[i] = t.dev;
+            a.visit[i] = ctx->gcMasks[i];
+            a.hasChain[i] = nullptr;
+            if (mapUsed + t.numBuckets <= ctx->gcCap * 4) {
+                a.hasChain[i] = reinterpret_cast<uint8_t*>(ctx->gcFront[0]) + mapUsed;
+                mapUsed += t.numBuckets;
+            }
+        }
         SVDAG_HIP(hipEventRecord(ctx->ev0, st));
         SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
-        for (size_t i = 0; i < nt; ++i)
-            gc_clear_visits<<<grid_for(ctx->ts.tables[i].slabCap, 256, 8192), 256, 0, st>>>(ctx->ts.tables[i].dev,
-                                                                                       ctx->gcMasks[i]);
+        gc_clear_visits<<<dim3(grid_for(maxCap, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a);
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dGcMasks, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L < dag.leafLevel; ++L)
             gc_mark_level<<<4096, 256, 0, st>>>(dag, ctx->dGcMasks, L, ctx->gcFront[L & 1],
                                                  ctx->gcFront[(L + 1) & 1],
                                                  (uint32_t)std::min<uint64_t>(ctx->gcCap, 0xFFFFFFFFull), ctr);
+        gc_sweep_lines<<<dim3(grid_for(maxCap, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
+        gc_sweep_chains<<<dim3(grid_for(maxBuckets, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
+        SVDAG_HIP(hipEventRecord(ctx->ev1, st));
         GcCounters h {};
         SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         SVDAG_HIP(hipStreamSynchronize(st));
         if (h.overflow) {
+            // The mark ran out of frontier space and the sweep did nothing:
+            // run again with larger frontiers.
             cap = std::min<uint64_t>(cap * 2, 1ull << 31);
             continue;
         }
-        // The frontiers are free after the mark; the first one holds the
-        // sweep's per-bucket chain bytes, one table at a time.
-        for (size_t i = 0; i < nt; ++i) {
-            const HostTable& t = ctx->ts.tables[i];
-            uint8_t* hasChain = t.numBuckets <= ctx->gcCap * 4 ? reinterpret_cast<uint8_t*>(ctx->gcFront[0]) : nullptr;
-            gc_sweep_lines<<<grid_for(t.slabCap, 256, 2048), 256, 0, st>>>(t.dev, ctx->gcMasks[i], hasChain, ctr);
-            gc_sweep_chains<<<grid_for(t.numBuckets, 256, 2048), 256, 0, st>>>(t.dev, hasChain, ctr);
-        }
-        SVDAG_HIP(hipEventRecord(ctx->ev1, st));
-        SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
-        SVDAG_HIP(hipStreamSynchronize(st));
         if (stats) {
             stats->items_freed = h.itemsFreed;
             stats->slabs_freed = h.slabsFreed;

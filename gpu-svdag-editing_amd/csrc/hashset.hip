@@ -23,7 +23,7 @@ __global__ void table_init_lines(TableDev t, uint32_t first, uint32_t last)
     uint32_t* base = t.lines + (size_t)first * kLineWords;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const uint32_t w = (uint32_t)(i % kLineWords);
-        if (w == kNextWord)
+        if (w == t.tagWords)
             base[i] = kNoSlab;
         else if (w < t.tagWords)
             base[i] = 0u;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) bulk_find_multi_kernel(const TableDev* ta
         LineHead lines[N];
 #pragma unroll
         for (int k = 0; k < N; ++k)
-            line_head_load(lines[k], tp[k]->lines + (size_t)slab[k] * kLineWords);
+            line_head_load(lines[k], tp[k]->lines + (size_t)slab[k] * kLineWords, tp[k]->tagWords);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             const uint32_t r = probe_chain<G, MW>(*tp[k], n[k], s[k], make_tag(h[k]) * 0x01010101u, slab[k], lines[k],
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) find_binned(const TableDev* tables, uint3
             LineHead lines[N];
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                line_head_load(lines[k], t.lines + (size_t)slab[k] * kLineWords);
+                line_head_load(lines[k], t.lines + (size_t)slab[k] * kLineWords, t.tagWords);
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 const uint32_t r = probe_chain<G, MW>(t, n[k], s, make_tag(h[k]) * 0x01010101u, slab[k], lines[k], gl,
@@ -426,7 +426,7 @@ __device__ inline uint32_t probe_lane(const TableDev& t, const uint32_t (&nw)[MW
         if (L.next >= kPendingSlab)
             return kNotFound;
         slab = L.next;
-        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+        line_head_load(L, t.lines + (size_t)slab * kLineWords, t.tagWords);
     }
 }
 
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(256) find_lanes_seg(const TableDev* tables, co
     }
 #pragma unroll
     for (int k = 0; k < NPL; ++k)
-        line_head_load(L[k], t.lines + (size_t)slab[k] * kLineWords);
+        line_head_load(L[k], t.lines + (size_t)slab[k] * kLineWords, t.tagWords);
 #pragma unroll
     for (int k = 0; k < NPL; ++k) {
         const uint32_t idx = first + (uint32_t)k * 256u;

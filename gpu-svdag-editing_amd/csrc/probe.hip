@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) probe_lines(const uint32_t* lines, uint32
         const uint32_t line = mix32((uint32_t)i * 0x9E3779B1u + 0x7F4A7C15u) % nLines;
         const uint32_t* l = lines + (size_t)line * kLineWords;
         const uint4 v = *reinterpret_cast<const uint4*>(l);
-        acc ^= v.x ^ v.y ^ v.z ^ v.w ^ l[kNextWord];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w ^ l[4]; // a probe's line head: tags and link, one 32-byte sector
     }
     for (int m = 32; m >= 1; m >>= 1)
         acc ^= (uint32_t)__shfl_xor((int)acc, m, 64);

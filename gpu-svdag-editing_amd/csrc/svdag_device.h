@@ -12,13 +12,14 @@
 //
 // Table layout (own design, not the reference's 32-slot slab): one table per
 // item size s; a slab is one 128-byte line holding everything a probe needs:
-//   words [0, tagWords)                 one-byte tags (0 = empty), one per slot
-//   words [tagWords, tagWords + slots*s) the items themselves
-//   word 31                             next slab id of the bucket's chain
-// where slots is the most items of s words that fit beside their tags (13 at
-// s = 2, 5 at s = 5, 3 at s = 9, 1 at s >= 16).  A probe fetches one line,
-// and the compare of a candidate reads the same line, which the fetch left
-// in cache.  Head slabs 0..numBuckets-1 are the buckets; overflow slabs come
+//   words [0, tagWords)                        one-byte tags (0 = empty), one per slot
+//   word  tagWords                             next slab id of the bucket's chain
+//   words [tagWords + 1, tagWords + 1 + slots*s) the items themselves
+// where slots is the most items of s words that fit beside their tags and
+// the link (13 at s = 2, 5 at s = 5, 3 at s = 9, 1 at s >= 16).  The tags and
+// the link sit in the line's first 32 bytes, so a probe that matches no tag
+// (every miss in a chainless bucket) reads one 32-byte sector, and the
+// compare of a candidate reads the same line.  Head slabs 0..numBuckets-1 are the buckets; overflow slabs come
 // from the slab allocator.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -27,7 +28,7 @@
 namespace svdag {
 
 constexpr uint32_t kLineWords = 32;     // words per line (128 B)
-constexpr uint32_t kNextWord = 31;      // line word that holds the next slab id
+constexpr uint32_t kItemBudget = 31;    // words for tags and items (the link takes one)
 constexpr uint32_t kSlotBits = 4;       // pointer = slab << 4 | slot
 constexpr uint32_t kMaxSlots = 1u << kSlotBits; // bound on slots per line (13 at most)
 constexpr uint32_t kNoSlab = 0xFFFFFFFFu;
@@ -40,7 +41,7 @@ constexpr uint32_t kMaxItemWords = 18;  // 2 + 64*8/32 (8 material bits)
 __host__ __device__ inline uint32_t line_slots(uint32_t s)
 {
     uint32_t k = 1;
-    while ((k + 1) * s + (k + 4) / 4 <= kNextWord)
+    while ((k + 1) * s + (k + 4) / 4 <= kItemBudget)
         ++k;
     return k;
 }
@@ -229,7 +230,7 @@ __device__ inline void wave_slab_free(const TableDev& t, bool give, uint32_t id)
 // try_ = false only take part in the wave's allocation and get kNoSlab.
 __device__ inline uint32_t chain_grow(const TableDev& t, uint32_t tail, bool try_)
 {
-    uint32_t* link = t.lines + (size_t)tail * kLineWords + kNextWord;
+    uint32_t* link = t.lines + (size_t)tail * kLineWords + t.tagWords;
     const bool mine = try_ && atomicCAS(link, kNoSlab, kPendingSlab) == kNoSlab;
     const uint32_t id = wave_slab_alloc(t, mine);
     if (!mine)
@@ -309,20 +310,21 @@ __device__ inline bool needle_equal_lane(const Needle<G, MW>& n, const uint32_t*
     return eq;
 }
 
-// The tag words of a line (at most 4) and its next slab id.  One 16-byte
-// load: the line is 128-byte aligned and tagWords <= 4.
+// The tag words of a line (at most 4) and its next slab id (word tagWords):
+// a 16-byte load and a word of the same 32-byte sector (the line is
+// 128-byte aligned and tagWords <= 4).
 struct LineHead {
     uint32_t tag[4];
     uint32_t next;
 };
-__device__ inline void line_head_load(LineHead& L, const uint32_t* line)
+__device__ inline void line_head_load(LineHead& L, const uint32_t* line, uint32_t tagWords)
 {
     const uint4 v = *reinterpret_cast<const uint4*>(line);
     L.tag[0] = v.x;
     L.tag[1] = v.y;
     L.tag[2] = v.z;
     L.tag[3] = v.w;
-    L.next = line[kNextWord];
+    L.next = line[tagWords];
 }
 // Bit 4j+b set where tag byte b of tag word j equals tag4's byte and the
 // slot exists.
@@ -347,7 +349,7 @@ __device__ inline uint32_t line_empty(const TableDev& t, const uint32_t* tagWord
 }
 __host__ __device__ inline const uint32_t* line_item(const TableDev& t, uint32_t slab, uint32_t slot)
 {
-    return t.lines + (size_t)slab * kLineWords + t.tagWords + slot * t.itemWords;
+    return t.lines + (size_t)slab * kLineWords + t.tagWords + 1u + slot * t.itemWords;
 }
 __host__ __device__ inline const uint32_t* table_item_ptr(const TableDev& t, uint32_t ptr)
 {
@@ -372,7 +374,7 @@ __device__ inline uint32_t probe_chain(const TableDev& t, const Needle<G, MW>& n
         if (L.next >= kPendingSlab)
             return kNotFound;
         slab = L.next;
-        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+        line_head_load(L, t.lines + (size_t)slab * kLineWords, t.tagWords);
     }
 }
 
@@ -384,7 +386,7 @@ __device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, M
     static_assert(G >= 4 && G <= 32, "group size");
     const uint32_t slab = h % t.numBuckets;
     LineHead L;
-    line_head_load(L, t.lines + (size_t)slab * kLineWords);
+    line_head_load(L, t.lines + (size_t)slab * kLineWords, t.tagWords);
     return probe_chain<G, MW>(t, n, s, make_tag(h) * 0x01010101u, slab, L, gl, gbase);
 }
 
@@ -403,7 +405,7 @@ __device__ inline uint32_t table_find_group_hint(const TableDev& t, const Needle
     uint32_t slab = h % t.numBuckets;
     hint = kNoSlab;
     LineHead L;
-    line_head_load(L, t.lines + (size_t)slab * kLineWords);
+    line_head_load(L, t.lines + (size_t)slab * kLineWords, t.tagWords);
     while (true) {
         uint32_t cand = line_match(t, L, tag4);
         while (cand) {
@@ -422,7 +424,7 @@ __device__ inline uint32_t table_find_group_hint(const TableDev& t, const Needle
             return kNotFound;
         }
         slab = L.next;
-        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+        line_head_load(L, t.lines + (size_t)slab * kLineWords, t.tagWords);
     }
 }
 
@@ -433,7 +435,7 @@ __device__ inline uint32_t table_find_lane(const TableDev& t, const uint32_t* ne
     uint32_t slab = h % t.numBuckets;
     while (true) {
         LineHead L;
-        line_head_load(L, t.lines + (size_t)slab * kLineWords);
+        line_head_load(L, t.lines + (size_t)slab * kLineWords, t.tagWords);
         uint32_t cand = line_match(t, L, tag4);
         while (cand) {
             const uint32_t slot = __ffs(cand) - 1u;
@@ -471,7 +473,7 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G,
             uint32_t tw[4];
             for (uint32_t j = 0; j < 4; ++j)
                 tw[j] = j < t.tagWords ? ld_relaxed(line + j) : 0u;
-            next = ld_relaxed(line + kNextWord);
+            next = ld_relaxed(line + t.tagWords);
             uint32_t emp;
             line_empty(t, tw, emp);
             if (emp) {
@@ -492,7 +494,7 @@ __device__ inline uint32_t table_insert_group(const TableDev& t, const Needle<G,
         slot = group_bcast<G>(slot, 0);
         next = group_bcast<G>(next, 0);
         if (ok) {
-            uint32_t* item = line + t.tagWords + slot * s;
+            uint32_t* item = line + t.tagWords + 1u + slot * s;
 #pragma unroll
             for (int k = 0; k < Needle<G, MW>::K; ++k) {
                 const uint32_t j = gl + (uint32_t)k * G;
@@ -537,12 +539,12 @@ __device__ inline uint32_t table_insert_lane(const TableDev& t, const uint32_t* 
                     old = tw[q];
             if (atomicCAS(line + j, old, old | (tag << (8u * b))) != old)
                 continue;
-            uint32_t* dst = line + t.tagWords + bit * s;
+            uint32_t* dst = line + t.tagWords + 1u + bit * s;
             for (uint32_t k = 0; k < s; ++k)
                 dst[k] = item[k];
             return (slab << kSlotBits) | bit;
         }
-        const uint32_t next = ld_relaxed(line + kNextWord);
+        const uint32_t next = ld_relaxed(line + t.tagWords);
         if (next == kPendingSlab)
             continue; // a successor is being linked: re-read the line
         if (next != kNoSlab) {
