@@ -64,8 +64,11 @@ __global__ void init_fully_filled(DagDev dag)
             h ^= hash_part(item[j], j);
         const TableDev& t = dag.tables[s - 2];
         uint32_t r = table_find_lane(t, item, s, h);
-        if (r == kNotFound)
+        if (r == kNotFound) {
             r = table_insert_lane(t, item, s, h);
+            t.roles[r] = 0u;
+            ++*dag.stored;
+        }
         dag.fullyFilled[dag.leafLevel * dag.numMaterials + m] = leaf_handle(dag, r, s);
     }
     for (int level = (int)dag.leafLevel - 1; level >= 0; --level) {
@@ -78,8 +81,18 @@ __global__ void init_fully_filled(DagDev dag)
                 h ^= hash_part(item[j], j);
             const TableDev& t = dag.tables[9 - 2];
             uint32_t r = table_find_lane(t, item, 9, h);
-            if (r == kNotFound)
+            if (r == kNotFound) {
                 r = table_insert_lane(t, item, 9, h);
+                const uint8_t role = node_role(dag, (uint32_t)level);
+                t.roles[r] = role;
+                for (uint32_t c = 0; c < 8; ++c)
+                    child_ref_add(dag, item[1 + c], role);
+                ++*dag.stored;
+            }
+            // The level-0 nodes are collection roots for good: one
+            // permanent reference keeps their chains stored.
+            if (level == 0)
+                atomicAdd(&t.refs[r], 1u);
             dag.fullyFilled[level * dag.numMaterials + m] = node_handle(r, 9);
         }
     }
@@ -545,23 +558,56 @@ __global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32
 {
     if (merge_skipped(ws, dag))
         return;
+    __shared__ uint32_t sInserted;
+    __shared__ RefArrays sRa;
+    if (threadIdx.x == 0)
+        sInserted = 0u;
+    ref_arrays_stage(sRa, dag);
+    __syncthreads();
     const LevelDesc me = ws.lvl[L];
     const bool isLeaf = L == dag.leafLevel;
+    const uint8_t role = isLeaf ? 0u : node_role(dag, L);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gl = lane & (G - 1);
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
     const uint32_t groupsPerBlock = blockDim.x / G;
     for (uint32_t i = blockIdx.x * groupsPerBlock + threadIdx.x / G; i < me.count; i += gridDim.x * groupsPerBlock) {
         const uint32_t w = me.start + i;
-        if (ws.rep[w] != w || ws.handle[w] != kNotFound)
+        if (ws.rep[w] != w)
             continue;
+        uint32_t hint = ws.hint[w];
+        const uint32_t found = ws.handle[w];
+        // A found node that lacks the role it is needed in counts as a miss
+        // (the reference-count rule in dag_internal.h).  A leaf the edit
+        // left unchanged (created 2) is the DAG's own leaf there: plain.
+        if (found != kNotFound) {
+            if (isLeaf) {
+                if (ws.created[w] == 2u
+                    || sRa.roles[leaf_handle_size(dag, found) - 2][leaf_handle_ptr(dag, found)] == 0u)
+                    continue;
+            } else if (sRa.roles[node_handle_size(found) - 2][node_handle_ptr(found)] & role) {
+                continue;
+            }
+        }
         const uint32_t* rec = isLeaf ? ws.leaves + (size_t)i * ws.leafStride : ws.nodes + (size_t)w * kNodeStride;
         const uint32_t w0 = rec[0], w1 = rec[1];
         const uint32_t s = isLeaf ? leaf_size_from_words(w0, w1, dag.M) : node_size_from_header(w0);
         Needle<G> n;
         needle_load(n, rec, s, gl);
         const uint32_t h = needle_hash(n, s, gl);
-        const uint32_t r = table_insert_group<G>(dag.tables[s - 2], n, s, h, gl, gbase, ws.hint[w]);
+        if (found != kNotFound) {
+            // An equal item stored with the right role may follow in the chain.
+            const uint32_t again = table_find_group_hint<G>(dag.tables[s - 2], n, s, h, gl, gbase, hint,
+                                                            isLeaf ? kRoleLeafUse : role);
+            if (again != kNotFound) {
+                if (gl == 0)
+                    ws.handle[w] = isLeaf ? leaf_handle(dag, again, s) : node_handle(again, s);
+                continue;
+            }
+        }
+        const uint32_t r = table_insert_group<G>(dag.tables[s - 2], n, s, h, gl, gbase, hint);
+        if (r != kNotFound)
+            refs_on_insert(dag, sRa, n, s, r, role, gl);
         if (gl == 0) {
             if (r == kNotFound) {
                 atomicOr(&ws.status->oom, 1u);
@@ -569,9 +615,13 @@ __global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32
             } else {
                 ws.handle[w] = isLeaf ? leaf_handle(dag, r, s) : node_handle(r, s);
                 ws.created[w] = 1;
+                atomicAdd(&sInserted, 1u);
             }
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && sInserted)
+        atomicAdd(dag.stored, (unsigned long long)sInserted);
 }
 
 // Publish the new root (my_gpu_hash_dag_edits.cu:626) unless the merge was
@@ -1032,6 +1082,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
                 break;
             }
         for (uint32_t k = first; k < bad; ++k) {
+            root_log(ctx);
             ctx->root = hSt[k].rootAfter;
             if (stats) {
                 svdag_edit_stats& o = stats[k];
@@ -1049,6 +1100,10 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
             break;
         }
         const EditStatus s = hSt[bad];
+        // The failed edit and those after it may have inserted items no root
+        // reaches; the next collection marks and recounts.
+        ctx->refsDirty = true;
+        root_log(ctx);
         ctx->root = s.rootBefore;
         SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
         if (s.fallback) {
@@ -1214,11 +1269,14 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
         const uint32_t buckets = (uint32_t)std::min<uint64_t>(want, slabLimit / 2);
         uint64_t cap = buckets + (cfg.arena_slabs ? cfg.arena_slabs : buckets / 2 + 4096);
         cap = std::min<uint64_t>(cap, slabLimit);
-        SVDAG_TRY(table_alloc(ctx->ts.tables[i], 2 + i, buckets, (uint32_t)cap, ctx->stream));
+        SVDAG_TRY(table_alloc(ctx->ts.tables[i], 2 + i, buckets, (uint32_t)cap, ctx->stream, true));
     }
     SVDAG_TRY(ctx_refresh_tables(ctx));
     SVDAG_HIP(hipMalloc(&ctx->dFullyFilled, (size_t)d.levels * d.numMaterials * 4));
     d.fullyFilled = ctx->dFullyFilled;
+    SVDAG_HIP(hipMalloc(&ctx->dStored, sizeof(unsigned long long)));
+    SVDAG_HIP(hipMemsetAsync(ctx->dStored, 0, sizeof(unsigned long long), ctx->stream));
+    d.stored = ctx->dStored;
     SVDAG_HIP(hipMalloc(&ctx->dRoot, 4));
     SVDAG_HIP(hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream));
     init_fully_filled<<<1, 64, 0, ctx->stream>>>(d);
@@ -1238,6 +1296,9 @@ int svdag_destroy(svdag_ctx* ctx)
         (void)svdag_server_stop(ctx, nullptr);
     (void)hipStreamSynchronize(ctx->stream);
     release_wave_graphs(ctx);
+    for (uint32_t* p : ctx->gcTouched)
+        if (p)
+            (void)hipFree(p);
     for (uint8_t* p : ctx->gcMasks)
         if (p)
             (void)hipFree(p);
@@ -1249,7 +1310,8 @@ int svdag_destroy(svdag_ctx* ctx)
                      (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits,
                      (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
-                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade, ctx->dSpine })
+                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade, ctx->dSpine,
+                     (void*)ctx->dStored, (void*)ctx->dGcCand, (void*)ctx->dGcTouchedList })
         if (p)
             (void)hipFree(p);
     for (void* p : { ctx->hBatch, ctx->hFused })
@@ -1343,6 +1405,7 @@ int svdag_build_voxels(svdag_ctx* ctx, const uint8_t* materials, uint32_t sideLo
             set_error(std::string("svdag_build_voxels: ") + hipGetErrorString(e));
             return SVDAG_E_DEVICE;
         }
+        root_log(ctx);
         ctx->root = root;
         ctx->undo.clear();
         ctx->redo.clear();
@@ -1361,6 +1424,7 @@ int svdag_build_voxels(svdag_ctx* ctx, const uint8_t* materials, uint32_t sideLo
     for (uint32_t k = 0; k <= sideLog2; ++k)
         ed.pyrOffset[k] = off[k];
     // The grid replaces whatever the DAG held: build from the empty root.
+    root_log(ctx);
     ctx->root = kSentinel;
     e = hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream);
     if (e != hipSuccess) {
@@ -1424,6 +1488,7 @@ int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)
     for (uint32_t k = 0; k < kMaxLevels; ++k)
         e.pyrOffset[k] = ctx->pyrOffset[k];
     // The terrain replaces whatever the DAG held: build from the empty root.
+    root_log(ctx);
     ctx->root = kSentinel;
     SVDAG_HIP(hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream));
     svdag_edit_stats stats;
@@ -1511,6 +1576,7 @@ int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, sv
     const uint32_t g0 = ctx->groupSize;
     std::vector<svdag_edit_stats> st(n);
     auto restore = [&]() -> int {
+        root_log(ctx);
         ctx->root = r0;
         SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
         SVDAG_HIP(hipStreamSynchronize(ctx->stream));
@@ -1560,6 +1626,7 @@ int svdag_set_batch_mode(svdag_ctx* ctx, uint32_t mode)
 
 static int set_root(svdag_ctx* ctx, uint32_t r)
 {
+    root_log(ctx);
     ctx->root = r;
     SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));

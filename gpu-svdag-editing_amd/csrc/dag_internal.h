@@ -76,6 +76,88 @@ struct FusedDev {
 // Bytes of device scratch the parallel spine needs (fused.hip).
 size_t spine_scratch_bytes();
 
+// ---------------------------------------------------------------- references
+// Every stored DAG item has a count of the references stored items hold to
+// it (one per child word that decodes to it) and a role byte: the decodings
+// under which its own child words were counted.  A node's children are nodes
+// unless it sits directly above the leaf level, where they are leaves, and
+// the same words can be a valid node in both places, or a leaf.  An item
+// is stored with the role of the insert that made it (none for a leaf), and
+// a find that returns an item of another role counts as a miss, so a new
+// copy with that role is inserted: every stored item is used in one role,
+// the one its children were counted under.  Roots hold no count:
+// collections release the roots no longer in use and free what drops to
+// zero (gc.hip).  A count that reaches kStickyRefs stays there: the most
+// shared items (the full leaf, common surface patterns) are referenced
+// millions of times, and atomics on one word serialise, so a cached load in
+// front of every update skips them.  Such an item is kept until a marking
+// collection.
+constexpr uint8_t kRoleParentOfNodes = 1, kRoleParentOfLeaves = 2;
+constexpr uint32_t kStickyRefs = 2048;
+__device__ inline uint8_t node_role(const DagDev& dag, uint32_t L)
+{
+    return L + 1u == dag.leafLevel ? kRoleParentOfLeaves : kRoleParentOfNodes;
+}
+// The tables' count and role arrays, staged where a kernel reads them often
+// (LDS): one dependent load less per count or role access.
+constexpr uint32_t kRefTables = 20; // item sizes 2 .. 18 words use 17
+struct RefArrays {
+    uint32_t* refs[kRefTables];
+    uint8_t* roles[kRefTables];
+};
+__device__ inline void ref_arrays_stage(RefArrays& r, const DagDev& dag)
+{
+    if (threadIdx.x < dag.numTables) {
+        r.refs[threadIdx.x] = dag.tables[threadIdx.x].refs;
+        r.roles[threadIdx.x] = dag.tables[threadIdx.x].roles;
+    }
+}
+__device__ inline void child_ref_add(const DagDev& dag, const RefArrays& ra, uint32_t h, uint8_t role)
+{
+    const bool leaf = role == kRoleParentOfLeaves;
+    const uint32_t s = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
+    const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h);
+    uint32_t* c = &ra.refs[s - 2][ptr];
+    if (ld_filter(c) < kStickyRefs)
+        atomicAdd(c, 1u);
+}
+__device__ inline void child_ref_add(const DagDev& dag, uint32_t h, uint8_t role)
+{
+    const bool leaf = role == kRoleParentOfLeaves;
+    const uint32_t s = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
+    const uint32_t ptr = leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h);
+    uint32_t* c = &dag.tables[s - 2].refs[ptr];
+    if (ld_filter(c) < kStickyRefs)
+        atomicAdd(c, 1u);
+}
+// node handle h may be used where its children decode under `role`.
+__device__ inline bool node_has_role(const DagDev& dag, uint32_t h, uint8_t role)
+{
+    return (dag.tables[node_handle_size(h) - 2].roles[node_handle_ptr(h)] & role) != 0;
+}
+// leaf handle h names an item stored as a leaf (no roles).
+__device__ inline bool leaf_is_plain(const DagDev& dag, uint32_t h)
+{
+    return dag.tables[leaf_handle_size(dag, h) - 2].roles[leaf_handle_ptr(dag, h)] == 0;
+}
+// After a group inserted the item held in n (s words) at pointer res of
+// table s - 2: record its role (0 for a leaf) and count its children.
+template <int G, int MW>
+__device__ inline void refs_on_insert(const DagDev& dag, const RefArrays& ra, const Needle<G, MW>& n, uint32_t s,
+                                      uint32_t res, uint8_t role, uint32_t gl)
+{
+    if (gl == 0)
+        ra.roles[s - 2][res] = role;
+    if (role) {
+#pragma unroll
+        for (int k = 0; k < Needle<G, MW>::K; ++k) {
+            const uint32_t j = gl + (uint32_t)k * G;
+            if (j >= 1u && j < s)
+                child_ref_add(dag, ra, n.w[k], role);
+        }
+    }
+}
+
 // A leaf is fully filled iff all 64 voxels are set with one material
 // (my_gpu_hash_dag_factory.cpp:357-360).
 __device__ inline bool leaf_is_full(const uint32_t* leaf, uint32_t M, uint32_t& mat)
@@ -162,6 +244,17 @@ struct svdag_ctx {
     uint8_t* dSurf = nullptr;
     float* dShade = nullptr; // per-pixel output of the shadow / occlusion passes (hitCap floats)
     void* dSpine = nullptr;  // parallel spine scratch, allocated with the batch slots
+    // Reference counting (gc.hip): items stored in the tables, every root the
+    // context has held since the last collection, and whether the counts
+    // may be off (an edit rolled back after inserting, an import), in which
+    // case the next collection marks from the roots and recounts.
+    unsigned long long* dStored = nullptr;
+    std::vector<uint32_t> rootLog;
+    bool refsDirty = false;
+    uint32_t* dGcCand = nullptr;         // released roots of a collection
+    uint32_t gcCandCap = 0;
+    uint2* dGcTouchedList = nullptr;     // (table, bucket) pairs for the chain pass
+    std::vector<uint32_t*> gcTouched;    // per table: a bit per bucket
     uint32_t traceW = 0, traceH = 0, traceRoot = 0xFFFFFFFFu;
     std::mutex mu;
     // Serving (serve.hip).  While a server runs, frame traces read the tables
@@ -186,7 +279,15 @@ struct FrameQuiesce {
     }
 };
 int ctx_refresh_tables(svdag_ctx* ctx);
-int gc_run(svdag_ctx* ctx, svdag_gc_stats* st);
+// Every root the context holds is logged when it is replaced: the next
+// collection releases the logged roots (gc.hip).
+inline void root_log(svdag_ctx* ctx)
+{
+    if (ctx->root != 0xFFFFFFFFu)
+        ctx->rootLog.push_back(ctx->root);
+}
+// mark: force the marking collection (else reference counts, when valid).
+int gc_run(svdag_ctx* ctx, svdag_gc_stats* st, bool mark = false);
 // surfDev (optional) receives one surface code per pixel (see dSurf).
 int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
                  svdag_hit* outDev, uint8_t* surfDev = nullptr);

@@ -346,6 +346,10 @@ static int import_dump(svdag_ctx* ctx, const uint32_t* d, size_t len)
         set_error("import: bad root id or trailing words");
         return SVDAG_E_INVALID;
     }
+    // Bulk inserts keep no reference counts: the next collection marks and
+    // recounts.
+    ctx->refsDirty = true;
+    root_log(ctx);
     ctx->root = rid == kSentinel ? kSentinel : below[rid];
     SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));

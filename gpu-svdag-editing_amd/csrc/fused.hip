@@ -167,7 +167,9 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
     __shared__ unsigned long long sPosEdits[kParItems];
     __shared__ uint32_t sFullCnt[kParEdits], sDropped[kParEdits];
     __shared__ uint32_t sOk, sBad, sOom, sNumReps;
+    __shared__ RefArrays sRa;
     const uint32_t t = threadIdx.x;
+    ref_arrays_stage(sRa, dag); // read after the barriers below
     const uint2 wv = f.waves[*f.waveIdx];
     const uint32_t K = wv.y - wv.x, S = f.S;
     const uint32_t leafLevel = dag.leafLevel;
@@ -473,7 +475,8 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
             const uint32_t hsh = needle_hash(nd, s, gl);
             uint32_t res = kNotFound, hint = kNoSlab;
             if (sNew[ri] == 0u)
-                res = table_find_group_hint<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, hint);
+                res = table_find_group_hint<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, hint,
+                                               node_role(dag, (uint32_t)l));
             if (gl == 0) {
                 sResH[ri] = res == kNotFound ? hint : node_handle(res, s);
                 sResC[ri] = res == kNotFound ? 2u : 0u; // 2: to insert, from the line in sResH
@@ -490,6 +493,11 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
             needle_load(nd, rec, s, gl);
             const uint32_t hsh = needle_hash(nd, s, gl);
             const uint32_t res = table_insert_group<8>(dag.tables[s - 2], nd, s, hsh, gl, gbase, sResH[ri]);
+            if (res != kNotFound) {
+                refs_on_insert(dag, sRa, nd, s, res, node_role(dag, (uint32_t)l), gl);
+                if (gl == 0)
+                    atomicAdd(dag.stored, 1ull);
+            }
             // Every lane has read sResH above; the group's lanes are in
             // lockstep within the wave, so lane 0 may overwrite it.
             if (gl == 0) {
@@ -560,7 +568,9 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
     __shared__ uint32_t sCreated[kMaxLevels][kSpineItems]; // 1: the spine inserted it
     __shared__ uint32_t sHasNew[kSpineItems];               // item has a created child
     __shared__ uint32_t sCnt[kMaxLevels];
+    __shared__ RefArrays sRa;
     const uint32_t lane = threadIdx.x;
+    ref_arrays_stage(sRa, dag); // read after the barriers below
     if (*f.spineDone) {
         // The parallel spine took this wave.
         __syncthreads();
@@ -770,20 +780,32 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                 }
             }
             uint32_t handle = kSentinel, created = 0;
-            if (i8 < n && !empty && repIdx == i8) {
+            const bool mine = i8 < n && !empty && repIdx == i8;
+            const uint8_t role = node_role(dag, (uint32_t)l);
+            uint32_t r = kNotFound, hint = kNoSlab, s = 0, h = 0;
+            Needle<8> nd;
+            if (mine) {
                 const uint32_t* rec = sRec[l][i8];
-                const uint32_t s = node_size_from_header(rec[0]);
-                Needle<8> nd;
+                s = node_size_from_header(rec[0]);
                 needle_load(nd, rec, s, c8);
-                const uint32_t h = needle_hash(nd, s, c8);
+                h = needle_hash(nd, s, c8);
                 // A node with a child inserted by this merge is new: no find.
-                uint32_t r = kNotFound, hint = kNoSlab;
                 if (sHasNew[i8] == 0u)
-                    r = table_find_group_hint<8>(dag.tables[s - 2], nd, s, h, c8, gbase, hint);
-                if (r == kNotFound) {
-                    r = table_insert_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase, hint);
-                    created = 1u;
+                    r = table_find_group_hint<8>(dag.tables[s - 2], nd, s, h, c8, gbase, hint, role);
+            }
+            // Every find before any insert: an insert claims a slot's tag
+            // before it writes the item.
+            __syncthreads();
+            if (mine && r == kNotFound) {
+                r = table_insert_group<8>(dag.tables[s - 2], nd, s, h, c8, gbase, hint);
+                created = 1u;
+                if (r != kNotFound) {
+                    refs_on_insert(dag, sRa, nd, s, r, role, c8);
+                    if (c8 == 0)
+                        atomicAdd(dag.stored, 1ull);
                 }
+            }
+            if (mine) {
                 if (r == kNotFound)
                     oom = true;
                 else

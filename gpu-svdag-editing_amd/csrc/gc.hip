@@ -29,9 +29,9 @@
 namespace svdag {
 
 struct GcCounters {
-    uint32_t count[kMaxLevels]; // frontier size per level
+    uint32_t count[kMaxLevels]; // frontier size per level (marking) / dead items per round (reference counts)
     uint32_t overflow;
-    uint32_t pad;
+    uint32_t touched;           // buckets listed for the chain pass (reference counts)
     unsigned long long itemsFreed, slabsFreed, itemsLive;
 };
 
@@ -42,6 +42,7 @@ struct GcTables {
     TableDev t[kGcMaxTables];
     uint8_t* visit[kGcMaxTables];
     uint8_t* hasChain[kGcMaxTables]; // per-bucket "has overflow lines" bytes, or nullptr: walk every bucket
+    uint32_t* touched[kGcMaxTables]; // per-bucket bits: listed for the chain pass (zero between collections)
 };
 
 // Clears the visit bytes of the lines handed out so far (16 bytes a line).
@@ -55,8 +56,11 @@ __global__ void __launch_bounds__(256) gc_clear_visits(GcTables a)
     if (blockIdx.x == 0 && threadIdx.x == 0 && t.alloc->freeCount < 0)
         t.alloc->freeCount = 0;
     uint4* v = reinterpret_cast<uint4*>(visit);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fresh; i += (uint64_t)gridDim.x * blockDim.x)
+    uint4* r = reinterpret_cast<uint4*>(t.roles);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fresh; i += (uint64_t)gridDim.x * blockDim.x) {
         v[i] = make_uint4(0u, 0u, 0u, 0u);
+        r[i] = make_uint4(0u, 0u, 0u, 0u); // the mark sets the roles items are reached in
+    }
 }
 
 // Visit (item, level): true if the caller is to expand the item at L (the
@@ -142,7 +146,10 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint8_t* const*
         const uint32_t i = i0 + threadIdx.x;
         uint32_t ch[8], keep = 0;
         if (i < n) {
-            const uint32_t* node = dag_node_ptr(dag, in[i]);
+            const uint32_t hn = in[i];
+            const uint32_t* node = dag_node_ptr(dag, hn);
+            uint8_t* role = &dag.tables[node_handle_size(hn) - 2].roles[node_handle_ptr(hn)];
+            *role = *role | node_role(dag, L); // one level per kernel: equal values race
             const uint32_t cnt = __builtin_popcount(node[0] & 0xFFu);
             uint8_t* b[8];
             uint8_t v[8];
@@ -290,6 +297,239 @@ __global__ void __launch_bounds__(256) gc_sweep_chains(GcTables a, GcCounters* c
     gc_block_add(freedSlabs, &ctr->slabsFreed);
 }
 
+// ---------------------------------------------------------------- recount
+// After a marking collection: zero every count, then each stored item counts
+// its children under each role the mark reached it in, and the level-0
+// fully-filled nodes get back their permanent reference.  Nothing runs after
+// a frontier overflow.
+__global__ void __launch_bounds__(256) gc_refs_clear(GcTables a, GcCounters* ctr)
+{
+    if (ctr->overflow)
+        return;
+    const TableDev& t = a.t[blockIdx.y];
+    const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
+    uint4* r = reinterpret_cast<uint4*>(t.refs); // 64 bytes a line
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)fresh * 4u;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        r[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void __launch_bounds__(256) gc_refs_count(DagDev dag, GcTables a, GcCounters* ctr)
+{
+    if (ctr->overflow)
+        return;
+    const TableDev& t = a.t[blockIdx.y];
+    const uint32_t fresh = min(t.alloc->fresh, t.slabCap);
+    const uint32_t s = t.itemWords;
+    for (uint32_t slab = blockIdx.x * blockDim.x + threadIdx.x; slab < fresh; slab += gridDim.x * blockDim.x) {
+        const uint32_t* line = t.lines + (size_t)slab * kLineWords;
+        const uint4 tv = *reinterpret_cast<const uint4*>(line);
+        const uint4 rv = reinterpret_cast<const uint4*>(t.roles)[slab];
+        const uint32_t tw[4] = { tv.x, tv.y, tv.z, tv.w }, rw[4] = { rv.x, rv.y, rv.z, rv.w };
+        for (uint32_t slot = 0; slot < t.slots; ++slot) {
+            const uint32_t sh = 8u * (slot & 3u);
+            const uint32_t tag = (tw[slot >> 2] >> sh) & 0xFFu, roles = (rw[slot >> 2] >> sh) & 0xFFu;
+            if (!tag || !roles)
+                continue;
+            const uint32_t* item = line + t.tagWords + 1u + slot * s;
+            for (uint32_t c = 1; c < s; ++c) {
+                if (roles & kRoleParentOfNodes)
+                    child_ref_add(dag, item[c], kRoleParentOfNodes);
+                if (roles & kRoleParentOfLeaves)
+                    child_ref_add(dag, item[c], kRoleParentOfLeaves);
+            }
+        }
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < dag.numMaterials) {
+        const uint32_t h = dag.fullyFilled[threadIdx.x];
+        const bool leaf = dag.leafLevel == 0;
+        const uint32_t hs = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
+        atomicAdd(&dag.tables[hs - 2].refs[leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h)], 1u);
+    }
+}
+
+__global__ void gc_set_stored(DagDev dag, GcCounters* ctr)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctr->overflow)
+        *dag.stored = ctr->itemsLive;
+}
+
+// ---------------------------------------------------------------- reference-count collection
+// Releases the roots held since the last collection.  A released root that
+// no stored item refers to dies; every dying item releases its children
+// under each of its roles, and those that drop to zero die in the next
+// round (a path from a root is at most a level per round).  Dead items lose
+// their tags; the overflow lines of their buckets are checked afterwards and
+// unlinked when empty.  The work is proportional to what died.
+
+// Wave-aggregated append: every lane of the wave calls it; a lane with
+// n = 1 gets its index, from one atomic per wave.
+__device__ inline uint32_t wave_append(uint32_t* counter, bool n)
+{
+    const unsigned long long m = __ballot(n);
+    uint32_t base = 0;
+    if (m != 0ull) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+        if (__lane_id() == leader)
+            base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    }
+    return base + (uint32_t)__popcll(m & lanes_below());
+}
+
+__global__ void __launch_bounds__(256) rc_seed(DagDev dag, const uint32_t* cand, uint32_t n, uint2* out,
+                                               uint32_t outCap, GcCounters* ctr)
+{
+    // A pop past the free stack's end leaves its count below zero; the chain
+    // pass pushes onto an empty stack then.
+    if (threadIdx.x < dag.numTables && dag.tables[threadIdx.x].alloc->freeCount < 0)
+        dag.tables[threadIdx.x].alloc->freeCount = 0;
+    const bool leaf = dag.leafLevel == 0;
+    for (uint32_t base = 0; base < n; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        bool dies = false;
+        uint2 e = make_uint2(0u, 0u);
+        if (i < n) {
+            const uint32_t h = cand[i];
+            const uint32_t hs = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
+            e = make_uint2(hs - 2u, leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h));
+            dies = dag.tables[e.x].refs[e.y] == 0u;
+        }
+        const uint32_t at = wave_append(&ctr->count[0], dies);
+        if (dies) {
+            if (at < outCap)
+                out[at] = e;
+            else
+                atomicOr(&ctr->overflow, 1u);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32_t rootTable, uint32_t rootPtr,
+                                                  const uint2* in, const uint32_t* inCount, uint2* out,
+                                                  uint32_t* outCount, uint32_t cap, uint2* touchedList,
+                                                  GcCounters* ctr)
+{
+    const uint32_t n = min(*inCount, cap);
+    uint32_t freed = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool have = i < n;
+        const uint2 e = have ? in[i] : make_uint2(0u, 0u);
+        const TableDev& t = dag.tables[e.x];
+        const uint32_t s = e.x + 2u;
+        const uint32_t* item = have ? table_item_ptr(t, e.y) : nullptr;
+        const uint32_t roles = have ? t.roles[e.y] : 0u;
+        // Children, a word and a role at a time (a node has at most 8).
+        for (uint32_t j = 1; j < 9; ++j) {
+#pragma unroll
+            for (uint32_t rb = 0; rb < 2; ++rb) {
+                const uint8_t role = rb == 0 ? kRoleParentOfNodes : kRoleParentOfLeaves;
+                bool dies = false;
+                uint2 ce = make_uint2(0u, 0u);
+                if (have && j < s && (roles & role)) {
+                    const uint32_t ch = item[j];
+                    const bool lf = role == kRoleParentOfLeaves;
+                    const uint32_t cs = lf ? leaf_handle_size(dag, ch) : node_handle_size(ch);
+                    ce = make_uint2(cs - 2u, lf ? leaf_handle_ptr(dag, ch) : node_handle_ptr(ch));
+                    uint32_t* c = &dag.tables[ce.x].refs[ce.y];
+                    if (ld_filter(c) < kStickyRefs) {
+                        const uint32_t old = atomicSub(c, 1u);
+                        if (old == 0u)
+                            atomicOr(&ctr->overflow, 1u); // counts off: the host falls back to marking
+                        dies = old == 1u && !(ce.x == rootTable && ce.y == rootPtr);
+                    }
+                }
+                const uint32_t at = wave_append(outCount, dies);
+                if (dies) {
+                    if (at < cap)
+                        out[at] = ce;
+                    else
+                        atomicOr(&ctr->overflow, 1u);
+                }
+            }
+        }
+        // Free the item; an overflow line's bucket is listed once for the
+        // chain pass (the bucket hash of the words, as at insert).
+        bool list = false;
+        uint32_t bucket = 0;
+        if (have) {
+            const uint32_t slab = e.y >> kSlotBits, slot = e.y & ((1u << kSlotBits) - 1u);
+            if (slab >= t.numBuckets) {
+                uint32_t h = 0;
+                for (uint32_t q = 0; q < s; ++q)
+                    h ^= hash_part(item[q], q);
+                bucket = h % t.numBuckets;
+                const uint32_t bit = 1u << (bucket & 31u);
+                list = (atomicOr(&a.touched[e.x][bucket >> 5], bit) & bit) == 0u;
+            }
+            atomicAnd(&t.lines[(size_t)slab * kLineWords + (slot >> 2)], ~(0xFFu << (8u * (slot & 3u))));
+            t.roles[e.y] = 0u;
+            ++freed;
+        }
+        const uint32_t at = wave_append(&ctr->touched, list);
+        if (list) {
+            if (at < cap)
+                touchedList[at] = make_uint2(e.x, bucket);
+            else
+                atomicOr(&ctr->overflow, 1u);
+        }
+    }
+    gc_block_add(freed, &ctr->itemsFreed);
+}
+
+// The listed buckets of table blockIdx.y: unlink the overflow lines left
+// without items and push them on the free stack (as gc_sweep_chains).
+__global__ void __launch_bounds__(256) rc_chains(GcTables a, const uint2* list, uint32_t cap, GcCounters* ctr)
+{
+    const uint32_t ti = blockIdx.y;
+    const TableDev& t = a.t[ti];
+    const uint32_t n = min(ctr->touched, cap);
+    uint32_t freedSlabs = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t prev = kNoSlab, cur = kNoSlab;
+        if (i < n) {
+            const uint2 e = list[i];
+            if (e.x == ti) {
+                atomicAnd(&a.touched[ti][e.y >> 5], ~(1u << (e.y & 31u)));
+                prev = e.y;
+                cur = t.lines[(size_t)e.y * kLineWords + t.tagWords];
+            }
+        }
+        while (__ballot(cur != kNoSlab) != 0ull) {
+            bool give = false;
+            const uint32_t id = cur;
+            if (cur != kNoSlab) {
+                uint32_t* line = t.lines + (size_t)cur * kLineWords;
+                const uint4 v = *reinterpret_cast<const uint4*>(line);
+                const uint32_t next = line[t.tagWords];
+                const uint32_t tags = v.x | (t.tagWords > 1 ? v.y : 0u) | (t.tagWords > 2 ? v.z : 0u)
+                                      | (t.tagWords > 3 ? v.w : 0u);
+                if (tags == 0u) {
+                    t.lines[(size_t)prev * kLineWords + t.tagWords] = next;
+                    line[t.tagWords] = kNoSlab;
+                    give = true;
+                    ++freedSlabs;
+                } else {
+                    prev = cur;
+                }
+                cur = next;
+            }
+            wave_slab_free(t, give, id);
+        }
+    }
+    gc_block_add(freedSlabs, &ctr->slabsFreed);
+}
+
+__global__ void rc_finish(DagDev dag, GcCounters* ctr)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *dag.stored -= ctr->itemsFreed;
+        ctr->itemsLive = *dag.stored;
+    }
+}
+
 // Visit bytes for every table (16 per line), kept between collections;
 // regrown when an arena has grown.
 static int gc_masks(svdag_ctx* ctx)
@@ -318,46 +558,74 @@ static int gc_masks(svdag_ctx* ctx)
     return SVDAG_OK;
 }
 
-int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
+// Frontier buffers (also the reference-count worklists, as uint2 pairs)
+// of at least cap words, the list of buckets for the chain pass, and the
+// per-table bucket bits.
+static int gc_fronts(svdag_ctx* ctx, uint64_t cap)
 {
-    // A collection may free the nodes of the last traced root, so the shadow
-    // and occlusion passes need a new primary trace afterwards.
-    ctx->traceW = ctx->traceH = 0;
+    if (cap > ctx->gcCap) {
+        for (int k = 0; k < 2; ++k) {
+            if (ctx->gcFront[k])
+                (void)hipFree(ctx->gcFront[k]);
+            ctx->gcFront[k] = nullptr;
+            SVDAG_HIP(hipMalloc(&ctx->gcFront[k], cap * 4));
+        }
+        if (ctx->dGcTouchedList)
+            (void)hipFree(ctx->dGcTouchedList);
+        ctx->dGcTouchedList = nullptr;
+        SVDAG_HIP(hipMalloc(&ctx->dGcTouchedList, (cap / 2) * sizeof(uint2)));
+        ctx->gcCap = cap;
+    }
+    const size_t nt = ctx->ts.tables.size();
+    if (ctx->gcTouched.size() != nt) {
+        for (uint32_t* p : ctx->gcTouched)
+            if (p)
+                (void)hipFree(p);
+        ctx->gcTouched.assign(nt, nullptr);
+        for (size_t i = 0; i < nt; ++i) {
+            const size_t words = (ctx->ts.tables[i].numBuckets + 31u) / 32u;
+            SVDAG_HIP(hipMalloc(&ctx->gcTouched[i], words * 4));
+            SVDAG_HIP(hipMemsetAsync(ctx->gcTouched[i], 0, words * 4, ctx->stream));
+        }
+    }
+    return SVDAG_OK;
+}
+
+static GcTables gc_tables(const svdag_ctx* ctx)
+{
+    GcTables a {};
+    for (size_t i = 0; i < ctx->ts.tables.size(); ++i) {
+        a.t[i] = ctx->ts.tables[i].dev;
+        a.visit[i] = ctx->gcMasks[i];
+        a.hasChain[i] = nullptr;
+        a.touched[i] = ctx->gcTouched.size() > i ? ctx->gcTouched[i] : nullptr;
+    }
+    return a;
+}
+
+
+// The marking collection (also the fallback of the reference-count one):
+// mark from the roots, sweep, then recount every stored item's references.
+static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
+{
     const hipStream_t st = ctx->stream;
     const DagDev& dag = ctx->dag;
     uint64_t cap = std::max<uint64_t>(ctx->gcCap, 1ull << 22);
-    if (!ctx->gcCounters)
-        SVDAG_HIP(hipMalloc(&ctx->gcCounters, sizeof(GcCounters)));
     GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
-    SVDAG_TRY(gc_masks(ctx));
     const size_t nt = ctx->ts.tables.size();
-    if (nt > kGcMaxTables) {
-        set_error("too many item tables for the collection");
-        return SVDAG_E_INVALID;
-    }
     uint32_t maxCap = 1, maxBuckets = 1;
     for (size_t i = 0; i < nt; ++i) {
         maxCap = std::max(maxCap, ctx->ts.tables[i].slabCap);
         maxBuckets = std::max(maxBuckets, ctx->ts.tables[i].numBuckets);
     }
     for (int attempt = 0; attempt < 6; ++attempt) {
-        if (cap > ctx->gcCap) {
-            for (int k = 0; k < 2; ++k) {
-                if (ctx->gcFront[k])
-                    (void)hipFree(ctx->gcFront[k]);
-                ctx->gcFront[k] = nullptr;
-                SVDAG_HIP(hipMalloc(&ctx->gcFront[k], cap * 4));
-            }
-            ctx->gcCap = cap;
-        }
+        SVDAG_TRY(gc_fronts(ctx, cap));
         // The frontiers are free after the mark; the first one holds the
         // sweep's per-bucket chain bytes, the tables' maps one after another.
-        GcTables a {};
+        GcTables a = gc_tables(ctx);
         uint64_t mapUsed = 0;
         for (size_t i = 0; i < nt; ++i) {
             const HostTable& t = ctx->ts.tables[i];
-            a.t[i] = t.dev;
-            a.visit[i] = ctx->gcMasks[i];
             a.hasChain[i] = nullptr;
             if (mapUsed + t.numBuckets <= ctx->gcCap * 4) {
                 a.hasChain[i] = reinterpret_cast<uint8_t*>(ctx->gcFront[0]) + mapUsed;
@@ -366,6 +634,9 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
         }
         SVDAG_HIP(hipEventRecord(ctx->ev0, st));
         SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
+        // Bucket bits a cut-short counting collection may have left set.
+        for (size_t i = 0; i < nt; ++i)
+            SVDAG_HIP(hipMemsetAsync(ctx->gcTouched[i], 0, (ctx->ts.tables[i].numBuckets + 31u) / 32u * 4, st));
         gc_clear_visits<<<dim3(grid_for(maxCap, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a);
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dGcMasks, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L < dag.leafLevel; ++L)
@@ -374,12 +645,15 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
                                                  (uint32_t)std::min<uint64_t>(ctx->gcCap, 0xFFFFFFFFull), ctr);
         gc_sweep_lines<<<dim3(grid_for(maxCap, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
         gc_sweep_chains<<<dim3(grid_for(maxBuckets, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
+        gc_refs_clear<<<dim3(grid_for((uint64_t)maxCap * 4, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
+        gc_refs_count<<<dim3(grid_for(maxCap, 256, 1024), (uint32_t)nt), 256, 0, st>>>(dag, a, ctr);
+        gc_set_stored<<<1, 64, 0, st>>>(dag, ctr);
         SVDAG_HIP(hipEventRecord(ctx->ev1, st));
         GcCounters h {};
         SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         SVDAG_HIP(hipStreamSynchronize(st));
         if (h.overflow) {
-            // The mark ran out of frontier space and the sweep did nothing:
+            // The mark ran out of frontier space and nothing after it ran:
             // run again with larger frontiers.
             cap = std::min<uint64_t>(cap * 2, 1ull << 31);
             continue;
@@ -398,6 +672,104 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats)
     return SVDAG_E_CAPACITY;
 }
 
+// The reference-count collection.  done = false when it could not finish
+// (a worklist overflowed, or counts were found off); the caller then runs
+// the marking collection, which repairs both.
+static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
+{
+    done = false;
+    const hipStream_t st = ctx->stream;
+    const DagDev& dag = ctx->dag;
+    GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
+    const size_t nt = ctx->ts.tables.size();
+    SVDAG_TRY(gc_fronts(ctx, std::max<uint64_t>(ctx->gcCap, 1ull << 22)));
+    // Released roots: every root held since the last collection, but the
+    // current one (the undo / redo history is cleared by now).
+    std::vector<uint32_t> cand;
+    for (uint32_t r : ctx->rootLog)
+        if (r != kSentinel && r != ctx->root)
+            cand.push_back(r);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    if (cand.size() > ctx->gcCandCap) {
+        if (ctx->dGcCand)
+            (void)hipFree(ctx->dGcCand);
+        ctx->dGcCand = nullptr;
+        ctx->gcCandCap = 0;
+        SVDAG_HIP(hipMalloc(&ctx->dGcCand, std::max<size_t>(cand.size(), 1024) * 4));
+        ctx->gcCandCap = (uint32_t)std::max<size_t>(cand.size(), 1024);
+    }
+    uint32_t rootTable = ~0u, rootPtr = ~0u;
+    if (ctx->root != kSentinel) {
+        const bool leaf = dag.leafLevel == 0;
+        rootTable = (leaf ? leaf_handle_size(dag, ctx->root) : node_handle_size(ctx->root)) - 2u;
+        rootPtr = leaf ? leaf_handle_ptr(dag, ctx->root) : node_handle_ptr(ctx->root);
+    }
+    GcTables a = gc_tables(ctx);
+    const uint32_t cap2 = (uint32_t)std::min<uint64_t>(ctx->gcCap / 2, 0xFFFFFFFFull); // uint2 entries per frontier
+    uint2* w[2] = { reinterpret_cast<uint2*>(ctx->gcFront[0]), reinterpret_cast<uint2*>(ctx->gcFront[1]) };
+    const uint32_t rounds = dag.leafLevel + 2u;
+    SVDAG_HIP(hipEventRecord(ctx->ev0, st));
+    if (!cand.empty())
+        SVDAG_HIP(hipMemcpyAsync(ctx->dGcCand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, st));
+    SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
+    rc_seed<<<1, 256, 0, st>>>(dag, ctx->dGcCand, (uint32_t)cand.size(), w[0], cap2, ctr);
+    for (uint32_t r = 0; r < rounds; ++r)
+        rc_release<<<256, 256, 0, st>>>(dag, a, rootTable, rootPtr, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
+                                         &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
+    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(a, ctx->dGcTouchedList, cap2, ctr);
+    rc_finish<<<1, 64, 0, st>>>(dag, ctr);
+    SVDAG_HIP(hipEventRecord(ctx->ev1, st));
+    GcCounters h {};
+    SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+    SVDAG_HIP(hipStreamSynchronize(st));
+    if (h.overflow || h.count[rounds] != 0u)
+        return SVDAG_OK;
+    done = true;
+    if (stats) {
+        stats->items_freed = h.itemsFreed;
+        stats->slabs_freed = h.slabsFreed;
+        stats->items_live = h.itemsLive;
+        float ms = 0;
+        SVDAG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->gpu_ms = ms;
+    }
+    return SVDAG_OK;
+}
+
+int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
+{
+    // A collection may free the nodes of the last traced root, so the shadow
+    // and occlusion passes need a new primary trace afterwards.
+    ctx->traceW = ctx->traceH = 0;
+    if (!ctx->gcCounters)
+        SVDAG_HIP(hipMalloc(&ctx->gcCounters, sizeof(GcCounters)));
+    if (ctx->ts.tables.size() > kGcMaxTables) {
+        set_error("too many item tables for the collection");
+        return SVDAG_E_INVALID;
+    }
+    SVDAG_TRY(gc_masks(ctx));
+    bool done = false;
+    float releaseMs = 0;
+    if (!mark && !ctx->refsDirty) {
+        svdag_gc_stats rs {};
+        SVDAG_TRY(gc_release(ctx, &rs, done));
+        if (done && stats)
+            *stats = rs;
+        releaseMs = rs.gpu_ms;
+    }
+    if (!done) {
+        SVDAG_TRY(gc_mark_sweep(ctx, stats));
+        if (stats)
+            stats->gpu_ms += releaseMs;
+    }
+    ctx->refsDirty = false;
+    ctx->rootLog.clear();
+    if (ctx->root != kSentinel)
+        ctx->rootLog.push_back(ctx->root);
+    return SVDAG_OK;
+}
+
 } // namespace svdag
 
 using namespace svdag;
@@ -412,5 +784,17 @@ extern "C" int svdag_gc(svdag_ctx* ctx, svdag_gc_stats* stats)
     ctx->redo.clear();
     svdag_gc_stats local {};
     FrameQuiesce q(ctx);
-    return gc_run(ctx, stats ? stats : &local);
+    return gc_run(ctx, stats ? stats : &local, false);
+}
+
+extern "C" int svdag_gc_mark(svdag_ctx* ctx, svdag_gc_stats* stats)
+{
+    if (!ctx)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->undo.clear();
+    ctx->redo.clear();
+    svdag_gc_stats local {};
+    FrameQuiesce q(ctx);
+    return gc_run(ctx, stats ? stats : &local, true);
 }

@@ -30,7 +30,8 @@ __global__ void table_init_lines(TableDev t, uint32_t first, uint32_t last)
     }
 }
 
-int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t slabCap, hipStream_t stream)
+int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t slabCap, hipStream_t stream,
+                bool withRefs)
 {
     if (itemWords == 0 || itemWords > kMaxItemWords || numBuckets == 0 || slabCap < numBuckets) {
         set_error("table_alloc: bad size");
@@ -52,6 +53,12 @@ int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t 
     SVDAG_HIP(hipMalloc(&d.lines, (size_t)slabCap * kLineWords * 4));
     SVDAG_HIP(hipMalloc(&d.freeList, (size_t)slabCap * 4));
     SVDAG_HIP(hipMalloc(&d.alloc, sizeof(AllocState)));
+    if (withRefs) {
+        SVDAG_HIP(hipMalloc(&d.refs, (size_t)slabCap * kMaxSlots * 4));
+        SVDAG_HIP(hipMalloc(&d.roles, (size_t)slabCap * kMaxSlots));
+        SVDAG_HIP(hipMemsetAsync(d.refs, 0, (size_t)slabCap * kMaxSlots * 4, stream));
+        SVDAG_HIP(hipMemsetAsync(d.roles, 0, (size_t)slabCap * kMaxSlots, stream));
+    }
     AllocState a { numBuckets, 0, 0u, 0u };
     SVDAG_HIP(hipMemcpyAsync(d.alloc, &a, sizeof(a), hipMemcpyHostToDevice, stream));
     table_init_lines<<<grid_for((uint64_t)slabCap * kLineWords, 256, 8192), 256, 0, stream>>>(d, 0u, slabCap);
@@ -65,6 +72,8 @@ void table_free(HostTable& t)
     if (d.lines) (void)hipFree(d.lines);
     if (d.freeList) (void)hipFree(d.freeList);
     if (d.alloc) (void)hipFree(d.alloc);
+    if (d.refs) (void)hipFree(d.refs);
+    if (d.roles) (void)hipFree(d.roles);
     d = TableDev {};
     t = HostTable {};
 }
@@ -81,14 +90,29 @@ int table_grow(HostTable& t, uint32_t newCap, hipStream_t stream)
     }
     TableDev& d = t.dev;
     const uint32_t old = t.slabCap;
-    uint32_t *lines = nullptr, *freeList = nullptr;
+    uint32_t *lines = nullptr, *freeList = nullptr, *refs = nullptr;
+    uint8_t* roles = nullptr;
     SVDAG_HIP(hipMalloc(&lines, (size_t)newCap * kLineWords * 4));
     SVDAG_HIP(hipMalloc(&freeList, (size_t)newCap * 4));
     SVDAG_HIP(hipMemcpyAsync(lines, d.lines, (size_t)old * kLineWords * 4, hipMemcpyDeviceToDevice, stream));
     SVDAG_HIP(hipMemcpyAsync(freeList, d.freeList, (size_t)old * 4, hipMemcpyDeviceToDevice, stream));
+    if (d.refs) {
+        SVDAG_HIP(hipMalloc(&refs, (size_t)newCap * kMaxSlots * 4));
+        SVDAG_HIP(hipMalloc(&roles, (size_t)newCap * kMaxSlots));
+        SVDAG_HIP(hipMemcpyAsync(refs, d.refs, (size_t)old * kMaxSlots * 4, hipMemcpyDeviceToDevice, stream));
+        SVDAG_HIP(hipMemcpyAsync(roles, d.roles, (size_t)old * kMaxSlots, hipMemcpyDeviceToDevice, stream));
+        SVDAG_HIP(hipMemsetAsync(refs + (size_t)old * kMaxSlots, 0, (size_t)(newCap - old) * kMaxSlots * 4, stream));
+        SVDAG_HIP(hipMemsetAsync(roles + (size_t)old * kMaxSlots, 0, (size_t)(newCap - old) * kMaxSlots, stream));
+    }
     SVDAG_HIP(hipStreamSynchronize(stream));
     (void)hipFree(d.lines);
     (void)hipFree(d.freeList);
+    if (d.refs) {
+        (void)hipFree(d.refs);
+        (void)hipFree(d.roles);
+        d.refs = refs;
+        d.roles = roles;
+    }
     d.lines = lines;
     d.freeList = freeList;
     d.slabCap = newCap;

@@ -36,6 +36,7 @@ constexpr uint32_t kPendingSlab = 0xFFFFFFFEu; // tail link while its successor 
 constexpr uint32_t kNotFound = 0xFFFFFFFFu;
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;
 constexpr uint32_t kMaxItemWords = 18;  // 2 + 64*8/32 (8 material bits)
+constexpr uint8_t kRoleLeafUse = 0x80;   // find filter: an item without roles (never stored)
 
 // Items of s words per line: the largest k with k*s + ceil(k/4) <= 31.
 __host__ __device__ inline uint32_t line_slots(uint32_t s)
@@ -105,15 +106,17 @@ struct AllocState {
 // Every line that is not on a chain (the arena's never-used tail and the
 // free stack) is empty: tag words 0 and next = kNoSlab.
 struct TableDev {
-    uint32_t* lines;     // [slabCap][32]: tags, items, next
+    uint32_t* lines;     // [slabCap][32]: tags, next, items
     uint32_t* freeList;  // [slabCap] free stack of slab ids
     AllocState* alloc;
+    uint32_t* refs;      // DAG tables: [slabCap][16] stored references to each slot's item
+    uint8_t* roles;      // DAG tables: [slabCap][16] kRoleParentOf* bits of each slot's item
     uint32_t itemWords;
     uint32_t numBuckets;
     uint32_t slabCap;
     uint32_t slots;      // items per line
     uint32_t tagWords;   // tag words at the line start
-    uint32_t pad[7];
+    uint32_t pad[3];
 };
 
 __device__ inline uint32_t ld_relaxed(const uint32_t* p)
@@ -396,9 +399,12 @@ __device__ inline uint32_t table_find_group(const TableDev& t, const Needle<G, M
 // lines leave a chain only in a collection), so an insert that starts there
 // and walks on finds every free slot it would have found from the head
 // after that line -- it only skips lines the find saw full.
+// role != 0: a filter on the item's role byte (DAG tables, see
+// node_has_role): kRoleLeafUse matches items without roles, any other value
+// items with one of its bits.
 template <int G, int MW>
 __device__ inline uint32_t table_find_group_hint(const TableDev& t, const Needle<G, MW>& n, uint32_t s, uint32_t h,
-                                                 uint32_t gl, uint32_t gbase, uint32_t& hint)
+                                                 uint32_t gl, uint32_t gbase, uint32_t& hint, uint8_t role = 0)
 {
     static_assert(G >= 4 && G <= 32, "group size");
     const uint32_t tag4 = make_tag(h) * 0x01010101u;
@@ -412,8 +418,11 @@ __device__ inline uint32_t table_find_group_hint(const TableDev& t, const Needle
             const uint32_t slot = __ffs(cand) - 1u;
             cand &= cand - 1u;
             const bool eq = needle_equal_lane<G>(n, line_item(t, slab, slot), s, gl);
-            if (group_ballot<G>(!eq, gbase) == 0)
-                return (slab << kSlotBits) | slot;
+            if (group_ballot<G>(!eq, gbase) == 0) {
+                const uint32_t ptr = (slab << kSlotBits) | slot;
+                if (role == 0 || (role == kRoleLeafUse ? t.roles[ptr] == 0 : (t.roles[ptr] & role) != 0))
+                    return ptr;
+            }
         }
         uint32_t emp;
         if (hint == kNoSlab && line_empty(t, L.tag, emp))
@@ -569,6 +578,7 @@ struct DagDev {
     uint32_t maxLeafWords;    // 2 + 64*M/32
     uint32_t leafSizeBits;    // bit_width(maxLeaf - minLeaf)
     uint32_t numTables;
+    unsigned long long* stored; // items stored in the tables (inserts minus frees)
 };
 
 constexpr uint32_t kNodeSizeBits = 3; // bit_width(9 - 2)

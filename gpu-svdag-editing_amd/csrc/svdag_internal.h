@@ -38,7 +38,10 @@ struct HostTable {
     uint32_t slabCap = 0;
 };
 
-int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t slabCap, hipStream_t stream);
+// withRefs: the table holds DAG items, with per-slot reference counts and
+// roles (dev.refs, dev.roles) kept beside the lines and grown with them.
+int table_alloc(HostTable& t, uint32_t itemWords, uint32_t numBuckets, uint32_t slabCap, hipStream_t stream,
+                bool withRefs = false);
 void table_free(HostTable& t);
 // Grow the slab arena of a table to newCap slabs, preserving slab ids.
 int table_grow(HostTable& t, uint32_t newCap, hipStream_t stream);

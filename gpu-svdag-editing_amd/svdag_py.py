@@ -101,7 +101,7 @@ EXPORTED = [
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_terrain_heights", "svdag_root",
-    "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_query_voxels",
+    "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
@@ -346,9 +346,11 @@ class MyGPUHashDAG:
         check(lib().svdag_history(self.h, out))
         return tuple(out)
 
-    def garbage_collect(self):
+    def garbage_collect(self, mark=False):
+        """Free what the root and the fully-filled chain do not reach.
+        mark=True forces the marking collection (svdag_gc_mark)."""
         st = GcStats()
-        check(lib().svdag_gc(self.h, ctypes.byref(st)))
+        check((lib().svdag_gc_mark if mark else lib().svdag_gc)(self.h, ctypes.byref(st)))
         return st
 
     def query(self, xyz):

@@ -227,8 +227,17 @@ int svdag_redo(svdag_ctx* ctx);
 int svdag_history(svdag_ctx* ctx, uint32_t out[2]);
 
 /* reference: MyGPUHashDAGUndoRedo::garbageCollect -> MyGPUHashDAG::garbageCollect
- * (my_gpu_base_dag.h:66-77; my_gpu_hash_dag.cu:364-475).  Clears the history,
- * marks from the root and the fully-filled chain, and sweeps (always). */
+ * (my_gpu_base_dag.h:66-77; my_gpu_hash_dag.cu:364-475).  Clears the history
+ * and frees every stored item that the root and the fully-filled chain do
+ * not reach (the reference marks and, only under CAPTURE_MEMORY_STATS_SLOW,
+ * sweeps; this always frees).  The same items are freed either way, but the
+ * work differs: every stored item keeps a count of the references other
+ * stored items hold to it, so the collection releases the roots held since
+ * the last one and frees what drops to zero, in time proportional to what
+ * died.  After an edit was rolled back or a dump was imported the counts are
+ * rebuilt: that collection marks from the roots, sweeps and recounts, like
+ * svdag_gc_mark.  items_live counts the stored items after the collection
+ * (all reachable then). */
 typedef struct {
     uint64_t items_freed;
     uint64_t slabs_freed;
@@ -236,6 +245,12 @@ typedef struct {
     float gpu_ms;
 } svdag_gc_stats;
 int svdag_gc(svdag_ctx* ctx, svdag_gc_stats* stats);
+/* The marking collection itself, whatever the counts say: marks from the
+ * roots level by level (the reference's scheme, my_gpu_hash_dag.cu:377-463),
+ * sweeps every line and recounts the references.  No reference counterpart
+ * as a separate call; it checks svdag_gc (a marking collection right after a
+ * counting one frees nothing). */
+int svdag_gc_mark(svdag_ctx* ctx, svdag_gc_stats* stats);
 
 /* Voxel queries: xyz = n*3 uint32 (host); out_material[i] = material or 0xFFFFFFFF.
  * reference: DAGUtils::get_value (dag_utils.h:146-182) / DAGTracer::get_voxel_values

@@ -506,3 +506,50 @@ def test_other_material_widths_match_oracle(material_bits):
     gh2, _ = g.trace(cam["position"], rot, cam["fov"], cam["bmin"], cam["bmax"], 128, 128)
     assert np.array_equal(gh2.view(np.uint32), oh.view(np.uint32))
     g.close()
+
+
+@pytest.mark.gpu
+def test_reference_count_collection_frees_what_marking_frees():
+    # svdag_gc releases the roots held since the last collection and frees
+    # what drops to zero; svdag_gc_mark marks from the roots, sweeps and
+    # recounts.  After every counting collection a marking one must free
+    # nothing and agree on the live count, which must equal the distinct
+    # stored items reachable from the roots; the DAG stays equal to the
+    # oracle's through edits in both batch modes, undo / redo, a copy and
+    # a paint, across several collections of each kind.
+    levels = L10
+    edits = _mixed_stream(levels, seed=77)
+    g, o = _pair(levels=levels)
+    first = g.garbage_collect()
+    check = g.garbage_collect(mark=True)
+    assert check.items_freed == 0 and check.items_live == first.items_live == _stored_items(g)
+    chunks = [edits[:12], edits[12:30], edits[30:]]
+    for step, chunk in enumerate(chunks):
+        g.set_batch_mode(step % 2)
+        g.edit_batch(chunk)
+        for e in chunk:
+            o.edit(*e)
+        if step == 1:
+            # Roots visited by undo / redo are released too.
+            g.undo()
+            g.undo()
+            g.redo()
+            g.redo()
+        st = g.garbage_collect()
+        assert st.items_freed > 0
+        assert st.items_live == _stored_items(g)
+        _same(g, o)
+        check = g.garbage_collect(mark=True)
+        assert check.items_freed == 0, (step, check.items_freed)
+        assert check.items_live == st.items_live, step
+    # An abandoned undo branch: its roots die at the next collection.
+    g.edit(svdag_py.SPHERE_ADD, 3, (300.0, 520.0, 300.0), 40.0)
+    g.undo()
+    g.edit(svdag_py.SPHERE_REMOVE, 0, (700.0, 500.0, 700.0), 30.0)
+    o.edit(svdag_py.SPHERE_REMOVE, 0, (700.0, 500.0, 700.0), 30.0)
+    st = g.garbage_collect()
+    assert st.items_live == _stored_items(g)
+    check = g.garbage_collect(mark=True)
+    assert check.items_freed == 0 and check.items_live == st.items_live
+    _same(g, o)
+    g.close()
