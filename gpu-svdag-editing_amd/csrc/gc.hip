@@ -420,32 +420,67 @@ __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32
         const uint32_t s = e.x + 2u;
         const uint32_t* item = have ? table_item_ptr(t, e.y) : nullptr;
         const uint32_t roles = have ? t.roles[e.y] : 0u;
-        // Children, a word and a role at a time (a node has at most 8).
-        for (uint32_t j = 1; j < 9; ++j) {
+        // Children, all of one role at once (a node has at most 8): their
+        // count loads, then the decrements, then one append per wave.
 #pragma unroll
-            for (uint32_t rb = 0; rb < 2; ++rb) {
-                const uint8_t role = rb == 0 ? kRoleParentOfNodes : kRoleParentOfLeaves;
-                bool dies = false;
-                uint2 ce = make_uint2(0u, 0u);
-                if (have && j < s && (roles & role)) {
-                    const uint32_t ch = item[j];
-                    const bool lf = role == kRoleParentOfLeaves;
+        for (uint32_t rb = 0; rb < 2; ++rb) {
+            const uint8_t role = rb == 0 ? kRoleParentOfNodes : kRoleParentOfLeaves;
+            const bool lf = role == kRoleParentOfLeaves;
+            const bool mine = have && (roles & role);
+            uint2 ce[8];
+            uint32_t* cp[8];
+            uint32_t cv[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) {
+                cv[j] = kStickyRefs;
+                if (mine && j + 1u < s) {
+                    const uint32_t ch = item[j + 1u];
                     const uint32_t cs = lf ? leaf_handle_size(dag, ch) : node_handle_size(ch);
-                    ce = make_uint2(cs - 2u, lf ? leaf_handle_ptr(dag, ch) : node_handle_ptr(ch));
-                    uint32_t* c = &dag.tables[ce.x].refs[ce.y];
-                    if (ld_filter(c) < kStickyRefs) {
-                        const uint32_t old = atomicSub(c, 1u);
-                        if (old == 0u)
-                            atomicOr(&ctr->overflow, 1u); // counts off: the host falls back to marking
-                        dies = old == 1u && !(ce.x == rootTable && ce.y == rootPtr);
-                    }
+                    ce[j] = make_uint2(cs - 2u, lf ? leaf_handle_ptr(dag, ch) : node_handle_ptr(ch));
+                    cp[j] = &dag.tables[ce[j].x].refs[ce[j].y];
+                    cv[j] = ld_filter(cp[j]);
                 }
-                const uint32_t at = wave_append(outCount, dies);
-                if (dies) {
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (cv[j] < kStickyRefs)
+                    cv[j] = atomicSub(cp[j], 1u);
+            uint32_t die = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) {
+                if (mine && j + 1u < s) {
+                    if (cv[j] == 0u)
+                        atomicOr(&ctr->overflow, 1u); // counts off: the host falls back to marking
+                    if (cv[j] == 1u && !(ce[j].x == rootTable && ce[j].y == rootPtr))
+                        die |= 1u << j;
+                }
+            }
+            // Lanes' dying children go to consecutive slots of the round's
+            // list: a wave scan of the counts, one atomic per wave.
+            const uint32_t cnt = (uint32_t)__builtin_popcount(die);
+            uint32_t incl = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t v = (uint32_t)__shfl_up((int)incl, (unsigned)d, 64);
+                if (__lane_id() >= (uint32_t)d)
+                    incl += v;
+            }
+            const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+            uint32_t base0 = 0;
+            if (total != 0u) {
+                if (__lane_id() == 63u)
+                    base0 = atomicAdd(outCount, total);
+                base0 = (uint32_t)__shfl((int)base0, 63, 64);
+            }
+            uint32_t at = base0 + incl - cnt;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) {
+                if ((die >> j) & 1u) {
                     if (at < cap)
-                        out[at] = ce;
+                        out[at] = ce[j];
                     else
                         atomicOr(&ctr->overflow, 1u);
+                    ++at;
                 }
             }
         }

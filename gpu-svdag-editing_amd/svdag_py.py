@@ -139,7 +139,8 @@ PROBE_STREAM, PROBE_RANDOM_LINES = 0, 1
 
 def hbm_probe(kind, nbytes, reads=0):
     """Best kernel ms of a stream read of nbytes (PROBE_STREAM) or of `reads`
-    random 128-byte line reads in an nbytes buffer (PROBE_RANDOM_LINES)."""
+    random line-head reads (a 128-byte line's first 32 bytes) in an nbytes
+    buffer (PROBE_RANDOM_LINES)."""
     ms = ctypes.c_float(0.0)
     check(lib().svdag_hbm_probe(ctypes.c_uint32(kind), ctypes.c_uint64(nbytes), ctypes.c_uint64(reads),
                                 ctypes.byref(ms)))

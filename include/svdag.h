@@ -49,7 +49,8 @@ int svdag_device_synchronize(void);
  * No reference counterpart: the practical ceilings the slab-hash find is
  * measured against.  SVDAG_PROBE_STREAM reads `bytes` sequentially with
  * 16-byte loads; SVDAG_PROBE_RANDOM_LINES reads `reads` random 128-byte lines
- * of a `bytes` buffer, 20 bytes of each (the words a find probe loads).
+ * of a `bytes` buffer, their first 20 bytes (the tags and the link: the
+ * words a find probe loads, all in the line's first 32-byte sector).
  * *ms is the best of five launches (HIP events on the probe's own stream).    */
 #define SVDAG_PROBE_STREAM 0u
 #define SVDAG_PROBE_RANDOM_LINES 1u
@@ -210,10 +211,10 @@ int svdag_set_batch_mode(svdag_ctx* ctx, uint32_t mode);
 /* Autotuner (no reference counterpart; the reference fixes warp-cooperative
  * probing at compile time).  Times a sample batch under each lanes-per-item
  * choice for the merge kernels (4, 8, 16, 32) and keeps the fastest for the
- * context.  The sample runs once untimed first so every candidate sees the
- * same tables; the root and the history are restored after each run, so the
- * DAG's contents do not change (items the sample inserted stay until the
- * next garbage collection). */
+ * context.  The sample runs once untimed first; after every run the root is
+ * restored and a garbage collection frees what the sample inserted, so each
+ * timed run inserts its items as a real edit would and the DAG's contents
+ * do not change.  Like svdag_gc, this clears the undo / redo history. */
 typedef struct {
     uint32_t group_size;       /* chosen lanes per item */
     float ms[4];               /* best device ms of the sample for 4, 8, 16, 32 */
