@@ -580,7 +580,11 @@ __global__ void __launch_bounds__(256) merge_insert(WsDev ws, DagDev dag, uint32
         // A found node that lacks the role it is needed in counts as a miss
         // (the reference-count rule in dag_internal.h).  A leaf the edit
         // left unchanged (created 2) is the DAG's own leaf there: plain.
+        // The item the DAG holds at this position already has this position's
+        // role (unchanged leaves are that case too): no role load.
         if (found != kNotFound) {
+            if (found == ws.work[w].w)
+                continue;
             if (isLeaf) {
                 if (ws.created[w] == 2u
                     || sRa.roles[leaf_handle_size(dag, found) - 2][leaf_handle_ptr(dag, found)] == 0u)
@@ -1575,12 +1579,20 @@ int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, sv
     const uint32_t r0 = ctx->root;
     const uint32_t g0 = ctx->groupSize;
     std::vector<svdag_edit_stats> st(n);
+    // Each run starts from the same root and, after a collection, the same
+    // tables: the sample's own items are freed again, so every timed run
+    // inserts them as a real edit would (a run over tables that still hold
+    // them would time finds that hit instead).
+    ctx->undo.clear();
+    ctx->redo.clear();
     auto restore = [&]() -> int {
         root_log(ctx);
         ctx->root = r0;
         SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
         SVDAG_HIP(hipStreamSynchronize(ctx->stream));
-        return SVDAG_OK;
+        svdag_gc_stats gs {};
+        FrameQuiesce q(ctx);
+        return gc_run(ctx, &gs);
     };
     float warm = 0.0f;
     SVDAG_TRY(run_edits(ctx, eds.data(), n, st.data(), &warm));
@@ -1591,7 +1603,7 @@ int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, sv
     for (int c = 0; c < 4; ++c) {
         ctx->groupSize = cand[c];
         float m = 0.0f;
-        for (int rep = 0; rep < 2; ++rep) {
+        for (int rep = 0; rep < 3; ++rep) {
             float t = 0.0f;
             const int rc = run_edits(ctx, eds.data(), n, st.data(), &t);
             if (rc != SVDAG_OK) {
