@@ -449,7 +449,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "
-              "(see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "(the DAG would shard by space across GPUs; not built)", file=sys.stderr)
         sys.exit(2)
     if args.editing_profile:
         editing_profile(args)

@@ -1,8 +1,8 @@
-// Memory-system probes: the practical ceilings the slab-hash find is judged
+// Memory-system probes: the practical ceilings the slab-hash find is measured
 // against.  A stream read gives the sequential HBM rate; a random-line read
-// touches random 128-byte lines the way a find probe does (16 bytes at the
-// line start and the link word at its end), so it gives the rate at which
-// the memory system serves random lines.  No reference counterpart.
+// touches random 128-byte lines the way a find probe does (the tags and the
+// link, in the line's first 32 bytes), so it gives the rate at which the
+// memory system serves random lines.  No reference counterpart.
 #include "svdag_internal.h"
 #include "svdag_device.h"
 #include "svdag.h"
