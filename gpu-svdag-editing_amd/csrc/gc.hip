@@ -25,6 +25,7 @@
 // the slab free stack.
 #include "dag_internal.h"
 #include <algorithm>
+#include <cstdlib>
 
 namespace svdag {
 
@@ -133,12 +134,20 @@ __global__ void __launch_bounds__(256) gc_mark_level(DagDev dag, uint8_t* const*
                                                      GcCounters* ctr)
 {
     __shared__ uint32_t sStage[kGcStage];
-    __shared__ uint32_t sFill, sBase;
+    __shared__ uint32_t sFill, sBase, sStop;
+    // After an overflow the last level's frontier has a hole: the batch
+    // that crossed the end was not written, so nothing more is expanded
+    // (the collection is retried with larger frontiers).  One read per
+    // block: other blocks of this level may set the flag meanwhile.
+    if (threadIdx.x == 0) {
+        sFill = 0u;
+        sStop = ctr->overflow;
+    }
+    __syncthreads();
+    if (sStop)
+        return;
     const uint32_t n = min(ctr->count[L], outCap); // both frontier buffers hold outCap entries
     const bool childLeaf = L + 1 == dag.leafLevel;
-    if (threadIdx.x == 0)
-        sFill = 0u;
-    __syncthreads();
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
         // One pass adds at most 8 entries per thread.
         if (sFill > kGcStage - 8u * blockDim.x)
@@ -410,6 +419,14 @@ __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32
                                                   uint32_t* outCount, uint32_t cap, uint2* touchedList,
                                                   GcCounters* ctr)
 {
+    // Once a list overflowed or a count was found off, the marking
+    // collection takes over: release nothing more.  One read per block.
+    __shared__ uint32_t sStop;
+    if (threadIdx.x == 0)
+        sStop = ctr->overflow;
+    __syncthreads();
+    if (sStop)
+        return;
     const uint32_t n = min(*inCount, cap);
     uint32_t freed = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -645,7 +662,19 @@ static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
 {
     const hipStream_t st = ctx->stream;
     const DagDev& dag = ctx->dag;
+    // A level's frontier holds each node reached at that level once (twice
+    // at most for simultaneous visits, which are rare), so the stored item
+    // count sizes the first attempt; an overflow still retries larger.
+    unsigned long long live = 0;
+    SVDAG_HIP(hipMemcpyAsync(&live, dag.stored, sizeof(live), hipMemcpyDeviceToHost, st));
+    SVDAG_HIP(hipStreamSynchronize(st));
     uint64_t cap = std::max<uint64_t>(ctx->gcCap, 1ull << 22);
+    cap = std::max<uint64_t>(cap, std::min<uint64_t>(live + live / 16 + (1ull << 16), 1ull << 31));
+    // SVDAG_GC_FRONTIER_ENTRIES sets the first attempt's frontier instead
+    // (small values make the retries run on small DAGs, for tests).  At
+    // least 4096: the seed writes the root and one node per material.
+    if (const char* e = std::getenv("SVDAG_GC_FRONTIER_ENTRIES"))
+        cap = std::min<uint64_t>(std::max<uint64_t>(std::strtoull(e, nullptr, 10), 4096), 1ull << 31);
     GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
     const size_t nt = ctx->ts.tables.size();
     uint32_t maxCap = 1, maxBuckets = 1;
@@ -653,7 +682,7 @@ static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
         maxCap = std::max(maxCap, ctx->ts.tables[i].slabCap);
         maxBuckets = std::max(maxBuckets, ctx->ts.tables[i].numBuckets);
     }
-    for (int attempt = 0; attempt < 6; ++attempt) {
+    for (;;) {
         SVDAG_TRY(gc_fronts(ctx, cap));
         // The frontiers are free after the mark; the first one holds the
         // sweep's per-bucket chain bytes, the tables' maps one after another.
@@ -676,8 +705,7 @@ static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
         gc_seed<<<1, 64, 0, st>>>(dag, ctx->dGcMasks, ctx->dRoot, ctx->gcFront[0], ctr);
         for (uint32_t L = 0; L < dag.leafLevel; ++L)
             gc_mark_level<<<4096, 256, 0, st>>>(dag, ctx->dGcMasks, L, ctx->gcFront[L & 1],
-                                                 ctx->gcFront[(L + 1) & 1],
-                                                 (uint32_t)std::min<uint64_t>(ctx->gcCap, 0xFFFFFFFFull), ctr);
+                                                 ctx->gcFront[(L + 1) & 1], (uint32_t)cap, ctr);
         gc_sweep_lines<<<dim3(grid_for(maxCap, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
         gc_sweep_chains<<<dim3(grid_for(maxBuckets, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
         gc_refs_clear<<<dim3(grid_for((uint64_t)maxCap * 4, 256, 1024), (uint32_t)nt), 256, 0, st>>>(a, ctr);
@@ -690,6 +718,8 @@ static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
         if (h.overflow) {
             // The mark ran out of frontier space and nothing after it ran:
             // run again with larger frontiers.
+            if (cap >= (1ull << 31))
+                break;
             cap = std::min<uint64_t>(cap * 2, 1ull << 31);
             continue;
         }

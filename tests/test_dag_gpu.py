@@ -509,6 +509,38 @@ def test_other_material_widths_match_oracle(material_bits):
 
 
 @pytest.mark.gpu
+def test_marking_collection_retries_after_frontier_overflow(monkeypatch):
+    # With the first attempt's frontier capped at 4096 entries the mark
+    # overflows at several levels and reruns with doubled frontiers; a level
+    # after an overflow must expand nothing (its input has an unwritten
+    # hole).  The result must equal an unforced collection's: a marking one
+    # right after frees nothing, and the DAG stays equal to the oracle's.
+    levels = L10
+    edits = _mixed_stream(levels, seed=5)
+    g, o = _pair(levels=levels)
+    g.edit_batch(edits)
+    for e in edits:
+        o.edit(*e)
+    monkeypatch.setenv("SVDAG_GC_FRONTIER_ENTRIES", "4096")
+    forced = g.garbage_collect(mark=True)
+    monkeypatch.delenv("SVDAG_GC_FRONTIER_ENTRIES")
+    assert forced.items_freed > 0
+    assert forced.items_live == _stored_items(g)
+    check = g.garbage_collect(mark=True)
+    assert check.items_freed == 0 and check.items_live == forced.items_live
+    _same(g, o)
+    # Some level's frontier is larger than the cap, so the retries ran.
+    rec = g.export_records()
+    per_level, p = {}, 4
+    for _ in range(int(rec[3])):
+        level, handle, size = int(rec[p]), int(rec[p + 1]), int(rec[p + 2])
+        per_level.setdefault(level, set()).add(handle)
+        p += 3 + size
+    assert max(len(v) for lv, v in per_level.items() if lv < g.levels - 2) > 4096
+    g.close()
+
+
+@pytest.mark.gpu
 def test_reference_count_collection_frees_what_marking_frees():
     # svdag_gc releases the roots held since the last collection and frees
     # what drops to zero; svdag_gc_mark marks from the roots, sweeps and
