@@ -253,6 +253,9 @@ struct svdag_ctx {
     unsigned long long* dStored = nullptr;
     std::vector<uint32_t> rootLog;
     bool refsDirty = false;
+    // Live items after the last marking collection (or the first one) and
+    // after the last collection of either kind.
+    unsigned long long liveAtMark = 0, liveAfterGc = 0;
     uint32_t* dGcCand = nullptr;         // released roots of a collection
     uint32_t gcCandCap = 0;
     uint2* dGcTouchedList = nullptr;     // (table, bucket) pairs for the chain pass

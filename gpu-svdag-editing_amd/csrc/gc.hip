@@ -804,6 +804,15 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
 
 int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
 {
+    svdag_gc_stats local {};
+    if (!stats)
+        stats = &local;
+    // A count that reached kStickyRefs never drops, so its item outlives
+    // its last referrer until a marking collection.  One runs once the
+    // stored items exceed 1.5 times the live count the last marking
+    // collection found, which bounds what such items can hold.
+    if (ctx->liveAtMark != 0 && ctx->liveAfterGc > ctx->liveAtMark + ctx->liveAtMark / 2)
+        mark = true;
     // A collection may free the nodes of the last traced root, so the shadow
     // and occlusion passes need a new primary trace afterwards.
     ctx->traceW = ctx->traceH = 0;
@@ -819,15 +828,18 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
     if (!mark && !ctx->refsDirty) {
         svdag_gc_stats rs {};
         SVDAG_TRY(gc_release(ctx, &rs, done));
-        if (done && stats)
+        if (done)
             *stats = rs;
         releaseMs = rs.gpu_ms;
     }
     if (!done) {
         SVDAG_TRY(gc_mark_sweep(ctx, stats));
-        if (stats)
-            stats->gpu_ms += releaseMs;
+        stats->gpu_ms += releaseMs;
     }
+    stats->marked = done ? 0u : 1u;
+    if (!done || ctx->liveAtMark == 0)
+        ctx->liveAtMark = stats->items_live;
+    ctx->liveAfterGc = stats->items_live;
     ctx->refsDirty = false;
     ctx->rootLog.clear();
     if (ctx->root != kSentinel)

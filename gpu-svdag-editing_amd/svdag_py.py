@@ -56,7 +56,7 @@ class EditStats(ctypes.Structure):
 
 class GcStats(ctypes.Structure):
     _fields_ = [("items_freed", ctypes.c_uint64), ("slabs_freed", ctypes.c_uint64),
-                ("items_live", ctypes.c_uint64), ("gpu_ms", ctypes.c_float)]
+                ("items_live", ctypes.c_uint64), ("gpu_ms", ctypes.c_float), ("marked", ctypes.c_uint32)]
 
 
 class Camera(ctypes.Structure):

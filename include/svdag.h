@@ -237,13 +237,17 @@ int svdag_history(svdag_ctx* ctx, uint32_t out[2]);
  * the last one and frees what drops to zero, in time proportional to what
  * died.  After an edit was rolled back or a dump was imported the counts are
  * rebuilt: that collection marks from the roots, sweeps and recounts, like
- * svdag_gc_mark.  items_live counts the stored items after the collection
- * (all reachable then). */
+ * svdag_gc_mark.  A count that reached its ceiling (64) stays there, so its
+ * item is freed only by a marking collection; svdag_gc marks instead once
+ * the stored items exceed 1.5 times the live count of the last marking
+ * collection.  items_live counts the stored items after the collection
+ * (all reachable after a marking one); marked = 1 if it marked. */
 typedef struct {
     uint64_t items_freed;
     uint64_t slabs_freed;
     uint64_t items_live;
     float gpu_ms;
+    uint32_t marked;
 } svdag_gc_stats;
 int svdag_gc(svdag_ctx* ctx, svdag_gc_stats* stats);
 /* The marking collection itself, whatever the counts say: marks from the

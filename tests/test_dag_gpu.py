@@ -541,6 +541,35 @@ def test_marking_collection_retries_after_frontier_overflow(monkeypatch):
 
 
 @pytest.mark.gpu
+def test_counting_collection_marks_after_growth():
+    # A count that reached its ceiling never drops, so only a marking
+    # collection frees such an item.  Once the stored items exceed 1.5 times
+    # the live count of the last marking collection, svdag_gc marks
+    # (stats.marked), and the next one counts again.  12 sphere adds of new
+    # materials grow the depth-10 terrain's distinct items 1.79x (oracle).
+    g, o = _pair()
+    base = g.garbage_collect()
+    tp = workloads.terrain_params(L10)
+    heights = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                        0, 0, 1024, 1024)
+    rng = np.random.default_rng(3)
+    for i in range(12):
+        x, z = rng.integers(64, 960, 2)
+        e = (svdag_py.SPHERE_ADD, 1 + i % 14, (float(x), float(heights[z, x]), float(z)), float(rng.uniform(50, 90)))
+        g.edit(*e)
+        o.edit(*e)
+    grown = g.garbage_collect()
+    assert grown.marked == 0 and grown.items_live > 1.5 * base.items_live
+    again = g.garbage_collect()
+    assert again.marked == 1 and again.items_live <= grown.items_live
+    assert again.items_live == _stored_items(g)
+    after = g.garbage_collect()
+    assert after.marked == 0 and after.items_freed == 0 and after.items_live == again.items_live
+    _same(g, o)
+    g.close()
+
+
+@pytest.mark.gpu
 def test_reference_count_collection_frees_what_marking_frees():
     # svdag_gc releases the roots held since the last collection and frees
     # what drops to zero; svdag_gc_mark marks from the roots, sweeps and
@@ -568,11 +597,11 @@ def test_reference_count_collection_frees_what_marking_frees():
             g.redo()
             g.redo()
         st = g.garbage_collect()
-        assert st.items_freed > 0
+        assert st.items_freed > 0 and st.marked == 0
         assert st.items_live == _stored_items(g)
         _same(g, o)
         check = g.garbage_collect(mark=True)
-        assert check.items_freed == 0, (step, check.items_freed)
+        assert check.items_freed == 0 and check.marked == 1, (step, check.items_freed)
         assert check.items_live == st.items_live, step
     # An abandoned undo branch: its roots die at the next collection.
     g.edit(svdag_py.SPHERE_ADD, 3, (300.0, 520.0, 300.0), 40.0)
