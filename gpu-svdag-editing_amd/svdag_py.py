@@ -5,6 +5,7 @@ there is no fallback path.  The classes mirror the reference's interface
 names (MyGPUHashDAG, MyGPUHashDAGUndoRedo, DAGTracer) over the C ABI.
 """
 import ctypes
+import struct
 import os
 
 import numpy as np
@@ -47,6 +48,10 @@ class Terrain(ctypes.Structure):
 class EditDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("material", ctypes.c_uint32), ("center", ctypes.c_float * 3),
                 ("radius", ctypes.c_float), ("src", ctypes.c_float * 3), ("dst", ctypes.c_float * 3)]
+
+
+_EDIT_PACK = struct.Struct("<II3ff3f3f")
+assert _EDIT_PACK.size == ctypes.sizeof(EditDesc)
 
 
 class EditStats(ctypes.Structure):
@@ -302,14 +307,17 @@ class MyGPUHashDAG:
 
     @staticmethod
     def _descs(edits):
-        descs = (EditDesc * len(edits))()
+        # Packed with struct (a third of the time of building each field as a
+        # ctypes object); kind and material wrap to uint32 as ctypes would.
+        n = len(edits)
+        buf = bytearray(_EDIT_PACK.size * n)
         for i, e in enumerate(edits):
-            kind, mat, c, r = e[:4]
-            src = e[4] if len(e) > 4 else (0, 0, 0)
-            dst = e[5] if len(e) > 5 else (0, 0, 0)
-            descs[i] = EditDesc(kind, mat, (ctypes.c_float * 3)(*c), float(r), (ctypes.c_float * 3)(*src),
-                                (ctypes.c_float * 3)(*dst))
-        return descs
+            c = e[2]
+            src = e[4] if len(e) > 4 else (0.0, 0.0, 0.0)
+            dst = e[5] if len(e) > 5 else (0.0, 0.0, 0.0)
+            _EDIT_PACK.pack_into(buf, i * _EDIT_PACK.size, int(e[0]) & 0xFFFFFFFF, int(e[1]) & 0xFFFFFFFF,
+                                 c[0], c[1], c[2], e[3], src[0], src[1], src[2], dst[0], dst[1], dst[2])
+        return (EditDesc * n).from_buffer(buf)
 
     @staticmethod
     def _camera(position, rotation, fov, bmin, bmax):
