@@ -330,6 +330,75 @@ def editing_bench(args):
             "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"]}
 
 
+def large_scene_bench(args):
+    """Depth-17 scene with 8-bit materials: the fBm terrain of terrain_params(17)
+    (base 65536, amplitude 24000, 8 octaves) with height-band materials (bands
+    of 512 voxels plus a seeded 0..3 offset per 512-column tile), built one
+    16384-wide column tile per edit.  Then `large_copies` copy/paste edits, each
+    a 512^3 region (centre = source = a seeded point near the surface, radius
+    256) pasted at a seeded point near the surface elsewhere, in groups of 8;
+    after each group the last 8 edits are undone and redone.  Translation
+    only: the reference's copy tool has no rotation (my_gpu_dag_editors.h:127-253).
+    A collection ends the stream.  Tables start sized for `large_expected`
+    items and grow as the allocators run short."""
+    import svdag_py
+    import workloads
+
+    D, M = args.large_levels, 8
+    side = 1 << D
+    tp = dict(workloads.terrain_params(D))
+    t0 = time.perf_counter()
+    g = svdag_py.MyGPUHashDAG(D, M, expected_items=int(args.large_expected), workspace_items=1 << 26)
+    g.build_terrain(**tp, band_log2=9, band_seed=0xBA5E, tile_log2=D - 3)
+    build_s = time.perf_counter() - t0
+    gc0 = g.garbage_collect()
+    log(f"[bench] depth {D} banded terrain: {build_s:.2f} s, {gc0.items_live} live items")
+    rng = np.random.default_rng(10)
+
+    def near_surface():
+        x, z = (int(v) for v in rng.integers(256, side - 256, 2))
+        y = int(g.terrain_heights(x, z, 1, 1)[0, 0]) + int(rng.integers(-256, 257))
+        return (float(x), float(min(max(y, 256), side - 256)), float(z))
+
+    copies = [(near_surface(), near_surface()) for _ in range(args.large_copies)]
+    lat, undo_ms, redo_ms = [], [], []
+    roots = []
+    t1 = time.perf_counter()
+    for i, (src, dst) in enumerate(copies):
+        s0 = time.perf_counter()
+        g.edit(svdag_py.COPY, 0, src, 256.0, src, dst)
+        lat.append(time.perf_counter() - s0)
+        roots.append(g.root)
+        if i % 8 == 7:
+            for _ in range(8):
+                s0 = time.perf_counter()
+                g.undo()
+                undo_ms.append((time.perf_counter() - s0) * 1e3)
+            for _ in range(8):
+                s0 = time.perf_counter()
+                g.redo()
+                redo_ms.append((time.perf_counter() - s0) * 1e3)
+            if g.root != roots[-1]:
+                raise RuntimeError("redo did not restore the root")
+    gc = g.garbage_collect()
+    dt = time.perf_counter() - t1
+    mem = g.memory_usage()
+    g.close()
+    vox = float(len(copies)) * 512.0 ** 3
+    out = {"levels": D, "material_bits": M, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
+           "copies": len(copies), "region": "512^3", "copies_per_s": len(copies) / dt,
+           "voxel_edits_per_s": vox / dt,
+           "copy_latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+           "copy_latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
+           "undo_ms_p99": float(np.percentile(undo_ms, 99)) if undo_ms else None,
+           "redo_ms_p99": float(np.percentile(redo_ms, 99)) if redo_ms else None,
+           "gc_gpu_ms": round(gc.gpu_ms, 3), "live_items_after": gc.items_live,
+           "table_bytes_reserved": int(mem.bytes_reserved), "slabs_used": int(mem.slabs_used),
+           "slab_capacity": int(mem.slab_capacity)}
+    log(f"[bench] large scene: {out}")
+    return out
+
+
 def editing_profile(args):
     """The batched-editing stream alone (depth 15 terrain, 256 edits in
     16-edit fused batches, one collection after the stream), for a kernel
@@ -444,6 +513,11 @@ def main():
     ap.add_argument("--no-editing", action="store_true", help="slab-hash line only")
     ap.add_argument("--no-probe", action="store_true", help="skip the stream / random-line memory probes")
     ap.add_argument("--no-shading", action="store_true", help="skip the shadow / occlusion timing")
+    ap.add_argument("--large-scene", action="store_true", help="add the depth-17 8-bit-material copy/paste leg")
+    ap.add_argument("--large-scene-only", action="store_true", help="only the depth-17 copy/paste leg")
+    ap.add_argument("--large-copies", type=int, default=32)
+    ap.add_argument("--large-levels", type=int, default=17)
+    ap.add_argument("--large-expected", type=float, default=4e8, help="items the depth-17 tables start sized for")
     ap.add_argument("--editing-profile", action="store_true",
                     help="only the batched-editing stream and its collection (for rocprofv3 --kernel-trace)")
     args = ap.parse_args()
@@ -454,6 +528,9 @@ def main():
     if args.editing_profile:
         editing_profile(args)
         return
+    if args.large_scene_only:
+        print(json.dumps({"large_scene": large_scene_bench(args)}), flush=True)
+        return
     out = slab_hash_bench(args)
     if out is None:
         return
@@ -461,6 +538,8 @@ def main():
         out["editing"] = editing_bench(args)
         if not args.no_cpu_baseline:
             out["editing"]["cpu_baseline"] = cpu_edit_baseline(args)
+    if args.large_scene:
+        out["large_scene"] = large_scene_bench(args)
     print(json.dumps(out), flush=True)
 
 

@@ -314,9 +314,11 @@ __global__ void __launch_bounds__(256) svo_leaf_level(WsDev ws, DagDev dag, cons
                 const Op op = decide(ed, vx, vy, vz, 0, hasOld ? NS_FILLED : NS_EMPTY, fm);
                 bool has = false;
                 uint32_t mat = 0;
-                if (ed.kind == EK_DENSE) {
-                    has = op == OP_FILL; // a set voxel of the grid, with its own material
-                    mat = fm;
+                if (ed.kind == EK_DENSE || ed.kind == EK_TERRAIN) {
+                    // a set voxel of the grid or below the surface, with its own material;
+                    // a tiled terrain keeps voxels outside its tile
+                    has = op == OP_FILL || (op == OP_KEEP && hasOld);
+                    mat = op == OP_FILL ? fm : oldMat;
                 } else if (op == OP_FILL || op == OP_SUBDIVIDE) {
                     if (ed.kind == EK_COPY)
                         has = copy_value(ed, dag, vx, vy, vz, ws.state[w], hasOld, oldMat, mat);
@@ -1027,6 +1029,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
     uint32_t first = 0;
     std::vector<uint8_t> alone(n, 0);
     std::vector<BatchStep> steps;
+    std::string why = "none";
     for (int attempt = 0; attempt < 16 && first < n; ++attempt) {
         bool needState = false;
         for (uint32_t k = first; k < n; ++k) {
@@ -1112,20 +1115,25 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
         if (s.fallback) {
             alone[bad] = 1;
+            why = "fused spine fallback";
         } else if (s.overflow) {
             uint64_t need = 0;
             for (uint32_t L = 0; L <= ctx->dag.leafLevel; ++L) {
                 const LevelDesc& d = hLv[(size_t)bad * kMaxLevels + L];
                 need = std::max<uint64_t>(need, (uint64_t)d.start + d.count);
             }
+            why = "work items: need " + std::to_string(need) + ", capacity " + std::to_string(ctx->ws.capWork);
             SVDAG_TRY(ws_alloc(ctx, std::max<uint64_t>(2ull * ctx->ws.capWork, need + need / 4), ctx->ws.state != nullptr));
         } else if (s.oom) {
             FrameQuiesce q(ctx);
             for (auto& t : ctx->ts.tables) {
                 AllocState a {};
                 SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
-                if (a.oom)
+                if (a.oom) {
+                    why = "table of " + std::to_string(t.itemWords) + "-word items full at " + std::to_string(t.slabCap)
+                        + " lines (limit " + std::to_string(ctx->slabLimit) + ")";
                     SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit), ctx->stream));
+                }
             }
             SVDAG_TRY(ctx_refresh_tables(ctx));
         } else {
@@ -1135,7 +1143,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         first = bad;
     }
     if (first < n) {
-        set_error("edit did not fit after repeated workspace / arena growth");
+        set_error("edit did not fit after repeated workspace / arena growth (last: " + why + ")");
         return SVDAG_E_CAPACITY;
     }
     if (totalMs)
@@ -1445,7 +1453,14 @@ int svdag_build_voxels(svdag_ctx* ctx, const uint8_t* materials, uint32_t sideLo
     return rc;
 }
 
-int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)
+// Terrain through the edit pipeline: a min/max height pyramid classifies the
+// nodes (editors.h, EK_TERRAIN).  Below 2^tileLog2 < 2^D the build runs as one
+// edit per column tile of that width, so that one edit's work items fit the
+// workspace at depth 17; the canonical DAG is the same as long as no
+// fully-filled node is wider than a tile (the merge only flags nodes whose
+// children sit in the edit that rebuilt them).
+static int build_terrain(svdag_ctx* ctx, const svdag_terrain* tp, bool banded, uint32_t bandLog2, uint32_t bandSeed,
+                         uint32_t tileLog2)
 {
     if (!ctx || !tp || tp->octaves == 0 || tp->octaves > 16 || !std::isfinite(tp->base) || !std::isfinite(tp->amplitude)
         || !std::isfinite(tp->frequency)) {
@@ -1458,6 +1473,16 @@ int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     const uint32_t D = ctx->dag.levels;
+    if (banded && (bandLog2 < 2 || bandLog2 > D)) {
+        set_error("terrain band_log2 must lie in [2, levels]");
+        return SVDAG_E_INVALID;
+    }
+    if (tileLog2 == 0)
+        tileLog2 = D > 15 ? D - 2 : D;
+    if (tileLog2 < 2 || tileLog2 > D) {
+        set_error("terrain tile_log2 must lie in [2, levels]");
+        return SVDAG_E_INVALID;
+    }
     const uint32_t side = 1u << D;
     // Pyramid: level k holds (side >> k)^2 cells.
     uint64_t off = 0;
@@ -1472,16 +1497,18 @@ int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)
     if (ctx->hmax)
         (void)hipFree(ctx->hmax);
     ctx->hmin = ctx->hmax = nullptr;
+    // Maxima start at level 1 (level 0's are the heights themselves): at depth
+    // 17 that saves the 64 GiB a second copy of the heights would take.
+    const uint64_t base1 = ctx->pyrOffset[1];
     SVDAG_HIP(hipMalloc(&ctx->hmin, off * 4));
-    SVDAG_HIP(hipMalloc(&ctx->hmax, off * 4));
+    SVDAG_HIP(hipMalloc(&ctx->hmax, std::max<uint64_t>(off - base1, 1) * 4));
     PerlinDev perlin;
     perlin_perm(tp->seed, perlin.perm);
     terrain_heights_kernel<<<grid_for((uint64_t)side * side, 256, 1u << 16), 256, 0, ctx->stream>>>(perlin, *tp, side, ctx->hmin);
-    SVDAG_HIP(hipMemcpyAsync(ctx->hmax, ctx->hmin, (uint64_t)side * side * 4, hipMemcpyDeviceToDevice, ctx->stream));
     for (uint32_t k = 1; k <= D; ++k)
         pyramid_level_kernel<<<grid_for((uint64_t)(side >> k) * (side >> k), 256, 1u << 16), 256, 0, ctx->stream>>>(
-            ctx->hmin + ctx->pyrOffset[k - 1], ctx->hmax + ctx->pyrOffset[k - 1], side >> (k - 1),
-            ctx->hmin + ctx->pyrOffset[k], ctx->hmax + ctx->pyrOffset[k]);
+            ctx->hmin + ctx->pyrOffset[k - 1], k == 1 ? ctx->hmin : ctx->hmax + (ctx->pyrOffset[k - 1] - base1),
+            side >> (k - 1), ctx->hmin + ctx->pyrOffset[k], ctx->hmax + (ctx->pyrOffset[k] - base1));
     SVDAG_HIP(hipGetLastError());
     EditorDev e {};
     e.kind = EK_TERRAIN;
@@ -1491,15 +1518,40 @@ int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp)
     e.hmax = ctx->hmax;
     for (uint32_t k = 0; k < kMaxLevels; ++k)
         e.pyrOffset[k] = ctx->pyrOffset[k];
+    e.banded = banded ? 1u : 0u;
+    e.bandLog2 = bandLog2;
+    e.bandSeed = bandSeed;
+    e.materialMask = ctx->dag.numMaterials - 1u;
     // The terrain replaces whatever the DAG held: build from the empty root.
     root_log(ctx);
     ctx->root = kSentinel;
     SVDAG_HIP(hipMemsetAsync(ctx->dRoot, 0xFF, 4, ctx->stream));
     svdag_edit_stats stats;
-    SVDAG_TRY(run_edit(ctx, e, &stats));
+    if (tileLog2 >= D) {
+        SVDAG_TRY(run_edit(ctx, e, &stats));
+    } else {
+        const uint32_t tiles = 1u << (D - tileLog2), w = 1u << tileLog2;
+        e.tiled = 1;
+        for (uint32_t tz = 0; tz < tiles; ++tz)
+            for (uint32_t tx = 0; tx < tiles; ++tx) {
+                e.tileMin[0] = tx * w;
+                e.tileMax[0] = tx * w + w;
+                e.tileMin[1] = tz * w;
+                e.tileMax[1] = tz * w + w;
+                SVDAG_TRY(run_edit(ctx, e, &stats));
+            }
+    }
     ctx->undo.clear();
     ctx->redo.clear();
     return SVDAG_OK;
+}
+
+int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* tp) { return build_terrain(ctx, tp, false, 0, 0, 0); }
+
+int svdag_build_terrain_banded(svdag_ctx* ctx, const svdag_terrain* tp, uint32_t band_log2, uint32_t band_seed,
+                               uint32_t tile_log2)
+{
+    return build_terrain(ctx, tp, true, band_log2, band_seed, tile_log2);
 }
 
 int svdag_terrain_heights(svdag_ctx* ctx, uint32_t x0, uint32_t z0, uint32_t w, uint32_t d, int32_t* out)

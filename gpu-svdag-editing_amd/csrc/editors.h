@@ -46,11 +46,20 @@ struct EditorDev {
     // copy
     float d2s[3];
     int32_t id2s[3];
-    // terrain: height pyramid, level k holds (2^D >> k)^2 cells, row-major in z
+    // terrain: height pyramid, level k holds (2^D >> k)^2 cells, row-major in z.
+    // hmax holds levels 1..D only (level 0's maxima are hmin's heights) and is
+    // indexed by pyrOffset[k] - pyrOffset[1].
     uint32_t levels;
     const int32_t* hmin;
     const int32_t* hmax;
     uint64_t pyrOffset[kMaxLevels];
+    // terrain height bands: cubes of 2^bandLog2 voxels hold one material
+    // (band_material); banded == 0 gives every voxel `material`
+    uint32_t banded, bandLog2, bandSeed, materialMask;
+    // terrain tile [tileMin, tileMax) in x and z: nodes outside keep their
+    // content, nodes straddling the edge subdivide (tiled == 0: no tile)
+    uint32_t tiled;
+    uint32_t tileMin[2], tileMax[2];
     // dense grid of side 2^denseLog2 at the origin: pyramid level k holds
     // (side >> k)^3 codes indexed (x * s + y) * s + z (pyrOffset as above)
     const uint16_t* dense;
@@ -122,6 +131,21 @@ __host__ __device__ inline void node_box(uint32_t x, uint32_t y, uint32_t z, uin
     mx[2] = mn[2] + size;
 }
 
+// Terrain material of voxel (x, y, z): with bands, the band index y >> b plus
+// a seeded 2-bit offset per 2^b column tile, over the material range.
+// No reference counterpart (a synthetic scene's material rule).
+__host__ __device__ inline uint32_t band_material(const EditorDev& e, uint32_t x, uint32_t y, uint32_t z)
+{
+    if (!e.banded)
+        return e.material;
+    const uint32_t b = e.bandLog2;
+    uint32_t h = e.bandSeed ^ ((x >> b) * 0x9E3779B1u) ^ ((z >> b) * 0x85EBCA77u);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return (e.material + (y >> b) + (h & 3u)) & e.materialMask;
+}
+
 // should_edit for the stateless tools (and the copy tool's voxel level).
 __device__ inline Op decide(const EditorDev& e, uint32_t x, uint32_t y, uint32_t z, uint32_t depth, NState st,
                             uint32_t& fillMat)
@@ -181,13 +205,26 @@ __device__ inline Op decide(const EditorDev& e, uint32_t x, uint32_t y, uint32_t
         return OP_FILL;
     }
     case EK_TERRAIN: {
+        if (e.tiled) {
+            const uint64_t x0 = (uint64_t)x << depth, z0 = (uint64_t)z << depth, w = (uint64_t)1 << depth;
+            if (x0 + w <= e.tileMin[0] || x0 >= e.tileMax[0] || z0 + w <= e.tileMin[1] || z0 >= e.tileMax[1])
+                return OP_KEEP;
+            if (x0 < e.tileMin[0] || x0 + w > e.tileMax[0] || z0 < e.tileMin[1] || z0 + w > e.tileMax[1])
+                return OP_SUBDIVIDE;
+        }
         const uint64_t width = (uint64_t)1 << (e.levels - depth);
         const uint64_t idx = e.pyrOffset[depth] + (uint64_t)z * width + x;
-        const int64_t lo = e.hmin[idx], hi = e.hmax[idx];
+        const int64_t lo = e.hmin[idx];
+        const int64_t hi = depth == 0 ? lo : e.hmax[idx - e.pyrOffset[1]];
         const int64_t y0 = (int64_t)y << depth;
         const int64_t y1 = y0 + ((int64_t)1 << depth) - 1;
-        if (y1 <= lo)
+        if (y1 <= lo) {
+            // A solid node wider than a band cube spans several materials.
+            if (e.banded && depth > e.bandLog2)
+                return OP_SUBDIVIDE;
+            fillMat = band_material(e, x << depth, y << depth, z << depth);
             return OP_FILL;
+        }
         if (y0 > hi)
             return OP_EMPTY;
         return OP_SUBDIVIDE;

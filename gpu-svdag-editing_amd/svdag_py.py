@@ -105,7 +105,7 @@ EXPORTED = [
     "svdag_memcpy_h2d", "svdag_memcpy_d2h", "svdag_memcpy_d2d", "svdag_device_synchronize",
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
-    "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_terrain_heights", "svdag_root",
+    "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_build_terrain_banded", "svdag_terrain_heights", "svdag_root",
     "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
@@ -283,9 +283,17 @@ class MyGPUHashDAG:
             raise ValueError("vox must be a cube of side 2^k")
         check(lib().svdag_build_voxels(self.h, vox.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(side.bit_length() - 1)))
 
-    def build_terrain(self, seed, base, amplitude, frequency, octaves, material):
+    def build_terrain(self, seed, base, amplitude, frequency, octaves, material, band_log2=0, band_seed=0,
+                      tile_log2=0):
+        """band_log2 > 0 selects height-band materials (svdag_build_terrain_banded)."""
         t = Terrain(seed, base, amplitude, frequency, octaves, material)
-        check(lib().svdag_build_terrain(self.h, ctypes.byref(t)))
+        if band_log2:
+            check(lib().svdag_build_terrain_banded(self.h, ctypes.byref(t), ctypes.c_uint32(band_log2),
+                                                   ctypes.c_uint32(band_seed), ctypes.c_uint32(tile_log2)))
+        elif tile_log2:
+            raise ValueError("tile_log2 needs band_log2 (svdag_build_terrain tiles by itself)")
+        else:
+            check(lib().svdag_build_terrain(self.h, ctypes.byref(t)))
 
     def terrain_heights(self, x0, z0, w, d):
         out = np.empty((d, w), np.int32)

@@ -136,6 +136,18 @@ typedef struct {
     uint32_t material;
 } svdag_terrain;
 int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* terrain);
+/* The same terrain with height-band materials (no reference counterpart: a
+ * synthetic scene with one material per cube of 2^band_log2 voxels, for the
+ * 8-bit-material depth-17 workload).  Voxel (x,y,z) below the surface holds
+ *   (material + (y >> b) + (mix32(band_seed ^ (x>>b)*0x9E3779B1 ^ (z>>b)*0x85EBCA77) & 3))
+ *   & (2^material_bits - 1),   b = band_log2 in [2, levels],
+ * mix32(h) = ((h ^ h>>15) * 0x2C1B3C6D) ^ (that >> 12).  The build runs one edit
+ * per column tile of 2^tile_log2 voxels (0 = levels-2 above depth 15, else one
+ * edit), which keeps one edit's work items within the workspace; the DAG is the
+ * same as a single-edit build when no fully-filled node is wider than a tile.
+ * svdag_build_terrain tiles the same way.                                   */
+int svdag_build_terrain_banded(svdag_ctx* ctx, const svdag_terrain* terrain, uint32_t band_log2, uint32_t band_seed,
+                               uint32_t tile_log2);
 /* Heights of a rectangle [x0,x0+w) x [z0,z0+d), row-major in z (out: w*d int32). */
 int svdag_terrain_heights(svdag_ctx* ctx, uint32_t x0, uint32_t z0, uint32_t w, uint32_t d, int32_t* out);
 

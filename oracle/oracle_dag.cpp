@@ -263,7 +263,24 @@ struct Built {
     uint32_t mat;
 };
 
-Built build_terrain(Dag& d, const Terrain& t, uint32_t mat, uint32_t level, uint32_t px, uint32_t py, uint32_t pz)
+// Height-band material rule of svdag_build_terrain_banded (a synthetic scene's
+// rule with no reference counterpart): band y >> b plus a seeded 2-bit offset
+// per 2^b column tile, masked to the material range.
+struct Bands {
+    bool on;
+    uint32_t b, seed, material, mask;
+    uint32_t at(uint32_t x, uint32_t y, uint32_t z) const
+    {
+        if (!on)
+            return material;
+        uint32_t h = seed ^ ((x >> b) * 0x9E3779B1u) ^ ((z >> b) * 0x85EBCA77u);
+        h = (h ^ (h >> 15)) * 0x2C1B3C6Du;
+        h ^= h >> 12;
+        return (material + (y >> b) + (h & 3u)) & mask;
+    }
+};
+
+Built build_terrain(Dag& d, const Terrain& t, const Bands& bands, uint32_t level, uint32_t px, uint32_t py, uint32_t pz)
 {
     const uint32_t depth = d.levels - level;
     const uint32_t S = 1u << depth;
@@ -277,8 +294,12 @@ Built build_terrain(Dag& d, const Terrain& t, uint32_t mat, uint32_t level, uint
         }
     if (y0 > hi)
         return { SENT, false, 0 };
-    if (y1 <= lo)
+    // A solid region within one band cube is that material's full node; a
+    // wider one is built from its children (the factory rule decides).
+    if (y1 <= lo && !(bands.on && depth > bands.b)) {
+        const uint32_t mat = bands.at(x0, (uint32_t)y0, z0);
         return { d.ff[level * d.numMat + mat], true, mat };
+    }
     if (level == d.leafLevel) {
         LeafBuilder b(d.M);
         for (uint32_t v = 0; v < 64; ++v) {
@@ -286,7 +307,7 @@ Built build_terrain(Dag& d, const Terrain& t, uint32_t mat, uint32_t level, uint
             const uint32_t y = (py << 2) | (((v >> 4) & 1u) << 1) | ((v >> 1) & 1u);
             const uint32_t z = (pz << 2) | (((v >> 3) & 1u) << 1) | (v & 1u);
             if ((int64_t)y <= t.at(x, z))
-                b.set(mat);
+                b.set(bands.at(x, y, z));
             b.next();
         }
         const auto w = b.finish();
@@ -301,7 +322,7 @@ Built build_terrain(Dag& d, const Terrain& t, uint32_t mat, uint32_t level, uint
     bool allFull = true;
     uint32_t fm = SENT;
     for (uint32_t c = 0; c < 8; ++c) {
-        const Built b = build_terrain(d, t, mat, level + 1, (px << 1) | ((c >> 2) & 1u), (py << 1) | ((c >> 1) & 1u),
+        const Built b = build_terrain(d, t, bands, level + 1, (px << 1) | ((c >> 2) & 1u), (py << 1) | ((c >> 1) & 1u),
                                       (pz << 1) | (c & 1u));
         if (b.handle == SENT) {
             allFull = false;
@@ -1109,7 +1130,9 @@ void ora_terrain_heights(uint64_t seed, double base, double amp, double freq, ui
             out[(size_t)z * w + x] = height_at(pn, t, x0 + x, z0 + z);
 }
 
-int ora_dag_build_terrain(void* dp, uint64_t seed, double base, double amp, double freq, uint32_t octaves, uint32_t material)
+// band_log2 == 0: one material; else the band rule of svdag_build_terrain_banded.
+int ora_dag_build_terrain(void* dp, uint64_t seed, double base, double amp, double freq, uint32_t octaves, uint32_t material,
+                          uint32_t band_log2, uint32_t band_seed)
 {
     Dag& d = *static_cast<Dag*>(dp);
     const Perlin pn(seed);
@@ -1120,7 +1143,8 @@ int ora_dag_build_terrain(void* dp, uint64_t seed, double base, double amp, doub
     for (uint32_t z = 0; z < t.side; ++z)
         for (uint32_t x = 0; x < t.side; ++x)
             t.h[(size_t)z * t.side + x] = height_at(pn, spec, x, z);
-    d.root = build_terrain(d, t, material, 0, 0, 0, 0).handle;
+    const Bands bands { band_log2 != 0, band_log2, band_seed, material, d.numMat - 1u };
+    d.root = build_terrain(d, t, bands, 0, 0, 0, 0).handle;
     return 0;
 }
 

@@ -99,7 +99,7 @@ def _dag_sigs(L):
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]
     L.ora_dag_build_terrain.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                        ctypes.c_uint32, ctypes.c_uint32]
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
     L.ora_dag_edit.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, f32p, ctypes.c_float, f32p, f32p, u32p]
     L.ora_dag_query.argtypes = [vp, u32p, ctypes.c_uint32, u32p]
     L.ora_dag_canonical.restype = ctypes.c_uint64
@@ -175,8 +175,8 @@ class OracleDag:
         if not self.h:
             raise ValueError("bad oracle DAG parameters")
 
-    def build_terrain(self, seed, base, amp, freq, octaves, material):
-        dlib().ora_dag_build_terrain(self.h, seed, base, amp, freq, octaves, material)
+    def build_terrain(self, seed, base, amp, freq, octaves, material, band_log2=0, band_seed=0):
+        dlib().ora_dag_build_terrain(self.h, seed, base, amp, freq, octaves, material, band_log2, band_seed)
 
     @property
     def root(self):

@@ -107,6 +107,39 @@ def test_surface_edit_stream_all_tools():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bits,band,tile", [(8, 5, 0), (8, 5, 7), (4, 3, 8)])
+def test_banded_terrain_and_copies_match_oracle(bits, band, tile):
+    # Height-band materials, built in one edit (tile 0) or one edit per
+    # column tile, then copy/paste edits with undo/redo, against the
+    # oracle's factory rule and copy restatement.
+    print(f"[case] banded terrain bits={bits} band={band} tile={tile}", flush=True)
+    g = svdag_py.MyGPUHashDAG(L10, bits)
+    o = oracle_py.OracleDag(L10, bits)
+    tp = dict(workloads.terrain_params(L10))
+    tp["material"] = 7 % (1 << bits)
+    g.build_terrain(**tp, band_log2=band, band_seed=0xBA5E, tile_log2=tile)
+    o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"],
+                    band, 0xBA5E)
+    _same(g, o)
+    hashes = [g.canonical()[0]]
+    for src, dst, c, r in [((300.0, 500.0, 300.0), (600.0, 520.0, 650.0), (310.0, 505.0, 305.0), 40.0),
+                           ((100.0, 470.0, 700.0), (140.0, 430.0, 690.0), (100.0, 480.0, 700.0), 31.0),
+                           ((512.0, 400.0, 512.0), (256.0, 600.0, 768.0), (512.0, 420.0, 512.0), 64.0)]:
+        st = g.edit(svdag_py.COPY, 0, c, r, src, dst)
+        applied, counts = o.edit(svdag_py.COPY, 0, c, r, src, dst)
+        assert st.applied == applied
+        assert list(st.svo_items)[:L10 - 1] == counts[:L10 - 1]
+        _same(g, o)
+        hashes.append(g.canonical()[0])
+    g.undo()
+    g.undo()
+    assert g.canonical()[0] == hashes[1]
+    g.redo()
+    assert g.canonical()[0] == hashes[2]
+    g.close()
+
+
+@pytest.mark.gpu
 def test_voxel_queries_match_oracle():
     g, o = _pair()
     for kind, mat, c, r in workloads.config1_edits():

@@ -330,3 +330,28 @@ def test_tracer_hits_first_set_voxel():
                 assert edge < 1e-3, (px, py, got, ref)
                 ambiguous += 1
     assert agree >= 0.98 * W * H
+
+
+def test_banded_terrain_voxels_follow_heights_and_bands():
+    # The factory-rule DAG over height bands holds exactly the voxel field:
+    # set iff y <= h(x, z), material by band and column tile.
+    L, bits, band, seed = 8, 8, 4, 0xBA5E
+    side = 1 << L
+    base, amp, freq, octaves, mat = 128.0, 50.0, 1 / 64, 5, 7
+    d = oracle_py.OracleDag(L, bits)
+    d.build_terrain(0x5EED, base, amp, freq, octaves, mat, band, seed)
+    h = oracle_py.terrain_heights(0x5EED, base, amp, freq, octaves, 0, 0, side, side)
+    rng = np.random.default_rng(11)
+    xz = rng.integers(0, side, size=(400, 2))
+    pts = np.array([(x, y, z) for x, z in xz for y in range(side)], np.uint32)
+    got = d.query(pts)
+    x, y, z = (pts[:, i].astype(np.uint64) for i in range(3))
+    t = (np.uint64(seed) ^ (((x >> band) * 0x9E3779B1) & 0xFFFFFFFF) ^ (((z >> band) * 0x85EBCA77) & 0xFFFFFFFF))
+    t = ((t ^ (t >> 15)) * 0x2C1B3C6D) & 0xFFFFFFFF
+    t ^= t >> 12
+    want_mat = (mat + (y >> band) + (t & 3)) & ((1 << bits) - 1)
+    solid = y.astype(np.int64) <= h[z.astype(np.int64), x.astype(np.int64)]
+    want = np.where(solid, want_mat, 0xFFFFFFFF).astype(np.uint32)
+    assert solid.any() and (~solid).any()
+    assert len(np.unique(want_mat[solid])) > 4
+    assert np.array_equal(got, want)
