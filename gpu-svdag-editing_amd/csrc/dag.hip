@@ -38,9 +38,12 @@ __device__ inline uint32_t dedup_cap(uint32_t count, uint32_t maxCap)
     return cap;
 }
 
+// The reference drops an edit whose SVO has no leaf (my_gpu_hash_dag_edits.cu:651).
+// A build is not an edit: a grid or terrain made only of fully-filled nodes
+// has no leaf either, and its merged root is the DAG.
 __device__ inline bool merge_skipped(const WsDev& ws, const DagDev& dag)
 {
-    return ws.status->overflow != 0 || ws.lvl[dag.leafLevel].count == 0;
+    return ws.status->overflow != 0 || (ws.lvl[dag.leafLevel].count == 0 && !ws.status->build);
 }
 
 // ---------------------------------------------------------------- fully-filled chain
@@ -99,7 +102,7 @@ __global__ void init_fully_filled(DagDev dag)
 }
 
 // ---------------------------------------------------------------- edit kernels
-__global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy)
+__global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy, uint32_t isBuild)
 {
     const uint32_t t = threadIdx.x;
     if (t < kMaxLevels) {
@@ -113,6 +116,7 @@ __global__ void edit_begin(WsDev ws, const uint32_t* root, uint32_t isCopy)
         EditStatus st {};
         st.rootBefore = r;
         st.rootAfter = r;
+        st.build = isBuild;
         *ws.status = st;
         ws.work[0] = make_uint4(0u, 0u, 0u, r);
         if (isCopy) {
@@ -638,7 +642,7 @@ __global__ void edit_finish(WsDev ws, DagDev dag, uint32_t* root)
         return;
     EditStatus* st = ws.status;
     st->leafCount = ws.lvl[dag.leafLevel].count;
-    if (st->overflow || st->leafCount == 0 || st->oom) {
+    if (st->overflow || (st->leafCount == 0 && !st->build) || st->oom) {
         st->done = st->overflow == 0 && st->oom == 0;
         st->applied = 0;
         st->rootAfter = st->rootBefore;
@@ -747,6 +751,19 @@ __global__ void pyramid_level_kernel(const int32_t* srcMin, const int32_t* srcMa
         dstMin[i] = min(min(srcMin[a], srcMin[a + 1]), min(srcMin[b], srcMin[b + 1]));
         dstMax[i] = max(max(srcMax[a], srcMax[a + 1]), max(srcMax[b], srcMax[b + 1]));
     }
+}
+
+// Largest min-height of a pyramid level: a node of width 2^k at y = 0 is
+// solid iff its column block's minimum height is at least 2^k - 1.
+__global__ void level_max_kernel(const int32_t* lvl, uint64_t n, int32_t* out)
+{
+    int32_t m = INT32_MIN;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, lvl[i]);
+    for (int o = 32; o >= 1; o >>= 1)
+        m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63u) == 0u)
+        atomicMax(out, m);
 }
 
 // Dense-grid pyramid, level 0: voxel materials (0xFF = empty) to codes;
@@ -862,9 +879,13 @@ static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
     return SVDAG_OK;
 }
 
+// After a table grew: upload the new descriptors and wait for the upload, so
+// that no frame trace launched once the caller's FrameQuiesce scope ends can
+// read descriptors that still point at freed lines.
 int ctx_refresh_tables(svdag_ctx* ctx)
 {
     SVDAG_TRY(ctx->ts.upload(ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
     ctx->dag.tables = ctx->ts.devTables;
     return SVDAG_OK;
 }
@@ -945,7 +966,8 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
     const hipStream_t st = ctx->stream;
     const Bounds b = edit_bounds(ctx, ed);
     const uint32_t kCap = 8192;
-    edit_begin<<<1, 64, 0, st>>>(ws, ctx->dRoot, ed.kind == EK_COPY ? 1u : 0u);
+    edit_begin<<<1, 64, 0, st>>>(ws, ctx->dRoot, ed.kind == EK_COPY ? 1u : 0u,
+                                 ed.kind == EK_DENSE || ed.kind == EK_TERRAIN ? 1u : 0u);
     for (uint32_t L = 0; L < dag.leafLevel; ++L) {
         const uint64_t items = std::min<uint64_t>(level_bound(ctx, b, L), ws.capWork);
         svo_inner_level<false><<<grid_for(items, 32, kCap), 256, 0, st>>>(ws, dag, edDev, L);
@@ -1183,6 +1205,10 @@ int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e)
         set_error("radius must be finite and >= 0");
         return SVDAG_E_INVALID;
     }
+    if (d.rotation > 3u || (d.rotation != 0u && d.kind != SVDAG_COPY)) {
+        set_error("rotation must be 0..3 quarter turns, and only on a copy");
+        return SVDAG_E_INVALID;
+    }
     // MyGpuSphereEditorBase (my_gpu_dag_editors.h:406-411).
     e.cx = d.center[0] + 0.5f;
     e.cy = d.center[1] + 0.5f;
@@ -1201,6 +1227,21 @@ int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e)
     for (int a = 0; a < 3; ++a) {
         e.bmin[a] = c[a] - d.radius;
         e.bmax[a] = d.radius == 0.0f ? c[a] + 1.0f : c[a] + d.radius;
+    }
+    // Rotated copy: the destination voxels overlapping [bmin, bmax) are
+    // floor(bmin) .. ceil(bmax) - 1; pivots p = floor(c + 0.5), q = p + id2s.
+    e.rot = d.kind == SVDAG_COPY ? d.rotation : 0u;
+    if (e.rot) {
+        for (int a = 0; a < 3; ++a) {
+            if (!(std::fabs(e.bmin[a]) <= 1e9f && std::fabs(e.bmax[a]) <= 1e9f && std::fabs(e.d2s[a]) <= 1e9f)) {
+                set_error("copy coordinates out of range");
+                return SVDAG_E_INVALID;
+            }
+            e.ivmin[a] = (int32_t)std::floor(e.bmin[a]);
+            e.ivmax[a] = (int32_t)std::ceil(e.bmax[a]) - 1;
+            e.pv[a] = (int32_t)std::floor(c[a] + 0.5f);
+            e.qv[a] = e.pv[a] + e.id2s[a];
+        }
     }
     return SVDAG_OK;
 }
@@ -1510,6 +1551,41 @@ static int build_terrain(svdag_ctx* ctx, const svdag_terrain* tp, bool banded, u
             ctx->hmin + ctx->pyrOffset[k - 1], k == 1 ? ctx->hmin : ctx->hmax + (ctx->pyrOffset[k - 1] - base1),
             side >> (k - 1), ctx->hmin + ctx->pyrOffset[k], ctx->hmax + (ctx->pyrOffset[k] - base1));
     SVDAG_HIP(hipGetLastError());
+    // A tiled build equals the one-edit build only if no fully-filled node is
+    // wider than a tile (the merge flags a node only from children rebuilt in
+    // the same edit).  The widest solid node sits at y = 0: width 2^k is solid
+    // somewhere iff level k's largest minimum height is >= 2^k - 1.  Banded
+    // terrains fill nothing wider than a band cube.  Widen the tile to it.
+    if (tileLog2 < D) {
+        const uint32_t kTop = banded ? std::min(bandLog2, D) : D;
+        int32_t* dMax = nullptr;
+        SVDAG_HIP(hipMalloc(&dMax, 4));
+        for (uint32_t k = kTop; k > tileLog2; --k) {
+            const int32_t lo = INT32_MIN;
+            int32_t best = INT32_MIN;
+            const uint64_t cells = (uint64_t)(side >> k) * (side >> k);
+            hipError_t he = hipMemcpyAsync(dMax, &lo, 4, hipMemcpyHostToDevice, ctx->stream);
+            if (he == hipSuccess) {
+                level_max_kernel<<<grid_for(cells, 256, 4096), 256, 0, ctx->stream>>>(ctx->hmin + ctx->pyrOffset[k],
+                                                                                       cells, dMax);
+                he = hipGetLastError();
+            }
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(&best, dMax, 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (he == hipSuccess)
+                he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) {
+                (void)hipFree(dMax);
+                set_error(std::string("terrain tile check: ") + hipGetErrorString(he));
+                return SVDAG_E_DEVICE;
+            }
+            if ((int64_t)best >= ((int64_t)1 << k) - 1) {
+                tileLog2 = k;
+                break;
+            }
+        }
+        (void)hipFree(dMax);
+    }
     EditorDev e {};
     e.kind = EK_TERRAIN;
     e.material = tp->material;

@@ -26,6 +26,9 @@ struct EditStatus {
     uint32_t leafCount;
     uint32_t done;      // the edit ran to completion (applied or skipped)
     uint32_t fallback;  // the fused pass could not take this edit: run it alone
+    uint32_t build;     // a DAG build (terrain, voxel grid): merged and published
+                        // even without leaves (fully-filled children only)
+    uint32_t pad[3];
 };
 
 // Edit workspace as seen by the kernels.

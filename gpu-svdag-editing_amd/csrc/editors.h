@@ -1,14 +1,19 @@
 // Edit tools evaluated on the device: node classification (should_edit) and
 // the per-voxel new value (get_new_value).
 //
-// The float arithmetic is the reference's, operation by operation, with FP
-// contraction off (every a*b and a-b rounded separately):
+// The float arithmetic is the reference's, operation by operation.  The build
+// contracts nothing by itself; the one site where nvcc's default
+// (--fmad=true) has a single possible fusion, `dist -= squared(d)` in the
+// sphere overlap test (my_gpu_dag_editors.h:425-437), is written as
+// fmaf(-d, d, dist).  Everything else is rounded operation by operation:
 //   sphere  my_gpu_dag_editors.h:401-496   (centre + 0.5, radius >= 0.1,
 //           overlap = r^2 - sum of squared axis gaps > 0, Fill/Empty when the
 //           box shrunk by one voxel is inside)
 //   box     my_gpu_dag_editors.h:45-119    ([c-r, c+r), r == 0 -> [c, c+1))
 //   paint   my_gpu_dag_editors.h:498-536   (Fill only over flagged full nodes)
-//   copy    my_gpu_dag_editors.h:127-253   (translation; up to 8 source parents)
+//   copy    my_gpu_dag_editors.h:127-253   (translation; up to 8 source parents;
+//           quarter turns about +Y are this build's own extension, done in
+//           integer voxel arithmetic: decide_copy_rot / rot_source)
 // The terrain tool (kind 6) has no reference counterpart: it classifies nodes
 // against a min/max height pyramid so that a heightfield builds through the
 // same pipeline as an edit.
@@ -46,6 +51,11 @@ struct EditorDev {
     // copy
     float d2s[3];
     int32_t id2s[3];
+    // rotated copy (rot != 0, no reference counterpart): quarter turns about
+    // +Y, destination pivot pv, source pivot qv = pv + id2s, and the
+    // destination voxels [ivmin, ivmax] whose unit boxes overlap the box
+    uint32_t rot;
+    int32_t pv[3], qv[3], ivmin[3], ivmax[3];
     // terrain: height pyramid, level k holds (2^D >> k)^2 cells, row-major in z.
     // hmax holds levels 1..D only (level 0's maxima are hmin's heights) and is
     // indexed by pyrOffset[k] - pyrOffset[1].
@@ -84,10 +94,10 @@ __host__ __device__ inline bool sphere_overlap(const EditorDev& e, const float m
     for (int a = 0; a < 3; ++a) {
         if (c[a] < mn[a]) {
             const float d = c[a] - mn[a];
-            dist = dist - d * d;
+            dist = fmaf(-d, d, dist);
         } else if (c[a] > mx[a]) {
             const float d = c[a] - mx[a];
-            dist = dist - d * d;
+            dist = fmaf(-d, d, dist);
         }
     }
     return dist > 0.0f;
@@ -259,12 +269,90 @@ __host__ __device__ inline bool aabb_overlap(const float lmn[3], const float lmx
              || rmx[1] <= lmn[1] || rmx[2] <= lmn[2]);
 }
 
+// Rotated copy: source voxel of destination voxel v, k quarter turns of
+// R(x, y, z) = (-z - 1, y, x) (the turn about +Y that carries voxel cells
+// onto voxel cells) about the pivots, clamped at 0 per axis like the
+// translating copy (my_gpu_dag_editors.h:231).  No reference counterpart.
+__host__ __device__ inline void rot_source(const EditorDev& e, const int64_t v[3], int64_t s[3])
+{
+    int64_t x = v[0] - e.pv[0], z = v[2] - e.pv[2];
+    for (uint32_t i = 0; i < e.rot; ++i) {
+        const int64_t nx = -z - 1;
+        z = x;
+        x = nx;
+    }
+    s[0] = e.qv[0] + x;
+    s[1] = e.qv[1] + (v[1] - e.pv[1]);
+    s[2] = e.qv[2] + z;
+    for (int a = 0; a < 3; ++a)
+        s[a] = s[a] > 0 ? s[a] : 0;
+}
+
+// Rotated copy, node level.  The node's destination voxels inside the edit
+// box are an integer range; its image under rot_source (both corners mapped,
+// then ordered) is the source range, and the children of the parents'
+// source nodes that overlap it are the next parents.  Integer arithmetic
+// throughout, so every destination voxel's source lies in a collected parent.
+__device__ inline Op decide_copy_rot(const EditorDev& e, const DagDev& dag, uint32_t x, uint32_t y, uint32_t z,
+                                     uint32_t depth, const CopyState& in, CopyState& out)
+{
+    for (int k = 0; k < 8; ++k)
+        out.p[k].h = kSentinel;
+    const uint32_t p[3] = { x, y, z };
+    int64_t lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n0 = (int64_t)p[a] << depth;
+        const int64_t n1 = n0 + ((int64_t)1 << depth) - 1;
+        lo[a] = n0 > e.ivmin[a] ? n0 : e.ivmin[a];
+        hi[a] = n1 < e.ivmax[a] ? n1 : e.ivmax[a];
+        if (lo[a] > hi[a])
+            return OP_KEEP;
+    }
+    int64_t a0[3], a1[3], slo[3], shi[3];
+    rot_source(e, lo, a0);
+    rot_source(e, hi, a1);
+    for (int a = 0; a < 3; ++a) {
+        slo[a] = a0[a] < a1[a] ? a0[a] : a1[a];
+        shi[a] = a0[a] < a1[a] ? a1[a] : a0[a];
+    }
+    int nOut = 0;
+    for (int k = 0; k < 8; ++k) {
+        const CopyParent& par = in.p[k];
+        if (par.h == kSentinel)
+            break;
+        const uint32_t* node = dag_node_ptr(dag, par.h);
+        const uint32_t mask = node[0] & 0xFFu;
+        for (uint32_t c = 0; c < 8; ++c) {
+            if (!(mask & (1u << c)))
+                continue;
+            const uint32_t cp[3] = { (par.x << 1) | ((c >> 2) & 1u), (par.y << 1) | ((c >> 1) & 1u),
+                                     (par.z << 1) | (c & 1u) };
+            bool ov = true;
+            for (int a = 0; a < 3; ++a) {
+                const int64_t c0 = (int64_t)cp[a] << depth;
+                const int64_t c1 = c0 + ((int64_t)1 << depth) - 1;
+                ov = ov && c0 <= shi[a] && c1 >= slo[a];
+            }
+            if (ov && nOut < 8) {
+                out.p[nOut].x = cp[0];
+                out.p[nOut].y = cp[1];
+                out.p[nOut].z = cp[2];
+                out.p[nOut].h = node[1 + __builtin_popcount(mask & ((1u << c) - 1u))];
+                ++nOut;
+            }
+        }
+    }
+    return nOut == 0 ? OP_KEEP : OP_SUBDIVIDE;
+}
+
 // Copy tool, stateful should_edit for an inner-level child at childDepth:
 // collects the source children of the parent's source nodes that overlap the
 // shifted destination region (my_gpu_dag_editors.h:173-217).
 __device__ inline Op decide_copy(const EditorDev& e, const DagDev& dag, uint32_t x, uint32_t y, uint32_t z,
                                  uint32_t depth, const CopyState& in, CopyState& out)
 {
+    if (e.rot)
+        return decide_copy_rot(e, dag, x, y, z, depth, in, out);
     float mn[3], mx[3];
     node_box(x, y, z, depth, mn, mx);
     for (int k = 0; k < 8; ++k)
@@ -309,9 +397,21 @@ __device__ inline Op decide_copy(const EditorDev& e, const DagDev& dag, uint32_t
 __device__ inline bool copy_value(const EditorDev& e, const DagDev& dag, uint32_t vx, uint32_t vy, uint32_t vz,
                                   const CopyState& st, bool hasOld, uint32_t oldMat, uint32_t& mat)
 {
-    int32_t s[3] = { (int32_t)vx + e.id2s[0], (int32_t)vy + e.id2s[1], (int32_t)vz + e.id2s[2] };
-    for (int a = 0; a < 3; ++a)
-        s[a] = s[a] > 0 ? s[a] : 0;
+    int32_t s[3];
+    if (e.rot) {
+        const int64_t v[3] = { vx, vy, vz };
+        int64_t r[3];
+        rot_source(e, v, r);
+        // beyond the volume no parent matches (coordinates stay below 2^31)
+        for (int a = 0; a < 3; ++a)
+            s[a] = (int32_t)(r[a] < 0x7FFFFFFF ? r[a] : 0x7FFFFFFF);
+    } else {
+        s[0] = (int32_t)vx + e.id2s[0];
+        s[1] = (int32_t)vy + e.id2s[1];
+        s[2] = (int32_t)vz + e.id2s[2];
+        for (int a = 0; a < 3; ++a)
+            s[a] = s[a] > 0 ? s[a] : 0;
+    }
     const uint32_t px = (uint32_t)s[0] >> 2, py = (uint32_t)s[1] >> 2, pz = (uint32_t)s[2] >> 2;
     const CopyParent* found = nullptr;
     for (int k = 0; k < 8; ++k) {

@@ -772,7 +772,7 @@ struct svdag_hashset {
 extern "C" {
 
 const char* svdag_last_error(void) { return g_lastError.c_str(); }
-uint32_t svdag_abi_version(void) { return 1u; }
+uint32_t svdag_abi_version(void) { return 2u; }
 
 int svdag_device_count(int* out)
 {

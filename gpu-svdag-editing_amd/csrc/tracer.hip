@@ -8,7 +8,13 @@
 // traversal stack (node handle, child mask, visit mask) is kept in LDS laid
 // out [level][lane] so that a wave's pushes and pops hit 64 consecutive
 // 8-byte slots.  Float arithmetic follows the reference operation by
-// operation with contraction off; the camera setup is double precision.
+// operation.  Nothing is contracted by the compiler; explicit fma/fmaf stand
+// where nvcc's default --fmad=true has exactly one possible fusion (the
+// per-pixel ray setup, the hit position, the shadow and occlusion ray
+// origins).  Sites with two products in one sum (the slab bounds
+// tmid +- radius*|invDir|, dot products) stay rounded per operation: which
+// product nvcc would fuse there is not fixed by its documentation.  The
+// camera setup is double precision.
 #include "dag_internal.h"
 #include <cfloat>
 #include <cmath>
@@ -110,9 +116,11 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
         return;
 
     // Per-pixel ray in double, normalised, then rounded to float (tracer.cu:402-405).
+    // rayMin + px*ddx + py*ddy has one fusion per add under --fmad=true:
+    // fma(py, ddy, fma(px, ddx, rayMin)).
     double a[3];
     for (int k = 0; k < 3; ++k)
-        a[k] = ((p.rayMin[k] + (double)px * p.ddx[k]) + (double)py * p.ddy[k]) - p.cam[k];
+        a[k] = fma((double)py, p.ddy[k], fma((double)px, p.ddx[k], p.rayMin[k])) - p.cam[k];
     const double len = sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
     const double inv = 1.0 / len;
     Ray ray;
@@ -211,9 +219,10 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
     h.z = z;
     h.material = material;
     h.t = t;
-    h.px = ray.o[0] + t * ray.d[0];
-    h.py = ray.o[1] + t * ray.d[1];
-    h.pz = ray.o[2] + t * ray.d[2];
+    // ray.origin + t * ray.direction (tracer.cu:388), one fusion possible.
+    h.px = fmaf(t, ray.d[0], ray.o[0]);
+    h.py = fmaf(t, ray.d[1], ray.o[1]);
+    h.pz = fmaf(t, ray.d[2], ray.o[2]);
     const size_t at = (size_t)(p.height - 1 - py) * p.width + px;
     out[at] = h;
     // The surface normal points against the ray on the entry axis; a null
@@ -403,9 +412,10 @@ __global__ void __launch_bounds__(kTraceBlock) shadow_kernel(DagDev dag, uint32_
     const svdag_hit h = hits[i];
     const float L[3] = { sun.x, sun.y, sun.z };
     const float p[3] = { h.px, h.py, h.pz };
+    // si.position + shadowBias * sun_direction() (tracer.cu:621), one fusion possible.
     float o[3];
     for (int k = 0; k < 3; ++k)
-        o[k] = p[k] + bias * L[k];
+        o[k] = fmaf(bias, L[k], p[k]);
     Ray r;
     make_ray(r, o, L, FLT_MAX);
     float nrm[3], du[3], dv[3];
@@ -432,9 +442,10 @@ __global__ void __launch_bounds__(kTraceBlock) ao_kernel(DagDev dag, uint32_t ro
     const float p[3] = { h.px, h.py, h.pz };
     float nrm[3], du[3], dv[3];
     surface_frame(code, nrm, du, dv);
+    // si.position + si.normal * shadowBias (tracer.cu:675), one fusion possible.
     float o[3];
     for (int k = 0; k < 3; ++k)
-        o[k] = p[k] + nrm[k] * bias;
+        o[k] = fmaf(nrm[k], bias, p[k]);
     // The sample stream is keyed by the pixel's index in the stored image.
     Pcg32 rng((uint64_t)i * (seed > 1ull ? seed : 1ull));
     uint32_t hitsUnoccluded = 0;
@@ -545,13 +556,16 @@ extern "C" int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* cam, uint
     return SVDAG_OK;
 }
 
-// Sun direction of the reference (tracer.cu:600-603): normalize(0.3f, 1, 0.5f),
-// here normalised in double and rounded once.
+// Sun direction of the reference (tracer.cu:600-603): normalize(float3(0.3f, 1, 0.5f))
+// with normalize(a) = (1 / length(a)) * a and length(a) = sqrt(dot(a, a)), all
+// in float (cuda_math.h:115-118).  The operands are constants, which the
+// compiler folds operation by operation, so nothing is fused.
 static float3 sun_direction()
 {
-    const double v[3] = { (double)0.3f, 1.0, 0.5 };
-    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    return make_float3((float)(v[0] / len), (float)(v[1] / len), (float)(v[2] / len));
+    const float v[3] = { 0.3f, 1.0f, 0.5f };
+    const float dot = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const float inv = 1.0f / std::sqrt(dot);
+    return make_float3(inv * v[0], inv * v[1], inv * v[2]);
 }
 
 static int shade_pass(svdag_ctx* ctx, int kind, uint32_t numSamples, float bias, float rayLength, uint64_t seed,

@@ -130,7 +130,7 @@ struct MyGpuSphereEditor {
     svdag_edit_desc desc() const
     {
         return { isAdding ? (uint32_t)SVDAG_SPHERE_ADD : (uint32_t)SVDAG_SPHERE_REMOVE, fillMaterial,
-                 { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 } };
+                 { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 }, 0u };
     }
 };
 
@@ -148,7 +148,7 @@ struct MyGpuBoxEditor {
     svdag_edit_desc desc() const
     {
         return { isAdding ? (uint32_t)SVDAG_BOX_ADD : (uint32_t)SVDAG_BOX_REMOVE, fillMaterial,
-                 { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 } };
+                 { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 }, 0u };
     }
 };
 
@@ -166,7 +166,8 @@ struct MyGpuSpherePaintEditor {
     }
     svdag_edit_desc desc() const
     {
-        return { (uint32_t)SVDAG_PAINT, paintMaterial, { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 } };
+        return { (uint32_t)SVDAG_PAINT, paintMaterial, { center.x, center.y, center.z }, radius, { 0, 0, 0 }, { 0, 0, 0 },
+                 0u };
     }
 };
 
@@ -174,19 +175,23 @@ template <typename TDag>
 struct MyGpuCopyEditor {
     float3 source, dest, center;
     float radius;
+    uint32_t quarterTurns;
     // The reference copies the DAG value into the editor to read the source;
-    // here the source is the same live DAG the edit is applied to.
-    MyGpuCopyEditor(const TDag&, float3 src, float3 dst, float3 c, float r)
+    // here the source is the same live DAG the edit is applied to.  The
+    // optional quarter turns about +Y have no reference counterpart (the
+    // reference's copy only translates).
+    MyGpuCopyEditor(const TDag&, float3 src, float3 dst, float3 c, float r, uint32_t turns = 0)
         : source(src)
         , dest(dst)
         , center(c)
         , radius(r)
+        , quarterTurns(turns)
     {
     }
     svdag_edit_desc desc() const
     {
         return { (uint32_t)SVDAG_COPY, 0u, { center.x, center.y, center.z }, radius, { source.x, source.y, source.z },
-                 { dest.x, dest.y, dest.z } };
+                 { dest.x, dest.y, dest.z }, quarterTurns };
     }
 };
 

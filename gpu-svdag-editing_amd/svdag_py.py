@@ -47,10 +47,11 @@ class Terrain(ctypes.Structure):
 
 class EditDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("material", ctypes.c_uint32), ("center", ctypes.c_float * 3),
-                ("radius", ctypes.c_float), ("src", ctypes.c_float * 3), ("dst", ctypes.c_float * 3)]
+                ("radius", ctypes.c_float), ("src", ctypes.c_float * 3), ("dst", ctypes.c_float * 3),
+                ("rotation", ctypes.c_uint32)]
 
 
-_EDIT_PACK = struct.Struct("<II3ff3f3f")
+_EDIT_PACK = struct.Struct("<II3ff3f3fI")
 assert _EDIT_PACK.size == ctypes.sizeof(EditDesc)
 
 
@@ -306,9 +307,9 @@ class MyGPUHashDAG:
         check(lib().svdag_root(self.h, ctypes.byref(r)))
         return r.value
 
-    def edit(self, kind, material, center, radius, src=(0, 0, 0), dst=(0, 0, 0)):
+    def edit(self, kind, material, center, radius, src=(0, 0, 0), dst=(0, 0, 0), rotation=0):
         d = EditDesc(kind, material, (ctypes.c_float * 3)(*center), float(radius),
-                     (ctypes.c_float * 3)(*src), (ctypes.c_float * 3)(*dst))
+                     (ctypes.c_float * 3)(*src), (ctypes.c_float * 3)(*dst), int(rotation))
         st = EditStats()
         check(lib().svdag_edit(self.h, ctypes.byref(d), ctypes.byref(st)))
         return st
@@ -323,8 +324,9 @@ class MyGPUHashDAG:
             c = e[2]
             src = e[4] if len(e) > 4 else (0.0, 0.0, 0.0)
             dst = e[5] if len(e) > 5 else (0.0, 0.0, 0.0)
+            rot = int(e[6]) & 0xFFFFFFFF if len(e) > 6 else 0
             _EDIT_PACK.pack_into(buf, i * _EDIT_PACK.size, int(e[0]) & 0xFFFFFFFF, int(e[1]) & 0xFFFFFFFF,
-                                 c[0], c[1], c[2], e[3], src[0], src[1], src[2], dst[0], dst[1], dst[2])
+                                 c[0], c[1], c[2], e[3], src[0], src[1], src[2], dst[0], dst[1], dst[2], rot)
         return (EditDesc * n).from_buffer(buf)
 
     @staticmethod
@@ -333,7 +335,7 @@ class MyGPUHashDAG:
                       (ctypes.c_double * 3)(*bmin), (ctypes.c_double * 3)(*bmax))
 
     def edit_batch(self, edits):
-        """edits: list of (kind, material, center, radius[, src, dst]); returns (stats list, gpu ms)."""
+        """edits: list of (kind, material, center, radius[, src, dst[, rotation]]); returns (stats list, gpu ms)."""
         n = len(edits)
         descs = self._descs(edits)
         stats = (EditStats * n)()

@@ -143,8 +143,9 @@ int svdag_build_terrain(svdag_ctx* ctx, const svdag_terrain* terrain);
  *   & (2^material_bits - 1),   b = band_log2 in [2, levels],
  * mix32(h) = ((h ^ h>>15) * 0x2C1B3C6D) ^ (that >> 12).  The build runs one edit
  * per column tile of 2^tile_log2 voxels (0 = levels-2 above depth 15, else one
- * edit), which keeps one edit's work items within the workspace; the DAG is the
- * same as a single-edit build when no fully-filled node is wider than a tile.
+ * edit), which keeps one edit's work items within the workspace.  The tile
+ * widens by itself to the widest fully solid node (read from the height
+ * pyramid), so the DAG always equals the single-edit build's.
  * svdag_build_terrain tiles the same way.                                   */
 int svdag_build_terrain_banded(svdag_ctx* ctx, const svdag_terrain* terrain, uint32_t band_log2, uint32_t band_seed,
                                uint32_t tile_log2);
@@ -184,6 +185,15 @@ typedef struct {
     float radius;
     float src[3];   /* COPY only */
     float dst[3];   /* COPY only */
+    /* COPY only: quarter turns (0..3) about +Y applied to the pasted region.
+     * 0 is the reference's translating copy, bit for bit.  No reference
+     * counterpart for 1..3 (MyGpuCopyEditor only translates): with
+     * c = dst + (center - src), pivot p = floor(c + 0.5) and
+     * q = p + trunc(src - dst), every destination voxel v whose unit box
+     * overlaps [c - radius, c + radius) takes source voxel
+     * max(q + R^k(v - p), 0), R(x, y, z) = (-z - 1, y, x), when that voxel is
+     * set, and keeps its own value otherwise.  Other kinds require 0.     */
+    uint32_t rotation;
 } svdag_edit_desc;
 
 typedef struct {

@@ -22,7 +22,11 @@
 //   GC mark             my_gpu_hash_dag.cu:364-458 (roots: root + level-0
 //                       fully-filled node of every material)
 // Float rules: every float/double operation is written out in the
-// reference's order; this file is compiled with -ffp-contract=off.
+// reference's order; this file is compiled with -ffp-contract=off.  std::fma
+// stands exactly where nvcc's default --fmad=true has a single possible
+// fusion (an add whose one operand is a product): the sphere overlap's
+// dist -= d*d, the per-pixel ray setup, the hit position and the shadow /
+// occlusion ray origins.  The same sites are fmaf / fma in the HIP kernels.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -352,9 +356,15 @@ struct Editor {
     float bmin[3], bmax[3];            // box / copy destination
     float d2s[3];
     int32_t id2s[3];
+    // Rotated copy (no reference counterpart): quarter turns about +Y, the
+    // destination pivot pv, the source pivot qv = pv + id2s, and the
+    // destination voxels [ivmin, ivmax] whose unit boxes overlap [bmin, bmax).
+    uint32_t rot;
+    int32_t pv[3], qv[3], ivmin[3], ivmax[3];
 };
 
-Editor make_editor(uint32_t kind, uint32_t material, const float* center, float radius, const float* src, const float* dst)
+Editor make_editor(uint32_t kind, uint32_t material, const float* center, float radius, const float* src, const float* dst,
+                   uint32_t rot = 0)
 {
     Editor e {};
     e.kind = kind;
@@ -376,7 +386,45 @@ Editor make_editor(uint32_t kind, uint32_t material, const float* center, float 
         e.bmin[a] = bc[a] - radius;
         e.bmax[a] = radius == 0 ? bc[a] + 1.0f : bc[a] + radius;
     }
+    e.rot = kind == COPY ? (rot & 3u) : 0u;
+    if (e.rot)
+        for (int a = 0; a < 3; ++a) {
+            // voxel v overlaps [bmin, bmax) iff bmin - 1 < v < bmax
+            e.ivmin[a] = (int32_t)std::floor(e.bmin[a]);
+            e.ivmax[a] = (int32_t)std::ceil(e.bmax[a]) - 1;
+            e.pv[a] = (int32_t)std::floor(bc[a] + 0.5f);
+            e.qv[a] = e.pv[a] + e.id2s[a];
+        }
     return e;
+}
+
+// Rotated copy map of an offset u from the destination pivot: k quarter turns
+// of R(x, y, z) = (-z - 1, y, x), the turn about +Y that carries voxel cells
+// onto voxel cells.  No reference counterpart.
+void rot_cell(uint32_t k, const int64_t* u, int64_t* out)
+{
+    int64_t x = u[0], z = u[2];
+    for (uint32_t i = 0; i < k; ++i) {
+        const int64_t nx = -z - 1, nz = x;
+        x = nx;
+        z = nz;
+    }
+    out[0] = x;
+    out[1] = u[1];
+    out[2] = z;
+}
+// Source voxel of destination voxel v under the rotated copy, clamped at 0
+// per axis like the translation (my_gpu_dag_editors.h:231).
+void rot_source(const Editor& e, const int64_t* v, int64_t* s)
+{
+    const int64_t u[3] = { v[0] - e.pv[0], v[1] - e.pv[1], v[2] - e.pv[2] };
+    int64_t r[3];
+    rot_cell(e.rot, u, r);
+    for (int a = 0; a < 3; ++a) {
+        s[a] = e.qv[a] + r[a];
+        if (s[a] < 0)
+            s[a] = 0;
+    }
 }
 
 float sq(float f) { return f * f; }
@@ -385,10 +433,14 @@ bool sphere_overlap(const Editor& e, const float* mn, const float* mx)
 {
     float dist = e.radius2;
     for (int a = 0; a < 3; ++a) {
-        if (e.c[a] < mn[a])
-            dist -= sq(e.c[a] - mn[a]);
-        else if (e.c[a] > mx[a])
-            dist -= sq(e.c[a] - mx[a]);
+        // dist -= squared(d), contracted as nvcc's --fmad=true does (one fusion possible).
+        if (e.c[a] < mn[a]) {
+            const float d = e.c[a] - mn[a];
+            dist = std::fma(-d, d, dist);
+        } else if (e.c[a] > mx[a]) {
+            const float d = e.c[a] - mx[a];
+            dist = std::fma(-d, d, dist);
+        }
     }
     return dist > 0;
 }
@@ -479,8 +531,57 @@ bool aabb_overlap(const float* lmn, const float* lmx, const float* rmn, const fl
     return true;
 }
 
+// Rotated copy, node level: the destination voxels of the node inside the
+// edit box form an integer range; its image under the map (corners mapped,
+// then ordered, then clamped at 0) is the source range, and the source
+// children overlapping it become the parents of the next level.  Exact
+// integer arithmetic, so every destination voxel's source cell lies in one
+// of the collected parents.
+Op should_edit_copy_rot(const Editor& e, const Dag& d, const uint32_t* p, uint32_t depth, const CopyState& in, CopyState& out)
+{
+    out.n = 0;
+    int64_t lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n0 = (int64_t)p[a] << depth, n1 = n0 + ((int64_t)1 << depth) - 1;
+        lo[a] = std::max<int64_t>(n0, e.ivmin[a]);
+        hi[a] = std::min<int64_t>(n1, e.ivmax[a]);
+        if (lo[a] > hi[a])
+            return KEEP;
+    }
+    int64_t a0[3], a1[3], slo[3], shi[3];
+    rot_source(e, lo, a0);
+    rot_source(e, hi, a1);
+    for (int a = 0; a < 3; ++a) {
+        slo[a] = std::min(a0[a], a1[a]);
+        shi[a] = std::max(a0[a], a1[a]);
+    }
+    for (int k = 0; k < in.n; ++k) {
+        const uint32_t* node = d.get(in.par[k].h);
+        const uint32_t mask = node[0] & 0xFFu;
+        for (uint32_t c = 0; c < 8; ++c) {
+            if (!(mask & (1u << c)))
+                continue;
+            const uint32_t cp[3] = { (in.par[k].p[0] << 1) | ((c >> 2) & 1u), (in.par[k].p[1] << 1) | ((c >> 1) & 1u),
+                                     (in.par[k].p[2] << 1) | (c & 1u) };
+            bool ov = true;
+            for (int a = 0; a < 3; ++a) {
+                const int64_t c0 = (int64_t)cp[a] << depth, c1 = c0 + ((int64_t)1 << depth) - 1;
+                ov = ov && c0 <= shi[a] && c1 >= slo[a];
+            }
+            if (ov && out.n < 8) {
+                Parent& q = out.par[out.n++];
+                std::memcpy(q.p, cp, sizeof(cp));
+                q.h = node[1 + popc(mask & ((1u << c) - 1u))];
+            }
+        }
+    }
+    return out.n == 0 ? KEEP : SUBDIVIDE;
+}
+
 Op should_edit_copy(const Editor& e, const Dag& d, const uint32_t* p, uint32_t depth, const CopyState& in, CopyState& out)
 {
+    if (e.rot)
+        return should_edit_copy_rot(e, d, p, depth, in, out);
     float mn[3], mx[3];
     box_of(p, depth, mn, mx);
     out.n = 0;
@@ -517,11 +618,16 @@ Op should_edit_copy(const Editor& e, const Dag& d, const uint32_t* p, uint32_t d
 bool copy_value(const Editor& e, const Dag& d, const uint32_t* v, const CopyState& st, bool hasOld, uint32_t oldMat,
                 uint32_t& mat)
 {
-    int32_t s[3];
-    for (int a = 0; a < 3; ++a) {
-        s[a] = (int32_t)v[a] + e.id2s[a];
-        if (s[a] < 0)
-            s[a] = 0;
+    int64_t s[3];
+    if (e.rot) {
+        const int64_t vv[3] = { v[0], v[1], v[2] };
+        rot_source(e, vv, s);
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            s[a] = (int32_t)v[a] + e.id2s[a];
+            if (s[a] < 0)
+                s[a] = 0;
+        }
     }
     const Parent* found = nullptr;
     for (int k = 0; k < st.n; ++k)
@@ -862,8 +968,10 @@ Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const dou
                 uint32_t* surfCode = nullptr)
 {
     double a[3];
+    // rayMin + px*ddx + py*ddy - cam (tracer.cu:404); each add has one product
+    // operand, which nvcc's --fmad=true fuses.
     for (int k = 0; k < 3; ++k)
-        a[k] = rayMin[k] + (double)px * ddx[k] + (double)py * ddy[k] - cam[k];
+        a[k] = std::fma((double)py, ddy[k], std::fma((double)px, ddx[k], rayMin[k])) - cam[k];
     const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
     const double inv = 1 / len;
     Ray r;
@@ -947,7 +1055,9 @@ Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const dou
     // entry axis; the null path (0,0,0) is no surface (path.h:61-64).
     if (surfCode)
         *surfCode = (path[0] | path[1] | path[2]) == 0 ? 0u : 1u + 2u * (uint32_t)axis + (r.d[axis] < 0.0f ? 1u : 0u);
-    return { path[0], path[1], path[2], material, t, r.o[0] + t * r.d[0], r.o[1] + t * r.d[1], r.o[2] + t * r.d[2] };
+    // origin + t * direction (tracer.cu:388), fused.
+    return { path[0], path[1], path[2], material, t, std::fma(t, r.d[0], r.o[0]), std::fma(t, r.d[1], r.o[1]),
+             std::fma(t, r.d[2], r.o[2]) };
 }
 
 // intersect_ray_node_out_of_order (tracer.cu:274-351): any set voxel along
@@ -1077,7 +1187,7 @@ float ao_at(const Dag& d, const float* p, uint32_t code, uint32_t samples, float
     surface_frame(code, n, du, dv);
     float o[3];
     for (int k = 0; k < 3; ++k)
-        o[k] = p[k] + n[k] * bias;
+        o[k] = std::fma(n[k], bias, p[k]); // position + normal * shadowBias, fused (tracer.cu:675)
     Pcg32 g = pcg_seeded(stream);
     uint32_t open = 0;
     for (uint32_t s = 0; s < samples; ++s) {
@@ -1096,13 +1206,16 @@ float ao_at(const Dag& d, const float* p, uint32_t code, uint32_t samples, float
     return (float)open / (float)samples;
 }
 
-// The reference's sun direction normalize(0.3f, 1, 0.5f), normalised in double.
+// sun_direction() (tracer.cu:600-603) = normalize(float3(0.3f, 1, 0.5f)):
+// (1 / sqrt(dot(v, v))) * v in float, rounded per operation (cuda_math.h:115-118;
+// constant operands are folded without fusion).
 void sun_dir(float* L)
 {
-    const double v[3] = { (double)0.3f, 1.0, 0.5 };
-    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const float v[3] = { 0.3f, 1.0f, 0.5f };
+    const float dot = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const float inv = 1.0f / std::sqrt(dot);
     for (int k = 0; k < 3; ++k)
-        L[k] = (float)(v[k] / len);
+        L[k] = inv * v[k];
 }
 
 } // namespace
@@ -1150,11 +1263,11 @@ int ora_dag_build_terrain(void* dp, uint64_t seed, double base, double amp, doub
 
 // Returns 1 if merged, 0 if skipped (no leaf).
 int ora_dag_edit(void* dp, uint32_t kind, uint32_t material, const float* center, float radius, const float* src,
-                 const float* dst, uint32_t* svoCounts)
+                 const float* dst, uint32_t rotation, uint32_t* svoCounts)
 {
     Dag& d = *static_cast<Dag*>(dp);
     const float zero[3] = { 0, 0, 0 };
-    const Editor e = make_editor(kind, material, center, radius, src ? src : zero, dst ? dst : zero);
+    const Editor e = make_editor(kind, material, center, radius, src ? src : zero, dst ? dst : zero, rotation);
     return apply_edit(d, e, svoCounts);
 }
 
@@ -1296,7 +1409,7 @@ void ora_dag_shade(void* dp, const double* position, const double* rotation, dou
             surface_frame(code, n, du, dv);
             float o[3];
             for (int k = 0; k < 3; ++k)
-                o[k] = p[k] + bias * L[k];
+                o[k] = std::fma(bias, L[k], p[k]); // position + shadowBias * L, fused (tracer.cu:621)
             float ndl = (n[0] * L[0] + n[1] * L[1]) + n[2] * L[2];
             ndl = ndl < 0.0f ? 0.0f : ndl;
             outSun[i] = occluded_ray(d, make_ray(o, L, FLT_MAX)) ? 0.0f : ndl;

@@ -100,7 +100,8 @@ def _dag_sigs(L):
                                       ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]
     L.ora_dag_build_terrain.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
-    L.ora_dag_edit.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, f32p, ctypes.c_float, f32p, f32p, u32p]
+    L.ora_dag_edit.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, f32p, ctypes.c_float, f32p, f32p, ctypes.c_uint32,
+                               u32p]
     L.ora_dag_query.argtypes = [vp, u32p, ctypes.c_uint32, u32p]
     L.ora_dag_canonical.restype = ctypes.c_uint64
     L.ora_dag_canonical.argtypes = [vp, u32p, szp, u32p]
@@ -186,9 +187,10 @@ class OracleDag:
     def root(self, r):
         dlib().ora_dag_set_root(self.h, r)
 
-    def edit(self, kind, material, center, radius, src=(0, 0, 0), dst=(0, 0, 0)):
+    def edit(self, kind, material, center, radius, src=(0, 0, 0), dst=(0, 0, 0), rotation=0):
         counts = (ctypes.c_uint32 * 32)()
-        applied = dlib().ora_dag_edit(self.h, kind, material, _f3(center), float(radius), _f3(src), _f3(dst), counts)
+        applied = dlib().ora_dag_edit(self.h, kind, material, _f3(center), float(radius), _f3(src), _f3(dst), rotation,
+                                      counts)
         return applied, list(counts)
 
     def query(self, xyz):

@@ -31,7 +31,7 @@ def test_python_binding_lists_every_declared_symbol():
 
 def test_abi_version_and_error_string():
     lib = svdag_py.lib()
-    assert lib.svdag_abi_version() == 1
+    assert lib.svdag_abi_version() == 2
     assert isinstance(lib.svdag_last_error(), bytes)
 
 

@@ -140,6 +140,103 @@ def test_banded_terrain_and_copies_match_oracle(bits, band, tile):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tile", [3, 5, 8])
+def test_tiled_terrain_widens_tiles_to_the_widest_solid_node(tile):
+    # A high base makes 512-wide nodes at y = 0 solid and fully filled.  A
+    # build tiled narrower than that widens its tiles, so the DAG still equals
+    # the factory rule's (one material: the band width is the whole volume).
+    print(f"[case] tiled terrain tile={tile}", flush=True)
+    g = svdag_py.MyGPUHashDAG(L10, 4)
+    o = oracle_py.OracleDag(L10, 4)
+    tp = dict(workloads.terrain_params(L10))
+    tp.update(base=820.0, amplitude=90.0, material=5)
+    g.build_terrain(**tp, band_log2=L10, band_seed=1, tile_log2=tile)
+    o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"], L10, 1)
+    # The canonical dump lists levels bottom up: [level, count, (size, words)*]*.
+    _, counts, dump = o.canonical(dump=True)
+    p, full_at = 0, {}
+    while p < len(dump) - 1:
+        lvl, n = int(dump[p]), int(dump[p + 1])
+        p += 2
+        for _ in range(n):
+            s = int(dump[p])
+            if lvl < L10 - 2 and (int(dump[p + 1]) >> 8) & 1:
+                full_at[lvl] = True
+            p += 1 + s
+    assert full_at.get(1) and not full_at.get(0)  # 512-wide nodes are full, the root is not
+    _same(g, o)
+    g.close()
+
+
+ROTATED_COPIES = [
+    # (src, dst, center, radius, quarter turns about +Y)
+    ((300.0, 500.0, 300.0), (600.0, 520.0, 650.0), (300.0, 500.0, 300.0), 40.0, 1),
+    ((512.0, 430.0, 512.0), (256.0, 560.0, 768.0), (512.0, 420.0, 512.0), 64.0, 2),
+    ((100.5, 470.25, 700.0), (140.0, 430.5, 690.75), (101.0, 480.0, 699.0), 31.5, 3),
+    ((620.0, 520.0, 640.0), (600.0, 520.0, 650.0), (610.0, 515.0, 655.0), 36.0, 1),  # overlapping regions
+    ((40.0, 500.0, 980.0), (1010.0, 505.0, 20.0), (40.0, 500.0, 980.0), 30.0, 3),    # past the volume edge
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bits", [8, 4])
+def test_rotated_copies_match_oracle(bits):
+    # Copy/paste with 90/180/270 degree turns about +Y on a height-band
+    # terrain, one edit at a time and in a batch, with undo/redo: canonical
+    # DAG and per-level edit-SVO sizes equal the oracle's after every copy.
+    print(f"[case] rotated copies bits={bits}", flush=True)
+    g = svdag_py.MyGPUHashDAG(L10, bits)
+    o = oracle_py.OracleDag(L10, bits)
+    tp = dict(workloads.terrain_params(L10))
+    tp["material"] = 7 % (1 << bits)
+    g.build_terrain(**tp, band_log2=4, band_seed=0xC0DE, tile_log2=0)
+    o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"],
+                    4, 0xC0DE)
+    _same(g, o)
+    hashes = [g.canonical()[0]]
+    for src, dst, c, r, k in ROTATED_COPIES:
+        st = g.edit(svdag_py.COPY, 0, c, r, src, dst, k)
+        applied, counts = o.edit(svdag_py.COPY, 0, c, r, src, dst, k)
+        assert st.applied == applied
+        assert list(st.svo_items)[:L10 - 1] == counts[:L10 - 1], (k, list(st.svo_items)[:L10 - 1], counts[:L10 - 1])
+        _same(g, o)
+        hashes.append(g.canonical()[0])
+    assert len(set(hashes)) == len(hashes)
+    for _ in range(3):
+        g.undo()
+    assert g.canonical()[0] == hashes[-4]
+    g.redo()
+    g.redo()
+    assert g.canonical()[0] == hashes[-2]
+    g.redo()
+    # The same copies as one batch from the first copy's input agree too.
+    o2 = oracle_py.OracleDag(L10, bits)
+    o2.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"],
+                     4, 0xC0DE)
+    g2 = svdag_py.MyGPUHashDAG(L10, bits)
+    g2.build_terrain(**tp, band_log2=4, band_seed=0xC0DE, tile_log2=0)
+    batch = [(svdag_py.COPY, 0, c, r, src, dst, k) for src, dst, c, r, k in ROTATED_COPIES]
+    batch.insert(2, (svdag_py.SPHERE_ADD, 3, (520.0, 440.0, 500.0), 20.0))
+    stats, _ = g2.edit_batch(batch)
+    for e, st in zip(batch, stats):
+        applied, _ = o2.edit(e[0], e[1], e[2], e[3], *e[4:])
+        assert st.applied == applied
+    _same(g2, o2)
+    g.close()
+    g2.close()
+
+
+@pytest.mark.gpu
+def test_rotation_rejected_on_other_tools():
+    g = svdag_py.MyGPUHashDAG(8, 4)
+    with pytest.raises(svdag_py.SvdagError):
+        g.edit(svdag_py.SPHERE_ADD, 1, (10.0, 10.0, 10.0), 4.0, rotation=1)
+    with pytest.raises(svdag_py.SvdagError):
+        g.edit(svdag_py.COPY, 0, (10.0, 10.0, 10.0), 4.0, (10.0, 10.0, 10.0), (40.0, 10.0, 10.0), 4)
+    g.close()
+
+
+@pytest.mark.gpu
 def test_voxel_queries_match_oracle():
     g, o = _pair()
     for kind, mat, c, r in workloads.config1_edits():

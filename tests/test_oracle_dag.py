@@ -209,6 +209,116 @@ def test_edits_match_brute_force_voxelisation():
         assert np.array_equal(got, vox), (step, kind, c, r)
 
 
+def _copy_rule(vox, src, dst, center, r, rot=0):
+    """The copy tool voxel by voxel (my_gpu_dag_editors.h:127-253): every
+    voxel whose unit box overlaps the destination box [c - r, c + r) (c =
+    dst + (center - src), float32) takes the material of the source voxel
+    s = v + trunc(src - dst) when that voxel is set (read from the DAG before
+    the edit) and keeps its own value otherwise.  rot = k quarter turns about
+    +Y instead map s = q + R^k(v - p), R(x, y, z) = (-z - 1, y, x), about the
+    destination pivot p = floor(c + 0.5) and the source pivot
+    q = p + trunc(src - dst)."""
+    f = np.float32
+    side = vox.shape[0]
+    c = [f(dst[a]) + (f(center[a]) - f(src[a])) for a in range(3)]
+    lo = [c[a] - f(r) for a in range(3)]
+    hi = [c[a] + (f(1.0) if r == 0 else f(r)) for a in range(3)]
+    off = [int(-f(dst[a]) + f(src[a])) for a in range(3)]
+    g = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    inbox = np.ones(len(g), bool)
+    for a in range(3):
+        inbox &= (lo[a] < g[:, a] + 1) & (hi[a] > g[:, a])
+    v = g[inbox].astype(np.int64)
+    if rot == 0:
+        s = v + np.array(off)
+    else:
+        p = np.array([int(np.floor(c[a] + f(0.5))) for a in range(3)])
+        q = p + np.array(off)
+        u = v - p
+        x, y, z = u[:, 0], u[:, 1], u[:, 2]
+        for _ in range(rot):
+            x, z = -z - 1, x
+        s = q + np.stack([x, y, z], 1)
+    s = np.maximum(s, 0)
+    inside = np.all(s < side, axis=1)
+    srcval = np.full(len(v), NONE, np.int64)
+    srcval[inside] = vox[s[inside, 0], s[inside, 1], s[inside, 2]]
+    out = vox.copy()
+    keep = srcval == NONE
+    out[v[~keep, 0], v[~keep, 1], v[~keep, 2]] = srcval[~keep]
+    return out
+
+
+COPY_CASES = [
+    # (src, dst, center, radius, rotation)
+    ((40.0, 60.0, 40.0), (84.0, 62.0, 70.0), (40.0, 60.0, 40.0), 11.0, 0),   # integer offset
+    ((40.5, 57.25, 40.75), (80.0, 62.5, 70.0), (41.0, 58.0, 39.5), 9.5, 0),  # fractional offset and box
+    ((70.0, 50.0, 30.0), (60.0, 55.0, 36.0), (70.0, 50.0, 30.0), 13.0, 0),   # overlapping source and destination
+    ((30.0, 58.0, 90.0), (96.0, 60.0, 30.0), (30.0, 58.0, 90.0), 12.0, 1),
+    ((30.0, 58.0, 90.0), (96.0, 60.0, 30.0), (30.0, 58.0, 90.0), 12.0, 2),
+    ((30.0, 58.0, 90.0), (96.0, 60.0, 30.0), (31.0, 57.0, 88.0), 10.5, 3),
+    ((52.25, 60.5, 47.0), (58.0, 61.0, 52.75), (52.0, 60.0, 47.0), 14.0, 1),  # fractional, overlapping, rotated
+    ((20.0, 55.0, 20.0), (120.0, 60.0, 118.0), (20.0, 55.0, 20.0), 15.0, 3),  # destination past the volume edge
+]
+
+
+@pytest.mark.parametrize("case", range(len(COPY_CASES)))
+def test_copy_matches_brute_force(case):
+    src, dst, center, r, rot = COPY_CASES[case]
+    levels = 7
+    side = 1 << levels
+    d, h = _terrain(levels)
+    # Varied materials around the source so that copies carry more than one.
+    rng = np.random.default_rng(5 + case)
+    for _ in range(6):
+        c = tuple(float(center[a]) + float(rng.integers(-10, 11)) for a in range(3))
+        d.edit([SPHERE_ADD, BOX_ADD, PAINT][int(rng.integers(0, 3))], int(rng.integers(2, 16)), c,
+               float(rng.integers(3, 8)))
+    before = _voxels(d, side).astype(np.int64)
+    want = _copy_rule(before, src, dst, center, r, rot)
+    assert not np.array_equal(want, before)
+    d.edit(COPY, 0, center, r, src, dst, rot)
+    got = _voxels(d, side).astype(np.int64)
+    assert np.array_equal(got, want), int((got != want).sum())
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_copy_rotations_compose(k):
+    # A cube with an integer centre and radius maps onto itself under the
+    # quarter turns.  Copied A -> B with k turns and B -> C with 4 - k turns,
+    # both into emptied cubes, C holds exactly A's voxels: the two maps
+    # compose to a translation.  B is A turned: its voxel u (from the cube's
+    # centre) is A's voxel R^k(u).
+    levels = 7
+    side = 1 << levels
+    d, _ = _terrain(levels)
+    rng = np.random.default_rng(40 + k)
+    for _ in range(5):
+        c = tuple(float(v) for v in rng.integers(24, 40, 3))
+        d.edit(BOX_ADD, int(rng.integers(2, 16)), (c[0], c[1] + 24.0, c[2]), float(rng.integers(2, 5)))
+    r = 10
+    A, B, C = (32, 58, 32), (90, 70, 40), (40, 96, 96)
+    for cen in (B, C):
+        d.edit(BOX_REMOVE, 0, tuple(float(v) for v in cen), float(r))
+    d.edit(COPY, 0, tuple(float(v) for v in A), float(r), tuple(float(v) for v in A), tuple(float(v) for v in B), k)
+    d.edit(COPY, 0, tuple(float(v) for v in B), float(r), tuple(float(v) for v in B), tuple(float(v) for v in C),
+           (4 - k) % 4)
+    vox = _voxels(d, side)
+
+    def cube(cen):
+        return vox[cen[0] - r:cen[0] + r, cen[1] - r:cen[1] + r, cen[2] - r:cen[2] + r]
+
+    a, b, c = cube(A), cube(B), cube(C)
+    assert (a != NONE).any() and (a == NONE).any()
+    assert np.array_equal(c, a)
+    u = np.arange(-r, r)
+    ux, uy, uz = np.meshgrid(u, u, u, indexing="ij")
+    x, z = ux, uz
+    for _ in range(k):
+        x, z = -z - 1, x
+    assert np.array_equal(b, a[x + r, uy + r, z + r])
+
+
 def test_paint_overlap_distance_bounds():
     # gpu_hash_dag_tests.cpp:168-196.
     center = (3599.0, 8440.0, 5045.0)
@@ -330,6 +440,110 @@ def test_tracer_hits_first_set_voxel():
                 assert edge < 1e-3, (px, py, got, ref)
                 ambiguous += 1
     assert agree >= 0.98 * W * H
+
+
+def _fma64(a, b, c):
+    """Correctly rounded double fma (exact rational, one rounding)."""
+    from fractions import Fraction
+    return float(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def _primary_rays(pos, rot, fov_deg, bmin, bmax, levels, W, H):
+    """get_trace_params (dag_tracer.cu:59-101) in double, then the per-pixel
+    ray of trace_paths (tracer.cu:402-405): (rayMin + px*ddx + py*ddy) - cam
+    with each add fused (nvcc --fmad=true), normalised as (1/length(a)) * a
+    and rounded to float."""
+    right, up, fwd = -np.array(rot[0:3]), np.array(rot[3:6]), np.array(rot[6:9])
+    fov = fov_deg / 2.0 * (np.pi / 180.0)
+    aspect = W / H
+    cam, rmin, ddx, ddy = [], [], [], []
+    for k in range(3):
+        X, Y, Z = right[k] * np.sin(fov) * aspect, up[k] * np.sin(fov), fwd[k] * np.cos(fov)
+        bl, br, tl = pos[k] + Z - Y - X, pos[k] + Z - Y + X, pos[k] + Z + Y - X
+        tr = -bmin[k]
+        sc = float(1 << levels) / (bmax[k] - bmin[k])
+        cam.append((pos[k] + tr) * sc)
+        rmin.append((bl + tr) * sc)
+        ddx.append(((br + tr) * sc - rmin[k]) * (1.0 / W))
+        ddy.append(((tl + tr) * sc - rmin[k]) * (1.0 / H))
+    rays = {}
+    for py in range(H):
+        for px in range(W):
+            a = [_fma64(py, ddy[k], _fma64(px, ddx[k], rmin[k])) - cam[k] for k in range(3)]
+            inv = 1.0 / np.sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2])
+            d = [np.float32(inv * a[k]) for k in range(3)]
+            rays[(px, py)] = ([np.float32(c) for c in cam], d)
+    return rays
+
+
+def test_tracer_hit_t_and_position_are_the_entry_face():
+    # The hit record's t is the ray's entry distance into the hit voxel's box
+    # (the largest of the per-axis near-slab distances), and its position is
+    # origin + t * dir on that entry face.  Both are checked against exact
+    # rational arithmetic over the float ray, with the rounding of t's three
+    # float operations (1/d, f - o, product) as the tolerance.
+    from fractions import Fraction
+    levels = 7
+    side = 1 << levels
+    d, h = _terrain(levels)
+    d.edit(SPHERE_REMOVE, 0, (60.0, float(h[60, 60]), 60.0), 12.0)
+    pos = (-0.2 * side, 0.9 * side, -0.15 * side)
+    tgt = np.array((0.55 * side, 0.3 * side, 0.5 * side))
+    f = tgt - np.array(pos)
+    f /= np.linalg.norm(f)
+    r = np.cross(f, (0.0, 1.0, 0.0))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    rot = np.concatenate([-r, u, f])
+    W = H = 24
+    hits = d.trace(pos, rot, 60.0, (0, 0, 0), (side, side, side), W, H)
+    rays = _primary_rays(pos, rot, 60.0, (0, 0, 0), (side, side, side), levels, W, H)
+    eps = Fraction(1, 1 << 24)
+    checked = 0
+    for py in range(H):
+        for px in range(W):
+            hit = hits[(H - 1 - py) * W + px]
+            if hit["material"] == NONE:
+                continue
+            o, dv = rays[(px, py)]
+            box = (int(hit["x"]), int(hit["y"]), int(hit["z"]))
+            near = []
+            for a in range(3):
+                assert dv[a] != 0
+                lo, hi = Fraction(box[a]) - Fraction(float(o[a])), Fraction(box[a] + 1) - Fraction(float(o[a]))
+                t1, t2 = lo / Fraction(float(dv[a])), hi / Fraction(float(dv[a]))
+                near.append(min(t1, t2))
+            t_exact = max(near)
+            axis = near.index(t_exact)
+            t = Fraction(float(hit["t"]))
+            assert abs(t - t_exact) <= 4 * eps * abs(t_exact), (px, py, float(t), float(t_exact))
+            p = [Fraction(float(hit[k])) for k in ("px", "py", "pz")]
+            for a in range(3):
+                scale = abs(t_exact * Fraction(float(dv[a]))) + abs(Fraction(float(o[a]))) + 1
+                tol = 8 * eps * scale
+                if a == axis:
+                    face = box[a] if dv[a] > 0 else box[a] + 1
+                    assert abs(p[a] - face) <= tol, (px, py, a, float(p[a]), face)
+                else:
+                    assert box[a] - tol <= p[a] <= box[a] + 1 + tol, (px, py, a, float(p[a]), box)
+            checked += 1
+    assert checked > W * H // 3
+
+
+def test_sun_direction_is_the_float_normalisation():
+    # sun_direction() = normalize(float3(0.3f, 1, 0.5f)) (tracer.cu:600-603)
+    # with normalize(a) = (1 / length(a)) * a in float (cuda_math.h:115-118):
+    # a unit vector within float rounding, parallel to (0.3, 1, 0.5), and not
+    # the double-normalised vector rounded once in at least one component.
+    L = oracle_py.sun_dir()
+    v = np.array([np.float32(0.3), 1.0, 0.5])
+    assert abs(float(np.dot(L.astype(np.float64), L.astype(np.float64))) - 1.0) < 4e-7
+    ref = v / np.linalg.norm(v)
+    assert np.all(np.abs(L.astype(np.float64) - ref) <= 2 * np.spacing(np.float32(1.0)))
+    f = np.float32
+    dot = (f(0.3) * f(0.3) + f(1.0) * f(1.0)) + f(0.5) * f(0.5)
+    inv = f(1.0) / np.sqrt(dot, dtype=np.float32)
+    assert np.array_equal(L, np.array([inv * f(0.3), inv * f(1.0), inv * f(0.5)], np.float32))
 
 
 def test_banded_terrain_voxels_follow_heights_and_bands():

@@ -72,3 +72,50 @@ def test_voxel_build_edge_cases():
     o.edit(svdag_py.SPHERE_REMOVE, 0, (32.0, 32.0, 32.0), 10.0)
     assert g.canonical()[0] == o.canonical()[0]
     g.close()
+
+
+LEAFLESS_GRIDS = ["octant", "half_spaces", "uniform_small", "octant_tree"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", LEAFLESS_GRIDS)
+def test_voxel_build_without_leaves(case):
+    # Grids whose every aligned 4^3 cell is uniform or empty make no leaf in
+    # the edit pipeline.  The reference drops such an edit
+    # (my_gpu_hash_dag_edits.cu:651), but a build is not an edit: the result
+    # must still be the factory rule's DAG (fully-filled nodes only).
+    print(f"[case] leafless voxel build {case}", flush=True)
+    levels = 7
+    side = 1 << levels
+    if case == "octant":
+        gs, vox = side, np.full((side,) * 3, 0xFF, np.uint8)
+        vox[:64, 64:, :64] = 6
+    elif case == "half_spaces":
+        gs, vox = side, np.full((side,) * 3, 3, np.uint8)
+        vox[:, 64:, :] = 9
+    elif case == "uniform_small":
+        gs, vox = 32, np.full((32,) * 3, 4, np.uint8)
+    else:
+        gs, vox = side, np.full((side,) * 3, 0xFF, np.uint8)
+        vox[:32, :32, :32] = 2
+        vox[96:, 32:64, 64:96] = 11
+        vox[64:80, :16, 16:32] = 5
+    g = svdag_py.MyGPUHashDAG(levels, 4)
+    g.build_voxels(vox)
+    full = np.full((side,) * 3, 0xFF, np.uint8)
+    full[:gs, :gs, :gs] = vox
+    o = oracle_py.OracleDag(levels, 4)
+    oracle_py.build_dense(o, full)
+    assert o.root != NONE
+    assert g.root != NONE
+    assert g.canonical()[0] == o.canonical()[0]
+    rng = np.random.default_rng(3)
+    pts = rng.integers(0, side, size=(4000, 3)).astype(np.uint32)
+    want = full[pts[:, 0], pts[:, 1], pts[:, 2]].astype(np.uint32)
+    want[want == 0xFF] = NONE
+    assert np.array_equal(g.query(pts), want)
+    # edits still skip when they make no leaf
+    before = g.canonical()[0]
+    st = g.edit(svdag_py.BOX_ADD, 7, (32.0, 96.0, 32.0), 32.0)
+    assert st.applied == 0 and g.canonical()[0] == before
+    g.close()
