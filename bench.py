@@ -221,6 +221,11 @@ def editing_bench(args):
 
     g, build_s, gc0 = fresh()
     log(f"[bench] depth {D} terrain: {build_s:.2f} s, {gc0.items_live} live items")
+    terrain_dump = None
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        terrain_hash, _, terrain_dump = g.canonical(dump=True)
+        log(f"[bench] terrain canonical dump: {terrain_dump.size} words, {time.perf_counter() - t:.1f} s")
 
     def h(x, z):
         return int(g.terrain_heights(x, z, 1, 1)[0, 0])
@@ -248,6 +253,16 @@ def editing_bench(args):
         log(f"[bench] shading: {shading}")
     g.close()
 
+    # Every leg applies the same 256-edit stream to the same terrain, so every
+    # leg must end on the same canonical DAG (hash over the renumbered dump);
+    # any difference fails the run.
+    hashes = {}
+
+    def record_hash(leg, g):
+        t = time.perf_counter()
+        hashes[leg] = g.canonical()[0]
+        log(f"[bench] canonical after {leg}: {hashes[leg]:#018x} ({time.perf_counter() - t:.1f} s)")
+
     # plain per-edit form
     g, _, _ = fresh()
     lat = []
@@ -262,11 +277,12 @@ def editing_bench(args):
              "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
              "latency_ms_p99": float(np.percentile(lat, 99) * 1e3)}
     log(f"[bench] plain: {plain}")
+    record_hash("plain", g)
     g.close()
 
-    def batched_leg(gc_each):
+    def batched_leg(gc_each, mode, name):
         g, _, _ = fresh()
-        g.set_batch_mode(1)
+        g.set_batch_mode(mode)
         tune = g.autotune(edits[:16])
         g.garbage_collect()
         blat = []
@@ -283,21 +299,37 @@ def editing_bench(args):
         if not gc_each:
             gc_ms += g.garbage_collect().gpu_ms
         dt = time.perf_counter() - t2
+        record_hash(name, g)
         g.close()
         return {"voxel_edits_per_s": float(counts.sum() / dt), "edits_per_s": len(edits) / dt,
                 "edit_gpu_ms_total": round(gpu_ms, 3), "gc_gpu_ms_total": round(gc_ms, 3),
                 "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
                 "latency_ms_p99": float(np.percentile(blat, 99) * 1e3)}, tune
 
-    batched, (tune_g, tune_ms) = batched_leg(False)
+    batched, (tune_g, tune_ms) = batched_leg(False, 1, "fused, one collection")
     batched.update({"edits": len(edits), "batch": 16, "gc": "once after the stream",
                     "autotuned_lanes_per_item": tune_g,
                     "autotune_ms": {str(k): round(v, 3) for k, v in tune_ms.items()}})
     batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
-    each, _ = batched_leg(True)
+    each, _ = batched_leg(True, 1, "fused, collection after each batch")
     each["speedup_vs_plain"] = each["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
     batched["gc_each_batch"] = each
+    graph, _ = batched_leg(True, 2, "graph, collection after each batch")
+    graph["speedup_vs_plain"] = graph["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
+    batched["graph_gc_each_batch"] = graph
     log(f"[bench] batched: {batched}")
+
+    # The same stream through the serving loop, all requests at t = 0.
+    g, _, _ = fresh()
+    g.autotune(edits[:16])
+    g.garbage_collect()
+    info, _ = g.serve_replay(edits, np.zeros(len(edits)), max_batch=64, allow_gc=True)
+    if not (info["status"] == 0).all():
+        raise RuntimeError("serving replay of the batched-editing stream: a request failed")
+    record_hash("serving replay", g)
+    g.close()
+    if len(set(hashes.values())) != 1:
+        raise RuntimeError(f"legs disagree on the final DAG: { {k: hex(v) for k, v in hashes.items()} }")
 
     serving = []
     g, _, _ = fresh()
@@ -325,7 +357,10 @@ def editing_bench(args):
         serve_row(rate, reqs, arr, 64)
     sat = serve_row("saturation", sat_reqs, np.zeros(len(sat_reqs)), 64)
     g.close()
+    cpu = cpu_edit_baseline(args, terrain_dump, edits, D) if not args.no_cpu_baseline else None
     return {"levels": D, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
+            "cpu_baseline": cpu,
+            "canonical_hash": f"{hashes['plain']:#018x}", "legs_with_that_hash": list(hashes),
             "plain": plain, "batched": batched, "serving": serving, "shading": shading,
             "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"]}
 
@@ -336,63 +371,104 @@ def large_scene_bench(args):
     of 512 voxels plus a seeded 0..3 offset per 512-column tile), built one
     16384-wide column tile per edit.  Then `large_copies` copy/paste edits, each
     a 512^3 region (centre = source = a seeded point near the surface, radius
-    256) pasted at a seeded point near the surface elsewhere, in groups of 8;
-    after each group the last 8 edits are undone and redone.  Translation
-    only: the reference's copy tool has no rotation (my_gpu_dag_editors.h:127-253).
-    A collection ends the stream.  Tables start sized for `large_expected`
-    items and grow as the allocators run short."""
+    256) pasted at a seeded point near the surface elsewhere with a 90 degree
+    turn about +Y, in groups of 8; after each group the last 8 edits are undone
+    and redone.  A collection ends the stream.  Tables start sized for
+    `large_expected` items and grow as the allocators run short.
+
+    Checks (outside the timed edits): terrain columns against the height field
+    and the band rule; for every copy, seeded destination voxels against the
+    copy rule (a set source voxel replaces the destination's value, an empty
+    one keeps it); after the undos, the first copy's destination voxels hold
+    their values from before the group; after the redos, the last copy's hold
+    their values from after it.  Any mismatch fails the leg."""
     import svdag_py
     import workloads
 
     D, M = args.large_levels, 8
     side = 1 << D
+    band_log2, band_seed = 9, 0xBA5E
     tp = dict(workloads.terrain_params(D))
     t0 = time.perf_counter()
     g = svdag_py.MyGPUHashDAG(D, M, expected_items=int(args.large_expected), workspace_items=1 << 26)
-    g.build_terrain(**tp, band_log2=9, band_seed=0xBA5E, tile_log2=D - 3)
+    g.build_terrain(**tp, band_log2=band_log2, band_seed=band_seed, tile_log2=D - 3)
     build_s = time.perf_counter() - t0
     gc0 = g.garbage_collect()
     log(f"[bench] depth {D} banded terrain: {build_s:.2f} s, {gc0.items_live} live items")
     rng = np.random.default_rng(10)
+
+    # Terrain columns: the surface voxel and one deeper voxel hold the band
+    # material, the voxel above the surface is empty.
+    cols = rng.integers(0, side, size=(2000, 2))
+    hts = np.array([int(g.terrain_heights(int(x), int(z), 1, 1)[0, 0]) for x, z in cols[:2000]], np.int64)
+    xs, zs = cols[:2000, 0], cols[:2000, 1]
+    ok = (hts >= 64) & (hts + 1 < side)
+    xs, zs, hts = xs[ok], zs[ok], hts[ok]
+    for dy, solid in ((0, True), (-63, True), (1, False)):
+        ys = hts + dy
+        got = g.query(np.stack([xs, ys, zs], axis=1))
+        want = (workloads.band_materials(xs, ys, zs, tp["material"], band_log2, band_seed, M) if solid
+                else np.full(len(xs), svdag_py.NOT_FOUND, np.uint32))
+        if not np.array_equal(got, want):
+            raise RuntimeError(f"terrain column check failed at dy={dy}: {int((got != want).sum())} mismatches")
+    log(f"[bench] terrain columns checked: {3 * len(xs)} voxels")
 
     def near_surface():
         x, z = (int(v) for v in rng.integers(256, side - 256, 2))
         y = int(g.terrain_heights(x, z, 1, 1)[0, 0]) + int(rng.integers(-256, 257))
         return (float(x), float(min(max(y, 256), side - 256)), float(z))
 
+    rotation = 1
     copies = [(near_surface(), near_surface()) for _ in range(args.large_copies)]
     lat, undo_ms, redo_ms = [], [], []
     roots = []
-    t1 = time.perf_counter()
+    checked = 0
+    group_before = None
+    after = []
     for i, (src, dst) in enumerate(copies):
+        v = workloads.copy_box_voxels(dst, 256.0, 4096, rng, D)
+        sv = workloads.copy_source_voxels(v, src, 256.0, src, dst, rotation)
+        old = g.query(v)
+        srcv = g.query(sv)
+        if i % 8 == 0:
+            group_before = (v, old)
         s0 = time.perf_counter()
-        g.edit(svdag_py.COPY, 0, src, 256.0, src, dst)
+        g.edit(svdag_py.COPY, 0, src, 256.0, src, dst, rotation=rotation)
         lat.append(time.perf_counter() - s0)
+        new = g.query(v)
+        want = np.where(srcv != svdag_py.NOT_FOUND, srcv, old)
+        if not np.array_equal(new, want):
+            raise RuntimeError(f"copy {i}: {int((new != want).sum())} of {len(v)} sampled voxels break the copy rule")
+        checked += len(v)
+        after = (v, new)
         roots.append(g.root)
         if i % 8 == 7:
             for _ in range(8):
                 s0 = time.perf_counter()
                 g.undo()
                 undo_ms.append((time.perf_counter() - s0) * 1e3)
+            if not np.array_equal(g.query(group_before[0]), group_before[1]):
+                raise RuntimeError("undo of 8 copies did not restore the sampled voxels")
             for _ in range(8):
                 s0 = time.perf_counter()
                 g.redo()
                 redo_ms.append((time.perf_counter() - s0) * 1e3)
-            if g.root != roots[-1]:
-                raise RuntimeError("redo did not restore the root")
+            if g.root != roots[-1] or not np.array_equal(g.query(after[0]), after[1]):
+                raise RuntimeError("redo did not restore the root and the sampled voxels")
     gc = g.garbage_collect()
-    dt = time.perf_counter() - t1
+    edit_s = sum(lat) + (sum(undo_ms) + sum(redo_ms)) * 1e-3 + gc.gpu_ms * 1e-3
     mem = g.memory_usage()
     g.close()
     vox = float(len(copies)) * 512.0 ** 3
     out = {"levels": D, "material_bits": M, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
-           "copies": len(copies), "region": "512^3", "copies_per_s": len(copies) / dt,
-           "voxel_edits_per_s": vox / dt,
+           "copies": len(copies), "region": "512^3", "rotation_quarter_turns": rotation,
+           "copies_per_s": len(copies) / edit_s, "voxel_edits_per_s": vox / edit_s,
            "copy_latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
            "copy_latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
            "undo_ms_p99": float(np.percentile(undo_ms, 99)) if undo_ms else None,
            "redo_ms_p99": float(np.percentile(redo_ms, 99)) if redo_ms else None,
            "gc_gpu_ms": round(gc.gpu_ms, 3), "live_items_after": gc.items_live,
+           "voxels_checked": int(checked + 3 * len(xs)),
            "table_bytes_reserved": int(mem.bytes_reserved), "slabs_used": int(mem.slabs_used),
            "slab_capacity": int(mem.slab_capacity)}
     log(f"[bench] large scene: {out}")
@@ -418,12 +494,20 @@ def editing_profile(args):
         return int(g.terrain_heights(x, z, 1, 1)[0, 0])
 
     edits = workloads.config3_edits(h, D, n=256, seed=6)
-    g.set_batch_mode(1)
+    g.set_batch_mode(args.batch_mode)
     tune_g, _ = g.autotune(edits[:16])
     g.garbage_collect()
     rows = []
+    marked = args.marked_batch
     for b in range(0, len(edits), 16):
+        if b // 16 == marked:
+            # Separator launches around the batch (a 1 MiB stream read, kernel
+            # probe_stream), so that per-dispatch counter rows of this one batch
+            # can be told apart from the build's and the other batches'.
+            svdag_py.hbm_probe(svdag_py.PROBE_STREAM, 1 << 20)
         st, ms = g.edit_batch(edits[b:b + 16])
+        if b // 16 == marked:
+            svdag_py.hbm_probe(svdag_py.PROBE_STREAM, 1 << 20)
         items = [sum(x.svo_items[l] for x in st) for l in range(D - 1)]
         log(f"[profile] batch {b // 16}: {ms:.3f} ms, SVO items per level {items}")
         rows.append(ms)
@@ -434,40 +518,111 @@ def editing_profile(args):
                       "gc_ms": round(gc.gpu_ms, 3), "lanes_per_item": tune_g}), flush=True)
 
 
-def cpu_edit_baseline(args):
-    """The oracle's per-edit path (a single-threaded restatement of the same
-    edit pipeline) on a bounded sample: the batched-editing stream generated
-    at depth 12, applied one edit at a time until `cpu_seconds` have passed."""
+def host_cpu_info():
+    """CPU model, online CPUs (nproc) and the CPUs this process may run on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_allowed": allowed}
+
+
+def cpu_edit_baseline(args, dump15, edits15, D):
+    """The reference's CPU HashDAG editing path (geometry only) restated in
+    C++ (oracle/cpu_hashdag.cpp), on the host cores, with NUM_THREADS = 32
+    worker threads (the reference default) and with 1.
+    config 1: the depth-10 terrain (imported from the GPU build's canonical
+      dump), sphere-add r=64 then sphere-remove r=48; median over 5 runs
+      after one warm-up, each on a fresh import.
+    stream:   the depth-15 terrain and the 256-edit batched-editing stream
+      applied one edit at a time, until `cpu_seconds` of edit time have
+      passed (a bounded sample).
+    voxel edits/s counts each edit's voxels by the CPU tool's own predicate
+    (its sphere has no +0.5 centre); paint / copy do not occur in the stream."""
     import oracle_py
+    import svdag_py
     import workloads
 
-    D = 12
-    tp = workloads.terrain_params(D)
-    o = oracle_py.OracleDag(D, 4)
-    o.build_terrain(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"], tp["material"])
-    side = 1 << D
-    hts = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
-                                    0, 0, side, side)
-    edits = workloads.config3_edits(lambda x, z: int(hts[z, x]), D, n=256, seed=6)
-    vox, n = 0, 0
-    t0 = time.perf_counter()
-    for e in edits:
-        o.edit(*e)
-        vox += workloads.voxel_edit_count(e[0], e[2], e[3], D)
-        n += 1
-        if time.perf_counter() - t0 > args.cpu_seconds:
-            break
-    dt = time.perf_counter() - t0
-    log(f"[bench] cpu edit baseline: {n} edits in {dt:.1f} s")
-    return {"value": round(vox / dt / 1e9, 3), "unit": "G voxel edits/s", "cores": 1, "kind": "port",
-            "sample": f"{n} edits of the batched-editing stream generated at depth 12 (same mix and radii), "
-                      f"oracle restatement of the GPU edit path, one edit at a time"}
+    kinds = {svdag_py.SPHERE_ADD: 0, svdag_py.SPHERE_REMOVE: 1, svdag_py.BOX_ADD: 2, svdag_py.BOX_REMOVE: 3}
+
+    def cpu_count(kind, c, r, levels):
+        if kind in (svdag_py.SPHERE_ADD, svdag_py.SPHERE_REMOVE):
+            return workloads.voxel_edit_count(kind, tuple(x - 0.5 for x in c), r, levels)
+        return workloads.voxel_edit_count(kind, c, r, levels)
+
+    def make(levels, dump):
+        top = 0
+        for _ in range(4):
+            d = oracle_py.CpuHashDag(levels, top_span=top)
+            try:
+                t = time.perf_counter()
+                d.import_canonical(dump, threads=16)
+                return d, top or 1024, time.perf_counter() - t
+            except RuntimeError:
+                top = 2 * (top or 1024)
+                del d
+        raise RuntimeError("CPU HashDAG: the terrain overflows the bucket spans")
+
+    info = host_cpu_info()
+    out = {"host": info, "kind": "port"}
+    # config 1 at depth 10
+    g = svdag_py.MyGPUHashDAG(10, 4)
+    g.build_terrain(**workloads.terrain_params(10))
+    _, _, dump10 = g.canonical(dump=True)
+    g.close()
+    c1 = workloads.config1_edits()
+    vox1 = sum(cpu_count(k, c, r, 10) for k, m, c, r in c1)
+    for threads in (32, 1):
+        times = []
+        for rep in range(6):
+            d, _, _ = make(10, dump10)
+            ms = sum(d.edit(kinds[k], c, r, threads)[0] for k, m, c, r in c1)
+            del d
+            if rep:
+                times.append(ms)
+        med = float(np.median(times))
+        out[f"config1_threads{threads}"] = {"ms_for_both_edits_median": round(med, 3),
+                                             "voxel_edits_per_s": vox1 / (med * 1e-3)}
+    # the depth-D stream
+    if dump15 is not None:
+        for threads, budget in ((32, args.cpu_seconds), (1, args.cpu_seconds / 2)):
+            d, top, imp_s = make(D, dump15)
+            ms_total, vox, n = 0.0, 0, 0
+            for k, m, c, r in edits15:
+                ms, _ = d.edit(kinds[k], c, r, threads)
+                ms_total += ms
+                vox += cpu_count(k, c, r, D)
+                n += 1
+                if ms_total * 1e-3 > budget:
+                    break
+            del d
+            out[f"stream_threads{threads}"] = {"edits": n, "edit_s": round(ms_total * 1e-3, 3),
+                                                "voxel_edits_per_s": vox / (ms_total * 1e-3),
+                                                "edits_per_s": n / (ms_total * 1e-3),
+                                                "import_s": round(imp_s, 2), "top_level_bucket_words": top}
+            log(f"[bench] cpu HashDAG stream, {threads} threads: {out[f'stream_threads{threads}']}")
+    s32 = out.get("stream_threads32")
+    out.update({"value": round((s32 or out["config1_threads32"])["voxel_edits_per_s"] / 1e9, 4),
+                "unit": "G voxel edits/s", "cores": 32,
+                "sample": (f"{s32['edits']} edits of the depth-{D} batched-editing stream, one at a time, "
+                           f"32 threads" if s32 else "config 1 at depth 10, 32 threads")
+                + " (reference CPU HashDAG edit path, C++ restatement)"})
+    log(f"[bench] cpu HashDAG baseline: {out}")
+    return out
 
 
 def cpu_baseline(items, finds, buckets, args):
     import oracle_py
 
-    threads = min(args.cpu_threads, os.cpu_count() or 1)
+    threads = args.cpu_threads
     log(f"[bench] cpu baseline: oracle tables, {threads} threads")
     tables = {}
     for s, a in items.items():
@@ -492,6 +647,7 @@ def cpu_baseline(items, finds, buckets, args):
         "kind": "port",
         "sample": f"{reps} pass(es) over the same {sum(len(a) for a in finds.values())} finds "
                   f"(oracle restatement of CompactAccelerationHashTable::find)",
+        "host": host_cpu_info(),
     }
 
 
@@ -520,6 +676,9 @@ def main():
     ap.add_argument("--large-expected", type=float, default=4e8, help="items the depth-17 tables start sized for")
     ap.add_argument("--editing-profile", action="store_true",
                     help="only the batched-editing stream and its collection (for rocprofv3 --kernel-trace)")
+    ap.add_argument("--batch-mode", type=int, default=1, help="editing profile: 1 fused, 2 fused + HIP graphs")
+    ap.add_argument("--marked-batch", type=int, default=-1,
+                    help="editing profile: bracket this batch with probe_stream launches (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this back-end has no multi-GPU path "
@@ -536,8 +695,6 @@ def main():
         return
     if not args.no_editing:
         out["editing"] = editing_bench(args)
-        if not args.no_cpu_baseline:
-            out["editing"]["cpu_baseline"] = cpu_edit_baseline(args)
     if args.large_scene:
         out["large_scene"] = large_scene_bench(args)
     print(json.dumps(out), flush=True)

@@ -1153,8 +1153,9 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
                 SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
                 if (a.oom) {
                     why = "table of " + std::to_string(t.itemWords) + "-word items full at " + std::to_string(t.slabCap)
-                        + " lines (limit " + std::to_string(ctx->slabLimit) + ")";
-                    SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit), ctx->stream));
+                        + " lines (limit " + std::to_string(ctx->slabLimit[t.itemWords - 2]) + ")";
+                    SVDAG_TRY(table_grow(
+                        t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit[t.itemWords - 2]), ctx->stream));
                 }
             }
             SVDAG_TRY(ctx_refresh_tables(ctx));
@@ -1288,6 +1289,7 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
     SVDAG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     SVDAG_HIP(hipEventCreate(&ctx->ev0));
     SVDAG_HIP(hipEventCreate(&ctx->ev1));
+    SVDAG_HIP(hipEventCreateWithFlags(&ctx->evWait, hipEventDisableTiming));
     SVDAG_HIP(hipHostMalloc((void**)&ctx->hStatus, sizeof(EditStatus) + kMaxLevels * sizeof(LevelDesc)));
     ctx->hLevels = reinterpret_cast<LevelDesc*>(ctx->hStatus + 1);
 
@@ -1301,9 +1303,9 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
     d.leafSizeBits = bit_width(d.maxLeafWords - d.minLeafWords);
     const uint32_t maxWords = std::max(9u, d.maxLeafWords);
     d.numTables = maxWords - 2 + 1;
-    const uint32_t indexBits = std::min(32u - kNodeSizeBits, 32u - d.leafSizeBits);
-    const uint64_t slabLimit = 1ull << (indexBits - kSlotBits);
-    ctx->slabLimit = slabLimit;
+    ctx->slabLimit.resize(d.numTables);
+    for (uint32_t i = 0; i < d.numTables; ++i)
+        ctx->slabLimit[i] = table_line_limit(2 + i, d.minLeafWords, d.maxLeafWords, d.leafSizeBits);
 
     // Buckets per table: the table's share of expected_items fills its head
     // lines to load_factor percent.  Half of the items are spread evenly over
@@ -1319,9 +1321,9 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
         const uint64_t share = (uint64_t)(frac * (double)cfg.expected_items);
         const uint64_t perLine = (uint64_t)line_slots(2 + i) * cfg.load_factor;
         const uint64_t want = std::max<uint64_t>(64, (share * 100 + perLine - 1) / perLine);
-        const uint32_t buckets = (uint32_t)std::min<uint64_t>(want, slabLimit / 2);
+        const uint32_t buckets = (uint32_t)std::min<uint64_t>(want, ctx->slabLimit[i] / 2);
         uint64_t cap = buckets + (cfg.arena_slabs ? cfg.arena_slabs : buckets / 2 + 4096);
-        cap = std::min<uint64_t>(cap, slabLimit);
+        cap = std::min<uint64_t>(cap, ctx->slabLimit[i]);
         SVDAG_TRY(table_alloc(ctx->ts.tables[i], 2 + i, buckets, (uint32_t)cap, ctx->stream, true));
     }
     SVDAG_TRY(ctx_refresh_tables(ctx));
@@ -1373,6 +1375,8 @@ int svdag_destroy(svdag_ctx* ctx)
     (void)hipHostFree(ctx->hStatus);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    if (ctx->evWait)
+        (void)hipEventDestroy(ctx->evWait);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SVDAG_OK;
@@ -1654,11 +1658,8 @@ int svdag_root(svdag_ctx* ctx, uint32_t* out)
     return SVDAG_OK;
 }
 
-int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* stats)
+static int edit_locked(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* stats)
 {
-    if (!ctx || !desc)
-        return SVDAG_E_INVALID;
-    std::lock_guard<std::mutex> lock(ctx->mu);
     EditorDev e;
     SVDAG_TRY(make_editor(ctx, *desc, e));
     // MyGPUHashDAGUndoRedo::add_frame (my_gpu_base_dag.h:23-34), recorded
@@ -1669,6 +1670,27 @@ int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* st
     }
     svdag_edit_stats local;
     return run_edit(ctx, e, stats ? stats : &local);
+}
+
+int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* stats)
+{
+    if (!ctx || !desc)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return edit_locked(ctx, desc, stats);
+}
+
+int svdag_edit_on_stream(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit_stats* stats, void* stream)
+{
+    if (!ctx || !desc)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (stream) {
+        // The edit starts after the work queued on the caller's stream so far.
+        SVDAG_HIP(hipEventRecord(ctx->evWait, static_cast<hipStream_t>(stream)));
+        SVDAG_HIP(hipStreamWaitEvent(ctx->stream, ctx->evWait, 0));
+    }
+    return edit_locked(ctx, desc, stats);
 }
 
 int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, svdag_edit_stats* stats, float* gpu_ms)

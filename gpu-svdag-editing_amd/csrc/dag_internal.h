@@ -195,9 +195,10 @@ struct svdag_ctx {
     svdag::WsDev ws {};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evWait = nullptr; // svdag_edit_on_stream: the caller's stream position
     std::vector<uint32_t> undo, redo;
     uint32_t groupSize = 16;
-    uint64_t slabLimit = 0; // most slabs a table may hold: slab ids must fit the handles' index bits
+    std::vector<uint64_t> slabLimit; // per table: most lines it may hold (table_line_limit)
     // Host-visible status block (pinned): EditStatus followed by the level table.
     svdag::EditStatus* hStatus = nullptr;
     svdag::LevelDesc* hLevels = nullptr;

@@ -78,7 +78,8 @@ static uint32_t h_leaf_size(const HostDag& h, uint32_t handle)
 }
 static uint32_t h_leaf_ptr(const HostDag& h, uint32_t handle)
 {
-    return h.leafSizeBits ? handle & ((1u << (32u - h.leafSizeBits)) - 1u) : handle;
+    const uint32_t idx = h.leafSizeBits ? handle & ((1u << (32u - h.leafSizeBits)) - 1u) : handle;
+    return index_to_ptr(idx, h_leaf_size(h, handle));
 }
 
 // Breadth-first walk of every (level, handle) reachable from the roots.
@@ -260,7 +261,8 @@ static int insert_level(svdag_ctx* ctx, const std::vector<std::vector<uint32_t>>
         {
             FrameQuiesce q(ctx);
             for (auto& t : ctx->ts.tables)
-                SVDAG_TRY(table_grow(t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit), ctx->stream));
+                SVDAG_TRY(table_grow(
+                    t, (uint32_t)std::min<uint64_t>(2ull * t.slabCap, ctx->slabLimit[t.itemWords - 2]), ctx->stream));
             SVDAG_TRY(ctx_refresh_tables(ctx));
         }
         std::vector<uint32_t> again;

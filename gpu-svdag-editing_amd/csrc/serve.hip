@@ -224,7 +224,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                 HostTable& t = ctx->ts.tables[i];
                 if (need * srv->slabsPerItem[i] <= room.room[i])
                     continue;
-                const uint64_t target = std::min<uint64_t>((uint64_t)t.slabCap * 2, ctx->slabLimit);
+                const uint64_t target = std::min<uint64_t>((uint64_t)t.slabCap * 2, ctx->slabLimit[i]);
                 if (target > t.slabCap && table_grow(t, (uint32_t)target, ctx->stream) == SVDAG_OK)
                     grew = true;
             }

@@ -592,9 +592,35 @@ __host__ __device__ inline uint32_t leaf_size_from_words(uint32_t w0, uint32_t w
     const uint32_t n = (uint32_t)__builtin_popcount(w0) + (uint32_t)__builtin_popcount(w1);
     return 2u + ((n * M + 31u) >> 5);
 }
+
+// Handle index <-> table pointer.  Inside the library a pointer is always
+// line << 4 | slot, so every per-slot array keeps 16 entries per line.  A
+// handle stores its index with only as many slot bits as the table's lines
+// have slots (bit_width(line_slots(s) - 1)): 4 bits at 2-3 words, 3 at 4-5,
+// 2 at 6-10, 1 at 11-15 and none from 16 words.  A table of 9-word items thus
+// addresses 2^27 lines with a node handle's 29 index bits, where a fixed
+// 4-bit slot code would stop it at 2^25 (2^24 at 8 material bits).  Packed
+// 3 bits per size, size 2 first.
+constexpr unsigned long long kSlotCodeBitsPacked =
+    (4ull << 0) | (4ull << 3) | (3ull << 6) | (3ull << 9) | (2ull << 12) | (2ull << 15) | (2ull << 18)
+    | (2ull << 21) | (2ull << 24) | (1ull << 27) | (1ull << 30) | (1ull << 33) | (1ull << 36) | (1ull << 39);
+__host__ __device__ inline uint32_t slot_code_bits(uint32_t s)
+{
+    return (uint32_t)(kSlotCodeBitsPacked >> (3u * (s - 2u))) & 7u;
+}
+__host__ __device__ inline uint32_t ptr_to_index(uint32_t ptr, uint32_t s)
+{
+    return ((ptr >> kSlotBits) << slot_code_bits(s)) | (ptr & (kMaxSlots - 1u));
+}
+__host__ __device__ inline uint32_t index_to_ptr(uint32_t idx, uint32_t s)
+{
+    const uint32_t b = slot_code_bits(s);
+    return ((idx >> b) << kSlotBits) | (idx & ((1u << b) - 1u));
+}
+
 __host__ __device__ inline uint32_t node_handle(uint32_t ptr, uint32_t size)
 {
-    return ptr | ((size - 2u) << (32u - kNodeSizeBits));
+    return ptr_to_index(ptr, size) | ((size - 2u) << (32u - kNodeSizeBits));
 }
 __host__ __device__ inline uint32_t node_handle_size(uint32_t h)
 {
@@ -602,13 +628,13 @@ __host__ __device__ inline uint32_t node_handle_size(uint32_t h)
 }
 __host__ __device__ inline uint32_t node_handle_ptr(uint32_t h)
 {
-    return h & ((1u << (32u - kNodeSizeBits)) - 1u);
+    return index_to_ptr(h & ((1u << (32u - kNodeSizeBits)) - 1u), node_handle_size(h));
 }
 __host__ __device__ inline uint32_t leaf_handle(const DagDev& d, uint32_t ptr, uint32_t size)
 {
     if (d.leafSizeBits == 0)
-        return ptr;
-    return ptr | ((size - d.minLeafWords) << (32u - d.leafSizeBits));
+        return ptr_to_index(ptr, size);
+    return ptr_to_index(ptr, size) | ((size - d.minLeafWords) << (32u - d.leafSizeBits));
 }
 __host__ __device__ inline uint32_t leaf_handle_size(const DagDev& d, uint32_t h)
 {
@@ -619,8 +645,22 @@ __host__ __device__ inline uint32_t leaf_handle_size(const DagDev& d, uint32_t h
 __host__ __device__ inline uint32_t leaf_handle_ptr(const DagDev& d, uint32_t h)
 {
     if (d.leafSizeBits == 0)
-        return h;
-    return h & ((1u << (32u - d.leafSizeBits)) - 1u);
+        return index_to_ptr(h, d.minLeafWords);
+    return index_to_ptr(h & ((1u << (32u - d.leafSizeBits)) - 1u), leaf_handle_size(d, h));
+}
+// Most lines a table of s-word items may hold: every item's handle index
+// must fit the index bits of each kind of item stored there, the index must
+// not be all ones (kSentinel), and a pointer (line << 4 | slot) must fit 32
+// bits.
+__host__ __device__ inline uint64_t table_line_limit(uint32_t s, uint32_t minLeafWords, uint32_t maxLeafWords,
+                                                     uint32_t leafSizeBits)
+{
+    uint32_t ib = 32u - kSlotBits + slot_code_bits(s); // pointer range, in handle-index bits
+    if (s <= 9u && ib > 32u - kNodeSizeBits)
+        ib = 32u - kNodeSizeBits;
+    if (s >= minLeafWords && s <= maxLeafWords && ib > 32u - leafSizeBits)
+        ib = 32u - leafSizeBits;
+    return (1ull << (ib - slot_code_bits(s))) - 1ull;
 }
 __device__ inline const uint32_t* dag_node_ptr(const DagDev& d, uint32_t h)
 {

@@ -17,7 +17,15 @@ int main(int argc, char** argv)
         MyGPUHashDAGUndoRedo undoRedo;
         const float s = (float)(side / 1024.0);
         editMyHashDag(MyGpuSphereEditor<true>({ 300 * s, 520 * s, 400 * s }, 40 * s, 2), dag, undoRedo);
-        editMyHashDag(MyGpuBoxEditor<false>({ 310 * s, 530 * s, 410 * s }, 20 * s, 0), dag, undoRedo);
+        // The reference's full signature: stats sink, memory pool, stream.
+        StatsRecorder stats;
+        GpuMemoryPool memPool;
+        editMyHashDag(MyGpuBoxEditor<false>({ 310 * s, 530 * s, 410 * s }, 20 * s, 0), dag, undoRedo, stats, memPool,
+                      nullptr);
+        if (stats.values.size() < 3 || stats.values[1].second != 1.0) {
+            std::fprintf(stderr, "editMyHashDag reported no statistics\n");
+            return 3;
+        }
         editMyHashDag(MyGpuSpherePaintEditor({ 600 * s, 500 * s, 600 * s }, 50 * s, { 1, 0, 0 }, 7), dag, undoRedo);
         editMyHashDag(MyGpuCopyEditor<MyGPUHashDAG>(dag, { 300 * s, 520 * s, 400 * s }, { 700 * s, 530 * s, 200 * s },
                                                     { 300 * s, 520 * s, 400 * s }, 30 * s),

@@ -226,6 +226,34 @@ svdag_edit_stats editMyHashDag(const Editor& editor, MyGPUHashDAG& dag, MyGPUHas
     return st;
 }
 
+// The reference's statistics sink (stats.h): named values reported per edit.
+struct StatsRecorder {
+    std::vector<std::pair<std::string, double>> values;
+    void report(const std::string& name, double value) { values.emplace_back(name, value); }
+};
+// The reference's transient device allocator (memory.h:192-224).  The context
+// allocates its edit workspace once and keeps it, so there is nothing to pass.
+struct GpuMemoryPool {
+};
+
+// The reference's full signature (my_gpu_hash_dag_edits.h:12-13): the edit
+// starts after the work queued on `stream` (a hipStream_t, or nullptr) and has
+// finished when the call returns; its GPU time, edit-SVO items per level and
+// whether it was applied are reported to `stats`.
+template <typename Editor, typename Stream>
+svdag_edit_stats editMyHashDag(const Editor& editor, MyGPUHashDAG& dag, MyGPUHashDAGUndoRedo&, StatsRecorder& stats,
+                               GpuMemoryPool&, Stream stream)
+{
+    const svdag_edit_desc d = editor.desc();
+    svdag_edit_stats st {};
+    check(svdag_edit_on_stream(dag.context(), &d, &st, (void*)stream));
+    stats.report("editing", st.gpu_ms);
+    stats.report("applied", st.applied);
+    for (uint32_t l = 0; l < 32 && st.svo_items[l]; ++l)
+        stats.report("edit SVO items, level " + std::to_string(l), st.svo_items[l]);
+    return st;
+}
+
 // ------------------------------------------------------------------ tracer
 struct CameraView {
     double position[3] = { 0, 0, 0 };

@@ -205,3 +205,43 @@ def camera_path(levels, n, seed=8):
                0.5 * side + float(rng.uniform(-0.05, 0.05)) * side)
         out.append((pos, tgt, 60.0, (0.0, 0.0, 0.0), (side, side, side)))
     return out
+
+
+def band_materials(x, y, z, material, band_log2, band_seed, material_bits):
+    """Material of solid voxels under svdag_build_terrain_banded's height-band
+    rule, vectorised: (material + (y >> b) + (mix(seed ^ (x>>b)*0x9E3779B1 ^
+    (z>>b)*0x85EBCA77) & 3)) & (2^M - 1), mix(h) = t ^ (t >> 12) with
+    t = (h ^ h >> 15) * 0x2C1B3C6D, all in u32 arithmetic."""
+    x, y, z = (np.asarray(v, np.uint64) for v in (x, y, z))
+    b = np.uint64(band_log2)
+    m32 = np.uint64(0xFFFFFFFF)
+    h = (np.uint64(band_seed) ^ (((x >> b) * np.uint64(0x9E3779B1)) & m32)
+         ^ (((z >> b) * np.uint64(0x85EBCA77)) & m32))
+    h = ((h ^ (h >> np.uint64(15))) * np.uint64(0x2C1B3C6D)) & m32
+    h ^= h >> np.uint64(12)
+    mask = np.uint64((1 << material_bits) - 1)
+    return ((np.uint64(material) + (y >> b) + (h & np.uint64(3))) & mask).astype(np.uint32)
+
+
+def copy_box_voxels(center, radius, n, rng, levels):
+    """n seeded destination voxels of a copy whose region is
+    [center - radius, center + radius): voxels whose unit box overlaps it."""
+    lo = [max(int(np.floor(c - radius - 1.0)) + 1, 0) for c in center]
+    hi = [min(int(np.ceil(c + radius)) - 1, (1 << levels) - 1) for c in center]
+    return np.stack([rng.integers(lo[a], hi[a] + 1, n) for a in range(3)], axis=1).astype(np.int64)
+
+
+def copy_source_voxels(v, center, radius, src, dst, rotation):
+    """Source voxel each destination voxel v of a copy/paste reads, as the
+    COPY tool defines it: c = dst + (center - src), pivot p = floor(c + 0.5),
+    q = p + trunc(src - dst), source = max(q + R^k(v - p), 0) with
+    R(x, y, z) = (-z - 1, y, x) (k quarter turns about +Y), in float32 for
+    the centre arithmetic as the tool computes it."""
+    f = np.float32
+    c = np.array([f(dst[a]) + (f(center[a]) - f(src[a])) for a in range(3)], np.float32)
+    p = np.floor(c + f(0.5)).astype(np.int64)
+    q = p + np.trunc(np.array([f(src[a]) - f(dst[a]) for a in range(3)], np.float32)).astype(np.int64)
+    d = np.asarray(v, np.int64) - p
+    for _ in range(int(rotation) & 3):
+        d = np.stack([-d[:, 2] - 1, d[:, 1], d[:, 0]], axis=1)
+    return np.maximum(q + d, 0)

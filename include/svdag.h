@@ -208,6 +208,12 @@ typedef struct {
  * reference: editMyHashDag<Editor>(editor, dag, undoRedo, stats, memPool, stream)
  * (my_gpu_hash_dag_edits.h:12-13; my_gpu_hash_dag_edits.cu:633-671).  stats may be NULL. */
 int svdag_edit(svdag_ctx* ctx, const svdag_edit_desc* edit, svdag_edit_stats* stats);
+/* svdag_edit ordered after the work already queued on the caller's HIP
+ * stream (a hipStream_t passed as void*; NULL = no ordering), the stream
+ * argument of editMyHashDag (my_gpu_hash_dag_edits.h:12-13).  The edit runs
+ * on the context's own stream and is complete when the call returns, as the
+ * reference's edit is (it synchronises the device several times per edit). */
+int svdag_edit_on_stream(svdag_ctx* ctx, const svdag_edit_desc* edit, svdag_edit_stats* stats, void* stream);
 
 /* Apply n edits in order with no host round trip between them (each edit's
  * kernels read the root the previous edit wrote on the device); one sync at

@@ -312,3 +312,71 @@ def sun_dir():
     L = (ctypes.c_float * 3)()
     dlib().ora_sun_dir(L)
     return np.array(list(L), np.float32)
+
+
+_hd_ready = False
+
+
+def hdlib():
+    """liboracle.so with the CPU HashDAG entry points typed."""
+    global _hd_ready
+    L = lib()
+    if not _hd_ready:
+        L.hd_create.restype = ctypes.c_void_p
+        L.hd_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+        L.hd_destroy.argtypes = [ctypes.c_void_p]
+        L.hd_import_canonical.restype = ctypes.c_int
+        L.hd_import_canonical.argtypes = [ctypes.c_void_p, u32p, ctypes.c_uint64, ctypes.c_uint32]
+        L.hd_edit.restype = ctypes.c_int
+        L.hd_edit.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_uint64)]
+        L.hd_query.argtypes = [ctypes.c_void_p, u32p, ctypes.c_uint64, ctypes.c_void_p]
+        L.hd_usage.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+        _hd_ready = True
+    return L
+
+
+class CpuHashDag:
+    """The reference's CPU HashDAG editing path (geometry only), restated:
+    the CPU timing baseline.  kinds: 0 sphere add, 1 sphere remove, 2 box
+    add, 3 box remove."""
+
+    SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE = range(4)
+
+    def __init__(self, levels, color_levels=0, top_span=0, low_span=0, pool_bytes=64 << 30):
+        self.levels = levels
+        self.h = hdlib().hd_create(levels, color_levels, top_span, low_span, pool_bytes)
+        if not self.h:
+            raise RuntimeError("hd_create failed (bad levels / bucket spans, or no address space for the pool)")
+
+    def import_canonical(self, dump, threads=1):
+        d = np.ascontiguousarray(dump, np.uint32)
+        rc = hdlib().hd_import_canonical(self.h, _p(d), d.size, threads)
+        if rc != 0:
+            raise RuntimeError("hd_import_canonical: " + ("malformed dump" if rc == -1 else "a bucket overflowed"))
+
+    def edit(self, kind, center, radius, threads=32):
+        """Returns (wall ms of the edit, leaf tasks)."""
+        ms, tasks = ctypes.c_double(), ctypes.c_uint64()
+        rc = hdlib().hd_edit(self.h, kind, float(center[0]), float(center[1]), float(center[2]), float(radius),
+                             threads, ctypes.byref(ms), ctypes.byref(tasks))
+        if rc != 0:
+            raise RuntimeError("hd_edit: a bucket or the pool overflowed")
+        return ms.value, tasks.value
+
+    def query(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.uint32).reshape(-1, 3)
+        out = np.empty(len(xyz), np.uint8)
+        hdlib().hd_query(self.h, _p(xyz), len(xyz), out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def usage(self):
+        out = (ctypes.c_uint64 * 2)()
+        hdlib().hd_usage(self.h, out)
+        return int(out[0]), int(out[1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            hdlib().hd_destroy(self.h)
+            self.h = None

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_py
+import workloads
 
 SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE, PAINT, COPY = range(6)
 NONE = 0xFFFFFFFF
@@ -569,3 +570,50 @@ def test_banded_terrain_voxels_follow_heights_and_bands():
     assert solid.any() and (~solid).any()
     assert len(np.unique(want_mat[solid])) > 4
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("case", range(len(COPY_CASES)))
+def test_sampled_copy_check_agrees_with_the_oracle(case):
+    # The large-scene leg checks copies by sampling: destination voxels from
+    # workloads.copy_box_voxels, their sources from copy_source_voxels, and
+    # the rule new = source if set else old.  That check must accept the
+    # oracle's copy and reject a copy that ignores the rotation.
+    src, dst, center, r, rot = COPY_CASES[case]
+    levels = 7
+    side = 1 << levels
+    d, _ = _terrain(levels)
+    rng = np.random.default_rng(5 + case)
+    for _ in range(6):
+        c = tuple(float(center[a]) + float(rng.integers(-10, 11)) for a in range(3))
+        d.edit([SPHERE_ADD, BOX_ADD, PAINT][int(rng.integers(0, 3))], int(rng.integers(2, 16)), c,
+               float(rng.integers(3, 8)))
+    c = [np.float32(dst[a]) + (np.float32(center[a]) - np.float32(src[a])) for a in range(3)]
+    v = workloads.copy_box_voxels([float(x) for x in c], r, 3000, np.random.default_rng(case), levels)
+    sv = workloads.copy_source_voxels(v, center, r, src, dst, rot)
+    inside = np.all(sv < side, axis=1)
+    v, sv = v[inside], sv[inside]
+    old = d.query(v.astype(np.uint32))
+    srcv = d.query(sv.astype(np.uint32))
+    d.edit(COPY, 0, center, r, src, dst, rot)
+    new = d.query(v.astype(np.uint32))
+    assert np.array_equal(new, np.where(srcv != 0xFFFFFFFF, srcv, old))
+    if rot:
+        wrong = workloads.copy_source_voxels(v, center, r, src, dst, 0)
+        srcw = d.query(np.minimum(wrong, side - 1).astype(np.uint32))
+        assert not np.array_equal(new, np.where(srcw != 0xFFFFFFFF, srcw, old))
+
+
+def test_band_materials_match_the_oracle_terrain():
+    L, bits, band, seed = 8, 8, 4, 0xBA5E
+    side = 1 << L
+    base, amp, freq, octaves, mat = 128.0, 50.0, 1 / 64, 5, 7
+    d = oracle_py.OracleDag(L, bits)
+    d.build_terrain(0x5EED, base, amp, freq, octaves, mat, band, seed)
+    h = oracle_py.terrain_heights(0x5EED, base, amp, freq, octaves, 0, 0, side, side)
+    rng = np.random.default_rng(12)
+    x, z = rng.integers(0, side, 5000), rng.integers(0, side, 5000)
+    y = np.minimum(h[z, x], side - 1) - rng.integers(0, 40, 5000)
+    keep = y >= 0
+    x, y, z = x[keep], y[keep], z[keep]
+    got = d.query(np.stack([x, y, z], 1).astype(np.uint32))
+    assert np.array_equal(got, workloads.band_materials(x, y, z, mat, band, seed, bits))
