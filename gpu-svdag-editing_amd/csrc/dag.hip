@@ -1247,6 +1247,37 @@ int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e)
     return SVDAG_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free voxel queries
+// Staging for svdag_query_voxels_noalloc: the query points (3 words each) of
+// at most kQueryScratchPoints points, then one result word per point.  One
+// copy per GPU, alive until the process ends; every call on a GPU shares it.
+constexpr uint32_t kQueryScratchPoints = 1u << 20;
+static __device__ __align__(256) uint32_t scratch[(size_t)kQueryScratchPoints * 4];
+
+// Device address of each GPU's copy of `scratch` (the array's name is only a
+// host-side stand-in), filled once per device.
+static std::mutex gScratchMu;
+static std::unordered_map<int, uint32_t*> gScratchByDevice;
+
+int query_scratch_attach(svdag_ctx* ctx)
+{
+    int dev = 0;
+    SVDAG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    auto it = gScratchByDevice.find(dev);
+    if (it == gScratchByDevice.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("query staging: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return SVDAG_E_DEVICE;
+        }
+        it = gScratchByDevice.emplace(dev, static_cast<uint32_t*>(p)).first;
+    }
+    ctx->queryScratch = it->second;
+    return SVDAG_OK;
+}
+
 } // namespace svdag
 
 using namespace svdag;
@@ -1337,6 +1368,7 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
     init_fully_filled<<<1, 64, 0, ctx->stream>>>(d);
     SVDAG_HIP(hipGetLastError());
     SVDAG_TRY(ws_alloc(ctx, cfg.workspace_items, false));
+    SVDAG_TRY(query_scratch_attach(ctx));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));
     ctx->root = kSentinel;
     *out = ctx;
@@ -1827,6 +1859,32 @@ int svdag_history(svdag_ctx* ctx, uint32_t out[2])
         return SVDAG_E_INVALID;
     out[0] = (uint32_t)ctx->undo.size();
     out[1] = (uint32_t)ctx->redo.size();
+    return SVDAG_OK;
+}
+
+int svdag_query_voxels_noalloc(svdag_ctx* ctx, const uint32_t* xyz, uint32_t n, uint32_t* out)
+{
+    if (!ctx || (n && (!xyz || !out)))
+        return SVDAG_E_INVALID;
+    if (n > kQueryScratchPoints) {
+        set_error("svdag_query_voxels_noalloc: " + std::to_string(n) + " points exceed the staging limit of "
+                  + std::to_string(kQueryScratchPoints));
+        return SVDAG_E_INVALID;
+    }
+    if (n == 0)
+        return SVDAG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->queryScratch) {
+        set_error("svdag_query_voxels_noalloc: no staging address for this context's device");
+        return SVDAG_E_DEVICE;
+    }
+    uint32_t* dIn = ctx->queryScratch;
+    uint32_t* dOut = ctx->queryScratch + (size_t)kQueryScratchPoints * 3;
+    SVDAG_HIP(hipMemcpyAsync(dIn, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    query_voxels_kernel<<<grid_for(n, 256, 1u << 20), 256, 0, ctx->stream>>>(ctx->dag, ctx->dRoot, dIn, n, dOut);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipMemcpyAsync(out, dOut, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
     return SVDAG_OK;
 }
 

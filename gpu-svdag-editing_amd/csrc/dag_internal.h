@@ -199,6 +199,9 @@ struct svdag_ctx {
     std::vector<uint32_t> undo, redo;
     uint32_t groupSize = 16;
     std::vector<uint64_t> slabLimit; // per table: most lines it may hold (table_line_limit)
+    // Device address of this GPU's copy of the file-scope query staging array
+    // (svdag_query_voxels_noalloc), taken once per device at create.
+    uint32_t* queryScratch = nullptr;
     // Host-visible status block (pinned): EditStatus followed by the level table.
     svdag::EditStatus* hStatus = nullptr;
     svdag::LevelDesc* hLevels = nullptr;

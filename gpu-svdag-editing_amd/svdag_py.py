@@ -18,6 +18,7 @@ ERRORS = {-1: "INVALID", -2: "OOM", -3: "DEVICE", -4: "CAPACITY", -5: "EMPTY", -
 
 SPHERE_ADD, SPHERE_REMOVE, BOX_ADD, BOX_REMOVE, PAINT, COPY = range(6)
 NOT_FOUND = 0xFFFFFFFF
+QUERY_NOALLOC_MAX = 1 << 20  # points per svdag_query_voxels_noalloc call
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 
@@ -108,7 +109,7 @@ EXPORTED = [
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_build_terrain_banded", "svdag_terrain_heights", "svdag_root",
     "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
-    "svdag_edit_on_stream",
+    "svdag_query_voxels_noalloc", "svdag_edit_on_stream",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
@@ -373,10 +374,14 @@ class MyGPUHashDAG:
         check((lib().svdag_gc_mark if mark else lib().svdag_gc)(self.h, ctypes.byref(st)))
         return st
 
-    def query(self, xyz):
+    def query(self, xyz, noalloc=False):
+        """Material per point (NOT_FOUND where empty).  noalloc=True stages
+        through the library's static per-GPU array (svdag_query_voxels_noalloc,
+        at most QUERY_NOALLOC_MAX points)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
         out = np.empty(len(xyz), np.uint32)
-        check(lib().svdag_query_voxels(self.h, xyz.ctypes.data_as(u32p), len(xyz), out.ctypes.data_as(u32p)))
+        fn = lib().svdag_query_voxels_noalloc if noalloc else lib().svdag_query_voxels
+        check(fn(self.h, xyz.ctypes.data_as(u32p), len(xyz), out.ctypes.data_as(u32p)))
         return out
 
     def memory_usage(self):

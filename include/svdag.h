@@ -290,6 +290,16 @@ int svdag_gc_mark(svdag_ctx* ctx, svdag_gc_stats* stats);
  * (dag_tracer.cu:361-390). */
 int svdag_query_voxels(svdag_ctx* ctx, const uint32_t* xyz, uint32_t n, uint32_t* out_material);
 
+/* svdag_query_voxels without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results; the points and results are staged through one
+ * static device array per GPU, set up at svdag_create, so n is at most
+ * 1048576 (larger n is rejected with SVDAG_E_INVALID).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (from different contexts
+ * or threads) race. */
+int svdag_query_voxels_noalloc(svdag_ctx* ctx, const uint32_t* xyz, uint32_t n, uint32_t* out_material);
+
 /* Primary rays.  reference: DAGTracer::resolve_paths -> Tracer::trace_paths
  * (dag_tracer.cu:59-134; tracer.cu:186-273, 353-411).  Camera as CameraView
  * (camera_view.h): position, 3x3 rotation (row 0 = -right, row 1 = up,
