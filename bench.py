@@ -50,10 +50,11 @@ def variant_name(v):
     return f"wave64 group of {v} lanes"
 
 
-def read_traffic(path, variant):
-    """HBM bytes per find launch of `variant` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a 256 MiB copy), if any."""
-    key = f"traffic_bytes_per_launch[{variant}]:"
+def read_traffic(path, variant, fill):
+    """HBM bytes per find launch of `variant` at head-line fill `fill` from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a 256 MiB copy), if it holds that case."""
+    key = f"traffic_bytes_per_launch[{variant}@{fill:.2f}]:"
     try:
         for line in open(path):
             if line.startswith(key):
@@ -71,7 +72,6 @@ def slab_hash_bench(args):
     log(f"[bench] generating {n_items} items / finds")
     items = workloads.slab_hash_items(n_items)
     finds = workloads.slab_hash_finds(items, n_items, hit_rate=0.5)
-    buckets = workloads.buckets_for(items, 0.75)
     sizes = list(range(2, 10))
     groups = sorted(items.items())
     fgroups = [(s, finds[s]) for s, _ in groups]
@@ -82,6 +82,26 @@ def slab_hash_bench(args):
     fdev = svdag_py.DeviceBuffer.from_array(fwords)
     odev = svdag_py.DeviceBuffer(4 * n)
     rdev = svdag_py.DeviceBuffer(4 * fn)
+
+    # Bucket count: the head-line fill each table is sized for, picked by
+    # timing the find (variant 1, best of three) at each candidate; within 2%
+    # of the fastest the fuller tables win (less memory).
+    fill_sweep = None
+    fill = args.fill if args.fill is not None else (0.75 if args.pmc_mode else None)
+    if fill is None:
+        fill_sweep = {}
+        for f in (0.5, 0.6, 0.75, 0.9):
+            bf = workloads.buckets_for(items, f)
+            t = svdag_py.HashSet(2, 9, [bf.get(s, 1) for s in sizes], [len(items.get(s, [])) for s in sizes])
+            t.add_dev(wdev, segs, odev, 16)
+            ms = min(t.find_dev(fdev, fsegs, rdev, 1) for _ in range(3))
+            lines_f = sum(t.usage(s)[2] for s in sizes if s in items)
+            t.close()
+            fill_sweep[f] = {"find_ms": round(ms, 4), "bytes_per_stored_item": round(lines_f * 128.0 / n_items, 1)}
+        fastest = min(v["find_ms"] for v in fill_sweep.values())
+        fill = max(f for f, v in fill_sweep.items() if v["find_ms"] <= 1.02 * fastest)
+        log(f"[bench] head-line fill sweep: {fill_sweep}, picked {fill}")
+    buckets = workloads.buckets_for(items, fill)
 
     log("[bench] bulk insert")
     hs = svdag_py.HashSet(2, 9, [buckets.get(s, 1) for s in sizes], [len(items.get(s, [])) for s in sizes])
@@ -144,12 +164,38 @@ def slab_hash_bench(args):
         }
         log(f"[bench] probes: {probe}")
 
+    # Bytes per stored item of this layout (every line handed out, head and
+    # overflow, over the items stored).
+    lines_used = sum(hs.usage(s)[2] for s in sizes if s in items)
+    bytes_per_item = lines_used * 128.0 / n_items
+    hs.close()
+
+    # The sizing as configs[1] writes it: ceil(n_s / 96) buckets per size
+    # (the reference's 96 items per 32-slot bucket).  Here a bucket is one
+    # 128-byte line plus its chain, so buckets hold 96 items in long chains.
+    as_written = None
+    if not args.no_as_written:
+        b96 = {s: max(1, -(-len(a) // 96)) for s, a in items.items()}
+        hs96 = svdag_py.HashSet(2, 9, [b96.get(s, 1) for s in sizes], [len(items.get(s, [])) for s in sizes])
+        hs96.add_dev(wdev, segs, odev, 16)
+        ms96 = {v: min(hs96.find_dev(fdev, fsegs, rdev, v) for _ in range(3)) for v in (1, 16, 208)}
+        res96 = rdev.to_array(np.uint32, fn)
+        if not np.array_equal(res96 != svdag_py.NOT_FOUND, hits):
+            raise RuntimeError("96-per-bucket tables disagree with the 75%-fill tables on which finds hit")
+        v96 = min(ms96, key=ms96.get)
+        lines96 = sum(hs96.usage(s)[2] for s in sizes if s in items)
+        hs96.close()
+        as_written = {"buckets": "ceil(n_s / 96) per item size", "find_ms_by_variant": {str(k): round(v, 4)
+                      for k, v in ms96.items()}, "G_finds_per_s": round(fn / (ms96[v96] * 1e-3) / 1e9, 2),
+                      "roofline_frac": round(alg_bytes / (ms96[v96] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "bytes_per_stored_item": round(lines96 * 128.0 / n_items, 1)}
+        log(f"[bench] 96 items per bucket: {as_written}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(items, finds, buckets, args)
 
-    hs.close()
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "slab_hash_find_traffic.txt"), best)
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "slab_hash_find_traffic.txt"), best, fill)
     return {
         "metric": METRIC,
         "value": round(value, 2),
@@ -165,12 +211,16 @@ def slab_hash_bench(args):
         "data": "synthetic",
         "config": {
             "workload": f"slab-hash find: 2^{args.log2_items} DAG node items of 2-9 u32 in per-size tables "
-                        f"with head lines 75% full, 2^{args.log2_items} finds, 50% hits",
+                        f"with head lines {round(100 * fill)}% full, 2^{args.log2_items} finds, 50% hits",
+            "head_line_fill": fill,
+            "head_line_fill_sweep": {str(k): v for k, v in fill_sweep.items()} if fill_sweep else None,
             "variant": variant_name(best),
             "find_kernel_ms_by_variant": {str(k): round(v, 4) for k, v in per_variant.items()},
             "bulk_insert_ms": round(add_ms, 3),
             "hit_rate": round(float(hits.mean()), 4),
             "kernel_ms": round(avg_kernel_ms, 4),
+            "bytes_per_stored_item": round(bytes_per_item, 1),
+            "as_written_96_items_per_bucket": as_written,
         },
         "roofline": {
             "bound": "hbm",
@@ -283,7 +333,7 @@ def editing_bench(args):
     def batched_leg(gc_each, mode, name):
         g, _, _ = fresh()
         g.set_batch_mode(mode)
-        tune = g.autotune(edits[:16])
+        tune = g.autotune_all(edits[:16])
         g.garbage_collect()
         blat = []
         gpu_ms = 0.0
@@ -306,10 +356,11 @@ def editing_bench(args):
                 "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
                 "latency_ms_p99": float(np.percentile(blat, 99) * 1e3)}, tune
 
-    batched, (tune_g, tune_ms) = batched_leg(False, 1, "fused, one collection")
+    batched, tune = batched_leg(False, 1, "fused, one collection")
     batched.update({"edits": len(edits), "batch": 16, "gc": "once after the stream",
-                    "autotuned_lanes_per_item": tune_g,
-                    "autotune_ms": {str(k): round(v, 3) for k, v in tune_ms.items()}})
+                    "autotune": {"lanes_per_item": tune["lanes_per_item"], "max_blocks": tune["max_blocks"],
+                                 "ms_by_lanes": {str(k): v for k, v in tune["ms_by_lanes"].items()},
+                                 "ms_by_max_blocks": {str(k): v for k, v in tune["ms_by_max_blocks"].items()}}})
     batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
     each, _ = batched_leg(True, 1, "fused, collection after each batch")
     each["speedup_vs_plain"] = each["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
@@ -321,7 +372,7 @@ def editing_bench(args):
 
     # The same stream through the serving loop, all requests at t = 0.
     g, _, _ = fresh()
-    g.autotune(edits[:16])
+    g.autotune_all(edits[:16])
     g.garbage_collect()
     info, _ = g.serve_replay(edits, np.zeros(len(edits)), max_batch=64, allow_gc=True)
     if not (info["status"] == 0).all():
@@ -333,7 +384,7 @@ def editing_bench(args):
 
     serving = []
     g, _, _ = fresh()
-    g.autotune(edits[:16])
+    g.autotune_all(edits[:16])
 
     def serve_row(name, reqs, arr, max_batch):
         vox = np.array([workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in reqs], np.float64)
@@ -494,8 +545,16 @@ def editing_profile(args):
         return int(g.terrain_heights(x, z, 1, 1)[0, 0])
 
     edits = workloads.config3_edits(h, D, n=256, seed=6)
+    if args.marked_batch >= 0:
+        # A device-to-device copy of known size: counter runs scale FETCH_SIZE
+        # and WRITE_SIZE by what the counters read for it.
+        nbytes = 256 << 20
+        a = svdag_py.DeviceBuffer(nbytes)
+        b = svdag_py.DeviceBuffer(nbytes)
+        svdag_py.check(svdag_py.lib().svdag_memcpy_d2d(b.ptr, a.ptr, svdag_py.ctypes.c_size_t(nbytes)))
+        del a, b
     g.set_batch_mode(args.batch_mode)
-    tune_g, _ = g.autotune(edits[:16])
+    tune_g = g.autotune_all(edits[:16])["lanes_per_item"]
     g.garbage_collect()
     rows = []
     marked = args.marked_batch
@@ -668,6 +727,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-editing", action="store_true", help="slab-hash line only")
     ap.add_argument("--no-probe", action="store_true", help="skip the stream / random-line memory probes")
+    ap.add_argument("--no-as-written", action="store_true", help="skip the 96-items-per-bucket find point")
+    ap.add_argument("--fill", type=float, default=None,
+                    help="head-line fill the find tables are sized for (default: picked by timing 0.5 .. 0.9)")
     ap.add_argument("--no-shading", action="store_true", help="skip the shadow / occlusion timing")
     ap.add_argument("--large-scene", action="store_true", help="add the depth-17 8-bit-material copy/paste leg")
     ap.add_argument("--large-scene-only", action="store_true", help="only the depth-17 copy/paste leg")

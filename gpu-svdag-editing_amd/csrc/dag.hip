@@ -930,7 +930,7 @@ uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L)
 }
 
 template <bool kMulti>
-static void launch_merge_level(svdag_ctx* ctx, const WsDev& ws, uint32_t L, uint64_t items, uint32_t kCap = 8192)
+static void launch_merge_level(svdag_ctx* ctx, const WsDev& ws, uint32_t L, uint64_t items, uint32_t kCap)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
@@ -965,7 +965,7 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
     const Bounds b = edit_bounds(ctx, ed);
-    const uint32_t kCap = 8192;
+    const uint32_t kCap = ctx->maxBlocks;
     edit_begin<<<1, 64, 0, st>>>(ws, ctx->dRoot, ed.kind == EK_COPY ? 1u : 0u,
                                  ed.kind == EK_DENSE || ed.kind == EK_TERRAIN ? 1u : 0u);
     for (uint32_t L = 0; L < dag.leafLevel; ++L) {
@@ -977,7 +977,8 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
         svo_leaf_level<false><<<grid_for(items, 4 * kLeavesPerWave, kCap), 256, 0, st>>>(ws, dag, edDev);
     }
     for (int L = (int)dag.leafLevel; L >= 0; --L)
-        launch_merge_level<false>(ctx, ws, (uint32_t)L, std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork));
+        launch_merge_level<false>(ctx, ws, (uint32_t)L, std::min<uint64_t>(level_bound(ctx, b, (uint32_t)L), ws.capWork),
+                                  kCap);
     edit_finish<<<1, 64, 0, st>>>(ws, dag, ctx->dRoot);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
@@ -1748,25 +1749,18 @@ int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, s
     return SVDAG_OK;
 }
 
-int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_result* out)
+// Times the sample batch under each value of one knob (three runs each, the
+// best kept) and leaves the fastest set.  Each run starts from the same root
+// and, after a collection, the same tables: the sample's own items are freed
+// again, so every timed run inserts them as a real edit would (a run over
+// tables that still hold them would time finds that hit instead).
+static int tune_knob(svdag_ctx* ctx, const std::vector<EditorDev>& eds, uint32_t* knob, const uint32_t* cand,
+                     uint32_t numCand, float* msOut)
 {
-    if (!ctx || !out || n == 0 || !sample) {
-        set_error("svdag_autotune: bad arguments");
-        return SVDAG_E_INVALID;
-    }
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    std::vector<EditorDev> eds(n);
-    for (uint32_t k = 0; k < n; ++k)
-        SVDAG_TRY(make_editor(ctx, sample[k], eds[k]));
+    const uint32_t n = (uint32_t)eds.size();
     const uint32_t r0 = ctx->root;
-    const uint32_t g0 = ctx->groupSize;
+    const uint32_t v0 = *knob;
     std::vector<svdag_edit_stats> st(n);
-    // Each run starts from the same root and, after a collection, the same
-    // tables: the sample's own items are freed again, so every timed run
-    // inserts them as a real edit would (a run over tables that still hold
-    // them would time finds that hit instead).
-    ctx->undo.clear();
-    ctx->redo.clear();
     auto restore = [&]() -> int {
         root_log(ctx);
         ctx->root = r0;
@@ -1779,31 +1773,73 @@ int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, sv
     float warm = 0.0f;
     SVDAG_TRY(run_edits(ctx, eds.data(), n, st.data(), &warm));
     SVDAG_TRY(restore());
-    const uint32_t cand[4] = { 4, 8, 16, 32 };
-    uint32_t best = g0;
+    uint32_t best = v0;
     float bestMs = 0.0f;
-    for (int c = 0; c < 4; ++c) {
-        ctx->groupSize = cand[c];
+    for (uint32_t c = 0; c < numCand; ++c) {
+        *knob = cand[c];
         float m = 0.0f;
         for (int rep = 0; rep < 3; ++rep) {
             float t = 0.0f;
             const int rc = run_edits(ctx, eds.data(), n, st.data(), &t);
             if (rc != SVDAG_OK) {
-                ctx->groupSize = g0;
+                *knob = v0;
                 (void)restore();
                 return rc;
             }
             SVDAG_TRY(restore());
             m = rep == 0 ? t : std::min(m, t);
         }
-        out->ms[c] = m;
+        msOut[c] = m;
         if (c == 0 || m < bestMs) {
             bestMs = m;
             best = cand[c];
         }
     }
-    ctx->groupSize = best;
-    out->group_size = best;
+    *knob = best;
+    return SVDAG_OK;
+}
+
+static int autotune_prepare(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, std::vector<EditorDev>& eds)
+{
+    eds.resize(n);
+    for (uint32_t k = 0; k < n; ++k)
+        SVDAG_TRY(make_editor(ctx, sample[k], eds[k]));
+    ctx->undo.clear();
+    ctx->redo.clear();
+    return SVDAG_OK;
+}
+
+int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_result* out)
+{
+    if (!ctx || !out || n == 0 || !sample) {
+        set_error("svdag_autotune: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::vector<EditorDev> eds;
+    SVDAG_TRY(autotune_prepare(ctx, sample, n, eds));
+    static constexpr uint32_t kGroups[4] = { 4, 8, 16, 32 };
+    SVDAG_TRY(tune_knob(ctx, eds, &ctx->groupSize, kGroups, 4, out->ms));
+    out->group_size = ctx->groupSize;
+    return SVDAG_OK;
+}
+
+int svdag_autotune_all(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_report* out)
+{
+    if (!ctx || !out || n == 0 || !sample) {
+        set_error("svdag_autotune_all: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::vector<EditorDev> eds;
+    SVDAG_TRY(autotune_prepare(ctx, sample, n, eds));
+    *out = svdag_tune_report {};
+    static constexpr uint32_t kGroups[4] = { 4, 8, 16, 32 };
+    static constexpr uint32_t kBlocks[5] = { 1024, 2048, 4096, 8192, 16384 };
+    SVDAG_TRY(tune_knob(ctx, eds, &ctx->groupSize, kGroups, 4, out->ms_group));
+    SVDAG_TRY(tune_knob(ctx, eds, &ctx->maxBlocks, kBlocks, 5, out->ms_blocks));
+    out->group_size = ctx->groupSize;
+    out->max_blocks = ctx->maxBlocks;
     return SVDAG_OK;
 }
 

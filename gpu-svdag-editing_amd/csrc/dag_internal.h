@@ -198,6 +198,7 @@ struct svdag_ctx {
     hipEvent_t evWait = nullptr; // svdag_edit_on_stream: the caller's stream position
     std::vector<uint32_t> undo, redo;
     uint32_t groupSize = 16;
+    uint32_t maxBlocks = 8192; // grid cap of the per-level launches (autotuned)
     std::vector<uint64_t> slabLimit; // per table: most lines it may hold (table_line_limit)
     // Device address of this GPU's copy of the file-scope query staging array
     // (svdag_query_voxels_noalloc), taken once per device at create.

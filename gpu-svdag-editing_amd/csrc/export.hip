@@ -13,6 +13,9 @@
 #include <map>
 #include <unordered_map>
 #include <unordered_set>
+#include <atomic>
+#include <functional>
+#include <thread>
 
 namespace svdag {
 
@@ -167,13 +170,230 @@ static void canonicalize(const HostDag& h, uint32_t root, const std::vector<std:
     dump.push_back(root == kSentinel ? kSentinel : idBelow.at(root));
 }
 
-static int canonical_dump(svdag_ctx* ctx, std::vector<uint32_t>& dump, std::vector<uint32_t>& cnt)
+// Host threads for the export's loops and sorts (at most 16).
+static uint32_t export_threads()
+{
+    const uint32_t hc = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, hc ? hc : 1u));
+}
+
+// fn(begin, end, thread) over [0, n) in contiguous chunks, one per thread.
+template <typename Fn>
+static void parallel_chunks(size_t n, uint32_t threads, Fn fn)
+{
+    if (n < 65536 || threads <= 1) {
+        fn((size_t)0, n, 0u);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t step = (n + threads - 1) / threads;
+    for (uint32_t t = 0; t < threads; ++t) {
+        const size_t b = std::min(n, (size_t)t * step), e = std::min(n, b + step);
+        pool.emplace_back([=, &fn] { fn(b, e, t); });
+    }
+    for (auto& th : pool)
+        th.join();
+}
+
+// Sorts v by `less`: sorted chunks, then rounds of pairwise merges, each
+// round's merges on their own threads.
+template <typename T, typename Less>
+static void parallel_sort(std::vector<T>& v, uint32_t threads, Less less)
+{
+    const size_t n = v.size();
+    if (n < 65536 || threads <= 1) {
+        std::sort(v.begin(), v.end(), less);
+        return;
+    }
+    const size_t step = (n + threads - 1) / threads;
+    std::vector<size_t> bounds;
+    for (size_t b = 0; b < n; b += step)
+        bounds.push_back(b);
+    bounds.push_back(n);
+    {
+        std::vector<std::thread> pool;
+        for (size_t c = 0; c + 1 < bounds.size(); ++c) {
+            const size_t a = bounds[c], e = bounds[c + 1];
+            pool.emplace_back([&v, a, e, &less] { std::sort(v.begin() + (ptrdiff_t)a, v.begin() + (ptrdiff_t)e, less); });
+        }
+        for (auto& th : pool)
+            th.join();
+    }
+    while (bounds.size() > 2) {
+        std::vector<size_t> next;
+        std::vector<std::thread> pool;
+        for (size_t c = 0; c + 1 < bounds.size(); c += 2) {
+            next.push_back(bounds[c]);
+            if (c + 2 < bounds.size()) {
+                const size_t a = bounds[c], m = bounds[c + 1], e = bounds[c + 2];
+                pool.emplace_back([&v, a, m, e, &less] {
+                    std::inplace_merge(v.begin() + (ptrdiff_t)a, v.begin() + (ptrdiff_t)m, v.begin() + (ptrdiff_t)e, less);
+                });
+            }
+        }
+        next.push_back(n);
+        for (auto& th : pool)
+            th.join();
+        bounds.swap(next);
+    }
+}
+
+// The canonical dump of a large DAG without per-item allocations: the
+// reachable handles of every level as a sorted array (children gathered,
+// sorted, deduplicated), contents in one flat array per level with the
+// children replaced by their canonical ids (a hash map of the level below),
+// and the level's items ordered by a 64-bit key of (size, word 0, top of
+// word 1) with full comparisons only on ties.  Gathers and sorts run on
+// host threads.  Same dump as canonicalize(): contents sorted by (size,
+// words), numbered by rank.
+static int canonical_dump(svdag_ctx* ctx, std::vector<uint32_t>& dump, std::vector<uint32_t>& counts)
 {
     HostDag h;
     SVDAG_TRY(copy_dag_to_host(ctx, h));
-    std::vector<std::vector<Rec>> levels;
-    SVDAG_TRY(reachable(h, { ctx->root }, levels));
-    canonicalize(h, ctx->root, levels, dump, cnt);
+    const uint32_t leafLevel = h.leafLevel;
+    const uint32_t T = export_threads();
+    std::vector<std::vector<uint32_t>> lvl(leafLevel + 1);
+    auto itemOf = [&](uint32_t L, uint32_t hd, uint32_t& s) -> const uint32_t* {
+        const bool leaf = L == leafLevel;
+        s = leaf ? h_leaf_size(h, hd) : node_handle_size(hd);
+        return h.item(s, leaf ? h_leaf_ptr(h, hd) : node_handle_ptr(hd));
+    };
+    std::atomic<bool> bad { false };
+    if (ctx->root != kSentinel)
+        lvl[0].push_back(ctx->root);
+    for (uint32_t L = 0; L <= leafLevel; ++L) {
+        std::vector<uint32_t>& cur = lvl[L];
+        parallel_sort(cur, T, std::less<uint32_t>());
+        cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+        if (L == leafLevel)
+            break;
+        std::vector<std::vector<uint32_t>> part(T);
+        parallel_chunks(cur.size(), T, [&](size_t b, size_t e, uint32_t t) {
+            for (size_t i = b; i < e; ++i) {
+                uint32_t s = 0;
+                const uint32_t* p = itemOf(L, cur[i], s);
+                if (!p) {
+                    bad = true;
+                    return;
+                }
+                part[t].insert(part[t].end(), p + 1, p + s);
+            }
+        });
+        if (bad) {
+            set_error("export: reachable handle points at a free or out-of-range slot");
+            return SVDAG_E_INVALID;
+        }
+        std::vector<uint32_t>& next = lvl[L + 1];
+        for (auto& q : part)
+            next.insert(next.end(), q.begin(), q.end());
+    }
+    counts.assign(leafLevel + 1, 0);
+    std::vector<uint32_t> idBelow; // canonical id of lvl[L + 1][i]
+    // Open-addressing map from the level below's handles to their canonical
+    // ids (handles are never kSentinel, which marks a free slot).
+    std::vector<uint64_t> idMap;
+    uint64_t idMask = 0;
+    auto idOf = [&](uint32_t w) -> uint32_t {
+        for (uint64_t q = mix32(w) & idMask;; q = (q + 1) & idMask) {
+            const uint32_t key = (uint32_t)(idMap[q] >> 32);
+            if (key == w)
+                return (uint32_t)idMap[q];
+            if (key == kSentinel)
+                return kSentinel;
+        }
+    };
+    for (int L = (int)leafLevel; L >= 0; --L) {
+        const std::vector<uint32_t>& hs = lvl[L];
+        const size_t n = hs.size();
+        const bool leaf = (uint32_t)L == leafLevel;
+        // Item sizes, then offsets, then the contents with children mapped.
+        std::vector<uint64_t> off(n + 1, 0);
+        parallel_chunks(n, T, [&](size_t b, size_t e, uint32_t) {
+            for (size_t i = b; i < e; ++i) {
+                uint32_t s = 0;
+                if (!itemOf((uint32_t)L, hs[i], s))
+                    bad = true;
+                off[i + 1] = s;
+            }
+        });
+        if (bad) {
+            set_error("export: reachable handle points at a free or out-of-range slot");
+            return SVDAG_E_INVALID;
+        }
+        for (size_t i = 0; i < n; ++i)
+            off[i + 1] += off[i];
+        std::vector<uint32_t> words(off[n]);
+        parallel_chunks(n, T, [&](size_t b, size_t e, uint32_t) {
+            for (size_t i = b; i < e; ++i) {
+                uint32_t s = 0;
+                const uint32_t* p = itemOf((uint32_t)L, hs[i], s);
+                uint32_t* w = words.data() + off[i];
+                w[0] = p[0];
+                for (uint32_t j = 1; j < s; ++j) {
+                    w[j] = leaf ? p[j] : idOf(p[j]);
+                    if (w[j] == kSentinel && !leaf)
+                        bad = true;
+                }
+            }
+        });
+        if (bad) {
+            set_error("export: a child handle is missing from the level below");
+            return SVDAG_E_INVALID;
+        }
+        struct Key {
+            uint64_t k;
+            uint32_t i;
+        };
+        std::vector<Key> order(n);
+        parallel_chunks(n, T, [&](size_t b, size_t e, uint32_t) {
+            for (size_t i = b; i < e; ++i) {
+                const uint32_t s = (uint32_t)(off[i + 1] - off[i]);
+                const uint32_t* w = words.data() + off[i];
+                order[i] = { ((uint64_t)s << 59) | ((uint64_t)w[0] << 27) | (uint64_t)(s > 1 ? w[1] >> 5 : 0u),
+                             (uint32_t)i };
+            }
+        });
+        auto less = [&](const Key& a, const Key& b) {
+            if (a.k != b.k)
+                return a.k < b.k;
+            const uint32_t* wa = words.data() + off[a.i];
+            const uint32_t* wb = words.data() + off[b.i];
+            const uint32_t s = (uint32_t)(off[a.i + 1] - off[a.i]); // equal keys: equal sizes
+            return std::lexicographical_compare(wa + 1, wa + s, wb + 1, wb + s);
+        };
+        parallel_sort(order, T, less);
+        std::vector<uint32_t> ids(n);
+        dump.push_back((uint32_t)L);
+        const size_t countAt = dump.size();
+        dump.push_back(0u);
+        uint32_t uniq = 0;
+        for (size_t r = 0; r < n; ++r) {
+            const Key& cur = order[r];
+            const bool same = r > 0 && !less(order[r - 1], cur);
+            if (!same) {
+                const uint32_t s = (uint32_t)(off[cur.i + 1] - off[cur.i]);
+                dump.push_back(s);
+                dump.insert(dump.end(), words.begin() + (ptrdiff_t)off[cur.i], words.begin() + (ptrdiff_t)off[cur.i + 1]);
+                ++uniq;
+            }
+            ids[cur.i] = uniq - 1u;
+        }
+        dump[countAt] = uniq;
+        counts[L] = uniq;
+        idBelow.swap(ids);
+        uint64_t cap = 64;
+        while (cap < 2 * (uint64_t)n)
+            cap <<= 1;
+        idMap.assign(cap, (uint64_t)kSentinel << 32);
+        idMask = cap - 1;
+        for (size_t i = 0; i < n; ++i) {
+            uint64_t q = mix32(hs[i]) & idMask;
+            while ((uint32_t)(idMap[q] >> 32) != kSentinel)
+                q = (q + 1) & idMask;
+            idMap[q] = ((uint64_t)hs[i] << 32) | idBelow[i];
+        }
+    }
+    dump.push_back(ctx->root == kSentinel ? kSentinel : idBelow.at(0));
     return SVDAG_OK;
 }
 

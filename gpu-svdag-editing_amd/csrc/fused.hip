@@ -128,6 +128,7 @@ constexpr uint32_t kParThreads = 1024;
 struct SpineItem {
     uint32_t x, y, z, orig; // position at its level; its handle in the wave's input DAG
     uint32_t sub;           // children this edit subdivides
+    uint32_t dep;           // paint: children whose decision read their state (empty or not)
     uint32_t ref[8];        // per subdivided child: its item one level down, or its seed work item at S
     uint32_t handle, created;
     uint32_t och[8];        // its children in the wave's input DAG (kSentinel: none)
@@ -179,7 +180,8 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
         uint32_t ok = K >= 1 && K <= kParEdits && S >= 1 && S <= leafLevel && !joint.overflow && !joint.oom;
         for (uint32_t k = 0; ok && k < K; ++k) {
             const uint32_t kind = eds[wv.x + k].kind;
-            ok = kind == EK_SPHERE_ADD || kind == EK_SPHERE_REMOVE || kind == EK_BOX_ADD || kind == EK_BOX_REMOVE;
+            ok = kind == EK_SPHERE_ADD || kind == EK_SPHERE_REMOVE || kind == EK_BOX_ADD || kind == EK_BOX_REMOVE
+                || kind == EK_PAINT;
         }
         sOk = ok;
         sBad = 0u;
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
     // ---- top-down: every edit's spine items on levels 0 .. S-1.
     const uint32_t k = t >> 3, i = t & 7u; // one (edit, item) pair per thread
     for (uint32_t l = 0; l < S; ++l) {
-        uint32_t sub = 0, origChild[8];
+        uint32_t sub = 0, dep = 0, origChild[8];
         const bool active = k < K && i < sc.cnt[k][l];
         if (active) {
             SpineItem& it = sc.item[k][l][i];
@@ -215,18 +217,34 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
                 node = dag_node_ptr(dag, it.orig);
                 mask = node[0] & 0xFFu;
             }
+            const bool paint = ed.kind == EK_PAINT;
             for (uint32_t c = 0; c < 8; ++c) {
                 origChild[c] = ((mask >> c) & 1u) ? node[1 + __builtin_popcount(mask & ((1u << c) - 1u))] : kSentinel;
                 const uint32_t cx = (it.x << 1) | ((c >> 2) & 1u), cy = (it.y << 1) | ((c >> 1) & 1u),
                                cz = (it.z << 1) | (c & 1u);
+                // Sphere and box decide by geometry alone above S.  Paint
+                // reads the child's state (svo_inner_level's NodeDesc), here
+                // from the wave's input DAG; the bottom-up checks that the
+                // edits before it in the wave left that state as it was.
+                NState st = NS_PARTIAL;
+                uint32_t stMat = 0;
+                if (paint) {
+                    if (origChild[c] == kSentinel)
+                        st = NS_EMPTY;
+                    else if (l + 1u != leafLevel)
+                        st = header_full(dag_node_ptr(dag, origChild[c])[0], dag.M, stMat) ? NS_FILLED : NS_PARTIAL;
+                }
                 uint32_t fm = 0;
-                const Op op = decide(ed, cx, cy, cz, dag.levels - (l + 1), NS_PARTIAL, fm);
+                const Op op = decide(ed, cx, cy, cz, dag.levels - (l + 1), st, fm);
                 if (op == OP_SUBDIVIDE)
                     sub |= 1u << c;
-                else if (op != OP_KEEP)
-                    atomicOr(&sBad, 1u);
+                else if (!(op == OP_KEEP || (paint && op == OP_EMPTY && st == NS_EMPTY)))
+                    atomicOr(&sBad, 1u); // a fill above S: the sequential spine takes the wave
+                if (paint && op != OP_KEEP)
+                    dep |= 1u << c;
             }
             it.sub = sub;
+            it.dep = dep;
             for (uint32_t c = 0; c < 8; ++c)
                 it.och[c] = origChild[c];
         }
@@ -426,7 +444,33 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
             if (full)
                 atomicAdd(&sFullCnt[k], 1u);
         }
+        // A paint item decided its state-dependent children on the input
+        // DAG's state; the edits before it in the wave may have made such a
+        // child empty or non-empty.  The DAG would come out the same (the
+        // paint's own cells are disjoint from those edits), but its SVO would
+        // not, so such a wave goes to the sequential spine, which decides on
+        // the current state.  Nothing of the wave is published before the
+        // last level, and what was inserted below is what the sequential
+        // spine builds again and finds.
+        if (k < K && i < sc.cnt[k][l] && eds[wv.x + k].kind == EK_PAINT) {
+            const SpineItem& it = sc.item[k][l][i];
+            uint32_t need = it.dep, prior[8];
+            for (uint32_t c = 0; c < 8; ++c)
+                prior[c] = it.och[c];
+            for (uint32_t o = sPrev[idx]; need != 0u && o != kSentinel; o = sPrev[o]) {
+                const uint32_t hit = sSub[o] & need;
+                for (uint32_t c = 0; c < 8; ++c)
+                    if ((hit >> c) & 1u)
+                        prior[c] = sChildH[o][c];
+                need &= ~hit;
+            }
+            for (uint32_t c = 0; c < 8; ++c)
+                if (((it.dep >> c) & 1u) && (prior[c] == kSentinel) != (it.och[c] == kSentinel))
+                    atomicOr(&sBad, 1u);
+        }
         __syncthreads();
+        if (sBad)
+            return; // the sequential spine takes the wave
         // In-level dedup over every edit's nodes: the first claimant of a
         // content is its representative.
         if (active && !empty) {
@@ -967,7 +1011,7 @@ static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, 
     wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
     fused_seed<<<(seedLanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
     if (levelItems) {
-        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems));
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems, ctx->maxBlocks));
     } else {
         uint64_t fixed[kMaxLevels];
         for (uint32_t L = 0; L < kMaxLevels; ++L)

@@ -76,6 +76,11 @@ class TuneResult(ctypes.Structure):
     _fields_ = [("group_size", ctypes.c_uint32), ("ms", ctypes.c_float * 4)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("group_size", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32), ("ms_group", ctypes.c_float * 4),
+                ("ms_blocks", ctypes.c_float * 5)]
+
+
 class Memory(ctypes.Structure):
     _fields_ = [("slab_capacity", ctypes.c_uint64), ("slabs_used", ctypes.c_uint64), ("slabs_free", ctypes.c_uint64),
                 ("bytes_reserved", ctypes.c_uint64), ("min_free_fraction", ctypes.c_double)]
@@ -109,7 +114,7 @@ EXPORTED = [
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_build_terrain_banded", "svdag_terrain_heights", "svdag_root",
     "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
-    "svdag_query_voxels_noalloc", "svdag_edit_on_stream",
+    "svdag_query_voxels_noalloc", "svdag_edit_on_stream", "svdag_autotune_all",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
@@ -350,6 +355,17 @@ class MyGPUHashDAG:
         r = TuneResult()
         check(lib().svdag_autotune(self.h, self._descs(sample), len(sample), ctypes.byref(r)))
         return r.group_size, dict(zip((4, 8, 16, 32), list(r.ms)))
+
+    def autotune_all(self, sample):
+        """Pick lanes per item, then the per-level grid cap, from a sample
+        batch; returns {"lanes_per_item": G, "max_blocks": B, "ms_by_lanes": {...},
+        "ms_by_max_blocks": {...}}."""
+        r = TuneReport()
+        check(lib().svdag_autotune_all(self.h, self._descs(sample), len(sample), ctypes.byref(r)))
+        return {"lanes_per_item": r.group_size, "max_blocks": r.max_blocks,
+                "ms_by_lanes": {g: round(m, 3) for g, m in zip((4, 8, 16, 32), list(r.ms_group))},
+                "ms_by_max_blocks": {b: round(m, 3) for b, m in zip((1024, 2048, 4096, 8192, 16384),
+                                                                      list(r.ms_blocks))}}
 
     def set_batch_mode(self, mode):
         """0: edits of a batch one after another; 1: fused waves (default);

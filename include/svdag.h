@@ -249,6 +249,19 @@ typedef struct {
 } svdag_tune_result;
 int svdag_autotune(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_result* out);
 
+/* The same tuner over two knobs in turn (no reference counterpart; the
+ * reference fixes both at compile time): lanes per item of the merge kernels,
+ * then the grid cap of the per-level launches (the most workgroups one level's
+ * kernel is launched with: 1024, 2048, 4096, 8192 or 16384; its grid-stride
+ * loops cover the rest).  Each value is timed as svdag_autotune times one. */
+typedef struct {
+    uint32_t group_size;       /* chosen lanes per item */
+    uint32_t max_blocks;       /* chosen grid cap */
+    float ms_group[4];         /* best device ms of the sample for 4, 8, 16, 32 lanes */
+    float ms_blocks[5];        /* the same for grid caps 1024 .. 16384, with the chosen lanes */
+} svdag_tune_report;
+int svdag_autotune_all(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_report* out);
+
 /* reference: MyGPUHashDAGUndoRedo::undo / redo (my_gpu_base_dag.h:35-64). */
 int svdag_undo(svdag_ctx* ctx);
 int svdag_redo(svdag_ctx* ctx);

@@ -438,6 +438,45 @@ def test_fused_waves_match_sequential_step_by_step(levels):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("gap", [40.0, 70.0, 130.0, 260.0])
+def test_paint_waves_match_sequential_when_earlier_edits_change_emptiness(gap):
+    # Sphere adds in the empty sky above the terrain, each followed by a paint
+    # `gap` voxels away: the paint's cells are its own, but its spine nodes
+    # are the ones the add made non-empty (and a remove then a paint on the
+    # surface, the other way round).  The fused form must give the sequential
+    # form's SVO sizes and applied flags edit by edit, and the oracle's DAG.
+    levels = L10
+    tp = workloads.terrain_params(levels)
+    side = 1 << levels
+    heights = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                        0, 0, side, side)
+    edits = []
+    for j, x in enumerate((150.0, 450.0, 750.0)):
+        edits.append((svdag_py.SPHERE_ADD, 3 + j, (x, 940.0, 300.0), 18.0))
+        edits.append((svdag_py.PAINT, 7 + j, (x, 940.0, 300.0 + gap), 18.0))
+        edits.append((svdag_py.PAINT, 9, (x, 940.0, 300.0 - gap), 25.0))
+    for x in (200.0, 600.0):
+        y = float(heights[600, int(x)])
+        edits.append((svdag_py.SPHERE_REMOVE, 0, (x, y, 600.0), 30.0))
+        edits.append((svdag_py.PAINT, 12, (x + gap, y, 600.0), 30.0))
+    a, _ = _pair()
+    b, o = _pair()
+    a.set_batch_mode(0)
+    b.set_batch_mode(1)
+    sa, _ = a.edit_batch(edits)
+    sb, _ = b.edit_batch(edits)
+    for k, (s1, s2) in enumerate(zip(sa, sb)):
+        assert s1.applied == s2.applied, k
+        assert list(s1.svo_items) == list(s2.svo_items), k
+    for e in edits:
+        o.edit(*e)
+    _same(b, o)
+    assert a.canonical() == b.canonical()
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("batch", [16, 64])
 def test_parallel_spine_waves_match_sequential(batch):
     # Sphere and box streams only (no paint): their waves take the parallel
@@ -557,6 +596,29 @@ def test_autotune_keeps_the_dag_and_later_edits_exact():
     assert g in (4, 8, 16, 32) and all(v > 0 for v in ms.values())
     assert a.canonical() == before and a.history() == (0, 0)
     a.edit_batch(edits)
+    for e in edits:
+        o.edit(*e)
+    _same(a, o)
+    a.close()
+
+
+@pytest.mark.gpu
+def test_autotune_all_picks_both_knobs_and_keeps_the_dag():
+    # Lanes per item, then the per-level grid cap: both picks come from the
+    # candidate sets, every candidate was timed, the DAG is unchanged, and
+    # later fused and single edits under the picks equal the oracle.
+    a, o = _pair()
+    edits = _mixed_stream(L10, seed=57)
+    before = a.canonical()
+    rep = a.autotune_all(edits[:16])
+    assert rep["lanes_per_item"] in (4, 8, 16, 32)
+    assert rep["max_blocks"] in (1024, 2048, 4096, 8192, 16384)
+    assert all(v > 0 for v in rep["ms_by_lanes"].values())
+    assert all(v > 0 for v in rep["ms_by_max_blocks"].values())
+    assert a.canonical() == before and a.history() == (0, 0)
+    a.edit_batch(edits[:40])
+    for e in edits[40:]:
+        a.edit(*e)
     for e in edits:
         o.edit(*e)
     _same(a, o)
@@ -743,4 +805,39 @@ def test_reference_count_collection_frees_what_marking_frees():
     check = g.garbage_collect(mark=True)
     assert check.items_freed == 0 and check.items_live == st.items_live
     _same(g, o)
+    g.close()
+
+
+@pytest.mark.gpu
+def test_config3_stream_depth13_fused_and_graph_match_oracle():
+    # The benchmark's batched-editing generator (40% sphere add, 40% sphere
+    # remove, 20% box fill, r log-uniform in [8, 512], centres within +-1024
+    # of the surface) at depth 13: 64 edits in fused batches of 16 with a
+    # collection after every batch, in eager fused mode and in graph mode,
+    # each equal to the oracle's sequential edits.
+    levels = 13
+    g, o = _pair(levels=levels, expected_items=4_000_000, workspace_items=1 << 24)
+    side = 1 << levels
+    tp = workloads.terrain_params(levels)
+    hts = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                    0, 0, side, side)
+    edits = workloads.config3_edits(lambda x, z: int(hts[z, x]), levels, n=64, seed=6)
+    for e in edits:
+        o.edit(*e)
+    oh, oc = o.canonical()
+    for mode in (1, 2):
+        if mode == 2:
+            g.close()
+            g = svdag_py.MyGPUHashDAG(levels, 4, expected_items=4_000_000, workspace_items=1 << 24)
+            g.build_terrain(**tp)
+        g.set_batch_mode(mode)
+        g.autotune(edits[:16])
+        g.garbage_collect()
+        for b in range(0, len(edits), 16):
+            st, _ = g.edit_batch(edits[b:b + 16])
+            assert len(st) == 16
+            gc = g.garbage_collect()
+            assert gc.marked == 0, (mode, b)
+        gh, gcnt = g.canonical()
+        assert gcnt == oc and gh == oh, mode
     g.close()

@@ -64,6 +64,39 @@ def test_bulk_add_find_parity(variant):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("variant", [1, 2, 3, 204, 408, 1004, 1208])
+def test_find_only_variants_match_oracle_at_2_20(variant):
+    # The find-only variants (the benchmark's timed one among them) against
+    # the oracle on 2^20 items and 2^20 finds at the benchmark's 75% fill:
+    # hit exactly where the oracle hits, and every hit decodes to its needle.
+    items = workloads.slab_hash_items(1 << 20)
+    finds = workloads.slab_hash_finds(items, 1 << 20, hit_rate=0.5)
+    hs, buckets = _build(items)
+    groups = sorted(items.items())
+    words, segs, n = svdag_py.make_segments(groups)
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    hs.add_dev(wdev, segs, odev, 16)
+    fgroups = [(s, finds[s]) for s, _ in groups]
+    fwords, fsegs, fn = svdag_py.make_segments(fgroups)
+    fdev = svdag_py.DeviceBuffer.from_array(fwords)
+    rdev = svdag_py.DeviceBuffer(4 * fn)
+    hs.find_dev(fdev, fsegs, rdev, variant)
+    res = rdev.to_array(np.uint32, fn)
+    expect = _oracle_found(items, finds, buckets)
+    hits = 0
+    for i, (s, a) in enumerate(fgroups):
+        seg = fsegs[i]
+        r = res[seg.out_offset:seg.out_offset + seg.count]
+        hit = r != svdag_py.NOT_FOUND
+        assert np.array_equal(hit, expect[s]), s
+        assert np.array_equal(hs.read(s, r[hit]), a[hit]), s
+        hits += int(hit.sum())
+    assert abs(hits / fn - 0.5) < 0.01
+    hs.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("variant", [0, 16])
 def test_long_chains_and_duplicates(variant):
     # Few buckets force multi-slab chains; a small alphabet forces duplicates.
