@@ -526,6 +526,41 @@ def large_scene_bench(args):
     return out
 
 
+def reference_format_bench(args):
+    """The reference's version-3 file format at the editing workload's depth
+    (terrain_params(D), 4 material bits): svdag_write_reference_dag of the
+    whole DAG, then svdag_read_reference_dag of those bytes into a fresh
+    context.  Host-side work on the GPU box's cores (the writer rebuilds the
+    reference's 32-slot tables, the reader walks them and imports through the
+    bulk insert).  Checks: both contexts give one canonical hash."""
+    import svdag_py
+    import workloads
+
+    D = args.edit_levels
+    expected = int(64e6 * 4 ** (D - 15)) if D >= 12 else 0
+    g = svdag_py.MyGPUHashDAG(D, 4, expected_items=expected)
+    g.build_terrain(**workloads.terrain_params(D))
+    live = g.garbage_collect().items_live
+    t0 = time.perf_counter()
+    data = g.write_reference_dag()
+    write_s = time.perf_counter() - t0
+    log(f"[bench] reference stream written: {len(data) / 1e6:.1f} MB in {write_s:.2f} s")
+    g2 = svdag_py.MyGPUHashDAG(D, 4, expected_items=expected)
+    t0 = time.perf_counter()
+    used = g2.read_reference_dag(data)
+    read_s = time.perf_counter() - t0
+    log(f"[bench] reference stream read: {read_s:.2f} s")
+    h1, h2 = g.canonical()[0], g2.canonical()[0]
+    if used != len(data) or h1 != h2:
+        raise RuntimeError(f"reference format round trip differs: {h1:#x} vs {h2:#x}, {used} of {len(data)} bytes")
+    g2.close()
+    g.close()
+    return {"levels": D, "material_bits": 4, "live_items": live, "bytes": len(data),
+            "write_s": round(write_s, 3), "read_s": round(read_s, 3),
+            "write_items_per_s": live / write_s, "read_items_per_s": live / read_s,
+            "canonical_hash": f"{h1:#018x}", "host": host_cpu_info()}
+
+
 def editing_profile(args):
     """The batched-editing stream alone (depth 15 terrain, 256 edits in
     16-edit fused batches, one collection after the stream), for a kernel
@@ -736,6 +771,8 @@ def main():
     ap.add_argument("--large-copies", type=int, default=32)
     ap.add_argument("--large-levels", type=int, default=17)
     ap.add_argument("--large-expected", type=float, default=4e8, help="items the depth-17 tables start sized for")
+    ap.add_argument("--ref-format-only", action="store_true",
+                    help="only the reference file-format write / read round trip at --edit-levels")
     ap.add_argument("--editing-profile", action="store_true",
                     help="only the batched-editing stream and its collection (for rocprofv3 --kernel-trace)")
     ap.add_argument("--batch-mode", type=int, default=1, help="editing profile: 1 fused, 2 fused + HIP graphs")
@@ -751,6 +788,9 @@ def main():
         return
     if args.large_scene_only:
         print(json.dumps({"large_scene": large_scene_bench(args)}), flush=True)
+        return
+    if args.ref_format_only:
+        print(json.dumps({"reference_format": reference_format_bench(args)}), flush=True)
         return
     out = slab_hash_bench(args)
     if out is None:

@@ -269,6 +269,14 @@ struct svdag_ctx {
     uint2* dGcTouchedList = nullptr;     // (table, bucket) pairs for the chain pass
     std::vector<uint32_t*> gcTouched;    // per table: a bit per bucket
     uint32_t traceW = 0, traceH = 0, traceRoot = 0xFFFFFFFFu;
+    // Collections run so far: between two collections no slot is freed, so
+    // a root handle names one DAG content.
+    uint64_t gcSerial = 0;
+    // The reference-format stream a size query built (refformat.hip), kept
+    // for the call that fetches it while (root, gcSerial) is unchanged.
+    std::vector<uint8_t> refStream;
+    uint32_t refStreamRoot = 0xFFFFFFFFu;
+    uint64_t refStreamGc = ~0ull;
     std::mutex mu;
     // Serving (serve.hip).  While a server runs, frame traces read the tables
     // on frameStream; whoever reallocates or sweeps tables first takes
@@ -308,6 +316,17 @@ int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H
 // overflow / OOM retry; used by svdag_edit, svdag_edit_batch and the server.
 int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats* stats, float* totalMs);
 int make_editor(const svdag_ctx* ctx, const svdag_edit_desc& d, EditorDev& e);
+
+// Canonical dump (export.hip): for level = leaf level down to 0,
+// [level, count, then per item size, words...] with children as indices into
+// the level below's list, then the root's index.  canonical_dump writes the
+// reachable DAG with each level's unique contents sorted; import_dump takes
+// any dump of that shape whose items are unique within each level, finds or
+// inserts them, and makes its root the context's root (the caller holds
+// ctx->mu).
+int canonical_dump(svdag_ctx* ctx, std::vector<uint32_t>& dump, std::vector<uint32_t>& counts);
+int import_dump(svdag_ctx* ctx, const uint32_t* d, size_t len);
+uint64_t fnv1a(const std::vector<uint32_t>& words);
 
 // Voxel bounds [lo, hi] (inclusive) outside which an edit changes nothing.
 struct Bounds {

@@ -7,6 +7,7 @@
 // size, words...], followed by the root's canonical id.  Allocation order and
 // handle values therefore do not affect it.  hash = FNV-1a 64 of the dump words.
 #include "dag_internal.h"
+#include "host_threads.h"
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -117,7 +118,7 @@ static int reachable(const HostDag& h, const std::vector<uint32_t>& roots, std::
     return SVDAG_OK;
 }
 
-static uint64_t fnv1a(const std::vector<uint32_t>& words)
+uint64_t fnv1a(const std::vector<uint32_t>& words)
 {
     uint64_t x = 1469598103934665603ull;
     for (uint32_t w : words)
@@ -170,74 +171,6 @@ static void canonicalize(const HostDag& h, uint32_t root, const std::vector<std:
     dump.push_back(root == kSentinel ? kSentinel : idBelow.at(root));
 }
 
-// Host threads for the export's loops and sorts (at most 16).
-static uint32_t export_threads()
-{
-    const uint32_t hc = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(16u, hc ? hc : 1u));
-}
-
-// fn(begin, end, thread) over [0, n) in contiguous chunks, one per thread.
-template <typename Fn>
-static void parallel_chunks(size_t n, uint32_t threads, Fn fn)
-{
-    if (n < 65536 || threads <= 1) {
-        fn((size_t)0, n, 0u);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const size_t step = (n + threads - 1) / threads;
-    for (uint32_t t = 0; t < threads; ++t) {
-        const size_t b = std::min(n, (size_t)t * step), e = std::min(n, b + step);
-        pool.emplace_back([=, &fn] { fn(b, e, t); });
-    }
-    for (auto& th : pool)
-        th.join();
-}
-
-// Sorts v by `less`: sorted chunks, then rounds of pairwise merges, each
-// round's merges on their own threads.
-template <typename T, typename Less>
-static void parallel_sort(std::vector<T>& v, uint32_t threads, Less less)
-{
-    const size_t n = v.size();
-    if (n < 65536 || threads <= 1) {
-        std::sort(v.begin(), v.end(), less);
-        return;
-    }
-    const size_t step = (n + threads - 1) / threads;
-    std::vector<size_t> bounds;
-    for (size_t b = 0; b < n; b += step)
-        bounds.push_back(b);
-    bounds.push_back(n);
-    {
-        std::vector<std::thread> pool;
-        for (size_t c = 0; c + 1 < bounds.size(); ++c) {
-            const size_t a = bounds[c], e = bounds[c + 1];
-            pool.emplace_back([&v, a, e, &less] { std::sort(v.begin() + (ptrdiff_t)a, v.begin() + (ptrdiff_t)e, less); });
-        }
-        for (auto& th : pool)
-            th.join();
-    }
-    while (bounds.size() > 2) {
-        std::vector<size_t> next;
-        std::vector<std::thread> pool;
-        for (size_t c = 0; c + 1 < bounds.size(); c += 2) {
-            next.push_back(bounds[c]);
-            if (c + 2 < bounds.size()) {
-                const size_t a = bounds[c], m = bounds[c + 1], e = bounds[c + 2];
-                pool.emplace_back([&v, a, m, e, &less] {
-                    std::inplace_merge(v.begin() + (ptrdiff_t)a, v.begin() + (ptrdiff_t)m, v.begin() + (ptrdiff_t)e, less);
-                });
-            }
-        }
-        next.push_back(n);
-        for (auto& th : pool)
-            th.join();
-        bounds.swap(next);
-    }
-}
-
 // The canonical dump of a large DAG without per-item allocations: the
 // reachable handles of every level as a sorted array (children gathered,
 // sorted, deduplicated), contents in one flat array per level with the
@@ -246,7 +179,7 @@ static void parallel_sort(std::vector<T>& v, uint32_t threads, Less less)
 // word 1) with full comparisons only on ties.  Gathers and sorts run on
 // host threads.  Same dump as canonicalize(): contents sorted by (size,
 // words), numbered by rank.
-static int canonical_dump(svdag_ctx* ctx, std::vector<uint32_t>& dump, std::vector<uint32_t>& counts)
+int canonical_dump(svdag_ctx* ctx, std::vector<uint32_t>& dump, std::vector<uint32_t>& counts)
 {
     HostDag h;
     SVDAG_TRY(copy_dag_to_host(ctx, h));
@@ -507,7 +440,7 @@ static int insert_level(svdag_ctx* ctx, const std::vector<std::vector<uint32_t>>
     return SVDAG_OK;
 }
 
-static int import_dump(svdag_ctx* ctx, const uint32_t* d, size_t len)
+int import_dump(svdag_ctx* ctx, const uint32_t* d, size_t len)
 {
     const DagDev& dag = ctx->dag;
     size_t p = 0;

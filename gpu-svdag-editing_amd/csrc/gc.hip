@@ -816,6 +816,7 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
     // A collection may free the nodes of the last traced root, so the shadow
     // and occlusion passes need a new primary trace afterwards.
     ctx->traceW = ctx->traceH = 0;
+    ++ctx->gcSerial;
     if (!ctx->gcCounters)
         SVDAG_HIP(hipMalloc(&ctx->gcCounters, sizeof(GcCounters)));
     if (ctx->ts.tables.size() > kGcMaxTables) {

@@ -1,6 +1,8 @@
 // Drives the C++ facade the way Engine::edit<T> drives the reference
 // (engine.h:104-163): build a terrain DAG, apply edits, undo/redo, collect
-// garbage, trace.  Prints "<canonical hash hex> <hits>" for a test to check.
+// garbage, trace, then write and read the reference's file format (argv[2]
+// names a scene file to write).  Prints "<canonical hash hex> <hits> <lit>
+// <unoccluded>" for a test to check.
 #include "my_gpu_hash_dag.hpp"
 #include <cstdio>
 #include <cstdlib>
@@ -67,7 +69,32 @@ int main(int argc, char** argv)
             lit += v > 0.0f;
         for (float v : tracer.ambientOcclusion())
             open += v > 0.0f;
-        std::printf("%016llx %u %u %u\n", (unsigned long long)dag.canonical_hash(), hits, lit, open);
+        // The reference's file format: MyGPUHashDAG::writeTo / readFrom, and the
+        // Engine scene file (argv[2]) with DAGInfo and the view.
+        const uint64_t hash = dag.canonical_hash();
+        MyGPUHashDAG copy(levels, 4);
+        const std::vector<uint8_t> stream = dag.writeTo();
+        if (copy.readFrom(stream.data(), stream.size()) != stream.size() || copy.canonical_hash() != hash) {
+            std::fprintf(stderr, "writeTo / readFrom round trip differs\n");
+            return 4;
+        }
+        if (argc > 2) {
+            write_scene(argv[2], info, dag, view);
+            MyGPUHashDAG loaded(levels, 4);
+            DAGInfo info2 {};
+            CameraView view2;
+            read_scene(argv[2], info2, loaded, view2);
+            bool same = loaded.canonical_hash() == hash;
+            for (int k = 0; k < 3; ++k)
+                same = same && info2.boundsAABBMax[k] == side && view2.position[k] == view.position[k];
+            for (int k = 0; k < 9; ++k)
+                same = same && view2.rotation[k] == view.rotation[k];
+            if (!same) {
+                std::fprintf(stderr, "scene file round trip differs\n");
+                return 5;
+            }
+        }
+        std::printf("%016llx %u %u %u\n", (unsigned long long)hash, hits, lit, open);
         return 0;
     } catch (const Error& e) {
         std::fprintf(stderr, "%s\n", e.what());

@@ -108,6 +108,25 @@ public:
         return h;
     }
 
+    // MyGPUHashDAG::writeTo / readFrom (my_gpu_hash_dag.cu:157-177): the
+    // reference's version-3 stream of the reachable DAG.  readFrom takes one
+    // stream from the front of data and returns the bytes it used.
+    std::vector<uint8_t> writeTo() const
+    {
+        size_t n = 0;
+        check(svdag_write_reference_dag(ctx_, nullptr, &n));
+        std::vector<uint8_t> out(n);
+        check(svdag_write_reference_dag(ctx_, out.data(), &n));
+        out.resize(n);
+        return out;
+    }
+    size_t readFrom(const uint8_t* data, size_t len)
+    {
+        size_t used = 0;
+        check(svdag_read_reference_dag(ctx_, data, len, &used));
+        return used;
+    }
+
     svdag_ctx* context() const { return ctx_; }
 
 private:
@@ -264,6 +283,34 @@ struct DAGInfo {
     double boundsAABBMin[3];
     double boundsAABBMax[3];
 };
+
+// Engine::readFrom / writeTo (engine.cpp:1543-1557): the scene file the
+// reference loads at start-up, for its DAG, DAGInfo and camera view (voxel
+// textures are skipped on read and written empty).
+inline void read_scene(const std::string& path, DAGInfo& info, MyGPUHashDAG& dag, CameraView& view)
+{
+    double b[6], v[12];
+    check(svdag_read_reference_scene(dag.context(), path.c_str(), b, v));
+    for (int k = 0; k < 3; ++k) {
+        info.boundsAABBMin[k] = b[k];
+        info.boundsAABBMax[k] = b[3 + k];
+        view.position[k] = v[k];
+    }
+    for (int k = 0; k < 9; ++k)
+        view.rotation[k] = v[3 + k];
+}
+inline void write_scene(const std::string& path, const DAGInfo& info, const MyGPUHashDAG& dag, const CameraView& view)
+{
+    double b[6], v[12];
+    for (int k = 0; k < 3; ++k) {
+        b[k] = info.boundsAABBMin[k];
+        b[3 + k] = info.boundsAABBMax[k];
+        v[k] = view.position[k];
+    }
+    for (int k = 0; k < 9; ++k)
+        v[3 + k] = view.rotation[k];
+    check(svdag_write_reference_scene(dag.context(), path.c_str(), b, v));
+}
 
 class DAGTracer {
 public:

@@ -119,6 +119,8 @@ EXPORTED = [
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
     "svdag_trace_shadows", "svdag_trace_ambient_occlusion", "svdag_build_voxels",
+    "svdag_read_reference_dag", "svdag_write_reference_dag", "svdag_read_reference_scene",
+    "svdag_write_reference_scene",
 ]
 
 _lib = None
@@ -484,6 +486,36 @@ class MyGPUHashDAG:
 
     def load(self, path):
         check(lib().svdag_load(self.h, os.fsencode(path)))
+
+    def write_reference_dag(self):
+        """The reference's MyGPUHashDAG::writeTo stream (version 3) of the reachable DAG, as bytes."""
+        n = ctypes.c_size_t(0)
+        check(lib().svdag_write_reference_dag(self.h, None, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        check(lib().svdag_write_reference_dag(self.h, buf.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+        return buf[:n.value].tobytes()
+
+    def read_reference_dag(self, data):
+        """MyGPUHashDAG::readFrom: imports one version-3 stream from the start of data (bytes);
+        returns the number of bytes it took."""
+        b = np.frombuffer(data, np.uint8)
+        used = ctypes.c_size_t(0)
+        check(lib().svdag_read_reference_dag(self.h, b.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(b.size),
+                                             ctypes.byref(used)))
+        return used.value
+
+    def write_reference_scene(self, path, bounds=None, view=None):
+        """Engine::writeTo: DAGInfo bounds (6 doubles), the DAG, CameraView (12 doubles), no textures."""
+        b = None if bounds is None else (ctypes.c_double * 6)(*bounds)
+        v = None if view is None else (ctypes.c_double * 12)(*view)
+        check(lib().svdag_write_reference_scene(self.h, os.fsencode(path), b, v))
+
+    def read_reference_scene(self, path):
+        """Engine::readFrom: imports the scene file's DAG; returns (bounds, view)."""
+        b = (ctypes.c_double * 6)()
+        v = (ctypes.c_double * 12)()
+        check(lib().svdag_read_reference_scene(self.h, os.fsencode(path), b, v))
+        return tuple(b), tuple(v)
 
     def export_records(self):
         n = ctypes.c_size_t(0)

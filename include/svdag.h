@@ -460,6 +460,43 @@ int svdag_import_canonical(svdag_ctx* ctx, const uint32_t* buf, size_t len_words
 int svdag_save(svdag_ctx* ctx, const char* path);
 int svdag_load(svdag_ctx* ctx, const char* path);
 
+/* The reference's own on-disk format (version 3).
+ *
+ * svdag_read_reference_dag replaces MyGPUHashDAG::readFrom
+ * (my_gpu_hash_dag.cu:167-177): it parses one MyGPUHashDAG stream from buf
+ * (version, fullyFilledNodes, root handle, one 32-slot slab table per item
+ * size with its slab allocator; gpu_hash_table_base.cu:138-162,
+ * dynamic_slab_alloc.cu:181-188), walks every item reachable from the root,
+ * and finds or inserts the same DAG in ctx, whose root it becomes (undo/redo
+ * cleared, as after svdag_import_canonical).  *consumed (optional) receives
+ * the stream's length in bytes.  The stream's depth (fullyFilledNodes width)
+ * and material bits (its height, 2^M) must equal the context's; a truncated
+ * stream, a header that does not describe 32-slot slabs of 10 + 32 s words,
+ * or a reachable handle that addresses no stored item is SVDAG_E_INVALID.
+ *
+ * svdag_write_reference_dag replaces MyGPUHashDAG::writeTo
+ * (my_gpu_hash_dag.cu:157-165) for a GPU_Malloc DAG: the reachable DAG and the
+ * fully-filled chain in tables of about 96 items per bucket
+ * (TARGET_LOAD_FACTOR), as MyGPUHashDAG::allocate and setLoadFactor lay them
+ * out.  Same calling pattern as svdag_export_canonical: *len receives the
+ * stream's length in bytes, and buf (if non-NULL and *len is large enough)
+ * the stream.  A table the reference's handles cannot address is
+ * SVDAG_E_CAPACITY. */
+int svdag_read_reference_dag(svdag_ctx* ctx, const void* buf, size_t len, size_t* consumed);
+int svdag_write_reference_dag(svdag_ctx* ctx, void* buf, size_t* len);
+
+/* Engine::readFrom / Engine::writeTo (engine.cpp:1543-1557): the scene file
+ * the reference loads at start-up (main.cpp:104-109): DAGInfo (AABB min and
+ * max, 6 doubles), the MyGPUHashDAG stream, CameraView (position then the
+ * row-major 3x3 rotation, 12 doubles) and VoxelTextures.  Reading imports the
+ * DAG as svdag_read_reference_dag does and returns the bounds and view
+ * (either may be NULL); the voxel textures are checked for extent and
+ * skipped (colour is not part of this library).  Writing takes bounds and
+ * view (NULL: [0, 2^depth]^3 and CameraView's defaults) and writes no
+ * materials. */
+int svdag_read_reference_scene(svdag_ctx* ctx, const char* path, double* bounds, double* view);
+int svdag_write_reference_scene(svdag_ctx* ctx, const char* path, const double* bounds, const double* view);
+
 #ifdef __cplusplus
 }
 #endif

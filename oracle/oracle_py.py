@@ -29,6 +29,7 @@ def lib():
         L.ora_accel_hash.argtypes = [u32p, ctypes.c_uint32]
         L.ora_table_overflow_slabs.restype = ctypes.c_uint32
         L.ora_table_overflow_slabs.argtypes = [ctypes.c_void_p]
+        L.ora_table_raw.argtypes = [ctypes.c_void_p, u32p, u32p]
         _lib = L
     return _lib
 
@@ -64,6 +65,14 @@ class OracleTable:
 
     def overflow_slabs(self):
         return lib().ora_table_overflow_slabs(self.h)
+
+    def raw(self, num_buckets):
+        """(head slabs, overflow slabs) as flat u32 arrays."""
+        sw = 10 + 32 * self.s
+        heads = np.empty(num_buckets * sw, np.uint32)
+        pool = np.empty(self.overflow_slabs() * sw, np.uint32)
+        lib().ora_table_raw(self.h, _p(heads), _p(pool))
+        return heads, pool
 
     def __del__(self):
         try:

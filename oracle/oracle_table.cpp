@@ -201,4 +201,15 @@ uint32_t ora_table_overflow_slabs(void* tp)
     return (uint32_t)(t->pool.size() / t->slabWords);
 }
 
+// Raw memory of the table: the head slabs (numBuckets * slabWords words) and
+// the overflow slabs in allocation order (overflow slab k is slab id k).
+void ora_table_raw(void* tp, uint32_t* heads, uint32_t* pool)
+{
+    const Table* t = static_cast<const Table*>(tp);
+    if (heads)
+        std::memcpy(heads, t->table.data(), t->table.size() * 4);
+    if (pool)
+        std::memcpy(pool, t->pool.data(), t->pool.size() * 4);
+}
+
 } // extern "C"

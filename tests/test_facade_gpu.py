@@ -21,8 +21,8 @@ def test_facade_header_compiles_without_gpu():
 
 
 @pytest.mark.gpu
-def test_facade_demo_matches_oracle():
-    r = subprocess.run([DEMO, "10"], capture_output=True, text=True, timeout=120)
+def test_facade_demo_matches_oracle(tmp_path):
+    r = subprocess.run([DEMO, "10", str(tmp_path / "scene.dag.bin")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     gpu_hash = int(r.stdout.split()[0], 16)
     tp = workloads.terrain_params(10)
