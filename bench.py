@@ -334,6 +334,11 @@ def editing_bench(args):
         g, _, _ = fresh()
         g.set_batch_mode(mode)
         tune = g.autotune_all(edits[:16])
+        if args.tune_load_factor:
+            t_lf = time.perf_counter()
+            tune.update(g.autotune_load_factor(edits[:16], (50, 75, 95)))
+            tune["load_factor_tuning_s"] = round(time.perf_counter() - t_lf, 1)
+            log(f"[bench] load factor: {tune['load_factor']} {tune['ms_by_load_factor']}")
         g.garbage_collect()
         blat = []
         gpu_ms = 0.0
@@ -360,7 +365,11 @@ def editing_bench(args):
     batched.update({"edits": len(edits), "batch": 16, "gc": "once after the stream",
                     "autotune": {"lanes_per_item": tune["lanes_per_item"], "max_blocks": tune["max_blocks"],
                                  "ms_by_lanes": {str(k): v for k, v in tune["ms_by_lanes"].items()},
-                                 "ms_by_max_blocks": {str(k): v for k, v in tune["ms_by_max_blocks"].items()}}})
+                                 "ms_by_max_blocks": {str(k): v for k, v in tune["ms_by_max_blocks"].items()},
+                                 **({"load_factor": tune["load_factor"],
+                                     "ms_by_load_factor": {str(k): v for k, v in tune["ms_by_load_factor"].items()},
+                                     "load_factor_tuning_s": tune["load_factor_tuning_s"]}
+                                    if "load_factor" in tune else {})}})
     batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
     each, _ = batched_leg(True, 1, "fused, collection after each batch")
     each["speedup_vs_plain"] = each["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
@@ -408,12 +417,28 @@ def editing_bench(args):
         serve_row(rate, reqs, arr, 64)
     sat = serve_row("saturation", sat_reqs, np.zeros(len(sat_reqs)), 64)
     g.close()
+    # The plain per-edit form over the saturation row's own requests (on a
+    # fresh terrain): voxel edits/s depends on the request mix (a request's
+    # voxels grow with r^3, and r is log-uniform), so the serving speedup is
+    # also given on one request set.
+    g, _, _ = fresh()
+    sat_vox = float(sum(workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in sat_reqs))
+    t1 = time.perf_counter()
+    for e in sat_reqs:
+        g.edit(*e)
+    same_s = time.perf_counter() - t1
+    g.close()
+    plain_same = {"requests": len(sat_reqs), "voxel_edits_per_s": sat_vox / same_s,
+                  "edits_per_s": len(sat_reqs) / same_s}
+    log(f"[bench] plain over the saturation requests: {plain_same}")
     cpu = cpu_edit_baseline(args, terrain_dump, edits, D) if not args.no_cpu_baseline else None
     return {"levels": D, "terrain_build_s": round(build_s, 2), "live_items": gc0.items_live,
             "cpu_baseline": cpu,
             "canonical_hash": f"{hashes['plain']:#018x}", "legs_with_that_hash": list(hashes),
             "plain": plain, "batched": batched, "serving": serving, "shading": shading,
-            "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"]}
+            "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"],
+            "plain_over_saturation_requests": plain_same,
+            "serving_saturation_speedup_vs_plain_same_requests": sat["voxel_edits_per_s"] / plain_same["voxel_edits_per_s"]}
 
 
 def large_scene_bench(args):
@@ -771,6 +796,8 @@ def main():
     ap.add_argument("--large-copies", type=int, default=32)
     ap.add_argument("--large-levels", type=int, default=17)
     ap.add_argument("--large-expected", type=float, default=4e8, help="items the depth-17 tables start sized for")
+    ap.add_argument("--tune-load-factor", action="store_true",
+                    help="editing legs also tune the DAG tables' load factor (50/75/95%%; rebuilds the tables)")
     ap.add_argument("--ref-format-only", action="store_true",
                     help="only the reference file-format write / read round trip at --edit-levels")
     ap.add_argument("--editing-profile", action="store_true",

@@ -1398,7 +1398,7 @@ int svdag_destroy(svdag_ctx* ctx)
                      (void*)ctx->gcFront[0], (void*)ctx->gcFront[1], (void*)ctx->gcCounters, ctx->dHits,
                      (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
-                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade, ctx->dSpine,
+                     (void*)ctx->dWaveIdx, ctx->dLastFrame, (void*)ctx->dSurf, (void*)ctx->dShade, (void*)ctx->dAo, (void*)ctx->dColor, (void*)ctx->dAccum, ctx->dSpine,
                      (void*)ctx->dStored, (void*)ctx->dGcCand, (void*)ctx->dGcTouchedList })
         if (p)
             (void)hipFree(p);
@@ -1840,6 +1840,161 @@ int svdag_autotune_all(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n
     SVDAG_TRY(tune_knob(ctx, eds, &ctx->maxBlocks, kBlocks, 5, out->ms_blocks));
     out->group_size = ctx->groupSize;
     out->max_blocks = ctx->maxBlocks;
+    return SVDAG_OK;
+}
+
+// MyGPUHashDAG::setLoadFactor (my_gpu_hash_dag.cu:139-155): new tables sized
+// for the DAG's own items, and the reachable DAG copied into them.  The
+// reference sizes for a number of items per bucket (TARGET_LOAD_FACTOR = 96
+// items in 32-slot slabs); loadFactor here is the percentage of head-line
+// slots the items fill, as svdag_config.load_factor.  The copy goes through
+// the canonical dump and the bulk find-or-insert, after the fully-filled
+// chain, as the reference re-adds every item after MyGPUHashDAG::allocate.
+// The old tables are kept until the copy succeeded, so a failure leaves the
+// DAG as it was.  Handles change: the undo / redo history is cleared and the
+// next collection marks.  The caller holds ctx->mu.
+static int set_load_factor(svdag_ctx* ctx, uint32_t loadFactor)
+{
+    DagDev& d = ctx->dag;
+    std::vector<uint32_t> dump, counts;
+    SVDAG_TRY(canonical_dump(ctx, dump, counts));
+    std::vector<uint64_t> per(d.numTables, 0);
+    size_t p = 0;
+    for (int L = (int)d.leafLevel; L >= 0; --L) {
+        const uint32_t count = dump[p + 1];
+        p += 2;
+        for (uint32_t i = 0; i < count; ++i) {
+            ++per[dump[p] - 2];
+            p += 1 + dump[p];
+        }
+    }
+    per[d.maxLeafWords - 2] += d.numMaterials;
+    per[9 - 2] += (uint64_t)d.numMaterials * d.leafLevel;
+    std::vector<HostTable> other(d.numTables);
+    auto freeOther = [&]() {
+        for (auto& t : other)
+            table_free(t);
+    };
+    for (uint32_t i = 0; i < d.numTables; ++i) {
+        const uint64_t slots = line_slots(2 + i);
+        const uint64_t want = std::max<uint64_t>(64, (per[i] * 100 + slots * loadFactor - 1) / (slots * loadFactor));
+        const uint32_t buckets = (uint32_t)std::min<uint64_t>(want, ctx->slabLimit[i] / 2);
+        const uint64_t cap = std::min<uint64_t>((uint64_t)buckets + (per[i] + slots - 1) / slots + buckets / 2 + 4096,
+                                                ctx->slabLimit[i]);
+        const int rc = table_alloc(other[i], 2 + i, buckets, (uint32_t)cap, ctx->stream, true);
+        if (rc != SVDAG_OK) {
+            freeOther();
+            return rc;
+        }
+    }
+    // No frame trace can read the tables meanwhile: the entry points refuse
+    // while a server (the only source of frame traces) runs.
+    std::vector<uint32_t> ff((size_t)d.levels * d.numMaterials);
+    unsigned long long stored = 0;
+    hipError_t he = hipMemcpyAsync(ff.data(), ctx->dFullyFilled, ff.size() * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(&stored, ctx->dStored, sizeof(stored), hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess)
+        he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) {
+        freeOther();
+        set_error(std::string("set_load_factor: ") + hipGetErrorString(he));
+        return SVDAG_E_DEVICE;
+    }
+    const uint32_t oldRoot = ctx->root;
+    const std::vector<uint32_t> oldLog = ctx->rootLog;
+    const bool oldDirty = ctx->refsDirty;
+    release_wave_graphs(ctx);
+    ++ctx->bufferGeneration; // captured wave graphs hold the old tables
+    ctx->ts.tables.swap(other); // `other` now holds the old tables
+    auto attempt = [&]() -> int {
+        SVDAG_TRY(ctx_refresh_tables(ctx));
+        SVDAG_HIP(hipMemsetAsync(ctx->dStored, 0, sizeof(unsigned long long), ctx->stream));
+        init_fully_filled<<<1, 64, 0, ctx->stream>>>(d);
+        SVDAG_HIP(hipGetLastError());
+        ctx->root = kSentinel;
+        ctx->rootLog.clear();
+        return import_dump(ctx, dump.data(), dump.size());
+    };
+    const int rc = attempt();
+    if (rc != SVDAG_OK) {
+        ctx->ts.tables.swap(other);
+        freeOther();
+        ctx->root = oldRoot;
+        ctx->rootLog = oldLog;
+        ctx->refsDirty = oldDirty;
+        SVDAG_TRY(ctx_refresh_tables(ctx));
+        SVDAG_HIP(hipMemcpyAsync(ctx->dFullyFilled, ff.data(), ff.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(ctx->dStored, &stored, sizeof(stored), hipMemcpyHostToDevice, ctx->stream));
+        SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
+        SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+        return rc;
+    }
+    freeOther();
+    // Per-bucket collection bits follow the new bucket counts.
+    for (uint32_t*& t : ctx->gcTouched)
+        if (t)
+            (void)hipFree(t);
+    ctx->gcTouched.clear();
+    ctx->traceW = ctx->traceH = 0;
+    ++ctx->gcSerial; // handles were reissued
+    ctx->cfg.load_factor = loadFactor;
+    return SVDAG_OK;
+}
+
+int svdag_set_load_factor(svdag_ctx* ctx, uint32_t load_factor)
+{
+    if (!ctx || load_factor < 5 || load_factor > 100) {
+        set_error("svdag_set_load_factor: load_factor must lie in [5, 100]");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->server) {
+        set_error("svdag_set_load_factor: a server is running on this context");
+        return SVDAG_E_INVALID;
+    }
+    return set_load_factor(ctx, load_factor);
+}
+
+int svdag_autotune_load_factor(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, const uint32_t* candidates,
+                               uint32_t num_candidates, uint32_t* chosen, float* ms)
+{
+    if (!ctx || !sample || n == 0 || !candidates || num_candidates == 0 || num_candidates > 8 || !chosen) {
+        set_error("svdag_autotune_load_factor: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    for (uint32_t c = 0; c < num_candidates; ++c)
+        if (candidates[c] < 5 || candidates[c] > 100) {
+            set_error("svdag_autotune_load_factor: load factors must lie in [5, 100]");
+            return SVDAG_E_INVALID;
+        }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->server) {
+        set_error("svdag_autotune_load_factor: a server is running on this context");
+        return SVDAG_E_INVALID;
+    }
+    std::vector<EditorDev> eds;
+    SVDAG_TRY(autotune_prepare(ctx, sample, n, eds));
+    uint32_t best = candidates[0];
+    float bestMs = 0.0f;
+    for (uint32_t c = 0; c < num_candidates; ++c) {
+        SVDAG_TRY(set_load_factor(ctx, candidates[c]));
+        // The sample timed as one knob value (a warm-up run, then the best
+        // of three, each from the same root after a collection).
+        uint32_t unused = 0;
+        const uint32_t one = 0;
+        float m = 0.0f;
+        SVDAG_TRY(tune_knob(ctx, eds, &unused, &one, 1, &m));
+        if (ms)
+            ms[c] = m;
+        if (c == 0 || m < bestMs) {
+            bestMs = m;
+            best = candidates[c];
+        }
+    }
+    if (best != candidates[num_candidates - 1])
+        SVDAG_TRY(set_load_factor(ctx, best));
+    *chosen = best;
     return SVDAG_OK;
 }
 

@@ -252,7 +252,22 @@ struct svdag_ctx {
     // 1 + 2 * axis + (normal points along +axis)), its size and root, for
     // the shadow and ambient-occlusion passes.
     uint8_t* dSurf = nullptr;
-    float* dShade = nullptr; // per-pixel output of the shadow / occlusion passes (hitCap floats)
+    float* dShade = nullptr; // per-pixel sun visibility of the last shadow pass (hitCap floats)
+    float* dAo = nullptr;    // per-pixel result of the last ambient-occlusion pass (hitCap floats)
+    uint32_t* dColor = nullptr; // per-pixel diffuse colour (RGBA8) of the last colour pass
+    // Which of those hold results for the current primary trace, and that
+    // trace's camera (position, first ray, per-pixel steps) for the lighting
+    // pass's fog.
+    bool sunValid = false, aoValid = false, colorValid = false;
+    double traceCam[12] = {};
+    // Path-tracing accumulation (3 floats per pixel) and what it was
+    // accumulated for.
+    float* dAccum = nullptr;
+    size_t accumCap = 0;
+    bool accumValid = false;
+    uint32_t numAccumulated = 0, accumW = 0, accumH = 0, accumRoot = 0xFFFFFFFFu, accumDepth = 0, accumIntegrate = 0;
+    float accumEnv = 0.0f;
+    double accumCam[12] = {};
     void* dSpine = nullptr;  // parallel spine scratch, allocated with the batch slots
     // Reference counting (gc.hip): items stored in the tables, every root the
     // context has held since the last collection, and whether the counts

@@ -532,8 +532,15 @@ __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32
 
 // The listed buckets of table blockIdx.y: unlink the overflow lines left
 // without items and push them on the free stack (as gc_sweep_chains).
-__global__ void __launch_bounds__(256) rc_chains(GcTables a, const uint2* list, uint32_t cap, GcCounters* ctr)
+__global__ void __launch_bounds__(256) rc_chains(DagDev dag, GcTables a, const uint2* list, uint32_t cap,
+                                                 GcCounters* ctr)
 {
+    // The release rounds have all run: the stored-item count loses what
+    // they freed (once, by the first block).
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        *dag.stored -= ctr->itemsFreed;
+        ctr->itemsLive = *dag.stored;
+    }
     const uint32_t ti = blockIdx.y;
     const TableDev& t = a.t[ti];
     const uint32_t n = min(ctr->touched, cap);
@@ -572,14 +579,6 @@ __global__ void __launch_bounds__(256) rc_chains(GcTables a, const uint2* list, 
         }
     }
     gc_block_add(freedSlabs, &ctr->slabsFreed);
-}
-
-__global__ void rc_finish(DagDev dag, GcCounters* ctr)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *dag.stored -= ctr->itemsFreed;
-        ctr->itemsLive = *dag.stored;
-    }
 }
 
 // Visit bytes for every table (16 per line), kept between collections;
@@ -782,8 +781,7 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
     for (uint32_t r = 0; r < rounds; ++r)
         rc_release<<<256, 256, 0, st>>>(dag, a, rootTable, rootPtr, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
                                          &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
-    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(a, ctx->dGcTouchedList, cap2, ctr);
-    rc_finish<<<1, 64, 0, st>>>(dag, ctr);
+    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, ctr);
     SVDAG_HIP(hipEventRecord(ctx->ev1, st));
     GcCounters h {};
     SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));

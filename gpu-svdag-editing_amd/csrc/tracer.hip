@@ -16,8 +16,11 @@
 // product nvcc would fuse there is not fixed by its documentation.  The
 // camera setup is double precision.
 #include "dag_internal.h"
+#include <numbers>
+#include <numeric>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 namespace svdag {
 
@@ -104,40 +107,18 @@ __device__ inline uint32_t intersection_mask(uint32_t levels, uint32_t level, ui
 
 constexpr uint32_t kTraceBlock = 256;
 
-__global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t root, TraceParams p, svdag_hit* out,
-                                                             uint8_t* surf)
+// intersect_ray (tracer.cu:186-273): the first set voxel along the ray, in
+// visit order, and its material; (0, 0, 0) and kSentinel when there is none.
+// stk is this lane's column of the block's [level][lane] LDS stack.
+__device__ inline void closest_voxel(const DagDev& dag, uint32_t root, const Ray& ray, uint2 (*stk)[kTraceBlock],
+                                     uint32_t tid, uint32_t& x, uint32_t& y, uint32_t& z, uint32_t& material)
 {
-    __shared__ uint2 stack[20][kTraceBlock]; // levels <= 20
-    const uint32_t tid = threadIdx.x;
-    const uint32_t tilesX = (p.width + 15) / 16;
-    const uint32_t px = (blockIdx.x % tilesX) * 16 + (tid & 15u);
-    const uint32_t py = (blockIdx.x / tilesX) * 16 + (tid >> 4);
-    if (px >= p.width || py >= p.height)
-        return;
-
-    // Per-pixel ray in double, normalised, then rounded to float (tracer.cu:402-405).
-    // rayMin + px*ddx + py*ddy has one fusion per add under --fmad=true:
-    // fma(py, ddy, fma(px, ddx, rayMin)).
-    double a[3];
-    for (int k = 0; k < 3; ++k)
-        a[k] = fma((double)py, p.ddy[k], fma((double)px, p.ddx[k], p.rayMin[k])) - p.cam[k];
-    const double len = sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
-    const double inv = 1.0 / len;
-    Ray ray;
-    for (int k = 0; k < 3; ++k) {
-        ray.o[k] = (float)p.cam[k];
-        ray.d[k] = (float)(inv * a[k]);
-        ray.inv[k] = 1.0f / ray.d[k];
-    }
-    ray.tmin = 0.0f;
-    ray.tmax = FLT_MAX;
-
     const uint32_t levels = dag.levels;
     const uint32_t leafLevel = dag.leafLevel;
     const uint32_t order = (ray.d[0] < 0.f ? 4u : 0u) + (ray.d[1] < 0.f ? 2u : 0u) + (ray.d[2] < 0.f ? 1u : 0u);
     uint32_t level = 0;
-    uint32_t x = 0, y = 0, z = 0;
-    uint32_t material = kSentinel;
+    x = y = z = 0;
+    material = kSentinel;
 
     if (root != kSentinel) {
         uint32_t cIndex = root;
@@ -150,7 +131,7 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
             uint32_t newLevel = level;
             while (newLevel > 0 && !cVisit) {
                 --newLevel;
-                const uint2 e = stack[newLevel][tid];
+                const uint2 e = stk[newLevel][tid];
                 cIndex = e.x;
                 cChild = e.y & 0xFFu;
                 cVisit = (e.y >> 8) & 0xFFu;
@@ -170,7 +151,7 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
             x = (x << 1) | ((nc >> 2) & 1u);
             y = (y << 1) | ((nc >> 1) & 1u);
             z = (z << 1) | (nc & 1u);
-            stack[level][tid] = make_uint2(cIndex, cChild | (cVisit << 8));
+            stk[level][tid] = make_uint2(cIndex, cChild | (cVisit << 8));
             ++level;
 
             if (level == levels) {
@@ -202,7 +183,15 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
         x = y = z = 0;
     }
 
-    // createSurfaceInteraction (tracer.cu:353-389): entry face of the hit voxel.
+}
+
+// createSurfaceInteraction (tracer.cu:353-389): the entry face of voxel
+// (x, y, z) along the ray: t, the position origin + t * direction (one
+// fusion possible), and the surface code 1 + 2 * axis + (the normal, which
+// points against the ray, is +axis); 0 for the null path (0, 0, 0), which
+// is no surface (Path::is_null, path.h:61-64).
+__device__ inline uint32_t entry_face(const Ray& ray, uint32_t x, uint32_t y, uint32_t z, float& t, float pos[3])
+{
     const float bmin[3] = { (float)x, (float)y, (float)z };
     float dists[3];
     for (int k = 0; k < 3; ++k) {
@@ -212,23 +201,57 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t
         dists[k] = fmin_ref(t1, t2);
     }
     const int axis = dists[0] > dists[1] ? (dists[0] > dists[2] ? 0 : 2) : (dists[1] > dists[2] ? 1 : 2);
-    const float t = dists[axis];
+    t = dists[axis];
+    for (int k = 0; k < 3; ++k)
+        pos[k] = fmaf(t, ray.d[k], ray.o[k]);
+    return (x | y | z) == 0u ? 0u : 1u + 2u * (uint32_t)axis + (ray.d[axis] < 0.0f ? 1u : 0u);
+}
+
+__global__ void __launch_bounds__(kTraceBlock) trace_kernel(DagDev dag, uint32_t root, TraceParams p, svdag_hit* out,
+                                                             uint8_t* surf)
+{
+    __shared__ uint2 stack[20][kTraceBlock]; // levels <= 20
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tilesX = (p.width + 15) / 16;
+    const uint32_t px = (blockIdx.x % tilesX) * 16 + (tid & 15u);
+    const uint32_t py = (blockIdx.x / tilesX) * 16 + (tid >> 4);
+    if (px >= p.width || py >= p.height)
+        return;
+
+    // Per-pixel ray in double, normalised, then rounded to float (tracer.cu:402-405).
+    // rayMin + px*ddx + py*ddy has one fusion per add under --fmad=true:
+    // fma(py, ddy, fma(px, ddx, rayMin)).
+    double a[3];
+    for (int k = 0; k < 3; ++k)
+        a[k] = fma((double)py, p.ddy[k], fma((double)px, p.ddx[k], p.rayMin[k])) - p.cam[k];
+    const double len = sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
+    const double inv = 1.0 / len;
+    Ray ray;
+    for (int k = 0; k < 3; ++k) {
+        ray.o[k] = (float)p.cam[k];
+        ray.d[k] = (float)(inv * a[k]);
+        ray.inv[k] = 1.0f / ray.d[k];
+    }
+    ray.tmin = 0.0f;
+    ray.tmax = FLT_MAX;
+
+    uint32_t x, y, z, material;
+    closest_voxel(dag, root, ray, stack, tid, x, y, z, material);
+
     svdag_hit h;
     h.x = x;
     h.y = y;
     h.z = z;
     h.material = material;
-    h.t = t;
-    // ray.origin + t * ray.direction (tracer.cu:388), one fusion possible.
-    h.px = fmaf(t, ray.d[0], ray.o[0]);
-    h.py = fmaf(t, ray.d[1], ray.o[1]);
-    h.pz = fmaf(t, ray.d[2], ray.o[2]);
+    float pos[3];
+    const uint32_t code = entry_face(ray, x, y, z, h.t, pos);
+    h.px = pos[0];
+    h.py = pos[1];
+    h.pz = pos[2];
     const size_t at = (size_t)(p.height - 1 - py) * p.width + px;
     out[at] = h;
-    // The surface normal points against the ray on the entry axis; a null
-    // path (0,0,0) is no surface (Path::is_null, path.h:61-64).
     if (surf)
-        surf[at] = (x | y | z) == 0u ? 0u : (uint8_t)(1u + 2u * (uint32_t)axis + (ray.d[axis] < 0.0f ? 1u : 0u));
+        surf[at] = (uint8_t)code;
 }
 
 // ------------------------------------------------------------------ sampling
@@ -464,6 +487,305 @@ __global__ void __launch_bounds__(kTraceBlock) ao_kernel(DagDev dag, uint32_t ro
     out[i] = (float)hitsUnoccluded / (float)numSamples;
 }
 
+// ---------------------------------------------------------------- colour and lighting
+// ColorUtils (color_utils.h:31-45, 233-252), clamp (cuda_math.h:44-45) and
+// the light colours of tracer.cu:598-611, in float as there.
+__device__ inline float clamp01(float f)
+{
+    return f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+}
+__device__ inline uint32_t to_rgb888(const float c[3])
+{
+    return (uint32_t)(clamp01(c[0]) * 255.0f) | ((uint32_t)(clamp01(c[1]) * 255.0f) << 8)
+           | ((uint32_t)(clamp01(c[2]) * 255.0f) << 16) | 0xff000000u;
+}
+__device__ inline float srgb_to_linear(float c)
+{
+    const float lo = c / 12.92f;
+    const float hi = powf((c + 0.055f) / 1.055f, 2.4f);
+    return c <= 0.04045f ? lo : hi;
+}
+// pow(|c|, 1/2.4) * 1.055 - 0.055: the one product-then-add, fused as nvcc
+// fuses it by default.
+__device__ inline float linear_to_srgb(float c)
+{
+    const float lo = c * 12.92f;
+    const float hi = fmaf(powf(fabsf(c), 1.0f / 2.4f), 1.055f, -0.055f);
+    return c <= 0.0031308f ? lo : hi;
+}
+__device__ inline float sky_channel(int k)
+{
+    return (k == 0 ? 187.0f : (k == 1 ? 242.0f : 250.0f)) / 255.0f;
+}
+__device__ inline float sun_channel(int k)
+{
+    return k == 0 ? 0.95f : (k == 1 ? 0.94f : 0.47f);
+}
+
+// Tracer::trace_colors -> computeColorAtSurface (tracer.cu:413-596) for a DAG
+// with materials, each material's textures being one colour (texture images
+// are not part of this library): no surface -> 0; a material without a
+// colour -> the invalid-colour checker (1, b, 1 - b), b = (x ^ y ^ z) & 1 of
+// the hit voxel; else the material's colour; stored as RGBA8.
+__global__ void __launch_bounds__(256) colors_kernel(const svdag_hit* hits, uint32_t n, const float* palette,
+                                                     uint32_t numColors, uint32_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const svdag_hit h = hits[i];
+    float c[3] = { 0.0f, 0.0f, 0.0f };
+    if (h.x != 0u || h.y != 0u || h.z != 0u) {
+        if (h.material >= numColors) {
+            const float b = (float)((h.x ^ h.y ^ h.z) & 1u);
+            c[0] = 1.0f;
+            c[1] = b;
+            c[2] = 1.0f - b;
+        } else {
+            for (int k = 0; k < 3; ++k)
+                c[k] = palette[3 * h.material + k];
+        }
+    }
+    out[i] = to_rgb888(c);
+}
+
+struct LightingDev {
+    double cam[3], rayMin[3], ddx[3], ddy[3];
+    float sunDir[3];
+    float sunBrightness, fogDensity;
+    uint32_t width, applyAo, applySun;
+};
+
+// Tracer::trace_lighting (tracer.cu:709-750) with applyFog (tracer.cu:693-707);
+// the tool overlay (TOOL_OVERLAY) is not built.  Pixel (x, y) of the stored
+// image reads the stored surface, occlusion and sun at its own index, and
+// reconstructs its fog ray from (x, y) as the reference does.
+__global__ void __launch_bounds__(256) lighting_kernel(const svdag_hit* hits, uint32_t n, const uint32_t* colors,
+                                                       const float* ao, const float* sun, LightingDev p,
+                                                       uint32_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const svdag_hit h = hits[i];
+    float c[3];
+    if (h.x == 0u && h.y == 0u && h.z == 0u) {
+        for (int k = 0; k < 3; ++k)
+            c[k] = sky_channel(k);
+    } else {
+        const uint32_t col = colors[i];
+        const float occl = p.applyAo ? ao[i] : 1.0f;
+        const float vis = p.applySun ? sun[i] : 1.0f;
+        for (int k = 0; k < 3; ++k) {
+            const float kd = srgb_to_linear((float)((col >> (8 * k)) & 0xFFu) / 255.0f);
+            const float ambient = (occl * 0.5f) * sky_channel(k);
+            const float direct = (vis * sun_channel(k)) * p.sunBrightness;
+            c[k] = (ambient + direct) * kd;
+        }
+        if (p.fogDensity > 0.0f) {
+            const uint32_t x = i % p.width, y = i / p.width;
+            double d[3];
+            for (int k = 0; k < 3; ++k)
+                d[k] = fma((double)y, p.ddy[k], fma((double)x, p.ddx[k], p.rayMin[k])) - p.cam[k];
+            const double len = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+            float dir[3], o[3];
+            for (int k = 0; k < 3; ++k) {
+                dir[k] = (float)(d[k] * (1.0 / len));
+                o[k] = (float)p.cam[k];
+            }
+            const float pos[3] = { h.px, h.py, h.pz };
+            float dx[3];
+            for (int k = 0; k < 3; ++k)
+                dx[k] = o[k] - pos[k];
+            const float dist = sqrtf((dx[0] * dx[0] + dx[1] * dx[1]) + dx[2] * dx[2]);
+            const float fogAmount = 1.0f - expf(-dist * (p.fogDensity * 0.00001f));
+            const float sd = (dir[0] * p.sunDir[0] + dir[1] * p.sunDir[1]) + dir[2] * p.sunDir[2];
+            const float sunAmount = 1.01f * fmaxf(sd, 0.0f);
+            const float w = powf(sunAmount, 30.0f);
+            const float f = clamp01(fogAmount);
+            for (int k = 0; k < 3; ++k) {
+                const float fogColor = sky_channel(k) * (1.0f - w) + 1.0f * w;
+                c[k] = c[k] * (1.0f - f) + fogColor * f;
+            }
+        }
+    }
+    float s[3];
+    for (int k = 0; k < 3; ++k)
+        s[k] = linear_to_srgb(c[k]);
+    out[i] = to_rgb888(s);
+}
+
+struct PathDev {
+    TraceParams cam;
+    float sunDir[3];
+    float env;
+    uint32_t numSamples, maxDepth, integrate, numAccumulated, numColors;
+    uint64_t seed;
+};
+
+// Tracer::trace_path_tracing (tracer.cu:758-842), one lane per stored pixel
+// i = y * W + x.  The PCG32 stream is i * max(seed, 1).  Sample camera rays
+// go through (x, H - 1 - y) (plus a jitter when integrating); at depth 0
+// without integration the path starts on the stored primary surface of
+// pixel i with the colour pass's colour.  Per bounce: nothing hit -> sky
+// times env, stop; else next-event estimation toward the sun from
+// position + 0.1 * normal, then a uniform hemisphere sample (z = u.x,
+// phi = 2 pi u.y, pdf 1 / (2 pi)) continues the path with
+// c *= brdf / pdf, brdf(d) = max(0, N.d) / pi * Kd, Kd the material's colour
+// made linear.  Sums with one product per add are fused as nvcc fuses them;
+// the products with the axis-aligned frame vectors are exact either way.
+__global__ void __launch_bounds__(kTraceBlock) path_kernel(DagDev dag, uint32_t root, PathDev p, const svdag_hit* hits,
+                                                           const uint8_t* surf, const uint32_t* colors,
+                                                           const float* palette, float* accum, uint32_t* out)
+{
+    __shared__ uint2 stack[20][kTraceBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t W = p.cam.width, H = p.cam.height;
+    const uint32_t i = blockIdx.x * kTraceBlock + tid;
+    if (i >= W * H)
+        return;
+    const uint32_t x = i % W, y = i / W;
+    Pcg32 rng((uint64_t)i * (p.seed > 1ull ? p.seed : 1ull));
+    float Lo[3] = { accum[3 * i], accum[3 * i + 1], accum[3 * i + 2] };
+    const float invPi = std::numbers::inv_pi_v<float>;
+    const float pdf = 0.5f * invPi;
+    const float eps = 0.1f;
+    for (uint32_t s = 0; s < p.numSamples; ++s) {
+        float sx = (float)x, sy = (float)y;
+        if (p.integrate) {
+            sx += rng.next_float();
+            sy += rng.next_float();
+        }
+        const double row = (double)((float)(H - 1) - sy);
+        double a[3];
+        for (int k = 0; k < 3; ++k)
+            a[k] = fma(row, p.cam.ddy[k], fma((double)sx, p.cam.ddx[k], p.cam.rayMin[k])) - p.cam.cam[k];
+        const double len = sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
+        const double inv = 1.0 / len;
+        float o[3], dir[3];
+        for (int k = 0; k < 3; ++k) {
+            o[k] = (float)p.cam.cam[k];
+            dir[k] = (float)(inv * a[k]);
+        }
+        float c[3] = { 1.0f, 1.0f, 1.0f };
+        for (uint32_t depth = 0; depth < p.maxDepth; ++depth) {
+            uint32_t code = 0;
+            float pos[3] = { 0.0f, 0.0f, 0.0f }, kd[3] = { 0.0f, 0.0f, 0.0f };
+            if (depth == 0 && !p.integrate) {
+                const svdag_hit h = hits[i];
+                code = surf[i];
+                pos[0] = h.px;
+                pos[1] = h.py;
+                pos[2] = h.pz;
+                for (int k = 0; k < 3; ++k)
+                    kd[k] = srgb_to_linear((float)((colors[i] >> (8 * k)) & 0xFFu) / 255.0f);
+            } else {
+                Ray r;
+                make_ray(r, o, dir, FLT_MAX);
+                uint32_t hx, hy, hz, m;
+                closest_voxel(dag, root, r, stack, tid, hx, hy, hz, m);
+                float t;
+                code = entry_face(r, hx, hy, hz, t, pos);
+                if (code) {
+                    float col[3];
+                    if (m >= p.numColors) {
+                        const float b = (float)((hx ^ hy ^ hz) & 1u);
+                        col[0] = 1.0f;
+                        col[1] = b;
+                        col[2] = 1.0f - b;
+                    } else {
+                        for (int k = 0; k < 3; ++k)
+                            col[k] = palette[3 * m + k];
+                    }
+                    for (int k = 0; k < 3; ++k)
+                        kd[k] = srgb_to_linear(col[k]);
+                }
+            }
+            if (!code) {
+                for (int k = 0; k < 3; ++k)
+                    Lo[k] = fmaf(c[k] * sky_channel(k), p.env, Lo[k]);
+                break;
+            }
+            float n[3], du[3], dv[3];
+            surface_frame(code, n, du, dv);
+            float so[3];
+            for (int k = 0; k < 3; ++k)
+                so[k] = fmaf(eps, n[k], pos[k]);
+            Ray sr;
+            make_ray(sr, so, p.sunDir, FLT_MAX);
+            if (!occluded(dag, root, sr, stack, tid)) {
+                float ndl = (n[0] * p.sunDir[0] + n[1] * p.sunDir[1]) + n[2] * p.sunDir[2];
+                ndl = 0.0f > ndl ? 0.0f : ndl;
+                for (int k = 0; k < 3; ++k)
+                    Lo[k] = fmaf((c[k] * ((ndl * invPi) * kd[k])) * sun_channel(k), p.env, Lo[k]);
+            }
+            const float ux = rng.next_float(), uy = rng.next_float();
+            const float q = fmaf(-ux, ux, 1.0f);
+            const float rr = sqrtf(0.0f < q ? q : 0.0f);
+            float sn, cs;
+            sincos_turns(uy, sn, cs);
+            const float local[3] = { rr * cs, rr * sn, ux };
+            for (int k = 0; k < 3; ++k) {
+                dir[k] = (local[0] * du[k] + local[1] * dv[k]) + local[2] * n[k];
+                o[k] = so[k];
+            }
+            float ndd = (n[0] * dir[0] + n[1] * dir[1]) + n[2] * dir[2];
+            ndd = 0.0f > ndd ? 0.0f : ndd;
+            for (int k = 0; k < 3; ++k)
+                c[k] = c[k] * (((ndd * invPi) * kd[k]) / pdf);
+        }
+    }
+    float v[3];
+    for (int k = 0; k < 3; ++k) {
+        accum[3 * i + k] = Lo[k];
+        const float m = Lo[k] / (float)(p.numSamples + p.numAccumulated);
+        v[k] = linear_to_srgb(m < 1.0f ? m : 1.0f);
+    }
+    out[i] = to_rgb888(v);
+}
+
+// BlurKernel (blur.cu:8-34): 2 * half + 1 weights of gaussian(half + 0.5,
+// sigma, x), sigma = 0.3 (half - 1) + 0.8 passed as the variance, each
+// divided by their std::reduce sum; host float arithmetic, as there.
+struct BlurWeights {
+    float w[9];
+};
+static BlurWeights blur_weights()
+{
+    constexpr uint32_t half = 4; // DAGTracer's ambientOcclusionBlurKernel(4) (dag_tracer.cu:19)
+    const float sigma = 0.3f * (float)(half - 1) + 0.8f;
+    const float mean = (float)half + 0.5f;
+    std::vector<float> k(2 * half + 1);
+    for (size_t x = 0; x < k.size(); ++x) {
+        const float norm = 1.0f / std::sqrt(2 * std::numbers::pi_v<float> * sigma);
+        const float d = (float)x - mean;
+        k[x] = norm * std::exp(-0.5f * d * d / sigma);
+    }
+    const float sum = std::reduce(k.begin(), k.end(), 0.0f, std::plus<float>());
+    BlurWeights b {};
+    for (size_t x = 0; x < k.size(); ++x)
+        b.w[x] = k[x] / sum;
+    return b;
+}
+
+// horizontalKernel / verticalKernel (blur.cu:36-66): taps outside the image
+// are skipped (no renormalisation); sum += weight * value, one fusion.
+template <bool kVertical>
+__global__ void __launch_bounds__(256) blur_kernel(const float* in, float* out, uint32_t W, uint32_t H, BlurWeights b)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H)
+        return;
+    const int x = (int)(i % W), y = (int)(i / W);
+    float sum = 0.0f;
+    for (int d = -4; d <= 4; ++d) {
+        const int nx = kVertical ? x : x + d, ny = kVertical ? y + d : y;
+        if (kVertical ? (ny >= 0 && ny < (int)H) : (nx >= 0 && nx < (int)W))
+            sum = fmaf(b.w[d + 4], in[(size_t)ny * W + nx], sum);
+    }
+    out[i] = sum;
+}
+
 // get_trace_params (dag_tracer.cu:59-101), double precision.
 static TraceParams trace_params(const svdag_camera& c, uint32_t levels, uint32_t W, uint32_t H)
 {
@@ -527,22 +849,31 @@ extern "C" int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* cam, uint
     std::lock_guard<std::mutex> lock(ctx->mu);
     const size_t n = (size_t)W * H;
     if (ctx->hitCap < n) {
-        if (ctx->dHits)
-            (void)hipFree(ctx->dHits);
-        if (ctx->dSurf)
-            (void)hipFree(ctx->dSurf);
-        if (ctx->dShade)
-            (void)hipFree(ctx->dShade);
-        ctx->dHits = nullptr;
-        ctx->dSurf = nullptr;
-        ctx->dShade = nullptr;
+        for (void** b : { (void**)&ctx->dHits, (void**)&ctx->dSurf, (void**)&ctx->dShade, (void**)&ctx->dAo,
+                          (void**)&ctx->dColor }) {
+            if (*b)
+                (void)hipFree(*b);
+            *b = nullptr;
+        }
         ctx->hitCap = 0;
         SVDAG_HIP(hipMalloc(&ctx->dHits, n * sizeof(svdag_hit)));
         SVDAG_HIP(hipMalloc(&ctx->dSurf, n));
         SVDAG_HIP(hipMalloc(&ctx->dShade, n * sizeof(float)));
+        SVDAG_HIP(hipMalloc(&ctx->dAo, n * sizeof(float)));
+        SVDAG_HIP(hipMalloc(&ctx->dColor, n * sizeof(uint32_t)));
         ctx->hitCap = n;
     }
     ctx->traceW = ctx->traceH = 0;
+    ctx->sunValid = ctx->aoValid = ctx->colorValid = false;
+    {
+        const TraceParams tp = trace_params(*cam, ctx->dag.levels, W, H);
+        for (int k = 0; k < 3; ++k) {
+            ctx->traceCam[k] = tp.cam[k];
+            ctx->traceCam[3 + k] = tp.rayMin[k];
+            ctx->traceCam[6 + k] = tp.ddx[k];
+            ctx->traceCam[9 + k] = tp.ddy[k];
+        }
+    }
     SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     SVDAG_TRY(launch_trace(ctx, *cam, W, H, ctx->root, ctx->stream, (svdag_hit*)ctx->dHits, ctx->dSurf));
     SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -578,17 +909,19 @@ static int shade_pass(svdag_ctx* ctx, int kind, uint32_t numSamples, float bias,
     }
     const uint32_t n = ctx->traceW * ctx->traceH;
     const uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    float* dst = kind == 0 ? ctx->dShade : ctx->dAo;
     SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (kind == 0)
         shadow_kernel<<<grid, kTraceBlock, 0, ctx->stream>>>(ctx->dag, ctx->traceRoot, (const svdag_hit*)ctx->dHits,
-                                                             ctx->dSurf, n, bias, sun_direction(), ctx->dShade);
+                                                             ctx->dSurf, n, bias, sun_direction(), dst);
     else
         ao_kernel<<<grid, kTraceBlock, 0, ctx->stream>>>(ctx->dag, ctx->traceRoot, (const svdag_hit*)ctx->dHits,
-                                                         ctx->dSurf, n, numSamples, bias, rayLength, seed, ctx->dShade);
+                                                         ctx->dSurf, n, numSamples, bias, rayLength, seed, dst);
     SVDAG_HIP(hipGetLastError());
     SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    SVDAG_HIP(hipMemcpyAsync(out, ctx->dShade, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipMemcpyAsync(out, dst, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    (kind == 0 ? ctx->sunValid : ctx->aoValid) = true;
     if (ms)
         SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
     return SVDAG_OK;
@@ -612,4 +945,227 @@ extern "C" int svdag_trace_ambient_occlusion(svdag_ctx* ctx, uint32_t num_sample
         return SVDAG_E_INVALID;
     }
     return shade_pass(ctx, 1, num_samples, shadow_bias, ao_ray_length, seed, out, ms);
+}
+
+extern "C" int svdag_trace_colors(svdag_ctx* ctx, const float* material_rgb, uint32_t num_materials, uint32_t* out,
+                                  float* ms)
+{
+    if (!ctx || (num_materials && !material_rgb) || num_materials > 65536) {
+        set_error("svdag_trace_colors: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->traceW == 0 || !ctx->dColor) {
+        set_error("svdag_trace_colors needs a primary trace first (svdag_trace_primary)");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t n = ctx->traceW * ctx->traceH;
+    float* dPal = nullptr;
+    SVDAG_HIP(hipMalloc(&dPal, std::max<size_t>(num_materials, 1) * 3 * sizeof(float)));
+    struct Free {
+        float* p;
+        ~Free() { (void)hipFree(p); }
+    } fr { dPal };
+    if (num_materials)
+        SVDAG_HIP(hipMemcpyAsync(dPal, material_rgb, (size_t)num_materials * 3 * sizeof(float), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    colors_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const svdag_hit*)ctx->dHits, n, dPal, num_materials,
+                                                            ctx->dColor);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (out)
+        SVDAG_HIP(hipMemcpyAsync(out, ctx->dColor, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->colorValid = true;
+    if (ms)
+        SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return SVDAG_OK;
+}
+
+extern "C" int svdag_trace_lighting(svdag_ctx* ctx, const svdag_lighting* params, uint32_t* out, float* ms)
+{
+    if (!ctx || !params || !out || !std::isfinite(params->fog_density) || !std::isfinite(params->sun_brightness)) {
+        set_error("svdag_trace_lighting: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->traceW == 0 || !ctx->colorValid) {
+        set_error("svdag_trace_lighting needs a primary trace and its colour pass first (svdag_trace_colors)");
+        return SVDAG_E_INVALID;
+    }
+    if ((params->apply_ambient_occlusion && !ctx->aoValid) || (params->apply_shadows && !ctx->sunValid)) {
+        set_error("svdag_trace_lighting: the occlusion / shadow pass it applies has not run on this trace");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t n = ctx->traceW * ctx->traceH;
+    LightingDev p {};
+    for (int k = 0; k < 3; ++k) {
+        p.cam[k] = ctx->traceCam[k];
+        p.rayMin[k] = ctx->traceCam[3 + k];
+        p.ddx[k] = ctx->traceCam[6 + k];
+        p.ddy[k] = ctx->traceCam[9 + k];
+    }
+    const float3 sd = sun_direction();
+    p.sunDir[0] = sd.x;
+    p.sunDir[1] = sd.y;
+    p.sunDir[2] = sd.z;
+    p.sunBrightness = params->sun_brightness;
+    p.fogDensity = params->fog_density;
+    p.width = ctx->traceW;
+    p.applyAo = params->apply_ambient_occlusion ? 1u : 0u;
+    p.applySun = params->apply_shadows ? 1u : 0u;
+    uint32_t* dOut = nullptr;
+    SVDAG_HIP(hipMalloc(&dOut, (size_t)n * 4));
+    struct Free {
+        uint32_t* p;
+        ~Free() { (void)hipFree(p); }
+    } fr { dOut };
+    SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    lighting_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>((const svdag_hit*)ctx->dHits, n, ctx->dColor, ctx->dAo,
+                                                              ctx->dShade, p, dOut);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    SVDAG_HIP(hipMemcpyAsync(out, dOut, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    if (ms)
+        SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return SVDAG_OK;
+}
+
+extern "C" int svdag_blur_ambient_occlusion(svdag_ctx* ctx, float* out, float* ms)
+{
+    if (!ctx)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->traceW == 0 || !ctx->aoValid) {
+        set_error("svdag_blur_ambient_occlusion needs an ambient-occlusion pass on the current trace first");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t W = ctx->traceW, H = ctx->traceH, n = W * H;
+    if (W % 8 || H % 8) {
+        set_error("svdag_blur_ambient_occlusion: width and height must be multiples of 8 (as BlurKernel::apply)");
+        return SVDAG_E_INVALID;
+    }
+    float* dScratch = nullptr;
+    SVDAG_HIP(hipMalloc(&dScratch, (size_t)n * sizeof(float)));
+    struct Free {
+        float* p;
+        ~Free() { (void)hipFree(p); }
+    } fr { dScratch };
+    const BlurWeights b = blur_weights();
+    SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    blur_kernel<false><<<(n + 255) / 256, 256, 0, ctx->stream>>>(ctx->dAo, dScratch, W, H, b);
+    blur_kernel<true><<<(n + 255) / 256, 256, 0, ctx->stream>>>(dScratch, ctx->dAo, W, H, b);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (out)
+        SVDAG_HIP(hipMemcpyAsync(out, ctx->dAo, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    if (ms)
+        SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return SVDAG_OK;
+}
+
+extern "C" int svdag_trace_path_tracing(svdag_ctx* ctx, const svdag_path_tracing* params, const float* material_rgb,
+                                        uint32_t num_materials, uint32_t* out, uint32_t* accumulated, float* ms)
+{
+    if (!ctx || !params || !out || (num_materials && !material_rgb) || num_materials > 65536
+        || !std::isfinite(params->environment_brightness) || params->num_samples > 4096
+        || params->max_path_depth > 64) {
+        set_error("svdag_trace_path_tracing: bad arguments");
+        return SVDAG_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->traceW == 0 || (!params->integrate_pixel && !ctx->colorValid)) {
+        set_error("svdag_trace_path_tracing needs a primary trace (and, without pixel integration, its colour pass)");
+        return SVDAG_E_INVALID;
+    }
+    const uint32_t n = ctx->traceW * ctx->traceH;
+    if (ctx->accumCap < n) {
+        if (ctx->dAccum)
+            (void)hipFree(ctx->dAccum);
+        ctx->dAccum = nullptr;
+        ctx->accumCap = 0;
+        SVDAG_HIP(hipMalloc(&ctx->dAccum, (size_t)n * 3 * sizeof(float)));
+        ctx->accumCap = n;
+        ctx->accumValid = false;
+    }
+    // A new accumulation when asked for, or when the image, the traced root
+    // or the parameters changed (the reference clears on any change of its
+    // parameters or root, dag_tracer.cu:286-296).
+    const bool same = params->accumulate && ctx->accumValid && ctx->accumW == ctx->traceW
+        && ctx->accumH == ctx->traceH && ctx->accumRoot == ctx->traceRoot
+        && std::memcmp(ctx->accumCam, ctx->traceCam, sizeof(ctx->traceCam)) == 0
+        && ctx->accumEnv == params->environment_brightness && ctx->accumDepth == params->max_path_depth
+        && ctx->accumIntegrate == (params->integrate_pixel ? 1u : 0u);
+    if (!same) {
+        SVDAG_HIP(hipMemsetAsync(ctx->dAccum, 0, (size_t)n * 3 * sizeof(float), ctx->stream));
+        ctx->numAccumulated = 0;
+    }
+    float* dPal = nullptr;
+    SVDAG_HIP(hipMalloc(&dPal, std::max<size_t>(num_materials, 1) * 3 * sizeof(float)));
+    uint32_t* dOut = nullptr;
+    if (hipMalloc(&dOut, (size_t)n * 4) != hipSuccess) {
+        (void)hipFree(dPal);
+        set_error("svdag_trace_path_tracing: out of device memory");
+        return SVDAG_E_DEVICE;
+    }
+    struct Free {
+        void* a;
+        void* b;
+        ~Free()
+        {
+            (void)hipFree(a);
+            (void)hipFree(b);
+        }
+    } fr { dPal, dOut };
+    if (num_materials)
+        SVDAG_HIP(hipMemcpyAsync(dPal, material_rgb, (size_t)num_materials * 3 * sizeof(float), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    PathDev p {};
+    for (int k = 0; k < 3; ++k) {
+        p.cam.cam[k] = ctx->traceCam[k];
+        p.cam.rayMin[k] = ctx->traceCam[3 + k];
+        p.cam.ddx[k] = ctx->traceCam[6 + k];
+        p.cam.ddy[k] = ctx->traceCam[9 + k];
+    }
+    p.cam.width = ctx->traceW;
+    p.cam.height = ctx->traceH;
+    const float3 sd = sun_direction();
+    p.sunDir[0] = sd.x;
+    p.sunDir[1] = sd.y;
+    p.sunDir[2] = sd.z;
+    p.env = params->environment_brightness;
+    p.numSamples = params->num_samples;
+    p.maxDepth = params->max_path_depth;
+    p.integrate = params->integrate_pixel ? 1u : 0u;
+    p.numAccumulated = ctx->numAccumulated;
+    // Past 256 accumulated paths the reference traces no more samples (the
+    // image only re-averages, dag_tracer.cu:297-302).
+    if (ctx->numAccumulated > 256)
+        p.numSamples = 0;
+    p.numColors = num_materials;
+    p.seed = params->seed;
+    SVDAG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    path_kernel<<<(n + kTraceBlock - 1) / kTraceBlock, kTraceBlock, 0, ctx->stream>>>(
+        ctx->dag, ctx->traceRoot, p, (const svdag_hit*)ctx->dHits, ctx->dSurf, ctx->dColor, dPal, ctx->dAccum, dOut);
+    SVDAG_HIP(hipGetLastError());
+    SVDAG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    SVDAG_HIP(hipMemcpyAsync(out, dOut, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->numAccumulated += p.numSamples;
+    ctx->accumValid = true;
+    ctx->accumW = ctx->traceW;
+    ctx->accumH = ctx->traceH;
+    ctx->accumRoot = ctx->traceRoot;
+    std::memcpy(ctx->accumCam, ctx->traceCam, sizeof(ctx->traceCam));
+    ctx->accumEnv = params->environment_brightness;
+    ctx->accumDepth = params->max_path_depth;
+    ctx->accumIntegrate = p.integrate;
+    if (accumulated)
+        *accumulated = ctx->numAccumulated;
+    if (ms)
+        SVDAG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return SVDAG_OK;
 }

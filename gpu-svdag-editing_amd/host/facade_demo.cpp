@@ -6,6 +6,7 @@
 #include "my_gpu_hash_dag.hpp"
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 int main(int argc, char** argv)
 {
@@ -63,12 +64,24 @@ int main(int argc, char** argv)
         // Shadows and ambient occlusion of the traced surface, as Engine
         // resolves them after the paths (engine.cpp:743-750).
         tracer.resolve_ambient_occlusion(dag, 3, 1.0f, 32.0f);
-        tracer.resolve_shadows(dag, 1.0f);
-        uint32_t lit = 0, open = 0;
-        for (float v : tracer.sunLight())
-            lit += v > 0.0f;
+        uint32_t open = 0;
         for (float v : tracer.ambientOcclusion())
             open += v > 0.0f;
+        tracer.resolve_ambient_occlusion_blur(dag);
+        tracer.resolve_shadows(dag, 1.0f);
+        std::vector<float> palette;
+        for (int m = 0; m < 16; ++m)
+            palette.insert(palette.end(), { (float)m / 15.0f, 0.5f, 1.0f - (float)m / 15.0f });
+        tracer.resolve_colors(dag, palette);
+        tracer.resolve_lighting(dag, true, true, 0.0f);
+        for (uint32_t c : tracer.colors())
+            if ((c >> 24) != 0xFFu) {
+                std::fprintf(stderr, "lighting wrote a pixel without alpha\n");
+                return 6;
+            }
+        uint32_t lit = 0;
+        for (float v : tracer.sunLight())
+            lit += v > 0.0f;
         // The reference's file format: MyGPUHashDAG::writeTo / readFrom, and the
         // Engine scene file (argv[2]) with DAGInfo and the view.
         const uint64_t hash = dag.canonical_hash();

@@ -357,13 +357,63 @@ public:
                                             &ms));
         return ms;
     }
+    // DAGTracer::resolve_ambient_occlusion_blur (dag_tracer.cu:189-196): the
+    // reference's 9-tap Gaussian over ambientOcclusion(), in place.
+    float resolve_ambient_occlusion_blur(const MyGPUHashDAG& dag)
+    {
+        ao_.resize(hits_.size());
+        float ms = 0.0f;
+        check(svdag_blur_ambient_occlusion(dag.context(), ao_.data(), &ms));
+        return ms;
+    }
+    // DAGTracer::resolve_colors (dag_tracer.cu:137-161) with one colour per
+    // material (rgb[3 m .. 3 m + 2]) in place of the voxel textures.
+    float resolve_colors(const MyGPUHashDAG& dag, const std::vector<float>& materialRgb)
+    {
+        colors_.resize(hits_.size());
+        float ms = 0.0f;
+        check(svdag_trace_colors(dag.context(), materialRgb.data(), (uint32_t)(materialRgb.size() / 3), colors_.data(),
+                                 &ms));
+        return ms;
+    }
+    // DAGTracer::resolve_lighting (dag_tracer.cu:223-254): the final RGBA8
+    // image in colors(); the reference's call leaves sunBrightness at 0.
+    float resolve_lighting(const MyGPUHashDAG& dag, bool applyAmbientOcclusion, bool applyShadows, float fogDensity,
+                           float sunBrightness = 0.0f)
+    {
+        final_.resize(hits_.size());
+        const svdag_lighting p { applyAmbientOcclusion ? 1u : 0u, applyShadows ? 1u : 0u, fogDensity, sunBrightness };
+        float ms = 0.0f;
+        check(svdag_trace_lighting(dag.context(), &p, final_.data(), &ms));
+        return ms;
+    }
+    // DAGTracer::resolve_path_tracing (dag_tracer.cu:256-313): the final RGBA8
+    // image in colors(); returns the samples accumulated so far (the
+    // reference's return value).  The random seed advances per call from
+    // 123, as the reference's shared occlusion / path seed does.
+    uint32_t resolve_path_tracing(const MyGPUHashDAG& dag, const std::vector<float>& materialRgb,
+                                  float environmentBrightness, uint32_t numSamples, uint32_t maxPathDepth,
+                                  bool integratePixel)
+    {
+        final_.resize(hits_.size());
+        const svdag_path_tracing p { environmentBrightness, numSamples, maxPathDepth, integratePixel ? 1u : 0u,
+                                     aoSeed_++, 1u, 0u };
+        uint32_t accumulated = 0;
+        float ms = 0.0f;
+        check(svdag_trace_path_tracing(dag.context(), &p, materialRgb.data(), (uint32_t)(materialRgb.size() / 3),
+                                       final_.data(), &accumulated, &ms));
+        return accumulated;
+    }
     const std::vector<float>& sunLight() const { return sun_; }
     const std::vector<float>& ambientOcclusion() const { return ao_; }
+    const std::vector<uint32_t>& diffuseColors() const { return colors_; }
+    const std::vector<uint32_t>& colors() const { return final_; }
 
 private:
     uint32_t width_, height_;
     std::vector<svdag_hit> hits_;
     std::vector<float> sun_, ao_;
+    std::vector<uint32_t> colors_, final_;
     uint64_t aoSeed_ = 123; // the reference starts its occlusion seed at 123 (dag_tracer.h:66)
 };
 

@@ -23,6 +23,17 @@ QUERY_NOALLOC_MAX = 1 << 20  # points per svdag_query_voxels_noalloc call
 u32p = ctypes.POINTER(ctypes.c_uint32)
 
 
+class Lighting(ctypes.Structure):
+    _fields_ = [("apply_ambient_occlusion", ctypes.c_uint32), ("apply_shadows", ctypes.c_uint32),
+                ("fog_density", ctypes.c_float), ("sun_brightness", ctypes.c_float)]
+
+
+class PathTracing(ctypes.Structure):
+    _fields_ = [("environment_brightness", ctypes.c_float), ("num_samples", ctypes.c_uint32),
+                ("max_path_depth", ctypes.c_uint32), ("integrate_pixel", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("accumulate", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 class SvdagError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"svdag error {code} ({ERRORS.get(code, '?')}): {msg}")
@@ -120,7 +131,8 @@ EXPORTED = [
     "svdag_export_tables", "svdag_import_canonical", "svdag_save", "svdag_load", "svdag_hbm_probe",
     "svdag_trace_shadows", "svdag_trace_ambient_occlusion", "svdag_build_voxels",
     "svdag_read_reference_dag", "svdag_write_reference_dag", "svdag_read_reference_scene",
-    "svdag_write_reference_scene",
+    "svdag_write_reference_scene", "svdag_set_load_factor", "svdag_autotune_load_factor",
+    "svdag_blur_ambient_occlusion", "svdag_trace_colors", "svdag_trace_lighting", "svdag_trace_path_tracing",
 ]
 
 _lib = None
@@ -369,6 +381,22 @@ class MyGPUHashDAG:
                 "ms_by_max_blocks": {b: round(m, 3) for b, m in zip((1024, 2048, 4096, 8192, 16384),
                                                                       list(r.ms_blocks))}}
 
+    def set_load_factor(self, load_factor):
+        """MyGPUHashDAG::setLoadFactor: rebuild the tables so the DAG's items fill
+        load_factor percent of the head lines (handles change; history cleared)."""
+        check(lib().svdag_set_load_factor(self.h, ctypes.c_uint32(int(load_factor))))
+
+    def autotune_load_factor(self, sample, candidates=(50, 75, 95)):
+        """Rebuild at each candidate load factor, time the sample batch, keep
+        the fastest; returns {"load_factor": chosen, "ms_by_load_factor": {...}}."""
+        c = (ctypes.c_uint32 * len(candidates))(*[int(v) for v in candidates])
+        ms = (ctypes.c_float * len(candidates))()
+        chosen = ctypes.c_uint32(0)
+        check(lib().svdag_autotune_load_factor(self.h, self._descs(sample), len(sample), c, len(candidates),
+                                               ctypes.byref(chosen), ms))
+        return {"load_factor": chosen.value,
+                "ms_by_load_factor": {int(v): round(m, 3) for v, m in zip(candidates, list(ms))}}
+
     def set_batch_mode(self, mode):
         """0: edits of a batch one after another; 1: fused waves (default);
         2: fused waves replayed from captured HIP graphs."""
@@ -464,6 +492,44 @@ class MyGPUHashDAG:
                                                   ctypes.c_float(ao_ray_length), ctypes.c_uint64(seed),
                                                   out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ms)))
         return out, ms.value
+
+    def blur_ambient_occlusion(self):
+        """The reference's 9-tap Gaussian over the last occlusion result, in place; returns it."""
+        out = np.empty(self._last_pixels(), np.float32)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_blur_ambient_occlusion(self.h, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ms)))
+        return out, ms.value
+
+    def trace_colors(self, material_rgb):
+        """Diffuse RGBA8 per pixel of the last trace; material_rgb: (num_materials, 3) floats."""
+        pal = np.ascontiguousarray(material_rgb, np.float32).reshape(-1, 3)
+        out = np.empty(self._last_pixels(), np.uint32)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_trace_colors(self.h, pal.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(len(pal)),
+                                       out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ms)))
+        return out, ms.value
+
+    def trace_lighting(self, apply_ambient_occlusion=True, apply_shadows=True, fog_density=0.0, sun_brightness=0.0):
+        """Final RGBA8 per pixel of the last trace (Tracer::trace_lighting)."""
+        p = Lighting(int(bool(apply_ambient_occlusion)), int(bool(apply_shadows)), float(fog_density),
+                     float(sun_brightness))
+        out = np.empty(self._last_pixels(), np.uint32)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_trace_lighting(self.h, ctypes.byref(p), out.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.byref(ms)))
+        return out, ms.value
+
+    def trace_path_tracing(self, material_rgb, env=1.0, samples=1, depth=2, integrate=False, seed=1, accumulate=True):
+        """Tracer::trace_path_tracing over the last trace; returns (RGBA8 image, samples accumulated, ms)."""
+        pal = np.ascontiguousarray(material_rgb, np.float32).reshape(-1, 3)
+        prm = PathTracing(float(env), int(samples), int(depth), int(bool(integrate)), int(seed), int(bool(accumulate)), 0)
+        out = np.empty(self._last_pixels(), np.uint32)
+        acc = ctypes.c_uint32(0)
+        ms = ctypes.c_float(0)
+        check(lib().svdag_trace_path_tracing(self.h, ctypes.byref(prm), pal.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.c_uint32(len(pal)), out.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.byref(acc), ctypes.byref(ms)))
+        return out, acc.value, ms.value
 
     def canonical(self, dump=False):
         hsh = ctypes.c_uint64()

@@ -262,6 +262,26 @@ typedef struct {
 } svdag_tune_report;
 int svdag_autotune_all(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, svdag_tune_report* out);
 
+/* MyGPUHashDAG::setLoadFactor (my_gpu_hash_dag.cu:139-155): rebuilds the item
+ * tables with bucket counts sized for the DAG's own items and copies the
+ * reachable DAG into them.  load_factor (5..100) is the percentage of
+ * head-line slots the items fill, as svdag_config.load_factor (the reference
+ * counts items per 32-slot bucket instead, TARGET_LOAD_FACTOR = 96).  The
+ * canonical DAG is unchanged; handles are reissued, so the undo / redo
+ * history is cleared and the next collection marks.  On failure the old
+ * tables and DAG stay.  SVDAG_E_INVALID while a server runs. */
+int svdag_set_load_factor(svdag_ctx* ctx, uint32_t load_factor);
+
+/* The autotuner's bucket-count knob (no reference counterpart; the reference
+ * sets TARGET_LOAD_FACTOR at compile time, gpu_hash_dag_definitions.h:4, and
+ * its hash-table sweeps pick it offline): for each candidate load factor
+ * (at most 8) the tables are rebuilt as svdag_set_load_factor does and the
+ * sample is timed as svdag_autotune times one knob value; the fastest is
+ * left set.  ms (optional, num_candidates entries) receives the best device
+ * ms of each. */
+int svdag_autotune_load_factor(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, const uint32_t* candidates,
+                               uint32_t num_candidates, uint32_t* chosen, float* ms);
+
 /* reference: MyGPUHashDAGUndoRedo::undo / redo (my_gpu_base_dag.h:35-64). */
 int svdag_undo(svdag_ctx* ctx);
 int svdag_redo(svdag_ctx* ctx);
@@ -353,6 +373,74 @@ int svdag_trace_primary(svdag_ctx* ctx, const svdag_camera* camera, uint32_t wid
 int svdag_trace_shadows(svdag_ctx* ctx, float shadow_bias, float* out_sun, float* gpu_ms);
 int svdag_trace_ambient_occlusion(svdag_ctx* ctx, uint32_t num_samples, float shadow_bias, float ao_ray_length,
                                   uint64_t seed, float* out_ao, float* gpu_ms);
+
+/* DAGTracer::resolve_ambient_occlusion_blur (dag_tracer.cu:189-196) ->
+ * BlurKernel::apply (blur.cu:68-85) with the tracer's 9-tap kernel
+ * (half size 4, dag_tracer.cu:19): a horizontal then a vertical pass over the
+ * last ambient-occlusion result, in place; taps outside the image are
+ * skipped.  width and height must be multiples of 8.  out (optional)
+ * receives the blurred buffer in the hit buffer's layout. */
+int svdag_blur_ambient_occlusion(svdag_ctx* ctx, float* out, float* gpu_ms);
+
+/* DAGTracer::resolve_colors -> Tracer::trace_colors -> computeColorAtSurface
+ * (dag_tracer.cu:137-161; tracer.cu:413-596) for a DAG with materials.  The
+ * reference samples each material's top / side / all textures; texture
+ * images are not part of this library, so each material is one colour:
+ * material_rgb[3 m .. 3 m + 2] (linear 0..1 floats, as the texture fetch
+ * returns), for m < num_materials.  Per pixel of the last primary trace: no
+ * surface -> 0; a material without a colour -> the reference's invalid-colour
+ * checker (1, b, 1 - b), b = (x ^ y ^ z) & 1 of the hit voxel; else the
+ * material's colour.  Stored (and copied to out, if given) as
+ * ColorUtils::float3_to_rgb888: R | G << 8 | B << 16 | 0xFF << 24. */
+int svdag_trace_colors(svdag_ctx* ctx, const float* material_rgb, uint32_t num_materials, uint32_t* out_rgba8,
+                       float* gpu_ms);
+
+/* DAGTracer::resolve_lighting -> Tracer::trace_lighting (dag_tracer.cu:223-254;
+ * tracer.cu:693-750): per pixel of the last primary trace, the sky colour
+ * where nothing was hit, else (AO * 0.5 * sky + sun_visibility * sun colour *
+ * sun_brightness) * Kd with Kd the colour pass's sRGB colour made linear,
+ * optionally fogged (fog_density > 0), then made sRGB and stored as RGBA8.
+ * AO is the last (blurred, if svdag_blur_ambient_occlusion ran) occlusion
+ * result and sun_visibility the last svdag_trace_shadows result of the same
+ * trace; each counts as 1 when not applied.  resolve_lighting leaves the
+ * reference's sunBrightness unset, so its frames have sun_brightness 0; pass
+ * 0 for the reference's image.  The tool overlay is not drawn. */
+typedef struct {
+    uint32_t apply_ambient_occlusion;
+    uint32_t apply_shadows;
+    float fog_density;       /* the reference's default is 0 (engine.h:214) */
+    float sun_brightness;
+} svdag_lighting;
+int svdag_trace_lighting(svdag_ctx* ctx, const svdag_lighting* params, uint32_t* out_rgba8, float* gpu_ms);
+
+/* DAGTracer::resolve_path_tracing -> Tracer::trace_path_tracing
+ * (dag_tracer.cu:256-313; tracer.cu:758-842) over the camera and root of the
+ * last primary trace: num_samples paths of at most max_path_depth bounces
+ * per pixel, next-event estimation toward the sun, uniform hemisphere
+ * continuation, sky light times environment_brightness; colours as
+ * svdag_trace_colors (material_rgb, one colour per material).  Without
+ * integrate_pixel each path starts on the stored primary surface with the
+ * colour pass's colour (svdag_trace_colors must have run); with it the
+ * camera ray is jittered in the pixel.  Sums accumulate across calls while
+ * accumulate is set and the image, root, camera, depth, brightness and
+ * integration are unchanged (else they restart, as the reference clears its
+ * buffer); *accumulated (optional) receives the samples summed so far.
+ * Once more than 256 are summed no new samples are traced (as there).
+ * Output: min(sum / samples, 1) made sRGB, RGBA8 per stored pixel.  The
+ * samples of stored pixel i come from PCG32 stream i * max(seed, 1); sin and
+ * cos of the sample angle use this library's own reduction (as the
+ * occlusion pass). */
+typedef struct {
+    float environment_brightness;
+    uint32_t num_samples;
+    uint32_t max_path_depth;
+    uint32_t integrate_pixel;
+    uint64_t seed;
+    uint32_t accumulate;
+    uint32_t reserved;
+} svdag_path_tracing;
+int svdag_trace_path_tracing(svdag_ctx* ctx, const svdag_path_tracing* params, const float* material_rgb,
+                             uint32_t num_materials, uint32_t* out_rgba8, uint32_t* accumulated, float* gpu_ms);
 
 /* Slab-arena usage over all tables of the DAG (no reference counterpart;
  * the reference reports memory_used_by_slabs, gpu_hash_table_base.cu:187-206).

@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <numbers>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -964,24 +965,11 @@ struct Hit {
     float t, px, py, pz;
 };
 
-Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const double* ddx, const double* ddy, uint32_t px, uint32_t py,
-                uint32_t* surfCode = nullptr)
+// intersect_ray + createSurfaceInteraction (tracer.cu:186-273, 353-389) for
+// any ray: the first set voxel along it, its material, the entry-face t and
+// position, and (optionally) the surface code (entry axis and normal sign).
+Hit trace_ray(const Dag& d, const Ray& r, uint32_t* surfCode = nullptr)
 {
-    double a[3];
-    // rayMin + px*ddx + py*ddy - cam (tracer.cu:404); each add has one product
-    // operand, which nvcc's --fmad=true fuses.
-    for (int k = 0; k < 3; ++k)
-        a[k] = std::fma((double)py, ddy[k], std::fma((double)px, ddx[k], rayMin[k])) - cam[k];
-    const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    const double inv = 1 / len;
-    Ray r;
-    for (int k = 0; k < 3; ++k) {
-        r.o[k] = (float)cam[k];
-        r.d[k] = (float)(inv * a[k]);
-        r.inv[k] = 1.0f / r.d[k];
-    }
-    r.tmin = 0.0f;
-    r.tmax = FLT_MAX;
     uint32_t path[3] = { 0, 0, 0 };
     uint32_t material = SENT;
     if (d.root != SENT) {
@@ -1058,6 +1046,27 @@ Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const dou
     // origin + t * direction (tracer.cu:388), fused.
     return { path[0], path[1], path[2], material, t, std::fma(t, r.d[0], r.o[0]), std::fma(t, r.d[1], r.o[1]),
              std::fma(t, r.d[2], r.o[2]) };
+}
+
+Hit trace_pixel(const Dag& d, const double* cam, const double* rayMin, const double* ddx, const double* ddy, uint32_t px, uint32_t py,
+                uint32_t* surfCode = nullptr)
+{
+    double a[3];
+    // rayMin + px*ddx + py*ddy - cam (tracer.cu:404); each add has one product
+    // operand, which nvcc's --fmad=true fuses.
+    for (int k = 0; k < 3; ++k)
+        a[k] = std::fma((double)py, ddy[k], std::fma((double)px, ddx[k], rayMin[k])) - cam[k];
+    const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    const double inv = 1 / len;
+    Ray r;
+    for (int k = 0; k < 3; ++k) {
+        r.o[k] = (float)cam[k];
+        r.d[k] = (float)(inv * a[k]);
+        r.inv[k] = 1.0f / r.d[k];
+    }
+    r.tmin = 0.0f;
+    r.tmax = FLT_MAX;
+    return trace_ray(d, r, surfCode);
 }
 
 // intersect_ray_node_out_of_order (tracer.cu:274-351): any set voxel along
@@ -1415,6 +1424,144 @@ void ora_dag_shade(void* dp, const double* position, const double* rotation, dou
             outSun[i] = occluded_ray(d, make_ray(o, L, FLT_MAX)) ? 0.0f : ndl;
             outAo[i] = aoSamples ? ao_at(d, p, code, aoSamples, bias, aoLength, (uint64_t)i * (seed > 1 ? seed : 1))
                                  : 0.0f;
+        }
+}
+
+// Tracer::trace_path_tracing (tracer.cu:758-842) for every stored pixel i =
+// y * W + x, with the same camera as the primary trace.  Colours are one per
+// material (palette, 3 floats each; a material without one gets the checker
+// (1, b, 1 - b)), made linear with accurateSRGBToLinear.  At depth 0 without
+// pixel integration the surface is the stored primary surface of pixel i and
+// its colour the colour pass's RGBA8 (colors); otherwise the camera ray of
+// sample (x, y) (+ a jitter from the pixel's PCG32 stream when integrating)
+// through row H - 1 - y.  Each bounce: sky -> Lo += c * sky * env, stop; else
+// a shadow ray from position + 0.1 * normal toward the sun adds
+// c * brdf(L) * sunColor * env when it reaches nothing, and the path goes on
+// along a uniform hemisphere sample (z = u.x, phi = 2 pi u.y, pdf 1 / (2 pi)):
+// c *= brdf(dir) / pdf, brdf(d) = max(0, N.d) / pi * Kd.  accum (3 floats per
+// pixel) holds the sums of earlier calls and receives the new ones; the
+// image is min(accum / (samples + accumulated), 1) made sRGB, as RGBA8.
+// Products whose sum nvcc fuses (one product per add) are fused here.
+extern "C" float ora_srgb_to_linear(float c);
+extern "C" float ora_linear_to_srgb(float c);
+extern "C" uint32_t ora_rgb888(const float* c);
+
+void ora_dag_path_trace(void* dp, const double* position, const double* rotation, double fovDeg, const double* bmin,
+                        const double* bmax, uint32_t W, uint32_t H, const uint32_t* colors, const float* palette,
+                        uint32_t numColors, float env, uint32_t numSamples, uint32_t maxDepth, uint32_t integrate,
+                        uint64_t seed, float* accum, uint32_t numAccumulated, uint32_t* out)
+{
+    const Dag& d = *static_cast<Dag*>(dp);
+    const double right[3] = { -rotation[0], -rotation[1], -rotation[2] };
+    const double up[3] = { rotation[3], rotation[4], rotation[5] };
+    const double fwd[3] = { rotation[6], rotation[7], rotation[8] };
+    const double fov = fovDeg / 2.0 * (double(M_PI) / 180.);
+    const double aspect = double(W) / double(H);
+    double cam[3], rayMin[3], ddx[3], ddy[3];
+    for (int k = 0; k < 3; ++k) {
+        const double X = right[k] * std::sin(fov) * aspect, Y = up[k] * std::sin(fov), Z = fwd[k] * std::cos(fov);
+        const double bl = position[k] + Z - Y - X, br = position[k] + Z - Y + X, tl = position[k] + Z + Y - X;
+        const double tr = -bmin[k], sc = double(1 << d.levels) / (bmax[k] - bmin[k]);
+        cam[k] = (position[k] + tr) * sc;
+        rayMin[k] = (bl + tr) * sc;
+        ddx[k] = ((br + tr) * sc - rayMin[k]) * (1.0 / W);
+        ddy[k] = ((tl + tr) * sc - rayMin[k]) * (1.0 / H);
+    }
+    float L[3];
+    sun_dir(L);
+    const float sky[3] = { 187.0f / 255.0f, 242.0f / 255.0f, 250.0f / 255.0f };
+    const float sunColor[3] = { 0.95f, 0.94f, 0.47f };
+    const float invPi = std::numbers::inv_pi_v<float>;
+    const float pdf = 0.5f * invPi;
+    const float eps = 0.1f;
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            Pcg32 g = pcg_seeded((uint64_t)(uint32_t)i * (seed > 1 ? seed : 1));
+            float Lo[3] = { accum[3 * i], accum[3 * i + 1], accum[3 * i + 2] };
+            for (uint32_t s = 0; s < numSamples; ++s) {
+                float sx = (float)x, sy = (float)y;
+                if (integrate) {
+                    sx += pcg_float(g);
+                    sy += pcg_float(g);
+                }
+                const double row = (double)((float)(H - 1) - sy);
+                double a[3];
+                for (int k = 0; k < 3; ++k)
+                    a[k] = std::fma(row, ddy[k], std::fma((double)sx, ddx[k], rayMin[k])) - cam[k];
+                const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+                const double inv = 1 / len;
+                float o[3], dir[3];
+                for (int k = 0; k < 3; ++k) {
+                    o[k] = (float)cam[k];
+                    dir[k] = (float)(inv * a[k]);
+                }
+                float c[3] = { 1.0f, 1.0f, 1.0f };
+                for (uint32_t depth = 0; depth < maxDepth; ++depth) {
+                    Hit h;
+                    uint32_t code = 0;
+                    float kd[3];
+                    if (depth == 0 && !integrate) {
+                        // The stored primary surface of pixel i (camera row H - 1 - y).
+                        h = trace_pixel(d, cam, rayMin, ddx, ddy, x, H - 1 - y, &code);
+                        for (int k = 0; k < 3; ++k)
+                            kd[k] = ora_srgb_to_linear((float)((colors[i] >> (8 * k)) & 0xFFu) / 255.0f);
+                    } else {
+                        h = trace_ray(d, make_ray(o, dir, FLT_MAX), &code);
+                        if (code) {
+                            float col[3];
+                            if (h.material >= numColors) {
+                                const float b = (float)((h.x ^ h.y ^ h.z) & 1u);
+                                col[0] = 1.0f;
+                                col[1] = b;
+                                col[2] = 1.0f - b;
+                            } else {
+                                for (int k = 0; k < 3; ++k)
+                                    col[k] = palette[3 * h.material + k];
+                            }
+                            for (int k = 0; k < 3; ++k)
+                                kd[k] = ora_srgb_to_linear(col[k]);
+                        }
+                    }
+                    if (!code) {
+                        for (int k = 0; k < 3; ++k)
+                            Lo[k] = std::fma(c[k] * sky[k], env, Lo[k]);
+                        break;
+                    }
+                    float n[3], du[3], dv[3];
+                    surface_frame(code, n, du, dv);
+                    const float p[3] = { h.px, h.py, h.pz };
+                    float so[3];
+                    for (int k = 0; k < 3; ++k)
+                        so[k] = std::fma(eps, n[k], p[k]);
+                    if (!occluded_ray(d, make_ray(so, L, FLT_MAX))) {
+                        float ndl = (n[0] * L[0] + n[1] * L[1]) + n[2] * L[2];
+                        ndl = 0.0f > ndl ? 0.0f : ndl;
+                        for (int k = 0; k < 3; ++k)
+                            Lo[k] = std::fma((c[k] * ((ndl * invPi) * kd[k])) * sunColor[k], env, Lo[k]);
+                    }
+                    const float ux = pcg_float(g), uy = pcg_float(g);
+                    const float rr = std::sqrt(std::max(0.0f, std::fma(-ux, ux, 1.0f)));
+                    float sn, cs;
+                    sincos_turns(uy, sn, cs);
+                    const float local[3] = { rr * cs, rr * sn, ux };
+                    for (int k = 0; k < 3; ++k) {
+                        dir[k] = (local[0] * du[k] + local[1] * dv[k]) + local[2] * n[k];
+                        o[k] = so[k];
+                    }
+                    float ndd = (n[0] * dir[0] + n[1] * dir[1]) + n[2] * dir[2];
+                    ndd = 0.0f > ndd ? 0.0f : ndd;
+                    for (int k = 0; k < 3; ++k)
+                        c[k] = c[k] * (((ndd * invPi) * kd[k]) / pdf);
+                }
+            }
+            float v[3];
+            for (int k = 0; k < 3; ++k) {
+                accum[3 * i + k] = Lo[k];
+                const float m = Lo[k] / (float)(numSamples + numAccumulated);
+                v[k] = ora_linear_to_srgb(m < 1.0f ? m : 1.0f);
+            }
+            out[i] = ora_rgb888(v);
         }
 }
 

@@ -142,6 +142,20 @@ def _dag_sigs(L):
     L.ora_header_decode.argtypes = [ctypes.c_uint32, ctypes.c_uint32] + [u32p] * 4
     L.ora_sphere_overlap.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_float,
                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.ora_dag_path_trace.argtypes = [vp, f64p, f64p, ctypes.c_double, f64p, f64p, ctypes.c_uint32, ctypes.c_uint32,
+                                     u32p, f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_uint64, f32p, ctypes.c_uint32, u32p]
+    L.ora_srgb_to_linear.restype = ctypes.c_float
+    L.ora_srgb_to_linear.argtypes = [ctypes.c_float]
+    L.ora_linear_to_srgb.restype = ctypes.c_float
+    L.ora_linear_to_srgb.argtypes = [ctypes.c_float]
+    L.ora_rgb888.restype = ctypes.c_uint32
+    L.ora_rgb888.argtypes = [f32p]
+    L.ora_blur_weights.argtypes = [f32p]
+    L.ora_colors.argtypes = [u32p, ctypes.c_uint32, f32p, ctypes.c_uint32, u32p]
+    L.ora_blur.argtypes = [f32p, ctypes.c_uint32, ctypes.c_uint32, f32p]
+    L.ora_lighting.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p, f32p, f32p, f64p, f64p, ctypes.c_double,
+                               f64p, f64p, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, u32p]
 
 
 _dag_ready = False
@@ -235,6 +249,19 @@ class OracleDag:
         dlib().ora_dag_shade(self.h, _d(position), _d(rotation), float(fov), _d(bmin), _d(bmax), w, h, float(bias),
                              ao_samples, float(ao_length), seed, sun.ctypes.data_as(f32p), ao.ctypes.data_as(f32p))
         return sun, ao
+
+    def path_trace(self, position, rotation, fov, bmin, bmax, w, h, colors_rgba8, palette, env=1.0, samples=1,
+                   depth=2, integrate=False, seed=1, accum=None, accumulated=0):
+        """trace_path_tracing per stored pixel; returns (RGBA8 image, accumulation sums)."""
+        f32p = ctypes.POINTER(ctypes.c_float)
+        col = np.ascontiguousarray(colors_rgba8 if colors_rgba8 is not None else np.zeros(w * h), np.uint32)
+        pal = np.ascontiguousarray(palette, np.float32).reshape(-1, 3)
+        acc = np.zeros(3 * w * h, np.float32) if accum is None else np.ascontiguousarray(accum, np.float32).copy()
+        out = np.empty(w * h, np.uint32)
+        dlib().ora_dag_path_trace(self.h, _d(position), _d(rotation), float(fov), _d(bmin), _d(bmax), w, h, _p(col),
+                                  pal.ctypes.data_as(f32p), len(pal), float(env), samples, depth, int(bool(integrate)),
+                                  seed, acc.ctypes.data_as(f32p), accumulated, _p(out))
+        return out, acc
 
     def occluded(self, origin, direction, tmax=3.4e38):
         return bool(dlib().ora_dag_occluded(self.h, _f3(origin), _f3(direction), float(tmax)))
@@ -389,3 +416,60 @@ class CpuHashDag:
         if getattr(self, "h", None):
             hdlib().hd_destroy(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------- colour, blur, lighting (oracle_shading.cpp)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def srgb_to_linear(c):
+    return dlib().ora_srgb_to_linear(float(c))
+
+
+def linear_to_srgb(c):
+    return dlib().ora_linear_to_srgb(float(c))
+
+
+def rgb888(c):
+    a = np.ascontiguousarray(c, np.float32)
+    return dlib().ora_rgb888(a.ctypes.data_as(_f32p))
+
+
+def blur_weights():
+    w = np.empty(9, np.float32)
+    dlib().ora_blur_weights(w.ctypes.data_as(_f32p))
+    return w
+
+
+def _hits_u32(hits):
+    return np.ascontiguousarray(hits).view(np.uint32).reshape(-1)
+
+
+def colors(hits, palette):
+    """RGBA8 diffuse colour per stored pixel (trace_colors with one-colour materials)."""
+    h = _hits_u32(hits)
+    pal = np.ascontiguousarray(palette, np.float32).reshape(-1, 3)
+    out = np.empty(len(h) // 8, np.uint32)
+    dlib().ora_colors(_p(h), len(out), pal.ctypes.data_as(_f32p), len(pal), _p(out))
+    return out
+
+
+def blur(values, w, h):
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.empty(w * h, np.float32)
+    dlib().ora_blur(v.ctypes.data_as(_f32p), w, h, out.ctypes.data_as(_f32p))
+    return out
+
+
+def lighting(hits, w, h, colors_rgba8, ao, sun, position, rotation, fov, bmin, bmax, levels, sun_brightness=0.0,
+             fog_density=0.0):
+    """Final RGBA8 per stored pixel (trace_lighting); ao / sun None = not applied."""
+    hh = _hits_u32(hits)
+    col = np.ascontiguousarray(colors_rgba8, np.uint32)
+    a = None if ao is None else np.ascontiguousarray(ao, np.float32)
+    s = None if sun is None else np.ascontiguousarray(sun, np.float32)
+    out = np.empty(w * h, np.uint32)
+    dlib().ora_lighting(_p(hh), w, h, _p(col), None if a is None else a.ctypes.data_as(_f32p),
+                        None if s is None else s.ctypes.data_as(_f32p), _d(position), _d(rotation), float(fov),
+                        _d(bmin), _d(bmax), levels, float(sun_brightness), float(fog_density), _p(out))
+    return out

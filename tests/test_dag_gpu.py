@@ -626,6 +626,56 @@ def test_autotune_all_picks_both_knobs_and_keeps_the_dag():
 
 
 @pytest.mark.gpu
+def test_set_load_factor_rebuilds_the_tables_and_keeps_the_dag():
+    # MyGPUHashDAG::setLoadFactor: at every load factor the rebuilt tables
+    # hold the same canonical DAG, lower load factors use more head lines,
+    # and edits, undo / redo and collections after a rebuild equal the oracle.
+    a, o = _pair()
+    edits = _mixed_stream(L10, seed=71)
+    for e in edits[:20]:
+        a.edit(*e)
+        o.edit(*e)
+    before = a.canonical()
+    used = {}
+    for lf in (30, 95, 60):
+        a.set_load_factor(lf)
+        assert a.canonical() == before and a.history() == (0, 0)
+        used[lf] = a.memory_usage().slabs_used
+    assert used[30] > used[60] > used[95]
+    a.garbage_collect()
+    assert a.canonical() == before
+    a.edit_batch(edits[20:36])
+    for e in edits[36:44]:
+        a.edit(*e)
+    # undo, undo, redo: the DAG after edits[:43].
+    a.undo()
+    a.undo()
+    a.redo()
+    for e in edits[20:43]:
+        o.edit(*e)
+    _same(a, o)
+    assert a.garbage_collect().items_live > 0
+    _same(a, o)
+    a.close()
+
+
+@pytest.mark.gpu
+def test_autotune_load_factor_picks_a_candidate_and_keeps_the_dag():
+    a, o = _pair()
+    edits = _mixed_stream(L10, seed=73)
+    before = a.canonical()
+    rep = a.autotune_load_factor(edits[:16], (40, 75, 95))
+    assert rep["load_factor"] in (40, 75, 95)
+    assert all(v > 0 for v in rep["ms_by_load_factor"].values())
+    assert a.canonical() == before
+    a.edit_batch(edits[:32])
+    for e in edits[:32]:
+        o.edit(*e)
+    _same(a, o)
+    a.close()
+
+
+@pytest.mark.gpu
 def test_graph_waves_replay_exactly_across_batches():
     # Fused waves replayed from captured HIP graphs: several batches reuse one
     # graph per split level, with eager work (single edits, a collection that
