@@ -396,6 +396,7 @@ def editing_bench(args):
     g.autotune_all(edits[:16])
 
     def serve_row(name, reqs, arr, max_batch):
+        g = ctx[0]
         vox = np.array([workloads.voxel_edit_count(k, c, r, D) for k, m, c, r in reqs], np.float64)
         info, st = g.serve_replay(reqs, arr, max_batch=max_batch, frame=(1920, 1080), cameras=cams, allow_gc=True)
         latency = info["done_s"] - info["submit_s"]
@@ -412,11 +413,39 @@ def editing_bench(args):
         g.garbage_collect()
         return row
 
+    ctx = [g]
     for rate in args.serve_rates:
         reqs, arr = creq[rate]
         serve_row(rate, reqs, arr, 64)
+    # The same saturation requests on the context the rate rows have aged
+    # (a grown DAG, fuller tables: collections and arena growth fall inside
+    # the window), then on a fresh terrain, the state the plain form over the
+    # same requests starts from below.
+    aged = serve_row("saturation after the rate rows", sat_reqs, np.zeros(len(sat_reqs)), 64)
+    g.close()
+    g, _, _ = fresh()
+    g.autotune_all(edits[:16])
+    g.garbage_collect()
+    ctx[0] = g
     sat = serve_row("saturation", sat_reqs, np.zeros(len(sat_reqs)), 64)
     g.close()
+    sweep = []
+    if args.serve_sweep:
+        # Sensitivity of the saturation row to the server's sizing: tables
+        # sized for twice the items, and larger batches.
+        serving_rows = serving
+        serving = []
+        for scale, mb in ((2.0, 64), (2.0, 128), (2.0, 256)):
+            g = svdag_py.MyGPUHashDAG(D, 4, expected_items=int(expected * scale),
+                                      workspace_items=min(1 << 28, int(2 * 4 ** max(D - 10, 0) * 400000)))
+            g.build_terrain(**tp)
+            g.autotune_all(edits[:16])
+            g.garbage_collect()
+            ctx[0] = g
+            row = serve_row(f"saturation, tables x{scale:g}, batches of {mb}", sat_reqs, np.zeros(len(sat_reqs)), mb)
+            g.close()
+            sweep.append(row)
+        serving = serving_rows
     # The plain per-edit form over the saturation row's own requests (on a
     # fresh terrain): voxel edits/s depends on the request mix (a request's
     # voxels grow with r^3, and r is log-uniform), so the serving speedup is
@@ -438,7 +467,10 @@ def editing_bench(args):
             "plain": plain, "batched": batched, "serving": serving, "shading": shading,
             "serving_saturation_speedup_vs_plain": sat["voxel_edits_per_s"] / plain["voxel_edits_per_s"],
             "plain_over_saturation_requests": plain_same,
-            "serving_saturation_speedup_vs_plain_same_requests": sat["voxel_edits_per_s"] / plain_same["voxel_edits_per_s"]}
+            "serving_saturation_speedup_vs_plain_same_requests": sat["voxel_edits_per_s"] / plain_same["voxel_edits_per_s"],
+            "serving_aged_saturation_speedup_vs_plain_same_requests":
+                aged["voxel_edits_per_s"] / plain_same["voxel_edits_per_s"],
+            **({"serving_sweep": sweep} if sweep else {})}
 
 
 def large_scene_bench(args):
@@ -796,6 +828,8 @@ def main():
     ap.add_argument("--large-copies", type=int, default=32)
     ap.add_argument("--large-levels", type=int, default=17)
     ap.add_argument("--large-expected", type=float, default=4e8, help="items the depth-17 tables start sized for")
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="also serve the saturation requests with tables sized x2 and batches of 64 / 128 / 256")
     ap.add_argument("--tune-load-factor", action="store_true",
                     help="editing legs also tune the DAG tables' load factor (50/75/95%%; rebuilds the tables)")
     ap.add_argument("--ref-format-only", action="store_true",

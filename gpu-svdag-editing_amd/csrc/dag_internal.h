@@ -287,6 +287,9 @@ struct svdag_ctx {
     // Collections run so far: between two collections no slot is freed, so
     // a root handle names one DAG content.
     uint64_t gcSerial = 0;
+    // Items per released root in each release round of recent counting
+    // collections, a decaying maximum (sizes the next collection's grids).
+    std::vector<double> gcPerRoot;
     // The reference-format stream a size query built (refformat.hip), kept
     // for the call that fetches it while (root, gcSerial) is unchanged.
     std::vector<uint8_t> refStream;

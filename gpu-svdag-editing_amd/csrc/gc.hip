@@ -25,6 +25,7 @@
 // the slab free stack.
 #include "dag_internal.h"
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 namespace svdag {
@@ -778,9 +779,20 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
         SVDAG_HIP(hipMemcpyAsync(ctx->dGcCand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, st));
     SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
     rc_seed<<<1, 256, 0, st>>>(dag, ctx->dGcCand, (uint32_t)cand.size(), w[0], cap2, ctr);
-    for (uint32_t r = 0; r < rounds; ++r)
-        rc_release<<<256, 256, 0, st>>>(dag, a, rootTable, rootPtr, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
-                                         &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
+    // Each round's grid from recent collections' items per released root in
+    // that round, times this collection's released roots, doubled: the
+    // top rounds hold a few items per root, and a launch of 256 blocks for
+    // them costs more than their work.  The grid-stride loop covers any
+    // count, so the sizing only affects speed.
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t grid = 256;
+        if (r < ctx->gcPerRoot.size() && !cand.empty()) {
+            const double want = 2.0 * ctx->gcPerRoot[r] * (double)cand.size();
+            grid = (uint32_t)std::min(256.0, std::max(1.0, std::ceil(want / 256.0)));
+        }
+        rc_release<<<grid, 256, 0, st>>>(dag, a, rootTable, rootPtr, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
+                                          &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
+    }
     rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, ctr);
     SVDAG_HIP(hipEventRecord(ctx->ev1, st));
     GcCounters h {};
@@ -789,6 +801,13 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
     if (h.overflow || h.count[rounds] != 0u)
         return SVDAG_OK;
     done = true;
+    // A decaying maximum over recent collections: one batch of small edits
+    // must not shrink the grids the next batch's large edit needs.
+    if (!cand.empty()) {
+        ctx->gcPerRoot.resize(rounds, 0.0);
+        for (uint32_t r = 0; r < rounds; ++r)
+            ctx->gcPerRoot[r] = std::max((double)h.count[r] / (double)cand.size(), 0.9 * ctx->gcPerRoot[r]);
+    }
     if (stats) {
         stats->items_freed = h.itemsFreed;
         stats->slabs_freed = h.slabsFreed;
