@@ -235,6 +235,46 @@ def slab_hash_bench(args):
     }
 
 
+def shading_bench(g, cam):
+    """Frame passes at 1920x1080 on the DAG of g, each run three times (the
+    best kept): primary paths, shadows and ambient occlusion (3 samples, rays
+    of 2^(D-10) voxels: the reference's defaults, engine.h:202-213), the
+    occlusion blur, colours (16 one-colour materials), lighting and one
+    path-traced sample of depth 3."""
+    D = g.levels
+    pos, rot, fov, bmin, bmax = cam
+    tr = [g.trace(pos, rot, fov, bmin, bmax, 1920, 1080)[1] for _ in range(3)]
+    sh = [g.trace_shadows(1.0)[1] for _ in range(3)]
+    ao = [g.trace_ambient_occlusion(3, 1.0, float(1 << (D - 10)), 123 + i)[1] for i in range(3)]
+    blur = [g.blur_ambient_occlusion()[1] for _ in range(3)]
+    pal = np.random.default_rng(3).uniform(0.2, 0.9, (16, 3)).astype(np.float32)
+    col = [g.trace_colors(pal)[1] for _ in range(3)]
+    lit = [g.trace_lighting(True, True, 0.0, 0.0)[1] for _ in range(3)]
+    pt = [g.trace_path_tracing(pal, env=1.0, samples=1, depth=3, integrate=False, seed=1 + i,
+                               accumulate=False)[2] for i in range(3)]
+    shading = {"width": 1920, "height": 1080, "primary_ms": round(min(tr), 3), "shadows_ms": round(min(sh), 3),
+               "ambient_occlusion_3spp_ms": round(min(ao), 3), "ao_blur_ms": round(min(blur), 3),
+               "colors_ms": round(min(col), 3), "lighting_ms": round(min(lit), 3),
+               "path_tracing_1spp_depth3_ms": round(min(pt), 3)}
+    log(f"[bench] shading: {shading}")
+    return shading
+
+
+def shading_only(args):
+    """The frame passes on a fresh depth-D terrain (for a kernel trace of them alone)."""
+    import svdag_py
+    import workloads
+
+    D = args.edit_levels
+    expected = int(64e6 * 4 ** (D - 15)) if D >= 12 else 0
+    g = svdag_py.MyGPUHashDAG(D, 4, expected_items=expected)
+    g.build_terrain(**workloads.terrain_params(D))
+    c = workloads.camera_path(D, 64)[0]
+    out = shading_bench(g, (c[0], svdag_py.look_at(c[0], c[1]), c[2], c[3], c[4]))
+    g.close()
+    return out
+
+
 def editing_bench(args):
     """Depth-D editing on the fBm terrain (base 2^(D-1), amplitude 6000 * 2^D / 32768, 8 octaves).
 
@@ -292,15 +332,7 @@ def editing_bench(args):
     # Frame passes on the fresh terrain at 1920x1080: primary paths, then
     # shadows and ambient occlusion (3 samples, rays of 2^(D-10) voxels), the
     # reference's defaults (engine.h:202-213).
-    shading = None
-    if not args.no_shading:
-        pos, rot, fov, bmin, bmax = cams[0]
-        tr = [g.trace(pos, rot, fov, bmin, bmax, 1920, 1080)[1] for _ in range(3)]
-        sh = [g.trace_shadows(1.0)[1] for _ in range(3)]
-        ao = [g.trace_ambient_occlusion(3, 1.0, float(1 << (D - 10)), 123 + i)[1] for i in range(3)]
-        shading = {"width": 1920, "height": 1080, "primary_ms": round(min(tr), 3), "shadows_ms": round(min(sh), 3),
-                   "ambient_occlusion_3spp_ms": round(min(ao), 3)}
-        log(f"[bench] shading: {shading}")
+    shading = None if args.no_shading else shading_bench(g, cams[0])
     g.close()
 
     # Every leg applies the same 256-edit stream to the same terrain, so every
@@ -828,6 +860,8 @@ def main():
     ap.add_argument("--large-copies", type=int, default=32)
     ap.add_argument("--large-levels", type=int, default=17)
     ap.add_argument("--large-expected", type=float, default=4e8, help="items the depth-17 tables start sized for")
+    ap.add_argument("--shading-only", action="store_true",
+                    help="only the 1920x1080 frame passes on a fresh terrain at --edit-levels")
     ap.add_argument("--serve-sweep", action="store_true",
                     help="also serve the saturation requests with tables sized x2 and batches of 64 / 128 / 256")
     ap.add_argument("--tune-load-factor", action="store_true",
@@ -849,6 +883,9 @@ def main():
         return
     if args.large_scene_only:
         print(json.dumps({"large_scene": large_scene_bench(args)}), flush=True)
+        return
+    if args.shading_only:
+        print(json.dumps({"shading": shading_only(args)}), flush=True)
         return
     if args.ref_format_only:
         print(json.dumps({"reference_format": reference_format_bench(args)}), flush=True)

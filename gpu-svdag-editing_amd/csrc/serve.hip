@@ -53,6 +53,8 @@ struct Server {
     svdag_server_stats stats {};
     std::atomic<uint32_t> published { kSentinel };
     std::vector<double> slabsPerItem; // per table (batch thread only)
+    AllocState* hAlloc = nullptr;     // pinned: every table's allocator state (batch thread only)
+    size_t hAllocCap = 0;
     // Frame state (frame thread only, plus frameMu for launches).
     hipEvent_t fStart = nullptr, fEnd = nullptr;
     void* dFrame = nullptr;
@@ -100,15 +102,29 @@ struct TableRoom {
     std::vector<uint64_t> used;
     std::vector<double> room;
 };
-static int headroom(svdag_ctx* ctx, double reserveFraction, TableRoom& r)
+// Every table's allocator state in one pinned buffer: all copies queued,
+// then one synchronisation (a copy and a wait per table cost about as much
+// as a small batch's kernels).
+static int headroom(svdag_ctx* ctx, Server* srv, double reserveFraction, TableRoom& r)
 {
     const size_t nt = ctx->ts.tables.size();
     r.used.assign(nt, 0);
     r.room.assign(nt, 0.0);
+    if (!srv->hAlloc) {
+        SVDAG_HIP(hipHostMalloc((void**)&srv->hAlloc, std::max<size_t>(nt, 1) * sizeof(AllocState)));
+        srv->hAllocCap = nt;
+    }
+    if (srv->hAllocCap < nt) {
+        set_error("serving: more tables than the allocator-state buffer holds");
+        return SVDAG_E_INVALID;
+    }
+    for (size_t i = 0; i < nt; ++i)
+        SVDAG_HIP(hipMemcpyAsync(&srv->hAlloc[i], ctx->ts.tables[i].dev.alloc, sizeof(AllocState),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    SVDAG_HIP(hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < nt; ++i) {
         const HostTable& t = ctx->ts.tables[i];
-        AllocState a {};
-        SVDAG_TRY(table_read_alloc(t, a, ctx->stream));
+        const AllocState a = srv->hAlloc[i];
         const uint64_t fresh = std::min<uint64_t>(a.fresh, t.slabCap);
         const uint64_t freeSlabs = (t.slabCap - fresh) + (uint64_t)std::max(a.freeCount, 0);
         r.used[i] = t.slabCap - freeSlabs;
@@ -206,7 +222,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
             for (size_t i = 0; i < ctx->ts.tables.size(); ++i)
                 srv->slabsPerItem[i] = 1.0 / (double)line_slots(ctx->ts.tables[i].itemWords);
         }
-        (void)headroom(ctx, reserve, room);
+        (void)headroom(ctx, srv, reserve, room);
         const double need = batch.front().estItems;
         if (!fits(srv, room, need) && srv->cfg.allow_gc) {
             ctx->undo.clear();
@@ -215,7 +231,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
             FrameQuiesce q(ctx);
             if (gc_run(ctx, &gs) == SVDAG_OK)
                 srv->stats.gcs++;
-            (void)headroom(ctx, reserve, room);
+            (void)headroom(ctx, srv, reserve, room);
         }
         if (!fits(srv, room, need)) {
             FrameQuiesce q(ctx);
@@ -230,7 +246,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
             }
             if (grew && ctx_refresh_tables(ctx) == SVDAG_OK)
                 srv->stats.grows++;
-            (void)headroom(ctx, reserve, room);
+            (void)headroom(ctx, srv, reserve, room);
         }
         size_t admit = 1; // the head request always runs (the pipeline grows on OOM)
         double sum = need;
@@ -264,7 +280,7 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                 for (uint32_t L = 0; L < 32; ++L)
                     items += s.svo_items[L];
             TableRoom after;
-            if (items > 0 && headroom(ctx, reserve, after) == SVDAG_OK) {
+            if (items > 0 && headroom(ctx, srv, reserve, after) == SVDAG_OK) {
                 // Slabs each table gained per item; a table that gained none
                 // keeps a small floor so a sudden demand still counts.
                 for (size_t i = 0; i < after.used.size() && i < room.used.size(); ++i) {
@@ -300,6 +316,8 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
 
 static void server_release(svdag_ctx* ctx, Server* srv)
 {
+    if (srv->hAlloc)
+        (void)hipHostFree(srv->hAlloc);
     if (srv->fStart)
         (void)hipEventDestroy(srv->fStart);
     if (srv->fEnd)
