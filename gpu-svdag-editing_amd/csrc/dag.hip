@@ -1402,10 +1402,12 @@ int svdag_destroy(svdag_ctx* ctx)
                      (void*)ctx->dStored, (void*)ctx->dGcCand, (void*)ctx->dGcTouchedList })
         if (p)
             (void)hipFree(p);
-    for (void* p : { ctx->hBatch, ctx->hFused })
+    for (void* p : { ctx->hBatch, ctx->hFused, ctx->hGcCtr })
         if (p)
             (void)hipHostFree(p);
     (void)hipHostFree(ctx->hStatus);
+    if (ctx->evGc)
+        (void)hipEventDestroy(ctx->evGc);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     if (ctx->evWait)

@@ -290,6 +290,13 @@ struct svdag_ctx {
     // Items per released root in each release round of recent counting
     // collections, a decaying maximum (sizes the next collection's grids).
     std::vector<double> gcPerRoot;
+    // A counting collection nobody waited for (svdag_gc without stats): its
+    // counters land in pinned hGcCtr when evGc completes, and gc_settle
+    // takes them in before anything reads the collection's results.
+    bool gcPending = false;
+    uint32_t gcPendingRounds = 0, gcPendingCand = 0;
+    void* hGcCtr = nullptr;
+    hipEvent_t evGc = nullptr;
     // The reference-format stream a size query built (refformat.hip), kept
     // for the call that fetches it while (root, gcSerial) is unchanged.
     std::vector<uint8_t> refStream;
@@ -326,7 +333,8 @@ inline void root_log(svdag_ctx* ctx)
         ctx->rootLog.push_back(ctx->root);
 }
 // mark: force the marking collection (else reference counts, when valid).
-int gc_run(svdag_ctx* ctx, svdag_gc_stats* st, bool mark = false);
+int gc_run(svdag_ctx* ctx, svdag_gc_stats* st, bool mark = false, bool wait = true);
+int gc_settle(svdag_ctx* ctx);
 // surfDev (optional) receives one surface code per pixel (see dSurf).
 int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
                  svdag_hit* outDev, uint8_t* surfDev = nullptr);

@@ -387,20 +387,34 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool n)
     return base + (uint32_t)__popcll(m & lanes_below());
 }
 
-__global__ void __launch_bounds__(256) rc_seed(DagDev dag, const uint32_t* cand, uint32_t n, uint2* out,
-                                               uint32_t outCap, GcCounters* ctr)
+// Up to this many released roots travel in the seed's arguments, so a
+// collection after a batch of edits needs no upload.
+constexpr uint32_t kSeedInline = 256;
+struct SeedInline {
+    uint32_t h[kSeedInline];
+};
+
+// Clears the counters, then lists the released roots whose count is zero.
+// cand == nullptr: the roots are inl.h[0 .. n).  One block.
+__global__ void __launch_bounds__(256) rc_seed(DagDev dag, const uint32_t* cand, SeedInline inl, uint32_t n,
+                                               uint2* out, uint32_t outCap, GcCounters* ctr)
 {
+    // Atomic writes, performed where the appends below are.
+    uint32_t* words = reinterpret_cast<uint32_t*>(ctr);
+    for (uint32_t k = threadIdx.x; k < (uint32_t)(sizeof(GcCounters) / 4); k += blockDim.x)
+        (void)atomicExch(&words[k], 0u);
     // A pop past the free stack's end leaves its count below zero; the chain
     // pass pushes onto an empty stack then.
     if (threadIdx.x < dag.numTables && dag.tables[threadIdx.x].alloc->freeCount < 0)
         dag.tables[threadIdx.x].alloc->freeCount = 0;
+    __syncthreads();
     const bool leaf = dag.leafLevel == 0;
     for (uint32_t base = 0; base < n; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         bool dies = false;
         uint2 e = make_uint2(0u, 0u);
         if (i < n) {
-            const uint32_t h = cand[i];
+            const uint32_t h = cand ? cand[i] : inl.h[i];
             const uint32_t hs = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
             e = make_uint2(hs - 2u, leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h));
             dies = dag.tables[e.x].refs[e.y] == 0u;
@@ -415,21 +429,32 @@ __global__ void __launch_bounds__(256) rc_seed(DagDev dag, const uint32_t* cand,
     }
 }
 
+// One round: frees the items of the input list.  Each wave's memory
+// accesses form one chain of round trips, ordered so that nothing waits
+// twice: the list entry; the item's words and role; the children's counts,
+// with the slot's free and its bucket's listing issued alongside; the
+// decrements; the appends of the children that die.  The items freed are
+// not summed here: every listed item is freed, so the chain pass adds up
+// the rounds' list sizes.
 __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32_t rootTable, uint32_t rootPtr,
                                                   const uint2* in, const uint32_t* inCount, uint2* out,
                                                   uint32_t* outCount, uint32_t cap, uint2* touchedList,
                                                   GcCounters* ctr)
 {
     // Once a list overflowed or a count was found off, the marking
-    // collection takes over: release nothing more.  One read per block.
-    __shared__ uint32_t sStop;
-    if (threadIdx.x == 0)
-        sStop = ctr->overflow;
+    // collection takes over: release nothing more.  Both words are read
+    // together, once per block.
+    __shared__ uint32_t sStop, sN;
+    if (threadIdx.x == 0) {
+        const uint32_t stop = ctr->overflow;
+        const uint32_t cnt = *inCount;
+        sStop = stop;
+        sN = cnt;
+    }
     __syncthreads();
     if (sStop)
         return;
-    const uint32_t n = min(*inCount, cap);
-    uint32_t freed = 0;
+    const uint32_t n = min(sN, cap);
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const bool have = i < n;
@@ -437,14 +462,44 @@ __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32
         const TableDev& t = dag.tables[e.x];
         const uint32_t s = e.x + 2u;
         const uint32_t* item = have ? table_item_ptr(t, e.y) : nullptr;
+        const uint32_t slab = e.y >> kSlotBits, slot = e.y & ((1u << kSlotBits) - 1u);
+        const bool chained = have && slab >= t.numBuckets;
+        // The item's words: its children and, in an overflow line, the
+        // bucket hash (as at insert), all loaded before the slot is freed.
         const uint32_t roles = have ? t.roles[e.y] : 0u;
+        uint32_t cw[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            cw[j] = have && j + 1u < s ? item[j + 1u] : 0u;
+        uint32_t bucket = 0;
+        if (chained) {
+            uint32_t h = 0;
+            for (uint32_t w = 0; w < s; ++w)
+                h ^= hash_part(item[w], w);
+            bucket = h % t.numBuckets;
+        }
+        // Free the slot; an overflow line's bucket is listed once for the
+        // chain pass.  The listing's atomic returns with the count loads.
+        uint32_t tbit = 0, told = 0;
+        if (chained) {
+            tbit = 1u << (bucket & 31u);
+            told = atomicOr(&a.touched[e.x][bucket >> 5], tbit);
+        }
+        if (have) {
+            atomicAnd(&t.lines[(size_t)slab * kLineWords + (slot >> 2)], ~(0xFFu << (8u * (slot & 3u))));
+            t.roles[e.y] = 0u;
+        }
+        bool listed = false;
         // Children, all of one role at once (a node has at most 8): their
-        // count loads, then the decrements, then one append per wave.
+        // count loads, then the decrements, then one append per wave.  A
+        // role no lane of the wave holds is skipped.
 #pragma unroll
         for (uint32_t rb = 0; rb < 2; ++rb) {
             const uint8_t role = rb == 0 ? kRoleParentOfNodes : kRoleParentOfLeaves;
             const bool lf = role == kRoleParentOfLeaves;
             const bool mine = have && (roles & role);
+            if (__ballot(mine) == 0ull)
+                continue;
             uint2 ce[8];
             uint32_t* cp[8];
             uint32_t cv[8];
@@ -452,11 +507,23 @@ __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32
             for (uint32_t j = 0; j < 8; ++j) {
                 cv[j] = kStickyRefs;
                 if (mine && j + 1u < s) {
-                    const uint32_t ch = item[j + 1u];
+                    const uint32_t ch = cw[j];
                     const uint32_t cs = lf ? leaf_handle_size(dag, ch) : node_handle_size(ch);
                     ce[j] = make_uint2(cs - 2u, lf ? leaf_handle_ptr(dag, ch) : node_handle_ptr(ch));
                     cp[j] = &dag.tables[ce[j].x].refs[ce[j].y];
                     cv[j] = ld_filter(cp[j]);
+                }
+            }
+            if (!listed) {
+                // The bucket listing rides on this round trip.
+                listed = true;
+                const bool list = chained && (told & tbit) == 0u;
+                const uint32_t at = wave_append(&ctr->touched, list);
+                if (list) {
+                    if (at < cap)
+                        touchedList[at] = make_uint2(e.x, bucket);
+                    else
+                        atomicOr(&ctr->overflow, 1u);
                 }
             }
 #pragma unroll
@@ -502,44 +569,34 @@ __global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32
                 }
             }
         }
-        // Free the item; an overflow line's bucket is listed once for the
-        // chain pass (the bucket hash of the words, as at insert).
-        bool list = false;
-        uint32_t bucket = 0;
-        if (have) {
-            const uint32_t slab = e.y >> kSlotBits, slot = e.y & ((1u << kSlotBits) - 1u);
-            if (slab >= t.numBuckets) {
-                uint32_t h = 0;
-                for (uint32_t q = 0; q < s; ++q)
-                    h ^= hash_part(item[q], q);
-                bucket = h % t.numBuckets;
-                const uint32_t bit = 1u << (bucket & 31u);
-                list = (atomicOr(&a.touched[e.x][bucket >> 5], bit) & bit) == 0u;
+        if (!listed) {
+            // A wave whose items have no counted children (leaves).
+            const bool list = chained && (told & tbit) == 0u;
+            const uint32_t at = wave_append(&ctr->touched, list);
+            if (list) {
+                if (at < cap)
+                    touchedList[at] = make_uint2(e.x, bucket);
+                else
+                    atomicOr(&ctr->overflow, 1u);
             }
-            atomicAnd(&t.lines[(size_t)slab * kLineWords + (slot >> 2)], ~(0xFFu << (8u * (slot & 3u))));
-            t.roles[e.y] = 0u;
-            ++freed;
-        }
-        const uint32_t at = wave_append(&ctr->touched, list);
-        if (list) {
-            if (at < cap)
-                touchedList[at] = make_uint2(e.x, bucket);
-            else
-                atomicOr(&ctr->overflow, 1u);
         }
     }
-    gc_block_add(freed, &ctr->itemsFreed);
 }
 
 // The listed buckets of table blockIdx.y: unlink the overflow lines left
 // without items and push them on the free stack (as gc_sweep_chains).
 __global__ void __launch_bounds__(256) rc_chains(DagDev dag, GcTables a, const uint2* list, uint32_t cap,
-                                                 GcCounters* ctr)
+                                                 uint32_t rounds, GcCounters* ctr)
 {
-    // The release rounds have all run: the stored-item count loses what
-    // they freed (once, by the first block).
+    // The release rounds have all run and each freed its whole input list:
+    // the stored-item count loses the lists' sizes (once, by the first
+    // block).  After an overflow the marking collection recounts.
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        *dag.stored -= ctr->itemsFreed;
+        unsigned long long freed = 0;
+        for (uint32_t r = 0; r < rounds; ++r)
+            freed += min(ctr->count[r], cap);
+        ctr->itemsFreed = freed;
+        *dag.stored -= freed;
         ctr->itemsLive = *dag.stored;
     }
     const uint32_t ti = blockIdx.y;
@@ -740,9 +797,45 @@ static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
 // The reference-count collection.  done = false when it could not finish
 // (a worklist overflowed, or counts were found off); the caller then runs
 // the marking collection, which repairs both.
-static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
+// Items per released root in each round, a decaying maximum over recent
+// collections: one batch of small edits must not shrink the grids the next
+// batch's large edit needs.
+static void gc_take_rounds(svdag_ctx* ctx, const uint32_t* count, uint32_t rounds, size_t cand)
+{
+    if (cand == 0)
+        return;
+    ctx->gcPerRoot.resize(rounds, 0.0);
+    for (uint32_t r = 0; r < rounds; ++r)
+        ctx->gcPerRoot[r] = std::max((double)count[r] / (double)cand, 0.9 * ctx->gcPerRoot[r]);
+}
+
+// Takes in the counters of a collection nobody waited for.  One that
+// stopped short (a list overflowed, or counts were off) left what it did
+// not release stored; the next collection then marks and recounts.
+int gc_settle(svdag_ctx* ctx)
+{
+    if (!ctx->gcPending)
+        return SVDAG_OK;
+    ctx->gcPending = false;
+    SVDAG_HIP(hipEventSynchronize(ctx->evGc));
+    const GcCounters& h = *static_cast<const GcCounters*>(ctx->hGcCtr);
+    if (h.overflow || h.count[ctx->gcPendingRounds] != 0u) {
+        ctx->refsDirty = true;
+        return SVDAG_OK;
+    }
+    gc_take_rounds(ctx, h.count, ctx->gcPendingRounds, ctx->gcPendingCand);
+    if (ctx->liveAtMark == 0)
+        ctx->liveAtMark = h.itemsLive;
+    ctx->liveAfterGc = h.itemsLive;
+    return SVDAG_OK;
+}
+
+// wait = false: the collection is only enqueued (deferred = true); its
+// counters come back with the stream and gc_settle takes them in.
+static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done, bool wait, bool& deferred)
 {
     done = false;
+    deferred = false;
     const hipStream_t st = ctx->stream;
     const DagDev& dag = ctx->dag;
     GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
@@ -775,10 +868,15 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
     uint2* w[2] = { reinterpret_cast<uint2*>(ctx->gcFront[0]), reinterpret_cast<uint2*>(ctx->gcFront[1]) };
     const uint32_t rounds = dag.leafLevel + 2u;
     SVDAG_HIP(hipEventRecord(ctx->ev0, st));
-    if (!cand.empty())
+    SeedInline inl {};
+    const uint32_t* candDev = nullptr;
+    if (cand.size() <= kSeedInline) {
+        std::copy(cand.begin(), cand.end(), inl.h);
+    } else {
         SVDAG_HIP(hipMemcpyAsync(ctx->dGcCand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, st));
-    SVDAG_HIP(hipMemsetAsync(ctr, 0, sizeof(GcCounters), st));
-    rc_seed<<<1, 256, 0, st>>>(dag, ctx->dGcCand, (uint32_t)cand.size(), w[0], cap2, ctr);
+        candDev = ctx->dGcCand;
+    }
+    rc_seed<<<1, 256, 0, st>>>(dag, candDev, inl, (uint32_t)cand.size(), w[0], cap2, ctr);
     // Each round's grid from recent collections' items per released root in
     // that round, times this collection's released roots, doubled: the
     // top rounds hold a few items per root, and a launch of 256 blocks for
@@ -793,21 +891,29 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
         rc_release<<<grid, 256, 0, st>>>(dag, a, rootTable, rootPtr, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
                                           &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
     }
-    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, ctr);
+    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, rounds, ctr);
     SVDAG_HIP(hipEventRecord(ctx->ev1, st));
+    if (!wait) {
+        if (!ctx->hGcCtr)
+            SVDAG_HIP(hipHostMalloc(&ctx->hGcCtr, sizeof(GcCounters)));
+        if (!ctx->evGc)
+            SVDAG_HIP(hipEventCreateWithFlags(&ctx->evGc, hipEventDisableTiming));
+        SVDAG_HIP(hipMemcpyAsync(ctx->hGcCtr, ctr, sizeof(GcCounters), hipMemcpyDeviceToHost, st));
+        SVDAG_HIP(hipEventRecord(ctx->evGc, st));
+        ctx->gcPending = true;
+        ctx->gcPendingRounds = rounds;
+        ctx->gcPendingCand = (uint32_t)cand.size();
+        done = true;
+        deferred = true;
+        return SVDAG_OK;
+    }
     GcCounters h {};
     SVDAG_HIP(hipMemcpyAsync(&h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
     SVDAG_HIP(hipStreamSynchronize(st));
     if (h.overflow || h.count[rounds] != 0u)
         return SVDAG_OK;
     done = true;
-    // A decaying maximum over recent collections: one batch of small edits
-    // must not shrink the grids the next batch's large edit needs.
-    if (!cand.empty()) {
-        ctx->gcPerRoot.resize(rounds, 0.0);
-        for (uint32_t r = 0; r < rounds; ++r)
-            ctx->gcPerRoot[r] = std::max((double)h.count[r] / (double)cand.size(), 0.9 * ctx->gcPerRoot[r]);
-    }
+    gc_take_rounds(ctx, h.count, rounds, cand.size());
     if (stats) {
         stats->items_freed = h.itemsFreed;
         stats->slabs_freed = h.slabsFreed;
@@ -819,11 +925,12 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done)
     return SVDAG_OK;
 }
 
-int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
+int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark, bool wait)
 {
     svdag_gc_stats local {};
     if (!stats)
         stats = &local;
+    SVDAG_TRY(gc_settle(ctx));
     // A count that reached kStickyRefs never drops, so its item outlives
     // its last referrer until a marking collection.  One runs once the
     // stored items exceed 1.5 times the live count the last marking
@@ -841,11 +948,11 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
         return SVDAG_E_INVALID;
     }
     SVDAG_TRY(gc_masks(ctx));
-    bool done = false;
+    bool done = false, deferred = false;
     float releaseMs = 0;
     if (!mark && !ctx->refsDirty) {
         svdag_gc_stats rs {};
-        SVDAG_TRY(gc_release(ctx, &rs, done));
+        SVDAG_TRY(gc_release(ctx, &rs, done, wait, deferred));
         if (done)
             *stats = rs;
         releaseMs = rs.gpu_ms;
@@ -855,9 +962,11 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark)
         stats->gpu_ms += releaseMs;
     }
     stats->marked = done ? 0u : 1u;
-    if (!done || ctx->liveAtMark == 0)
-        ctx->liveAtMark = stats->items_live;
-    ctx->liveAfterGc = stats->items_live;
+    if (!deferred) {
+        if (!done || ctx->liveAtMark == 0)
+            ctx->liveAtMark = stats->items_live;
+        ctx->liveAfterGc = stats->items_live;
+    }
     ctx->refsDirty = false;
     ctx->rootLog.clear();
     if (ctx->root != kSentinel)
@@ -877,9 +986,12 @@ extern "C" int svdag_gc(svdag_ctx* ctx, svdag_gc_stats* stats)
     // MyGPUHashDAGUndoRedo::garbageCollect clears both history stacks first.
     ctx->undo.clear();
     ctx->redo.clear();
+    // Without stats nobody needs the collection's results now: it is only
+    // enqueued on the context's stream, and what follows on that stream
+    // runs after it.
     svdag_gc_stats local {};
     FrameQuiesce q(ctx);
-    return gc_run(ctx, stats ? stats : &local, false);
+    return gc_run(ctx, stats ? stats : &local, false, stats != nullptr);
 }
 
 extern "C" int svdag_gc_mark(svdag_ctx* ctx, svdag_gc_stats* stats)

@@ -413,9 +413,14 @@ class MyGPUHashDAG:
         check(lib().svdag_history(self.h, out))
         return tuple(out)
 
-    def garbage_collect(self, mark=False):
+    def garbage_collect(self, mark=False, wait=True):
         """Free what the root and the fully-filled chain do not reach.
-        mark=True forces the marking collection (svdag_gc_mark)."""
+        mark=True forces the marking collection (svdag_gc_mark).  wait=False
+        only enqueues the collection (svdag_gc without stats) and returns
+        None: later work on the context runs after it."""
+        if not wait and not mark:
+            check(lib().svdag_gc(self.h, None))
+            return None
         st = GcStats()
         check((lib().svdag_gc_mark if mark else lib().svdag_gc)(self.h, ctypes.byref(st)))
         return st

@@ -302,7 +302,12 @@ int svdag_history(svdag_ctx* ctx, uint32_t out[2]);
  * item is freed only by a marking collection; svdag_gc marks instead once
  * the stored items exceed 1.5 times the live count of the last marking
  * collection.  items_live counts the stored items after the collection
- * (all reachable after a marking one); marked = 1 if it marked. */
+ * (all reachable after a marking one); marked = 1 if it marked.
+ * stats == NULL: a counting collection is only enqueued on the context's
+ * stream and the call returns without waiting for it (everything later on
+ * the context runs after it); the next collection takes in its results, and
+ * if it stopped short (a list overflowed) that next collection marks.  A
+ * marking collection always completes before the call returns. */
 typedef struct {
     uint64_t items_freed;
     uint64_t slabs_freed;

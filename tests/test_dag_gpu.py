@@ -859,6 +859,39 @@ def test_reference_count_collection_frees_what_marking_frees():
 
 
 @pytest.mark.gpu
+def test_deferred_collections_free_what_waited_ones_free():
+    # svdag_gc without stats only enqueues the counting collection.  A
+    # stream of batches, each followed by a deferred collection, must leave
+    # the same DAG as the oracle and the same stored items as the same
+    # stream with waited collections; a marking collection afterwards frees
+    # nothing.  Queries between deferred collections see the edited DAG.
+    levels = L10
+    edits = _mixed_stream(levels, seed=91)
+    g, o = _pair(levels=levels)
+    w, _ = _pair(levels=levels)
+    g.garbage_collect()
+    w.garbage_collect()
+    for b in range(0, len(edits), 8):
+        chunk = edits[b:b + 8]
+        g.edit_batch(chunk)
+        w.edit_batch(chunk)
+        for e in chunk:
+            o.edit(*e)
+        assert g.garbage_collect(wait=False) is None
+        w.garbage_collect()
+        pts = np.random.default_rng(b).integers(0, 1 << levels, (2000, 3), dtype=np.uint32)
+        assert np.array_equal(g.query(pts), o.query(pts)), b
+    _same(g, o)
+    st = g.garbage_collect()
+    sw = w.garbage_collect()
+    assert st.items_live == sw.items_live == _stored_items(g) == _stored_items(w)
+    check = g.garbage_collect(mark=True)
+    assert check.items_freed == 0 and check.items_live == st.items_live
+    g.close()
+    w.close()
+
+
+@pytest.mark.gpu
 def test_config3_stream_depth13_fused_and_graph_match_oracle():
     # The benchmark's batched-editing generator (40% sphere add, 40% sphere
     # remove, 20% box fill, r log-uniform in [8, 512], centres within +-1024
