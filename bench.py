@@ -362,10 +362,11 @@ def editing_bench(args):
     record_hash("plain", g)
     g.close()
 
-    def batched_leg(gc_each, mode, name, wait=True):
-        """wait=False: the collection after each batch but the last is only
-        enqueued (svdag_gc without stats), so the host prepares the next batch
-        while it runs; the last one is waited for inside the timed region."""
+    def batched_leg(gc_each, mode, name, stream):
+        """stream=True: the 256 edits in one svdag_edit_stream call (16-edit
+        batches enqueued back to back, each followed by its collection when
+        gc_each, one wait at the end).  stream=False: one svdag_edit_batch
+        call per batch, and svdag_gc after it when gc_each, each waited for."""
         g, _, _ = fresh()
         g.set_batch_mode(mode)
         tune = g.autotune_all(edits[:16])
@@ -379,28 +380,32 @@ def editing_bench(args):
         gpu_ms = 0.0
         gc_ms = 0.0
         t2 = time.perf_counter()
-        for b in range(0, len(edits), 16):
-            s0 = time.perf_counter()
-            _, ms = g.edit_batch(edits[b:b + 16])
-            gpu_ms += ms
-            if gc_each:
-                if wait or b + 16 >= len(edits):
+        if stream:
+            _, gpu_ms = g.edit_stream(edits, batch=16, collect=gc_each)
+        else:
+            for b in range(0, len(edits), 16):
+                s0 = time.perf_counter()
+                _, ms = g.edit_batch(edits[b:b + 16])
+                gpu_ms += ms
+                if gc_each:
                     gc_ms += g.garbage_collect().gpu_ms
-                else:
-                    g.garbage_collect(wait=False)
-            blat += [time.perf_counter() - s0] * len(edits[b:b + 16])
+                blat += [time.perf_counter() - s0] * len(edits[b:b + 16])
         if not gc_each:
             gc_ms += g.garbage_collect().gpu_ms
         dt = time.perf_counter() - t2
         record_hash(name, g)
         g.close()
-        return {"voxel_edits_per_s": float(counts.sum() / dt), "edits_per_s": len(edits) / dt,
-                "edit_gpu_ms_total": round(gpu_ms, 3),
-                "gc_gpu_ms_total": round(gc_ms, 3) if wait else None, "collections_waited": "all" if wait else "last",
-                "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
-                "latency_ms_p99": float(np.percentile(blat, 99) * 1e3)}, tune
+        out = {"form": "one edit stream" if stream else "a call per batch",
+               "voxel_edits_per_s": float(counts.sum() / dt), "edits_per_s": len(edits) / dt}
+        if stream:
+            out["gpu_ms_total"] = round(gpu_ms + gc_ms, 3)
+        else:
+            out.update({"edit_gpu_ms_total": round(gpu_ms, 3), "gc_gpu_ms_total": round(gc_ms, 3),
+                        "latency_ms_p50": float(np.percentile(blat, 50) * 1e3),
+                        "latency_ms_p99": float(np.percentile(blat, 99) * 1e3)})
+        return out, tune
 
-    batched, tune = batched_leg(False, 1, "fused, one collection")
+    batched, tune = batched_leg(False, 1, "fused stream, one collection", True)
     batched.update({"edits": len(edits), "batch": 16, "gc": "once after the stream",
                     "autotune": {"lanes_per_item": tune["lanes_per_item"], "max_blocks": tune["max_blocks"],
                                  "ms_by_lanes": {str(k): v for k, v in tune["ms_by_lanes"].items()},
@@ -410,13 +415,16 @@ def editing_bench(args):
                                      "load_factor_tuning_s": tune["load_factor_tuning_s"]}
                                     if "load_factor" in tune else {})}})
     batched["speedup_vs_plain"] = batched["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
-    each, _ = batched_leg(True, 1, "fused, collection after each batch", wait=False)
+    calls, _ = batched_leg(False, 1, "fused, a call per batch, one collection", False)
+    calls["speedup_vs_plain"] = calls["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
+    batched["per_batch_calls"] = calls
+    each, _ = batched_leg(True, 1, "fused stream, collection after each batch", True)
     each["speedup_vs_plain"] = each["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
     batched["gc_each_batch"] = each
-    waited, _ = batched_leg(True, 1, "fused, collection after each batch, each waited for")
+    waited, _ = batched_leg(True, 1, "fused, a call per batch, collection after each, each waited for", False)
     waited["speedup_vs_plain"] = waited["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
-    batched["gc_each_batch_waited"] = waited
-    graph, _ = batched_leg(True, 2, "graph, collection after each batch", wait=False)
+    batched["gc_each_batch_per_batch_calls"] = waited
+    graph, _ = batched_leg(True, 2, "graph stream, collection after each batch", True)
     graph["speedup_vs_plain"] = graph["voxel_edits_per_s"] / plain["voxel_edits_per_s"]
     batched["graph_gc_each_batch"] = graph
     log(f"[bench] batched: {batched}")
@@ -701,6 +709,8 @@ def editing_profile(args):
         st, ms = g.edit_batch(edits[b:b + 16])
         if b // 16 == marked:
             svdag_py.hbm_probe(svdag_py.PROBE_STREAM, 1 << 20)
+        if args.profile_gc_each and b + 16 < len(edits):
+            g.garbage_collect(wait=False)
         items = [sum(x.svo_items[l] for x in st) for l in range(D - 1)]
         log(f"[profile] batch {b // 16}: {ms:.3f} ms, SVO items per level {items}")
         rows.append(ms)
@@ -881,6 +891,8 @@ def main():
     ap.add_argument("--editing-profile", action="store_true",
                     help="only the batched-editing stream and its collection (for rocprofv3 --kernel-trace)")
     ap.add_argument("--batch-mode", type=int, default=1, help="editing profile: 1 fused, 2 fused + HIP graphs")
+    ap.add_argument("--profile-gc-each", action="store_true",
+                    help="editing profile: a collection (not waited for) after every batch")
     ap.add_argument("--marked-batch", type=int, default=-1,
                     help="editing profile: bracket this batch with probe_stream launches (counter runs)")
     args = ap.parse_args()

@@ -646,6 +646,8 @@ __global__ void edit_finish(WsDev ws, DagDev dag, uint32_t* root)
         st->done = st->overflow == 0 && st->oom == 0;
         st->applied = 0;
         st->rootAfter = st->rootBefore;
+        if (!st->done)
+            *ws.poison = 1u;
         return;
     }
     const uint32_t r = ws.rep[ws.lvl[0].start];
@@ -839,7 +841,7 @@ static void ws_release(WsDev& ws)
 {
     for (void* p : { (void*)ws.work, (void*)ws.state, (void*)ws.nodes, (void*)ws.leaves, (void*)ws.rep,
                      (void*)ws.handle, (void*)ws.dedup, (void*)ws.lvl, (void*)ws.status, (void*)ws.editOf,
-                     (void*)ws.created, (void*)ws.hint })
+                     (void*)ws.created, (void*)ws.hint, (void*)ws.poison })
         if (p)
             (void)hipFree(p);
     ws = WsDev {};
@@ -875,6 +877,8 @@ static int ws_alloc(svdag_ctx* ctx, uint64_t capWork, bool withState)
     SVDAG_HIP(hipMalloc(&ws.dedup, (size_t)ws.dedupCap * 8));
     SVDAG_HIP(hipMalloc(&ws.lvl, kMaxLevels * sizeof(LevelDesc)));
     SVDAG_HIP(hipMalloc(&ws.status, sizeof(EditStatus)));
+    SVDAG_HIP(hipMalloc(&ws.poison, 4));
+    SVDAG_HIP(hipMemsetAsync(ws.poison, 0, 4, ctx->stream));
     SVDAG_HIP(hipMemsetAsync(ws.dedup, 0, (size_t)ws.dedupCap * 8, ctx->stream));
     return SVDAG_OK;
 }
@@ -985,7 +989,7 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
 }
 
 int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems,
-                        uint32_t kCap)
+                        uint32_t kCap, hipEvent_t mergeAfter)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
@@ -995,6 +999,10 @@ int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, u
     svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf), 4 * kLeavesPerWave, kCap),
                            256, 0, st>>>(
         ws, dag, eds);
+    // The build above only reads items the root reaches; the merge below
+    // finds and inserts, so it waits for whatever mergeAfter marks.
+    if (mergeAfter)
+        SVDAG_HIP(hipStreamWaitEvent(st, mergeAfter, 0));
     for (int L = (int)dag.leafLevel; L >= (int)S; --L)
         launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork), kCap);
     SVDAG_HIP(hipGetLastError());
@@ -1006,6 +1014,9 @@ static int ensure_batch_slots(svdag_ctx* ctx, uint32_t n)
 {
     if (n <= ctx->batchCap)
         return SVDAG_OK;
+    // At least 256 edits (a few hundred KB in all), so that batches and
+    // edit streams of the usual sizes never reallocate between calls.
+    n = std::max<uint32_t>(n, 256u);
     ++ctx->bufferGeneration;
     for (void* p : { (void*)ctx->dEditors, (void*)ctx->dStatusArr, (void*)ctx->dLevelArr, (void*)ctx->dEdCount,
                      (void*)ctx->dEdFull, (void*)ctx->dSeeds, (void*)ctx->dSeedWork, (void*)ctx->dWaves,
@@ -1322,6 +1333,15 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
     SVDAG_HIP(hipEventCreate(&ctx->ev0));
     SVDAG_HIP(hipEventCreate(&ctx->ev1));
     SVDAG_HIP(hipEventCreateWithFlags(&ctx->evWait, hipEventDisableTiming));
+    // The edit streams' collection stream and events for streams of up to 64
+    // batches, made here: creating a stream or events costs milliseconds,
+    // which an edit stream would otherwise pay on its first call.
+    SVDAG_HIP(hipStreamCreateWithFlags(&ctx->gcStream, hipStreamNonBlocking));
+    for (int k = 0; k < 128; ++k) {
+        hipEvent_t e = nullptr;
+        SVDAG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->streamEvents.push_back(e);
+    }
     SVDAG_HIP(hipHostMalloc((void**)&ctx->hStatus, sizeof(EditStatus) + kMaxLevels * sizeof(LevelDesc)));
     ctx->hLevels = reinterpret_cast<LevelDesc*>(ctx->hStatus + 1);
 
@@ -1406,8 +1426,18 @@ int svdag_destroy(svdag_ctx* ctx)
         if (p)
             (void)hipHostFree(p);
     (void)hipHostFree(ctx->hStatus);
+    if (ctx->hStream)
+        (void)hipHostFree(ctx->hStream);
+    if (ctx->dStreamFail)
+        (void)hipFree(ctx->dStreamFail);
     if (ctx->evGc)
         (void)hipEventDestroy(ctx->evGc);
+    if (ctx->gcStream) {
+        (void)hipStreamSynchronize(ctx->gcStream);
+        (void)hipStreamDestroy(ctx->gcStream);
+    }
+    for (hipEvent_t e : ctx->streamEvents)
+        (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     if (ctx->evWait)
@@ -1728,6 +1758,215 @@ int svdag_edit_on_stream(svdag_ctx* ctx, const svdag_edit_desc* desc, svdag_edit
     return edit_locked(ctx, desc, stats);
 }
 
+// Undo frames in edit order, as add_frame before every editMyHashDag.
+static void push_undo_frames(svdag_ctx* ctx, const svdag_edit_stats* st, uint32_t n)
+{
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ctx->undo.empty() || ctx->undo.back() != st[k].root_before) {
+            ctx->redo.clear();
+            ctx->undo.push_back(st[k].root_before);
+        }
+    }
+}
+
+// Edits [0, n) in batches of `batch`, each batch followed by a counting
+// collection when collect, all enqueued back to back with one wait at the
+// end: the host plans and enqueues every batch while the device runs the
+// first ones.  Each collection takes its released roots from its batch's
+// device-side results (rc_release_batch).  An edit that cannot complete on
+// the device (workspace overflow, arena exhausted, handed back by its wave)
+// sets the workspace's poison word, after which the stream's collections
+// release nothing; having seen the results, the host runs the rest from that
+// edit batch by batch (run_edits, then a collection), as it does a stream
+// that is not all fused waves from the start.
+static int run_edit_stream(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, uint32_t batch, bool collect,
+                           svdag_edit_stats* stats, float* totalMs)
+{
+    float ms = 0.0f;
+    // Batch by batch from edit k0, waiting for each.
+    auto serial = [&](uint32_t k0) -> int {
+        for (uint32_t a = k0; a < n;) {
+            const uint32_t b = std::min<uint32_t>(n, (a / batch + 1) * batch);
+            float part = 0.0f;
+            SVDAG_TRY(run_edits(ctx, eds + a, b - a, stats + a, &part));
+            ms += part;
+            if (collect) {
+                ctx->undo.clear();
+                ctx->redo.clear();
+                svdag_gc_stats gs {};
+                SVDAG_TRY(gc_run(ctx, &gs, false, false));
+            } else {
+                push_undo_frames(ctx, stats + a, b - a);
+            }
+            a = b;
+        }
+        return SVDAG_OK;
+    };
+    const uint32_t nb = (n + batch - 1) / batch;
+    std::vector<uint32_t> held;
+    bool pipe = ctx->batchMode >= SVDAG_BATCH_FUSED;
+    if (collect) {
+        SVDAG_TRY(gc_settle(ctx));
+        for (uint32_t r : ctx->rootLog)
+            if (r != kSentinel)
+                held.push_back(r);
+        std::sort(held.begin(), held.end());
+        held.erase(std::unique(held.begin(), held.end()), held.end());
+        // A collection that must mark (counts off, or the stored items past
+        // 1.5 times the last marking's live count) runs in the serial form.
+        const bool markDue = ctx->liveAtMark != 0 && ctx->liveAfterGc > ctx->liveAtMark + ctx->liveAtMark / 2;
+        pipe = pipe && !ctx->refsDirty && !markDue && held.size() + batch <= 256;
+    }
+    std::vector<std::vector<BatchStep>> steps(nb);
+    std::vector<uint32_t> S(nb, 0);
+    uint32_t nw = 0;
+    if (pipe) {
+        SVDAG_TRY(ensure_batch_slots(ctx, n));
+        const std::vector<uint8_t> alone(n, 0);
+        for (uint32_t j = 0; j < nb && pipe; ++j) {
+            const uint32_t a = j * batch, b = std::min<uint32_t>(n, a + batch);
+            S[j] = plan_batch(ctx, eds, a, b, steps[j], alone, nw);
+            pipe = S[j] != 0;
+            for (const BatchStep& st : steps[j]) {
+                pipe = pipe && st.fused;
+                nw += st.fused ? 1u : 0u;
+            }
+        }
+    }
+    if (!pipe) {
+        SVDAG_TRY(serial(0));
+        if (totalMs)
+            *totalMs = ms;
+        return SVDAG_OK;
+    }
+    EditorDev* hEd = reinterpret_cast<EditorDev*>(ctx->hBatch);
+    EditStatus* hSt = reinterpret_cast<EditStatus*>(hEd + ctx->batchCap);
+    LevelDesc* hLv = reinterpret_cast<LevelDesc*>(hSt + ctx->batchCap);
+    std::copy(eds, eds + n, hEd);
+    if (collect)
+        SVDAG_TRY(gc_stream_prepare(ctx));
+    if (!ctx->hStream) {
+        SVDAG_HIP(hipHostMalloc(&ctx->hStream, gc_counters_bytes() + 64));
+        SVDAG_HIP(hipMalloc(&ctx->dStreamFail, 4));
+    }
+    const hipStream_t st = ctx->stream;
+    const SeedDesc* seeds = reinterpret_cast<const SeedDesc*>(ctx->hFused);
+    const uint2* waves = reinterpret_cast<const uint2*>(seeds + ctx->batchCap);
+    SVDAG_HIP(hipMemcpyAsync(ctx->dEditors, hEd, n * sizeof(EditorDev), hipMemcpyHostToDevice, st));
+    SVDAG_HIP(hipMemcpyAsync(ctx->dSeeds, seeds, n * sizeof(SeedDesc), hipMemcpyHostToDevice, st));
+    SVDAG_HIP(hipMemcpyAsync(ctx->dWaves, waves, nw * sizeof(uint2), hipMemcpyHostToDevice, st));
+    SVDAG_HIP(hipMemsetAsync(ctx->dWaveIdx, 0, 8, st));
+    SVDAG_HIP(hipMemsetAsync(ctx->ws.poison, 0, 4, st));
+    SVDAG_HIP(hipMemsetAsync(ctx->dStreamFail, 0, 4, st));
+    // In fused mode each batch's collection runs on gcStream beside the next
+    // batch's first SVO build, which only reads items the root reaches
+    // (a collection frees only items no root reaches); that wave's merge,
+    // which finds and inserts, waits for the collection.  HIP-graph waves
+    // are replayed whole, so their collections stay on the context's stream.
+    // Each batch's two events are its own: recording an event again while
+    // its last record is still pending can hold the host until it completes.
+    const bool overlap = collect && ctx->batchMode == SVDAG_BATCH_FUSED;
+    if (overlap) {
+        if (!ctx->gcStream)
+            SVDAG_HIP(hipStreamCreateWithFlags(&ctx->gcStream, hipStreamNonBlocking));
+        while (ctx->streamEvents.size() < 2 * (size_t)nb) {
+            hipEvent_t e = nullptr;
+            SVDAG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ctx->streamEvents.push_back(e);
+        }
+    }
+    auto batchDone = [&](uint32_t j) { return ctx->streamEvents[2 * j]; };
+    auto gcDone = [&](uint32_t j) { return ctx->streamEvents[2 * j + 1]; };
+    SVDAG_HIP(hipEventRecord(ctx->ev0, st));
+    void* lastCtr = nullptr;
+    const std::vector<uint32_t> none;
+    for (uint32_t j = 0; j < nb; ++j) {
+        for (size_t w = 0; w < steps[j].size(); ++w)
+            SVDAG_TRY(launch_wave(ctx, hEd, steps[j][w], S[j], overlap && j > 0 && w == 0 ? gcDone(j - 1) : nullptr));
+        if (collect) {
+            const uint32_t a = j * batch, b = std::min<uint32_t>(n, a + batch);
+            hipStream_t gs = st;
+            if (overlap) {
+                SVDAG_HIP(hipEventRecord(batchDone(j), st));
+                SVDAG_HIP(hipStreamWaitEvent(ctx->gcStream, batchDone(j), 0));
+                gs = ctx->gcStream;
+            }
+            SVDAG_TRY(gc_enqueue_stream(ctx, a, b - a, j == 0 ? held : none, ctx->dStreamFail, &lastCtr, gs));
+            if (overlap)
+                SVDAG_HIP(hipEventRecord(gcDone(j), gs));
+        }
+    }
+    if (overlap)
+        SVDAG_HIP(hipStreamWaitEvent(st, gcDone(nb - 1), 0));
+    SVDAG_HIP(hipEventRecord(ctx->ev1, st));
+    SVDAG_HIP(hipMemcpyAsync(hSt, ctx->dStatusArr, n * sizeof(EditStatus), hipMemcpyDeviceToHost, st));
+    SVDAG_HIP(hipMemcpyAsync(hLv, ctx->dLevelArr, (size_t)n * kMaxLevels * sizeof(LevelDesc), hipMemcpyDeviceToHost,
+                             st));
+    uint8_t* hs = static_cast<uint8_t*>(ctx->hStream);
+    if (collect) {
+        SVDAG_HIP(hipMemcpyAsync(hs, lastCtr, gc_counters_bytes(), hipMemcpyDeviceToHost, st));
+        SVDAG_HIP(hipMemcpyAsync(hs + gc_counters_bytes(), ctx->dStreamFail, 4, hipMemcpyDeviceToHost, st));
+    }
+    SVDAG_HIP(hipStreamSynchronize(st));
+    float part = 0.0f;
+    SVDAG_HIP(hipEventElapsedTime(&part, ctx->ev0, ctx->ev1));
+    ms += part;
+    uint32_t bad = n;
+    for (uint32_t k = 0; k < n; ++k)
+        if (!hSt[k].done) {
+            bad = k;
+            break;
+        }
+    // The collections of batches before jb ran: all of them when every edit
+    // completed (the last batch may be short), else those before the batch
+    // of the first edit that did not.
+    const uint32_t jb = bad == n ? nb : bad / batch;
+    for (uint32_t k = 0; k < bad; ++k) {
+        root_log(ctx);
+        ctx->root = hSt[k].rootAfter;
+        svdag_edit_stats& o = stats[k];
+        std::memset(&o, 0, sizeof(o));
+        for (uint32_t L = 0; L <= ctx->dag.leafLevel && L < 32; ++L)
+            o.svo_items[L] = hLv[(size_t)k * kMaxLevels + L].count;
+        o.applied = hSt[k].applied;
+        o.root_before = hSt[k].rootBefore;
+        o.root_after = hSt[k].rootAfter;
+        o.gpu_ms = part / (float)n;
+        const bool batchEnd = k + 1 == n || (k + 1) % batch == 0;
+        if (collect && batchEnd && k / batch < jb) {
+            ctx->rootLog.clear();
+            root_log(ctx);
+        }
+    }
+    if (collect) {
+        if (jb > 0) {
+            // What svdag_gc does around a collection.
+            ctx->undo.clear();
+            ctx->redo.clear();
+            ctx->traceW = ctx->traceH = 0;
+            ++ctx->gcSerial;
+            uint32_t fail = 0;
+            std::memcpy(&fail, hs + gc_counters_bytes(), 4);
+            if (fail)
+                ctx->refsDirty = true;
+            else if (jb == nb)
+                gc_stream_take(ctx, hs, n - (nb - 1) * batch + (nb == 1 ? (uint32_t)held.size() : 0u));
+        }
+    } else {
+        push_undo_frames(ctx, stats, bad);
+    }
+    if (bad < n) {
+        // Later edits may have inserted items no root reaches; the next
+        // collection marks and recounts.
+        ctx->refsDirty = true;
+        SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, st));
+        SVDAG_TRY(serial(bad));
+    }
+    if (totalMs)
+        *totalMs = ms;
+    return SVDAG_OK;
+}
+
 int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, svdag_edit_stats* stats, float* gpu_ms)
 {
     if (!ctx || (n && !descs))
@@ -1741,14 +1980,29 @@ int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, s
     std::vector<svdag_edit_stats> local(stats ? 0 : n);
     svdag_edit_stats* st = stats ? stats : local.data();
     SVDAG_TRY(run_edits(ctx, eds.data(), n, st, gpu_ms));
-    // Undo frames in edit order, as add_frame before every editMyHashDag.
-    for (uint32_t k = 0; k < n; ++k) {
-        if (ctx->undo.empty() || ctx->undo.back() != st[k].root_before) {
-            ctx->redo.clear();
-            ctx->undo.push_back(st[k].root_before);
-        }
-    }
+    push_undo_frames(ctx, st, n);
     return SVDAG_OK;
+}
+
+int svdag_edit_stream(svdag_ctx* ctx, const svdag_edit_desc* descs, uint32_t n, uint32_t batch, uint32_t collect,
+                      svdag_edit_stats* stats, float* gpu_ms)
+{
+    if (!ctx || (n && !descs) || batch == 0)
+        return SVDAG_E_INVALID;
+    if (n == 0)
+        return SVDAG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->server) {
+        set_error("svdag_edit_stream: a server is running on this context");
+        return SVDAG_E_INVALID;
+    }
+    std::vector<EditorDev> eds(n);
+    for (uint32_t k = 0; k < n; ++k)
+        SVDAG_TRY(make_editor(ctx, descs[k], eds[k]));
+    std::vector<svdag_edit_stats> local(stats ? 0 : n);
+    svdag_edit_stats* st = stats ? stats : local.data();
+    // No server runs, so no frame traces beside the stream's collections.
+    return run_edit_stream(ctx, eds.data(), n, batch, collect != 0, st, gpu_ms);
 }
 
 // Times the sample batch under each value of one knob (three runs each, the

@@ -54,6 +54,10 @@ struct WsDev {
     // Where the insert of a representative the DAG find missed starts: the
     // first line of its chain the find saw with a free slot (kNoSlab: head).
     uint32_t* hint;              // [capWork]
+    // Set by an edit that could not complete (overflow, arena exhausted,
+    // handed back by a fused wave).  The collections of an edit stream read
+    // it and release nothing once it is set; the stream clears it first.
+    uint32_t* poison;
 };
 
 // Level-S cell range of an edit in a fused wave: cells lo .. lo + n - 1 per
@@ -214,6 +218,7 @@ struct svdag_ctx {
     uint32_t* gcFront[2] = { nullptr, nullptr };
     uint64_t gcCap = 0;
     uint32_t* gcCounters = nullptr;
+    uint32_t gcCtrIdx = 0; // the counter block the next counting collection uses
     // Per-slot visit bytes of every table (kept between collections and
     // regrown with the arenas) and their device pointer array.
     std::vector<uint8_t*> gcMasks;
@@ -297,6 +302,16 @@ struct svdag_ctx {
     uint32_t gcPendingRounds = 0, gcPendingCand = 0;
     void* hGcCtr = nullptr;
     hipEvent_t evGc = nullptr;
+    // An edit stream's results beside the edit statuses: its last
+    // collection's counters and its "a collection stopped short" word
+    // (pinned), and that word on the device.
+    void* hStream = nullptr;
+    uint32_t* dStreamFail = nullptr;
+    // An edit stream runs each batch's collection on gcStream beside the
+    // next batch's SVO build; per batch, one event marks the batch's end on
+    // the context's stream and one the collection's end on gcStream.
+    hipStream_t gcStream = nullptr;
+    std::vector<hipEvent_t> streamEvents;
     // The reference-format stream a size query built (refformat.hip), kept
     // for the call that fetches it while (root, gcSerial) is unchanged.
     std::vector<uint8_t> refStream;
@@ -335,6 +350,11 @@ inline void root_log(svdag_ctx* ctx)
 // mark: force the marking collection (else reference counts, when valid).
 int gc_run(svdag_ctx* ctx, svdag_gc_stats* st, bool mark = false, bool wait = true);
 int gc_settle(svdag_ctx* ctx);
+int gc_stream_prepare(svdag_ctx* ctx);
+int gc_enqueue_stream(svdag_ctx* ctx, uint32_t first, uint32_t count, const std::vector<uint32_t>& held,
+                      uint32_t* fail, void** ctrOut, hipStream_t st);
+void gc_stream_take(svdag_ctx* ctx, const void* counters, uint32_t roots);
+size_t gc_counters_bytes();
 // surfDev (optional) receives one surface code per pixel (see dSurf).
 int launch_trace(svdag_ctx* ctx, const svdag_camera& cam, uint32_t W, uint32_t H, uint32_t root, hipStream_t stream,
                  svdag_hit* outDev, uint8_t* surfDev = nullptr);
@@ -366,7 +386,7 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
 // Levels S .. leaf of a fused wave: SVO build and merge with per-item editors
 // (dag.hip).  levelItems[L] bounds the items of level L (grid sizing only).
 int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems,
-                        uint32_t gridCap = 8192);
+                        uint32_t gridCap = 8192, hipEvent_t mergeAfter = nullptr);
 
 // Fused multi-edit waves (fused.hip).
 struct BatchStep {
@@ -374,10 +394,13 @@ struct BatchStep {
     bool fused;           // a fused wave (else a single edit on its own)
 };
 // Splits edits [first, n) into steps; fills the seeds / waves host mirrors
-// and returns the split level S (0 when nothing can be fused).
-// Edits flagged in `alone` run on their own.
+// (the waves from index waveBase on) and returns the split level S (0 when
+// nothing can be fused).  Edits flagged in `alone` run on their own.
 uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32_t n, std::vector<BatchStep>& steps,
-                    const std::vector<uint8_t>& alone);
-int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S);
+                    const std::vector<uint8_t>& alone, uint32_t waveBase = 0);
+// mergeAfter: the wave's merge kernels wait for this event (its SVO build
+// does not); fused mode only.
+int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S,
+                hipEvent_t mergeAfter = nullptr);
 void release_wave_graphs(svdag_ctx* ctx);
 }

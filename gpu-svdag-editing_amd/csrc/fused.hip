@@ -573,6 +573,7 @@ __global__ void __launch_bounds__(kParThreads) fused_spine_par(WsDev ws, DagDev 
             st.oom = 1u;
             st.rootBefore = st.rootAfter = root0;
             status[wv.x] = st;
+            *ws.poison = 1u;
             *f.spineDone = 1u;
         }
         return;
@@ -638,6 +639,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
             st.oom = joint.oom;
             st.rootBefore = st.rootAfter = root;
             status[wv.x] = st;
+            *ws.poison = 1u;
         }
         if (lane < kMaxLevels)
             levels[wv.x * kMaxLevels + lane] = ws.lvl[lane];
@@ -730,6 +732,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                 st.fallback = 1u;
                 st.rootBefore = st.rootAfter = root;
                 status[k] = st;
+                *ws.poison = 1u;
             }
             break;
         }
@@ -874,6 +877,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
                 st.oom = 1u;
                 st.rootBefore = st.rootAfter = root;
                 status[k] = st;
+                *ws.poison = 1u;
             }
             break;
         }
@@ -928,7 +932,7 @@ static bool fusable(const svdag_ctx* ctx, const EditorDev& e)
 }
 
 uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32_t n, std::vector<BatchStep>& steps,
-                    const std::vector<uint8_t>& alone)
+                    const std::vector<uint8_t>& alone, uint32_t waveBase)
 {
     steps.clear();
     const uint32_t D = ctx->dag.levels;
@@ -945,7 +949,7 @@ uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32
     }
     SeedDesc* seeds = reinterpret_cast<SeedDesc*>(ctx->hFused);
     uint2* waves = reinterpret_cast<uint2*>(seeds + ctx->batchCap);
-    uint32_t nWaves = 0;
+    uint32_t nWaves = waveBase;
     const uint32_t sh = D - (uint32_t)S;
     std::vector<Bounds> members;
     uint32_t waveStart = first;
@@ -1000,7 +1004,8 @@ uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32
 // The wave's launch sequence.  levelItems == nullptr: fixed grids (every
 // kernel walks its device-side count with a grid-stride loop), as captured
 // into a graph that serves any wave of the same split level.
-static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, uint32_t seedLanes)
+static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, uint32_t seedLanes,
+                        hipEvent_t mergeAfter = nullptr)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
@@ -1011,7 +1016,7 @@ static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, 
     wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
     fused_seed<<<(seedLanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
     if (levelItems) {
-        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems, ctx->maxBlocks));
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems, ctx->maxBlocks, mergeAfter));
     } else {
         uint64_t fixed[kMaxLevels];
         for (uint32_t L = 0; L < kMaxLevels; ++L)
@@ -1064,7 +1069,7 @@ static int wave_graph(svdag_ctx* ctx, uint32_t S, hipGraphExec_t& out)
     return SVDAG_OK;
 }
 
-int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S)
+int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S, hipEvent_t mergeAfter)
 {
     if (ctx->batchMode == SVDAG_BATCH_FUSED_GRAPH) {
         hipGraphExec_t exec = nullptr;
@@ -1078,7 +1083,7 @@ int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, ui
         for (uint32_t L = S; L <= ctx->dag.leafLevel; ++L)
             levelItems[L] += level_bound(ctx, b, L);
     }
-    return enqueue_wave(ctx, S, levelItems, 8u * (step.end - step.first));
+    return enqueue_wave(ctx, S, levelItems, 8u * (step.end - step.first), mergeAfter);
 }
 
 } // namespace svdag

@@ -387,206 +387,293 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool n)
     return base + (uint32_t)__popcll(m & lanes_below());
 }
 
-// Up to this many released roots travel in the seed's arguments, so a
-// collection after a batch of edits needs no upload.
+// Up to this many released roots travel in the first round's arguments,
+// so a collection after a batch of edits needs no upload.
 constexpr uint32_t kSeedInline = 256;
 struct SeedInline {
     uint32_t h[kSeedInline];
 };
 
-// Clears the counters, then lists the released roots whose count is zero.
-// cand == nullptr: the roots are inl.h[0 .. n).  One block.
-__global__ void __launch_bounds__(256) rc_seed(DagDev dag, const uint32_t* cand, SeedInline inl, uint32_t n,
-                                               uint2* out, uint32_t outCap, GcCounters* ctr)
+// The table and slot of root handle h (~0u for none), as the host decodes
+// the current root.
+__device__ inline uint2 root_slot(const DagDev& dag, uint32_t h)
 {
-    // Atomic writes, performed where the appends below are.
-    uint32_t* words = reinterpret_cast<uint32_t*>(ctr);
-    for (uint32_t k = threadIdx.x; k < (uint32_t)(sizeof(GcCounters) / 4); k += blockDim.x)
-        (void)atomicExch(&words[k], 0u);
-    // A pop past the free stack's end leaves its count below zero; the chain
-    // pass pushes onto an empty stack then.
-    if (threadIdx.x < dag.numTables && dag.tables[threadIdx.x].alloc->freeCount < 0)
-        dag.tables[threadIdx.x].alloc->freeCount = 0;
-    __syncthreads();
+    if (h == kSentinel)
+        return make_uint2(~0u, ~0u);
     const bool leaf = dag.leafLevel == 0;
-    for (uint32_t base = 0; base < n; base += blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        bool dies = false;
-        uint2 e = make_uint2(0u, 0u);
-        if (i < n) {
-            const uint32_t h = cand ? cand[i] : inl.h[i];
-            const uint32_t hs = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
-            e = make_uint2(hs - 2u, leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h));
-            dies = dag.tables[e.x].refs[e.y] == 0u;
+    return make_uint2((leaf ? leaf_handle_size(dag, h) : node_handle_size(h)) - 2u,
+                      leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h));
+}
+
+// What a round reads of an item before freeing it: its role, its child
+// words, and for an item in an overflow line its bucket (the bucket hash of
+// its words, as at insert).
+struct ReleaseItem {
+    uint32_t roles;
+    uint32_t cw[8];
+    uint32_t bucket;
+    bool chained;
+};
+
+__device__ inline ReleaseItem rc_load(const DagDev& dag, bool load, uint2 e)
+{
+    ReleaseItem r {};
+    const TableDev& t = dag.tables[e.x];
+    const uint32_t s = e.x + 2u;
+    const uint32_t* item = load ? table_item_ptr(t, e.y) : nullptr;
+    r.roles = load ? t.roles[e.y] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+        r.cw[j] = load && j + 1u < s ? item[j + 1u] : 0u;
+    r.chained = load && (e.y >> kSlotBits) >= t.numBuckets;
+    if (r.chained) {
+        uint32_t h = 0;
+        for (uint32_t w = 0; w < s; ++w)
+            h ^= hash_part(item[w], w);
+        r.bucket = h % t.numBuckets;
+    }
+    return r;
+}
+
+// Releases item e (have) whose words v were read: frees its slot, lists an
+// overflow line's bucket once for the chain pass, and takes one count from
+// each child of each counted role; the children that reach zero go to the
+// output list.  The slot's free and the listing are issued alongside the
+// children's count loads, so each wave waits once per step: the counts, the
+// decrements, the appends.  Every lane of the wave calls it.
+__device__ inline void rc_release_loaded(const DagDev& dag, const GcTables& a, uint32_t rootTable, uint32_t rootPtr,
+                                         bool have, uint2 e, ReleaseItem v, uint2* out, uint32_t* outCount,
+                                         uint32_t cap, uint2* touchedList, GcCounters* ctr)
+{
+    const TableDev& t = dag.tables[e.x];
+    const uint32_t s = e.x + 2u;
+    const bool chained = have && v.chained;
+    uint32_t tbit = 0, told = 0;
+    if (chained) {
+        tbit = 1u << (v.bucket & 31u);
+        told = atomicOr(&a.touched[e.x][v.bucket >> 5], tbit);
+    }
+    if (have) {
+        const uint32_t slab = e.y >> kSlotBits, slot = e.y & ((1u << kSlotBits) - 1u);
+        atomicAnd(&t.lines[(size_t)slab * kLineWords + (slot >> 2)], ~(0xFFu << (8u * (slot & 3u))));
+        t.roles[e.y] = 0u;
+    }
+    bool listed = false;
+    // Children, all of one role at once (a node has at most 8): their count
+    // loads, then the decrements, then one append per wave.  A role no lane
+    // of the wave holds is skipped.
+#pragma unroll
+    for (uint32_t rb = 0; rb < 2; ++rb) {
+        const uint8_t role = rb == 0 ? kRoleParentOfNodes : kRoleParentOfLeaves;
+        const bool lf = role == kRoleParentOfLeaves;
+        const bool mine = have && (v.roles & role);
+        if (__ballot(mine) == 0ull)
+            continue;
+        uint2 ce[8];
+        uint32_t* cp[8];
+        uint32_t cv[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            cv[j] = kStickyRefs;
+            if (mine && j + 1u < s) {
+                const uint32_t ch = v.cw[j];
+                const uint32_t cs = lf ? leaf_handle_size(dag, ch) : node_handle_size(ch);
+                ce[j] = make_uint2(cs - 2u, lf ? leaf_handle_ptr(dag, ch) : node_handle_ptr(ch));
+                cp[j] = &dag.tables[ce[j].x].refs[ce[j].y];
+                cv[j] = ld_filter(cp[j]);
+            }
         }
-        const uint32_t at = wave_append(&ctr->count[0], dies);
-        if (dies) {
-            if (at < outCap)
-                out[at] = e;
+        if (!listed) {
+            // The bucket listing rides on this round trip.
+            listed = true;
+            const bool list = chained && (told & tbit) == 0u;
+            const uint32_t at = wave_append(&ctr->touched, list);
+            if (list) {
+                if (at < cap)
+                    touchedList[at] = make_uint2(e.x, v.bucket);
+                else
+                    atomicOr(&ctr->overflow, 1u);
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (cv[j] < kStickyRefs)
+                cv[j] = atomicSub(cp[j], 1u);
+        uint32_t die = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            if (mine && j + 1u < s) {
+                if (cv[j] == 0u)
+                    atomicOr(&ctr->overflow, 1u); // counts off: the host falls back to marking
+                if (cv[j] == 1u && !(ce[j].x == rootTable && ce[j].y == rootPtr))
+                    die |= 1u << j;
+            }
+        }
+        // Lanes' dying children go to consecutive slots of the round's
+        // list: a wave scan of the counts, one atomic per wave.
+        const uint32_t cnt = (uint32_t)__builtin_popcount(die);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t x = (uint32_t)__shfl_up((int)incl, (unsigned)d, 64);
+            if (__lane_id() >= (uint32_t)d)
+                incl += x;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+        uint32_t base0 = 0;
+        if (total != 0u) {
+            if (__lane_id() == 63u)
+                base0 = atomicAdd(outCount, total);
+            base0 = (uint32_t)__shfl((int)base0, 63, 64);
+        }
+        uint32_t at = base0 + incl - cnt;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            if ((die >> j) & 1u) {
+                if (at < cap)
+                    out[at] = ce[j];
+                else
+                    atomicOr(&ctr->overflow, 1u);
+                ++at;
+            }
+        }
+    }
+    if (!listed) {
+        // A wave whose items have no counted children (leaves).
+        const bool list = chained && (told & tbit) == 0u;
+        const uint32_t at = wave_append(&ctr->touched, list);
+        if (list) {
+            if (at < cap)
+                touchedList[at] = make_uint2(e.x, v.bucket);
             else
                 atomicOr(&ctr->overflow, 1u);
         }
     }
 }
 
-// One round: frees the items of the input list.  Each wave's memory
-// accesses form one chain of round trips, ordered so that nothing waits
-// twice: the list entry; the item's words and role; the children's counts,
-// with the slot's free and its bucket's listing issued alongside; the
-// decrements; the appends of the children that die.  The items freed are
-// not summed here: every listed item is freed, so the chain pass adds up
-// the rounds' list sizes.
-__global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32_t rootTable, uint32_t rootPtr,
-                                                  const uint2* in, const uint32_t* inCount, uint2* out,
-                                                  uint32_t* outCount, uint32_t cap, uint2* touchedList,
-                                                  GcCounters* ctr)
+// The first round: the released roots (cand, or inl.h when cand is null)
+// whose count is zero die; each root's count and words are read together.
+// count[0] gets the number that died.  The counters start at zero: the
+// previous collection's chain pass cleared them.
+__global__ void __launch_bounds__(256) rc_release_roots(DagDev dag, GcTables a, uint32_t rootTable, uint32_t rootPtr,
+                                                        const uint32_t* cand, SeedInline inl, uint32_t n, uint2* out,
+                                                        uint32_t* outCount, uint32_t cap, uint2* touchedList,
+                                                        GcCounters* ctr)
 {
-    // Once a list overflowed or a count was found off, the marking
-    // collection takes over: release nothing more.  Both words are read
-    // together, once per block.
-    __shared__ uint32_t sStop, sN;
-    if (threadIdx.x == 0) {
-        const uint32_t stop = ctr->overflow;
-        const uint32_t cnt = *inCount;
-        sStop = stop;
-        sN = cnt;
-    }
-    __syncthreads();
-    if (sStop)
-        return;
-    const uint32_t n = min(sN, cap);
+    // A pop past the free stack's end leaves its count below zero; the chain
+    // pass pushes onto an empty stack then.
+    if (blockIdx.x == 0 && threadIdx.x < dag.numTables && dag.tables[threadIdx.x].alloc->freeCount < 0)
+        dag.tables[threadIdx.x].alloc->freeCount = 0;
+    const bool leaf = dag.leafLevel == 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
+        const bool in = i < n;
+        uint2 e = make_uint2(0u, 0u);
+        if (in) {
+            const uint32_t h = cand ? cand[i] : inl.h[i];
+            const uint32_t hs = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
+            e = make_uint2(hs - 2u, leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h));
+        }
+        const uint32_t refs = in ? dag.tables[e.x].refs[e.y] : 1u;
+        const ReleaseItem v = rc_load(dag, in, e);
+        const bool dies = in && refs == 0u;
+        const unsigned long long m = __ballot(dies);
+        if (m != 0ull && __lane_id() == (uint32_t)__ffsll((long long)m) - 1u)
+            atomicAdd(&ctr->count[0], (uint32_t)__popcll(m));
+        rc_release_loaded(dag, a, rootTable, rootPtr, dies, e, v, out, outCount, cap, touchedList, ctr);
+    }
+}
+
+// The first round of a collection inside an edit stream, enqueued behind
+// the batch of edits [first, first + count) before the host has seen its
+// results.  The released roots are the batch's input roots
+// (status[k].rootBefore) and, for the stream's first collection, the roots
+// the host held before it (inl.h[0 .. n0)), less the current root
+// (*rootDev), each once.  Once an edit of the stream failed (*poison) the
+// collection releases nothing: the host runs the stream again from that
+// edit.  One block; n0 + count <= 256.
+__global__ void __launch_bounds__(256) rc_release_batch(DagDev dag, GcTables a, const EditStatus* status,
+                                                        uint32_t first, uint32_t count, SeedInline inl, uint32_t n0,
+                                                        const uint32_t* rootDev, const uint32_t* poison, uint2* out,
+                                                        uint32_t* outCount, uint32_t cap, uint2* touchedList,
+                                                        GcCounters* ctr)
+{
+    __shared__ uint32_t sCand[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t stop = *poison;
+    const uint32_t cur = *rootDev;
+    const uint32_t m = n0 + count;
+    uint32_t h = kSentinel;
+    if (t < n0)
+        h = inl.h[t];
+    else if (t < m)
+        h = status[first + (t - n0)].rootBefore;
+    sCand[t] = h;
+    // A pop past the free stack's end leaves its count below zero; the chain
+    // pass pushes onto an empty stack then.
+    if (!stop && t < dag.numTables && dag.tables[t].alloc->freeCount < 0)
+        dag.tables[t].alloc->freeCount = 0;
+    __syncthreads();
+    if (stop)
+        return;
+    bool in = t < m && h != kSentinel && h != cur;
+    for (uint32_t j = 0; j < t && in; ++j)
+        in = sCand[j] != h;
+    uint2 e = make_uint2(0u, 0u);
+    if (in) {
+        const bool leaf = dag.leafLevel == 0;
+        const uint32_t hs = leaf ? leaf_handle_size(dag, h) : node_handle_size(h);
+        e = make_uint2(hs - 2u, leaf ? leaf_handle_ptr(dag, h) : node_handle_ptr(h));
+    }
+    const uint32_t refs = in ? dag.tables[e.x].refs[e.y] : 1u;
+    const ReleaseItem v = rc_load(dag, in, e);
+    const bool dies = in && refs == 0u;
+    const unsigned long long mk = __ballot(dies);
+    if (mk != 0ull && __lane_id() == (uint32_t)__ffsll((long long)mk) - 1u)
+        atomicAdd(&ctr->count[0], (uint32_t)__popcll(mk));
+    const uint2 rs = root_slot(dag, cur);
+    rc_release_loaded(dag, a, rs.x, rs.y, dies, e, v, out, outCount, cap, touchedList, ctr);
+}
+
+// A later round: frees the items of the input list.  Each thread reads its
+// first entry together with the list's length (an entry past the length is
+// read but not used), so the round's chain is: entry, item, the children's
+// counts, the decrements, the appends.
+// rootDev (an edit stream's collections): the current root is read there
+// instead of taken from rootTable / rootPtr.
+__global__ void __launch_bounds__(256) rc_release(DagDev dag, GcTables a, uint32_t rootTable, uint32_t rootPtr,
+                                                  const uint32_t* rootDev, const uint2* in, const uint32_t* inCount,
+                                                  uint2* out, uint32_t* outCount, uint32_t cap, uint2* touchedList,
+                                                  GcCounters* ctr)
+{
+    const uint32_t first = blockIdx.x * blockDim.x;
+    const uint32_t i0 = first + threadIdx.x;
+    // Once a list overflowed or a count was found off, the marking
+    // collection takes over: release nothing more.
+    const uint32_t stop = ctr->overflow;
+    const uint32_t cnt = *inCount;
+    const uint2 spec = i0 < cap ? in[i0] : make_uint2(0u, 0u);
+    if (rootDev) {
+        const uint2 rs = root_slot(dag, *rootDev);
+        rootTable = rs.x;
+        rootPtr = rs.y;
+    }
+    if (stop)
+        return;
+    const uint32_t n = min(cnt, cap);
+    for (uint32_t base = first; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
         const bool have = i < n;
-        const uint2 e = have ? in[i] : make_uint2(0u, 0u);
-        const TableDev& t = dag.tables[e.x];
-        const uint32_t s = e.x + 2u;
-        const uint32_t* item = have ? table_item_ptr(t, e.y) : nullptr;
-        const uint32_t slab = e.y >> kSlotBits, slot = e.y & ((1u << kSlotBits) - 1u);
-        const bool chained = have && slab >= t.numBuckets;
-        // The item's words: its children and, in an overflow line, the
-        // bucket hash (as at insert), all loaded before the slot is freed.
-        const uint32_t roles = have ? t.roles[e.y] : 0u;
-        uint32_t cw[8];
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
-            cw[j] = have && j + 1u < s ? item[j + 1u] : 0u;
-        uint32_t bucket = 0;
-        if (chained) {
-            uint32_t h = 0;
-            for (uint32_t w = 0; w < s; ++w)
-                h ^= hash_part(item[w], w);
-            bucket = h % t.numBuckets;
-        }
-        // Free the slot; an overflow line's bucket is listed once for the
-        // chain pass.  The listing's atomic returns with the count loads.
-        uint32_t tbit = 0, told = 0;
-        if (chained) {
-            tbit = 1u << (bucket & 31u);
-            told = atomicOr(&a.touched[e.x][bucket >> 5], tbit);
-        }
-        if (have) {
-            atomicAnd(&t.lines[(size_t)slab * kLineWords + (slot >> 2)], ~(0xFFu << (8u * (slot & 3u))));
-            t.roles[e.y] = 0u;
-        }
-        bool listed = false;
-        // Children, all of one role at once (a node has at most 8): their
-        // count loads, then the decrements, then one append per wave.  A
-        // role no lane of the wave holds is skipped.
-#pragma unroll
-        for (uint32_t rb = 0; rb < 2; ++rb) {
-            const uint8_t role = rb == 0 ? kRoleParentOfNodes : kRoleParentOfLeaves;
-            const bool lf = role == kRoleParentOfLeaves;
-            const bool mine = have && (roles & role);
-            if (__ballot(mine) == 0ull)
-                continue;
-            uint2 ce[8];
-            uint32_t* cp[8];
-            uint32_t cv[8];
-#pragma unroll
-            for (uint32_t j = 0; j < 8; ++j) {
-                cv[j] = kStickyRefs;
-                if (mine && j + 1u < s) {
-                    const uint32_t ch = cw[j];
-                    const uint32_t cs = lf ? leaf_handle_size(dag, ch) : node_handle_size(ch);
-                    ce[j] = make_uint2(cs - 2u, lf ? leaf_handle_ptr(dag, ch) : node_handle_ptr(ch));
-                    cp[j] = &dag.tables[ce[j].x].refs[ce[j].y];
-                    cv[j] = ld_filter(cp[j]);
-                }
-            }
-            if (!listed) {
-                // The bucket listing rides on this round trip.
-                listed = true;
-                const bool list = chained && (told & tbit) == 0u;
-                const uint32_t at = wave_append(&ctr->touched, list);
-                if (list) {
-                    if (at < cap)
-                        touchedList[at] = make_uint2(e.x, bucket);
-                    else
-                        atomicOr(&ctr->overflow, 1u);
-                }
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < 8; ++j)
-                if (cv[j] < kStickyRefs)
-                    cv[j] = atomicSub(cp[j], 1u);
-            uint32_t die = 0;
-#pragma unroll
-            for (uint32_t j = 0; j < 8; ++j) {
-                if (mine && j + 1u < s) {
-                    if (cv[j] == 0u)
-                        atomicOr(&ctr->overflow, 1u); // counts off: the host falls back to marking
-                    if (cv[j] == 1u && !(ce[j].x == rootTable && ce[j].y == rootPtr))
-                        die |= 1u << j;
-                }
-            }
-            // Lanes' dying children go to consecutive slots of the round's
-            // list: a wave scan of the counts, one atomic per wave.
-            const uint32_t cnt = (uint32_t)__builtin_popcount(die);
-            uint32_t incl = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t v = (uint32_t)__shfl_up((int)incl, (unsigned)d, 64);
-                if (__lane_id() >= (uint32_t)d)
-                    incl += v;
-            }
-            const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
-            uint32_t base0 = 0;
-            if (total != 0u) {
-                if (__lane_id() == 63u)
-                    base0 = atomicAdd(outCount, total);
-                base0 = (uint32_t)__shfl((int)base0, 63, 64);
-            }
-            uint32_t at = base0 + incl - cnt;
-#pragma unroll
-            for (uint32_t j = 0; j < 8; ++j) {
-                if ((die >> j) & 1u) {
-                    if (at < cap)
-                        out[at] = ce[j];
-                    else
-                        atomicOr(&ctr->overflow, 1u);
-                    ++at;
-                }
-            }
-        }
-        if (!listed) {
-            // A wave whose items have no counted children (leaves).
-            const bool list = chained && (told & tbit) == 0u;
-            const uint32_t at = wave_append(&ctr->touched, list);
-            if (list) {
-                if (at < cap)
-                    touchedList[at] = make_uint2(e.x, bucket);
-                else
-                    atomicOr(&ctr->overflow, 1u);
-            }
-        }
+        const uint2 e = have ? (base == first ? spec : in[i]) : make_uint2(0u, 0u);
+        const ReleaseItem v = rc_load(dag, have, e);
+        rc_release_loaded(dag, a, rootTable, rootPtr, have, e, v, out, outCount, cap, touchedList, ctr);
     }
 }
 
 // The listed buckets of table blockIdx.y: unlink the overflow lines left
 // without items and push them on the free stack (as gc_sweep_chains).
 __global__ void __launch_bounds__(256) rc_chains(DagDev dag, GcTables a, const uint2* list, uint32_t cap,
-                                                 uint32_t rounds, GcCounters* ctr)
+                                                 uint32_t rounds, GcCounters* ctr, GcCounters* next, uint32_t* fail)
 {
     // The release rounds have all run and each freed its whole input list:
     // the stored-item count loses the lists' sizes (once, by the first
@@ -598,6 +685,16 @@ __global__ void __launch_bounds__(256) rc_chains(DagDev dag, GcTables a, const u
         ctr->itemsFreed = freed;
         *dag.stored -= freed;
         ctr->itemsLive = *dag.stored;
+        // An edit stream's collection that stopped short: the host's next
+        // collection marks.
+        if (fail && (ctr->overflow || ctr->count[rounds] != 0u))
+            *fail = 1u;
+    }
+    // The next counting collection's counters start at zero.
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        uint32_t* words = reinterpret_cast<uint32_t*>(next);
+        for (uint32_t k = threadIdx.x; k < (uint32_t)(sizeof(GcCounters) / 4); k += blockDim.x)
+            words[k] = 0u;
     }
     const uint32_t ti = blockIdx.y;
     const TableDev& t = a.t[ti];
@@ -732,7 +829,7 @@ static int gc_mark_sweep(svdag_ctx* ctx, svdag_gc_stats* stats)
     // least 4096: the seed writes the root and one node per material.
     if (const char* e = std::getenv("SVDAG_GC_FRONTIER_ENTRIES"))
         cap = std::min<uint64_t>(std::max<uint64_t>(std::strtoull(e, nullptr, 10), 4096), 1ull << 31);
-    GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
+    GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters) + 2; // its own block
     const size_t nt = ctx->ts.tables.size();
     uint32_t maxCap = 1, maxBuckets = 1;
     for (size_t i = 0; i < nt; ++i) {
@@ -830,6 +927,73 @@ int gc_settle(svdag_ctx* ctx)
     return SVDAG_OK;
 }
 
+// Buffers a counting collection needs, allocated before an edit stream
+// enqueues its collections.
+int gc_stream_prepare(svdag_ctx* ctx)
+{
+    if (ctx->ts.tables.size() > kGcMaxTables) {
+        set_error("too many item tables for the collection");
+        return SVDAG_E_INVALID;
+    }
+    if (!ctx->gcCounters) {
+        SVDAG_HIP(hipMalloc(&ctx->gcCounters, 3 * sizeof(GcCounters)));
+        SVDAG_HIP(hipMemsetAsync(ctx->gcCounters, 0, 3 * sizeof(GcCounters), ctx->stream));
+    }
+    SVDAG_TRY(gc_masks(ctx));
+    return gc_fronts(ctx, std::max<uint64_t>(ctx->gcCap, 1ull << 22));
+}
+
+// Enqueues the counting collection behind edits [first, first + count) of
+// an edit stream (rc_release_batch): held are the roots the host held before
+// the stream, for its first collection (empty for later ones), and
+// held.size() + count <= 256.  *fail is set if the collection stops short.
+// ctrOut: the counter block it uses.
+int gc_enqueue_stream(svdag_ctx* ctx, uint32_t first, uint32_t count, const std::vector<uint32_t>& held,
+                      uint32_t* fail, void** ctrOut, hipStream_t st)
+{
+    const DagDev& dag = ctx->dag;
+    GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters) + ctx->gcCtrIdx;
+    GcCounters* next = reinterpret_cast<GcCounters*>(ctx->gcCounters) + (ctx->gcCtrIdx ^ 1u);
+    ctx->gcCtrIdx ^= 1u;
+    *ctrOut = ctr;
+    const size_t nt = ctx->ts.tables.size();
+    SeedInline inl {};
+    std::copy(held.begin(), held.end(), inl.h);
+    GcTables a = gc_tables(ctx);
+    const uint32_t cap2 = (uint32_t)std::min<uint64_t>(ctx->gcCap / 2, 0xFFFFFFFFull);
+    uint2* w[2] = { reinterpret_cast<uint2*>(ctx->gcFront[0]), reinterpret_cast<uint2*>(ctx->gcFront[1]) };
+    const uint32_t rounds = dag.leafLevel + 2u;
+    const double roots = (double)(count + held.size());
+    auto grid = [&](uint32_t r) {
+        if (r >= ctx->gcPerRoot.size())
+            return 256u;
+        const double want = 2.0 * ctx->gcPerRoot[r] * roots;
+        return (uint32_t)std::min(256.0, std::max(1.0, std::ceil(want / 256.0)));
+    };
+    rc_release_batch<<<1, 256, 0, st>>>(dag, a, ctx->dStatusArr, first, count, inl, (uint32_t)held.size(), ctx->dRoot,
+                                        ctx->ws.poison, w[1], &ctr->count[1], cap2, ctx->dGcTouchedList, ctr);
+    for (uint32_t r = 1; r < rounds; ++r)
+        rc_release<<<grid(r), 256, 0, st>>>(dag, a, ~0u, ~0u, ctx->dRoot, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
+                                             &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
+    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, rounds, ctr, next, fail);
+    SVDAG_HIP(hipGetLastError());
+    return SVDAG_OK;
+}
+
+// The counters of an edit stream's last collection, copied to the host:
+// round sizes for the grids of later collections, and the live count.
+void gc_stream_take(svdag_ctx* ctx, const void* counters, uint32_t roots)
+{
+    const GcCounters& h = *static_cast<const GcCounters*>(counters);
+    const uint32_t rounds = ctx->dag.leafLevel + 2u;
+    gc_take_rounds(ctx, h.count, rounds, roots);
+    if (ctx->liveAtMark == 0)
+        ctx->liveAtMark = h.itemsLive;
+    ctx->liveAfterGc = h.itemsLive;
+}
+
+size_t gc_counters_bytes() { return sizeof(GcCounters); }
+
 // wait = false: the collection is only enqueued (deferred = true); its
 // counters come back with the stream and gc_settle takes them in.
 static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done, bool wait, bool& deferred)
@@ -838,7 +1002,11 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done, bool wa
     deferred = false;
     const hipStream_t st = ctx->stream;
     const DagDev& dag = ctx->dag;
-    GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters);
+    // Two counter blocks in turn: this collection's chain pass clears the
+    // other one for the next collection.
+    GcCounters* ctr = reinterpret_cast<GcCounters*>(ctx->gcCounters) + ctx->gcCtrIdx;
+    GcCounters* next = reinterpret_cast<GcCounters*>(ctx->gcCounters) + (ctx->gcCtrIdx ^ 1u);
+    ctx->gcCtrIdx ^= 1u;
     const size_t nt = ctx->ts.tables.size();
     SVDAG_TRY(gc_fronts(ctx, std::max<uint64_t>(ctx->gcCap, 1ull << 22)));
     // Released roots: every root held since the last collection, but the
@@ -876,22 +1044,23 @@ static int gc_release(svdag_ctx* ctx, svdag_gc_stats* stats, bool& done, bool wa
         SVDAG_HIP(hipMemcpyAsync(ctx->dGcCand, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, st));
         candDev = ctx->dGcCand;
     }
-    rc_seed<<<1, 256, 0, st>>>(dag, candDev, inl, (uint32_t)cand.size(), w[0], cap2, ctr);
     // Each round's grid from recent collections' items per released root in
     // that round, times this collection's released roots, doubled: the
     // top rounds hold a few items per root, and a launch of 256 blocks for
     // them costs more than their work.  The grid-stride loop covers any
     // count, so the sizing only affects speed.
-    for (uint32_t r = 0; r < rounds; ++r) {
-        uint32_t grid = 256;
-        if (r < ctx->gcPerRoot.size() && !cand.empty()) {
-            const double want = 2.0 * ctx->gcPerRoot[r] * (double)cand.size();
-            grid = (uint32_t)std::min(256.0, std::max(1.0, std::ceil(want / 256.0)));
-        }
-        rc_release<<<grid, 256, 0, st>>>(dag, a, rootTable, rootPtr, w[r & 1], &ctr->count[r], w[(r + 1) & 1],
-                                          &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
-    }
-    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, rounds, ctr);
+    auto grid = [&](uint32_t r) {
+        if (r >= ctx->gcPerRoot.size() || cand.empty())
+            return 256u;
+        const double want = 2.0 * ctx->gcPerRoot[r] * (double)cand.size();
+        return (uint32_t)std::min(256.0, std::max(1.0, std::ceil(want / 256.0)));
+    };
+    rc_release_roots<<<grid(0), 256, 0, st>>>(dag, a, rootTable, rootPtr, candDev, inl, (uint32_t)cand.size(), w[1],
+                                              &ctr->count[1], cap2, ctx->dGcTouchedList, ctr);
+    for (uint32_t r = 1; r < rounds; ++r)
+        rc_release<<<grid(r), 256, 0, st>>>(dag, a, rootTable, rootPtr, nullptr, w[r & 1], &ctr->count[r],
+                                             w[(r + 1) & 1], &ctr->count[r + 1], cap2, ctx->dGcTouchedList, ctr);
+    rc_chains<<<dim3(256, (uint32_t)nt), 256, 0, st>>>(dag, a, ctx->dGcTouchedList, cap2, rounds, ctr, next, nullptr);
     SVDAG_HIP(hipEventRecord(ctx->ev1, st));
     if (!wait) {
         if (!ctx->hGcCtr)
@@ -941,8 +1110,11 @@ int gc_run(svdag_ctx* ctx, svdag_gc_stats* stats, bool mark, bool wait)
     // and occlusion passes need a new primary trace afterwards.
     ctx->traceW = ctx->traceH = 0;
     ++ctx->gcSerial;
-    if (!ctx->gcCounters)
-        SVDAG_HIP(hipMalloc(&ctx->gcCounters, sizeof(GcCounters)));
+    if (!ctx->gcCounters) {
+        // Two blocks the counting collections use in turn, one for marking.
+        SVDAG_HIP(hipMalloc(&ctx->gcCounters, 3 * sizeof(GcCounters)));
+        SVDAG_HIP(hipMemsetAsync(ctx->gcCounters, 0, 3 * sizeof(GcCounters), ctx->stream));
+    }
     if (ctx->ts.tables.size() > kGcMaxTables) {
         set_error("too many item tables for the collection");
         return SVDAG_E_INVALID;

@@ -124,7 +124,7 @@ EXPORTED = [
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_build_terrain_banded", "svdag_terrain_heights", "svdag_root",
-    "svdag_edit", "svdag_edit_batch", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
+    "svdag_edit", "svdag_edit_batch", "svdag_edit_stream", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
     "svdag_query_voxels_noalloc", "svdag_edit_on_stream", "svdag_autotune_all",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
@@ -362,6 +362,16 @@ class MyGPUHashDAG:
         stats = (EditStats * n)()
         ms = ctypes.c_float(0)
         check(lib().svdag_edit_batch(self.h, descs, n, stats, ctypes.byref(ms)))
+        return list(stats), ms.value
+
+    def edit_stream(self, edits, batch=16, collect=True):
+        """edits in batches of `batch`, each followed by a collection when collect,
+        enqueued back to back (svdag_edit_stream); returns (stats list, gpu ms)."""
+        n = len(edits)
+        descs = self._descs(edits)
+        stats = (EditStats * n)()
+        ms = ctypes.c_float(0)
+        check(lib().svdag_edit_stream(self.h, descs, n, batch, 1 if collect else 0, stats, ctypes.byref(ms)))
         return list(stats), ms.value
 
     def autotune(self, sample):

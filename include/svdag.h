@@ -222,6 +222,22 @@ int svdag_edit_on_stream(svdag_ctx* ctx, const svdag_edit_desc* edit, svdag_edit
  * my_gpu_hash_dag_edits.cu:526-626).  stats: n entries or NULL. */
 int svdag_edit_batch(svdag_ctx* ctx, const svdag_edit_desc* edits, uint32_t n, svdag_edit_stats* stats, float* gpu_ms);
 
+/* An edit stream: edits [0, n) in batches of `batch`, each batch followed by
+ * a counting collection (svdag_gc) when collect != 0, enqueued back to back
+ * with one wait at the end, so the host plans the next batches while the
+ * device runs the first.  Same DAG as svdag_edit_batch per batch (and
+ * svdag_gc after each when collect).  With collect the history is cleared
+ * (every batch ends in a collection); without, each edit leaves its undo
+ * frame.  An edit that cannot complete on the device (a workspace or arena
+ * that must grow) stops the stream's collections there, and the rest runs
+ * batch by batch.  No reference counterpart: the reference's edit loop
+ * (editMyHashDag then garbageCollect, my_gpu_hash_dag_edits.cu:633-671,
+ * my_gpu_base_dag.h:66-77) waits for every step.  Refused while a server
+ * runs on the context.  stats: n entries or NULL; gpu_ms: the stream's
+ * device time. */
+int svdag_edit_stream(svdag_ctx* ctx, const svdag_edit_desc* edits, uint32_t n, uint32_t batch, uint32_t collect,
+                      svdag_edit_stats* stats, float* gpu_ms);
+
 /* How svdag_edit_batch schedules a batch (no reference counterpart; the
  * result is the same in every mode):
  *   SVDAG_BATCH_SEQUENTIAL  each edit's kernels one after another;

@@ -858,6 +858,96 @@ def test_reference_count_collection_frees_what_marking_frees():
     g.close()
 
 
+def _fusable_stream(levels, n, seed):
+    """Sphere / box / paint edits small enough for fused waves (split level
+    at least 2), near the terrain surface."""
+    tp = workloads.terrain_params(levels)
+    side = 1 << levels
+    heights = oracle_py.terrain_heights(tp["seed"], tp["base"], tp["amplitude"], tp["frequency"], tp["octaves"],
+                                        0, 0, side, side)
+
+    def h(x, z):
+        return float(heights[z, x])
+
+    edits = workloads.surface_edits(h, n, seed=seed, levels=levels, rmin=2.0, rmax=side / 32, near=side / 16)
+    rng = np.random.default_rng(seed)
+    for _ in range(n // 8):
+        x, z = (int(v) for v in rng.integers(0, side, 2))
+        edits.insert(int(rng.integers(0, len(edits))),
+                     (svdag_py.PAINT, int(rng.integers(1, 15)), (float(x), h(x, z), float(z)),
+                      float(rng.uniform(3, side / 32))))
+    return edits
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("collect,mode", [(True, 1), (False, 1), (True, 2)])
+def test_edit_stream_matches_batches_and_oracle(collect, mode):
+    # svdag_edit_stream enqueues every batch (and its collection) before
+    # waiting; in fused mode each collection runs beside the next batch's
+    # SVO build.  The DAG, the per-edit results and what is stored afterwards
+    # equal svdag_edit_batch per batch with svdag_gc after each, and the
+    # oracle's DAG.  The last batch is short (81 edits in batches of 8).
+    levels = L10
+    edits = _fusable_stream(levels, 72, seed=23)
+    g, o = _pair(levels=levels)
+    w, _ = _pair(levels=levels)
+    g.set_batch_mode(mode)
+    g.garbage_collect()
+    w.garbage_collect()
+    st, ms = g.edit_stream(edits, batch=8, collect=collect)
+    sw = []
+    for b in range(0, len(edits), 8):
+        sw += w.edit_batch(edits[b:b + 8])[0]
+        if collect:
+            w.garbage_collect()
+    for e in edits:
+        o.edit(*e)
+    _same(g, o)
+    assert ms > 0
+    assert [s.applied for s in st] == [s.applied for s in sw]
+    assert [list(s.svo_items) for s in st] == [list(s.svo_items) for s in sw]
+    if collect:
+        assert g.history() == (0, 0)
+        live, lw = g.garbage_collect(), w.garbage_collect()
+        print(f"[case] live after stream {live.items_live}, reachable {_stored_items(g)}; "
+              f"per batch {lw.items_live}, reachable {_stored_items(w)}", flush=True)
+        # The stream's collections released every root it replaced, the
+        # last (short) batch's included: nothing is left to free, and each
+        # context's stored items are exactly what its roots reach.
+        assert live.items_freed == 0
+        assert live.marked == 0 and live.items_live == _stored_items(g)
+        assert lw.items_live == _stored_items(w)
+        check = g.garbage_collect(mark=True)
+        assert check.items_freed == 0 and check.items_live == live.items_live
+    else:
+        assert g.history() == w.history()
+    g.close()
+    w.close()
+
+
+@pytest.mark.gpu
+def test_edit_stream_recovers_when_an_arena_runs_out():
+    # Arenas far too small: an edit of the pipelined stream runs out on the
+    # device, the stream's later collections release nothing, and the rest
+    # runs batch by batch after the arenas grow.  The DAG equals the
+    # oracle's, and the next collection marks and frees what the abandoned
+    # part inserted.
+    levels = L10
+    edits = _fusable_stream(levels, 96, seed=29)
+    g, o = _pair(levels=levels, expected_items=20000, arena_slabs=64)
+    g.garbage_collect()
+    cap0 = g.memory_usage().slab_capacity
+    g.edit_stream(edits, batch=8, collect=True)
+    assert g.memory_usage().slab_capacity > cap0
+    for e in edits:
+        o.edit(*e)
+    _same(g, o)
+    st = g.garbage_collect()
+    assert st.items_live == _stored_items(g)
+    check = g.garbage_collect(mark=True)
+    assert check.items_freed == 0 and check.items_live == st.items_live
+    g.close()
+
 @pytest.mark.gpu
 def test_deferred_collections_free_what_waited_ones_free():
     # svdag_gc without stats only enqueues the counting collection.  A
