@@ -700,6 +700,14 @@ def editing_profile(args):
     g.garbage_collect()
     rows = []
     marked = args.marked_batch
+    if args.profile_stream:
+        # The whole stream in one svdag_edit_stream call, a collection after
+        # every batch (each beside the next batch's SVO build).
+        _, ms = g.edit_stream(edits, batch=16, collect=True)
+        log(f"[profile] edit stream with a collection after each batch: {ms:.3f} ms")
+        g.close()
+        print(json.dumps({"edit_stream_gpu_ms": round(ms, 3), "lanes_per_item": tune_g}), flush=True)
+        return
     for b in range(0, len(edits), 16):
         if b // 16 == marked:
             # Separator launches around the batch (a 1 MiB stream read, kernel
@@ -893,6 +901,8 @@ def main():
     ap.add_argument("--batch-mode", type=int, default=1, help="editing profile: 1 fused, 2 fused + HIP graphs")
     ap.add_argument("--profile-gc-each", action="store_true",
                     help="editing profile: a collection (not waited for) after every batch")
+    ap.add_argument("--profile-stream", action="store_true",
+                    help="editing profile: the stream as one svdag_edit_stream call, a collection after each batch")
     ap.add_argument("--marked-batch", type=int, default=-1,
                     help="editing profile: bracket this batch with probe_stream launches (counter runs)")
     args = ap.parse_args()
