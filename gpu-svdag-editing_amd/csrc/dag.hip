@@ -989,22 +989,25 @@ int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, con
 }
 
 int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems,
-                        uint32_t kCap, hipEvent_t mergeAfter)
+                        uint32_t kCap, hipEvent_t mergeAfter, int part)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
-    for (uint32_t L = S; L < dag.leafLevel; ++L)
-        svo_inner_level<true><<<grid_for(std::min<uint64_t>(levelItems[L], ws.capWork), 32, kCap), 256, 0, st>>>(
-            ws, dag, eds, L);
-    svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf), 4 * kLeavesPerWave, kCap),
-                           256, 0, st>>>(
-        ws, dag, eds);
+    if (part != 2) {
+        for (uint32_t L = S; L < dag.leafLevel; ++L)
+            svo_inner_level<true><<<grid_for(std::min<uint64_t>(levelItems[L], ws.capWork), 32, kCap), 256, 0, st>>>(
+                ws, dag, eds, L);
+        svo_leaf_level<true><<<grid_for(std::min<uint64_t>(levelItems[dag.leafLevel], ws.capLeaf),
+                                        4 * kLeavesPerWave, kCap),
+                               256, 0, st>>>(ws, dag, eds);
+    }
     // The build above only reads items the root reaches; the merge below
     // finds and inserts, so it waits for whatever mergeAfter marks.
     if (mergeAfter)
         SVDAG_HIP(hipStreamWaitEvent(st, mergeAfter, 0));
-    for (int L = (int)dag.leafLevel; L >= (int)S; --L)
-        launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork), kCap);
+    if (part != 1)
+        for (int L = (int)dag.leafLevel; L >= (int)S; --L)
+            launch_merge_level<true>(ctx, ws, (uint32_t)L, std::min<uint64_t>(levelItems[L], ws.capWork), kCap);
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
@@ -1858,14 +1861,14 @@ static int run_edit_stream(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, uin
     SVDAG_HIP(hipMemsetAsync(ctx->dWaveIdx, 0, 8, st));
     SVDAG_HIP(hipMemsetAsync(ctx->ws.poison, 0, 4, st));
     SVDAG_HIP(hipMemsetAsync(ctx->dStreamFail, 0, 4, st));
-    // In fused mode each batch's collection runs on gcStream beside the next
-    // batch's first SVO build, which only reads items the root reaches
-    // (a collection frees only items no root reaches); that wave's merge,
-    // which finds and inserts, waits for the collection.  HIP-graph waves
-    // are replayed whole, so their collections stay on the context's stream.
+    // Each batch's collection runs on gcStream beside the next batch's first
+    // SVO build, which only reads items the root reaches (a collection frees
+    // only items no root reaches); that wave's merge, which finds and
+    // inserts, waits for the collection.  (HIP-graph waves are replayed as
+    // two graphs, the build and the merge, with the wait between them.)
     // Each batch's two events are its own: recording an event again while
     // its last record is still pending can hold the host until it completes.
-    const bool overlap = collect && ctx->batchMode == SVDAG_BATCH_FUSED;
+    const bool overlap = collect && ctx->batchMode >= SVDAG_BATCH_FUSED;
     if (overlap) {
         if (!ctx->gcStream)
             SVDAG_HIP(hipStreamCreateWithFlags(&ctx->gcStream, hipStreamNonBlocking));

@@ -246,7 +246,8 @@ struct svdag_ctx {
     struct WaveGraph {
         uint32_t S, groupSize;
         uint64_t generation;
-        hipGraphExec_t exec;
+        hipGraphExec_t exec;  // the wave's SVO build (from wave_begin)
+        hipGraphExec_t merge; // its merge and spine, replayed after exec
     };
     std::vector<WaveGraph> waveGraphs;
     uint64_t bufferGeneration = 0;
@@ -385,8 +386,9 @@ uint64_t level_bound(const svdag_ctx* ctx, const Bounds& b, uint32_t L);
 int launch_edit(svdag_ctx* ctx, const EditorDev& ed, const EditorDev* edDev, const WsDev& ws);
 // Levels S .. leaf of a fused wave: SVO build and merge with per-item editors
 // (dag.hip).  levelItems[L] bounds the items of level L (grid sizing only).
+// part: 0 the whole sequence, 1 the SVO build only, 2 the merge only.
 int launch_joint_levels(svdag_ctx* ctx, const WsDev& ws, const EditorDev* eds, uint32_t S, const uint64_t* levelItems,
-                        uint32_t gridCap = 8192, hipEvent_t mergeAfter = nullptr);
+                        uint32_t gridCap = 8192, hipEvent_t mergeAfter = nullptr, int part = 0);
 
 // Fused multi-edit waves (fused.hip).
 struct BatchStep {
@@ -399,7 +401,7 @@ struct BatchStep {
 uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32_t n, std::vector<BatchStep>& steps,
                     const std::vector<uint8_t>& alone, uint32_t waveBase = 0);
 // mergeAfter: the wave's merge kernels wait for this event (its SVO build
-// does not); fused mode only.
+// does not).
 int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S,
                 hipEvent_t mergeAfter = nullptr);
 void release_wave_graphs(svdag_ctx* ctx);

@@ -1004,8 +1004,11 @@ uint32_t plan_batch(svdag_ctx* ctx, const EditorDev* eds, uint32_t first, uint32
 // The wave's launch sequence.  levelItems == nullptr: fixed grids (every
 // kernel walks its device-side count with a grid-stride loop), as captured
 // into a graph that serves any wave of the same split level.
+// part: 0 the whole wave, 1 its SVO build (wave_begin .. leaf level), 2
+// its merge and spine.  The build only reads items the root reaches; the
+// merge, which finds and inserts, waits for mergeAfter when given.
 static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, uint32_t seedLanes,
-                        hipEvent_t mergeAfter = nullptr)
+                        hipEvent_t mergeAfter = nullptr, int part = 0)
 {
     const DagDev& dag = ctx->dag;
     const hipStream_t st = ctx->stream;
@@ -1013,46 +1016,42 @@ static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, 
     ws.edCount = ctx->dEdCount;
     ws.edFull = ctx->dEdFull;
     const FusedDev f { ctx->dWaves, ctx->dWaveIdx, ctx->dSeeds, ctx->dSeedWork, S, ctx->dWaveIdx + 1, ctx->dSpine };
-    wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
-    fused_seed<<<(seedLanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
+    if (part != 2) {
+        wave_begin<<<1, 256, 0, st>>>(ws, f, ctx->dStatusArr);
+        fused_seed<<<(seedLanes + 255) / 256, 256, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot);
+    }
     if (levelItems) {
-        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems, ctx->maxBlocks, mergeAfter));
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems, ctx->maxBlocks, mergeAfter, part));
     } else {
         uint64_t fixed[kMaxLevels];
         for (uint32_t L = 0; L < kMaxLevels; ++L)
             fixed[L] = ~0ull;
-        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, fixed, 2048));
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, fixed, 2048, mergeAfter, part));
     }
-    fused_spine_par<<<1, kParThreads, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
-    fused_spine<<<1, 64, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
+    if (part != 1) {
+        fused_spine_par<<<1, kParThreads, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr,
+                                                   ctx->dLevelArr);
+        fused_spine<<<1, 64, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr, ctx->dLevelArr);
+    }
     SVDAG_HIP(hipGetLastError());
     return SVDAG_OK;
 }
 
 void release_wave_graphs(svdag_ctx* ctx)
 {
-    for (auto& g : ctx->waveGraphs)
+    for (auto& g : ctx->waveGraphs) {
         (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphExecDestroy(g.merge);
+    }
     ctx->waveGraphs.clear();
 }
 
-static int wave_graph(svdag_ctx* ctx, uint32_t S, hipGraphExec_t& out)
+// One part of the wave's launch sequence captured and instantiated.
+static int capture_wave_part(svdag_ctx* ctx, uint32_t S, int part, hipGraphExec_t& out)
 {
-    for (auto it = ctx->waveGraphs.begin(); it != ctx->waveGraphs.end();) {
-        if (it->generation != ctx->bufferGeneration) {
-            (void)hipGraphExecDestroy(it->exec);
-            it = ctx->waveGraphs.erase(it);
-            continue;
-        }
-        if (it->S == S && it->groupSize == ctx->groupSize) {
-            out = it->exec;
-            return SVDAG_OK;
-        }
-        ++it;
-    }
     hipGraph_t graph = nullptr;
     SVDAG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_wave(ctx, S, nullptr, 8u * kMaxWave);
+    const int rc = enqueue_wave(ctx, S, nullptr, 8u * kMaxWave, nullptr, part);
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (rc != SVDAG_OK) {
         if (graph)
@@ -1060,21 +1059,49 @@ static int wave_graph(svdag_ctx* ctx, uint32_t S, hipGraphExec_t& out)
         return rc;
     }
     SVDAG_HIP(e);
-    hipGraphExec_t exec = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    const hipError_t ie = hipGraphInstantiate(&out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     SVDAG_HIP(ie);
-    ctx->waveGraphs.push_back({ S, ctx->groupSize, ctx->bufferGeneration, exec });
-    out = exec;
+    return SVDAG_OK;
+}
+
+// The wave's graphs for split level S: its SVO build and its merge, replayed
+// one after the other (so that the merge can wait for an event in between).
+static int wave_graph(svdag_ctx* ctx, uint32_t S, hipGraphExec_t& build, hipGraphExec_t& merge)
+{
+    for (auto it = ctx->waveGraphs.begin(); it != ctx->waveGraphs.end();) {
+        if (it->generation != ctx->bufferGeneration) {
+            (void)hipGraphExecDestroy(it->exec);
+            (void)hipGraphExecDestroy(it->merge);
+            it = ctx->waveGraphs.erase(it);
+            continue;
+        }
+        if (it->S == S && it->groupSize == ctx->groupSize) {
+            build = it->exec;
+            merge = it->merge;
+            return SVDAG_OK;
+        }
+        ++it;
+    }
+    SVDAG_TRY(capture_wave_part(ctx, S, 1, build));
+    const int rc = capture_wave_part(ctx, S, 2, merge);
+    if (rc != SVDAG_OK) {
+        (void)hipGraphExecDestroy(build);
+        return rc;
+    }
+    ctx->waveGraphs.push_back({ S, ctx->groupSize, ctx->bufferGeneration, build, merge });
     return SVDAG_OK;
 }
 
 int launch_wave(svdag_ctx* ctx, const EditorDev* hEds, const BatchStep& step, uint32_t S, hipEvent_t mergeAfter)
 {
     if (ctx->batchMode == SVDAG_BATCH_FUSED_GRAPH) {
-        hipGraphExec_t exec = nullptr;
-        SVDAG_TRY(wave_graph(ctx, S, exec));
-        SVDAG_HIP(hipGraphLaunch(exec, ctx->stream));
+        hipGraphExec_t build = nullptr, merge = nullptr;
+        SVDAG_TRY(wave_graph(ctx, S, build, merge));
+        SVDAG_HIP(hipGraphLaunch(build, ctx->stream));
+        if (mergeAfter)
+            SVDAG_HIP(hipStreamWaitEvent(ctx->stream, mergeAfter, 0));
+        SVDAG_HIP(hipGraphLaunch(merge, ctx->stream));
         return SVDAG_OK;
     }
     uint64_t levelItems[kMaxLevels] {};
