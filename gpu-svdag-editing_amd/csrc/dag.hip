@@ -1336,10 +1336,13 @@ int svdag_create(const svdag_config* cfgIn, svdag_ctx** out)
     SVDAG_HIP(hipEventCreate(&ctx->ev0));
     SVDAG_HIP(hipEventCreate(&ctx->ev1));
     SVDAG_HIP(hipEventCreateWithFlags(&ctx->evWait, hipEventDisableTiming));
-    // The edit streams' collection stream and events for streams of up to 64
-    // batches, made here: creating a stream or events costs milliseconds,
-    // which an edit stream would otherwise pay on its first call.
+    // The edit streams' collection stream, result buffers and events for
+    // streams of up to 64 batches, made here: creating a stream, events or
+    // pinned memory costs milliseconds, which an edit stream would otherwise
+    // pay on its first call.
     SVDAG_HIP(hipStreamCreateWithFlags(&ctx->gcStream, hipStreamNonBlocking));
+    SVDAG_HIP(hipHostMalloc(&ctx->hStream, gc_counters_bytes() + 64));
+    SVDAG_HIP(hipMalloc(&ctx->dStreamFail, 4));
     for (int k = 0; k < 128; ++k) {
         hipEvent_t e = nullptr;
         SVDAG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1848,10 +1851,6 @@ static int run_edit_stream(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, uin
     std::copy(eds, eds + n, hEd);
     if (collect)
         SVDAG_TRY(gc_stream_prepare(ctx));
-    if (!ctx->hStream) {
-        SVDAG_HIP(hipHostMalloc(&ctx->hStream, gc_counters_bytes() + 64));
-        SVDAG_HIP(hipMalloc(&ctx->dStreamFail, 4));
-    }
     const hipStream_t st = ctx->stream;
     const SeedDesc* seeds = reinterpret_cast<const SeedDesc*>(ctx->hFused);
     const uint2* waves = reinterpret_cast<const uint2*>(seeds + ctx->batchCap);
