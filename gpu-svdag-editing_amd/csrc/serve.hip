@@ -54,6 +54,7 @@ struct Server {
     std::atomic<uint32_t> published { kSentinel };
     std::vector<double> slabsPerItem; // per table (batch thread only)
     AllocState* hAlloc = nullptr;     // pinned: every table's allocator state (batch thread only)
+    AllocState* dAlloc = nullptr;     // the same, gathered on the device
     size_t hAllocCap = 0;
     // Frame state (frame thread only, plus frameMu for launches).
     hipEvent_t fStart = nullptr, fEnd = nullptr;
@@ -105,6 +106,14 @@ struct TableRoom {
 // Every table's allocator state in one pinned buffer: all copies queued,
 // then one synchronisation (a copy and a wait per table cost about as much
 // as a small batch's kernels).
+// Every table's allocator state into one array: one launch and one copy
+// back instead of a copy per table.
+__global__ void alloc_states_gather(DagDev dag, AllocState* out)
+{
+    if (threadIdx.x < dag.numTables)
+        out[threadIdx.x] = *dag.tables[threadIdx.x].alloc;
+}
+
 static int headroom(svdag_ctx* ctx, Server* srv, double reserveFraction, TableRoom& r)
 {
     const size_t nt = ctx->ts.tables.size();
@@ -112,15 +121,15 @@ static int headroom(svdag_ctx* ctx, Server* srv, double reserveFraction, TableRo
     r.room.assign(nt, 0.0);
     if (!srv->hAlloc) {
         SVDAG_HIP(hipHostMalloc((void**)&srv->hAlloc, std::max<size_t>(nt, 1) * sizeof(AllocState)));
+        SVDAG_HIP(hipMalloc((void**)&srv->dAlloc, std::max<size_t>(nt, 1) * sizeof(AllocState)));
         srv->hAllocCap = nt;
     }
-    if (srv->hAllocCap < nt) {
+    if (srv->hAllocCap < nt || nt > 64 || ctx->dag.numTables != nt) {
         set_error("serving: more tables than the allocator-state buffer holds");
         return SVDAG_E_INVALID;
     }
-    for (size_t i = 0; i < nt; ++i)
-        SVDAG_HIP(hipMemcpyAsync(&srv->hAlloc[i], ctx->ts.tables[i].dev.alloc, sizeof(AllocState),
-                                 hipMemcpyDeviceToHost, ctx->stream));
+    alloc_states_gather<<<1, 64, 0, ctx->stream>>>(ctx->dag, srv->dAlloc);
+    SVDAG_HIP(hipMemcpyAsync(srv->hAlloc, srv->dAlloc, nt * sizeof(AllocState), hipMemcpyDeviceToHost, ctx->stream));
     SVDAG_HIP(hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < nt; ++i) {
         const HostTable& t = ctx->ts.tables[i];
@@ -318,6 +327,8 @@ static void server_release(svdag_ctx* ctx, Server* srv)
 {
     if (srv->hAlloc)
         (void)hipHostFree(srv->hAlloc);
+    if (srv->dAlloc)
+        (void)hipFree(srv->dAlloc);
     if (srv->fStart)
         (void)hipEventDestroy(srv->fStart);
     if (srv->fEnd)
