@@ -37,6 +37,12 @@ struct Request {
     double estItems;
 };
 
+// Per table: slabs in use and headroom (free - reserve * capacity).
+struct TableRoom {
+    std::vector<uint64_t> used;
+    std::vector<double> room;
+};
+
 struct Server {
     svdag_server_config cfg {};
     std::vector<svdag_camera> cameras;
@@ -56,6 +62,13 @@ struct Server {
     AllocState* hAlloc = nullptr;     // pinned: every table's allocator state (batch thread only)
     AllocState* dAlloc = nullptr;     // the same, gathered on the device
     size_t hAllocCap = 0;
+    // The headroom read after the last batch, valid for the next admission
+    // while the root, the collections run and the tables' capacity are as
+    // they were then (batch thread only).
+    TableRoom lastRoom;
+    bool lastRoomOk = false;
+    uint32_t lastRoomRoot = 0;
+    uint64_t lastRoomGc = 0, lastRoomCap = 0;
     // Frame state (frame thread only, plus frameMu for launches).
     hipEvent_t fStart = nullptr, fEnd = nullptr;
     void* dFrame = nullptr;
@@ -98,11 +111,6 @@ static int read_memory(svdag_ctx* ctx, svdag_memory& m)
     return SVDAG_OK;
 }
 
-// Per table: slabs in use and headroom (free - reserve * capacity).
-struct TableRoom {
-    std::vector<uint64_t> used;
-    std::vector<double> room;
-};
 // Every table's allocator state in one pinned buffer: all copies queued,
 // then one synchronisation (a copy and a wait per table cost about as much
 // as a small batch's kernels).
@@ -225,13 +233,22 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
         std::unique_lock<std::mutex> ctxLock(ctx->mu);
         // Admission against every table's headroom.
         TableRoom room;
+        uint64_t capSum = 0;
+        for (const HostTable& t : ctx->ts.tables)
+            capSum += t.slabCap;
+        const bool reuse = srv->lastRoomOk && srv->lastRoomRoot == ctx->root && srv->lastRoomGc == ctx->gcSerial
+                           && srv->lastRoomCap == capSum;
+        srv->lastRoomOk = false;
         if (srv->slabsPerItem.size() != ctx->ts.tables.size()) {
             // Before any measurement: one slot per item in every table.
             srv->slabsPerItem.resize(ctx->ts.tables.size());
             for (size_t i = 0; i < ctx->ts.tables.size(); ++i)
                 srv->slabsPerItem[i] = 1.0 / (double)line_slots(ctx->ts.tables[i].itemWords);
         }
-        (void)headroom(ctx, srv, reserve, room);
+        if (reuse)
+            room = srv->lastRoom;
+        else
+            (void)headroom(ctx, srv, reserve, room);
         const double need = batch.front().estItems;
         if (!fits(srv, room, need) && srv->cfg.allow_gc) {
             ctx->undo.clear();
@@ -290,6 +307,14 @@ static void batch_loop(svdag_ctx* ctx, Server* srv)
                     items += s.svo_items[L];
             TableRoom after;
             if (items > 0 && headroom(ctx, srv, reserve, after) == SVDAG_OK) {
+                uint64_t capNow = 0;
+                for (const HostTable& t : ctx->ts.tables)
+                    capNow += t.slabCap;
+                srv->lastRoom = after;
+                srv->lastRoomOk = true;
+                srv->lastRoomRoot = ctx->root;
+                srv->lastRoomGc = ctx->gcSerial;
+                srv->lastRoomCap = capNow;
                 // Slabs each table gained per item; a table that gained none
                 // keeps a small floor so a sudden demand still counts.
                 for (size_t i = 0; i < after.used.size() && i < room.used.size(); ++i) {
