@@ -1023,10 +1023,18 @@ static int enqueue_wave(svdag_ctx* ctx, uint32_t S, const uint64_t* levelItems, 
     if (levelItems) {
         SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, levelItems, ctx->maxBlocks, mergeAfter, part));
     } else {
-        uint64_t fixed[kMaxLevels];
-        for (uint32_t L = 0; L < kMaxLevels; ++L)
-            fixed[L] = ~0ull;
-        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, fixed, 2048, mergeAfter, part));
+        // Grids for any wave of this split level: a wave holds at most
+        // kMaxWave edits of at most 8 level-S cells each, and every item has
+        // at most 8 children, so level L holds at most 8 kMaxWave 8^(L-S)
+        // items (the grid-stride loops cover the counts either way).
+        uint64_t bound[kMaxLevels];
+        uint64_t b = 8ull * kMaxWave;
+        for (uint32_t L = 0; L < kMaxLevels; ++L) {
+            bound[L] = L < S ? 0ull : b;
+            if (L >= S)
+                b = std::min<uint64_t>(b * 8ull, 1ull << 40);
+        }
+        SVDAG_TRY(launch_joint_levels(ctx, ws, ctx->dEditors, S, bound, 2048, mergeAfter, part));
     }
     if (part != 1) {
         fused_spine_par<<<1, kParThreads, 0, st>>>(ws, dag, ctx->dEditors, f, ctx->dRoot, ctx->dStatusArr,
