@@ -926,6 +926,35 @@ def test_edit_stream_matches_batches_and_oracle(collect, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 2])
+def test_edit_stream_recreates_what_the_previous_collection_frees(mode):
+    # Batches that add, remove and add again the same spheres: each
+    # collection frees the nodes the next batch then builds again, while that
+    # batch's SVO build runs beside the collection and its merge waits for
+    # it.  The DAG equals the oracle's after every step, and a collection
+    # after the stream frees nothing.
+    levels = L10
+    side = 1 << levels
+    rng = np.random.default_rng(97)
+    centers = [tuple(float(v) for v in rng.uniform(100, side - 100, 3)) for _ in range(6)]
+    add = [(svdag_py.SPHERE_ADD, 3 + k, c, 20.0 + 4 * k) for k, c in enumerate(centers)]
+    rem = [(svdag_py.SPHERE_REMOVE, 0, c, 20.0 + 4 * k) for k, c in enumerate(centers)]
+    edits = add + rem + add + rem + add
+    g, o = _pair(levels=levels)
+    g.set_batch_mode(mode)
+    g.garbage_collect()
+    g.edit_stream(edits, batch=6, collect=True)
+    for e in edits:
+        o.edit(*e)
+    _same(g, o)
+    st = g.garbage_collect()
+    assert st.items_freed == 0 and st.items_live == _stored_items(g)
+    check = g.garbage_collect(mark=True)
+    assert check.items_freed == 0 and check.items_live == st.items_live
+    g.close()
+
+
+@pytest.mark.gpu
 def test_edit_stream_recovers_when_an_arena_runs_out():
     # Arenas far too small: an edit of the pipelined stream runs out on the
     # device, the stream's later collections release nothing, and the rest
