@@ -170,7 +170,7 @@ def slab_hash_bench(args):
     bytes_per_item = lines_used * 128.0 / n_items
     hs.close()
 
-    # The sizing as configs[1] writes it: ceil(n_s / 96) buckets per size
+    # The slab-hash workload's sizing as written: ceil(n_s / 96) buckets per size
     # (the reference's 96 items per 32-slot bucket).  Here a bucket is one
     # 128-byte line plus its chain, so buckets hold 96 items in long chains.
     as_written = None
@@ -637,7 +637,7 @@ def reference_format_bench(args):
     """The reference's version-3 file format at the editing workload's depth
     (terrain_params(D), 4 material bits): svdag_write_reference_dag of the
     whole DAG, then svdag_read_reference_dag of those bytes into a fresh
-    context.  Host-side work on the GPU box's cores (the writer rebuilds the
+    context.  Host-side work on the host's cores (the writer rebuilds the
     reference's 32-slot tables, the reader walks them and imports through the
     bulk insert).  Checks: both contexts give one canonical hash."""
     import svdag_py

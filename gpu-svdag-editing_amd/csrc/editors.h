@@ -12,7 +12,7 @@
 //   box     my_gpu_dag_editors.h:45-119    ([c-r, c+r), r == 0 -> [c, c+1))
 //   paint   my_gpu_dag_editors.h:498-536   (Fill only over flagged full nodes)
 //   copy    my_gpu_dag_editors.h:127-253   (translation; up to 8 source parents;
-//           quarter turns about +Y are this build's own extension, done in
+//           quarter turns about +Y have no reference counterpart; done in
 //           integer voxel arithmetic: decide_copy_rot / rot_source)
 // The terrain tool (kind 6) has no reference counterpart: it classifies nodes
 // against a min/max height pyramid so that a heightfield builds through the
