@@ -36,6 +36,19 @@ int main(int argc, char** argv)
         undoRedo.undo(dag);
         undoRedo.redo(dag);
         undoRedo.garbageCollect(dag);
+        // Two batches of two edits, each followed by a collection, as one
+        // edit stream.
+        const auto streamed = editStream(dag,
+                                         { MyGpuSphereEditor<true>({ 150 * s, 560 * s, 800 * s }, 24 * s, 5).desc(),
+                                           MyGpuSphereEditor<false>({ 820 * s, 480 * s, 150 * s }, 18 * s, 0).desc(),
+                                           MyGpuBoxEditor<true>({ 500 * s, 600 * s, 500 * s }, 12 * s, 3).desc(),
+                                           MyGpuSpherePaintEditor({ 400 * s, 510 * s, 700 * s }, 30 * s, { 0, 1, 0 }, 9)
+                                               .desc() },
+                                         2, true);
+        if (streamed.size() != 4) {
+            std::fprintf(stderr, "editStream returned %zu results\n", streamed.size());
+            return 4;
+        }
         DAGTracer tracer(128, 128);
         CameraView view;
         const double pos[3] = { -0.1 * side, 0.85 * side, -0.1 * side };

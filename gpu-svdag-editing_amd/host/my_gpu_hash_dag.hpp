@@ -219,8 +219,8 @@ class MyGPUHashDAGUndoRedo {
 public:
     void undo(MyGPUHashDAG& dag) { check(svdag_undo(dag.context())); }
     void redo(MyGPUHashDAG& dag) { check(svdag_redo(dag.context())); }
-    // Clears the history, then marks from the root and the fully-filled
-    // chain and sweeps everything else.
+    // Clears the history, then frees every stored item the root and the
+    // fully-filled chain do not reach (by reference counts; svdag_gc).
     svdag_gc_stats garbageCollect(MyGPUHashDAG& dag)
     {
         svdag_gc_stats st {};
@@ -242,6 +242,20 @@ svdag_edit_stats editMyHashDag(const Editor& editor, MyGPUHashDAG& dag, MyGPUHas
     const svdag_edit_desc d = editor.desc();
     svdag_edit_stats st {};
     check(svdag_edit(dag.context(), &d, &st));
+    return st;
+}
+
+// An edit stream (svdag_edit_stream; no reference counterpart): the
+// reference's loop of editMyHashDag and garbageCollect, over batches of
+// `batch` edits (editor.desc() of any tool), enqueued back to back with one
+// wait at the end.  collectEachBatch: a collection after every batch (the
+// history is then cleared); otherwise each edit leaves its undo frame.
+inline std::vector<svdag_edit_stats> editStream(MyGPUHashDAG& dag, const std::vector<svdag_edit_desc>& edits,
+                                                uint32_t batch, bool collectEachBatch, float* gpuMs = nullptr)
+{
+    std::vector<svdag_edit_stats> st(edits.size());
+    check(svdag_edit_stream(dag.context(), edits.data(), (uint32_t)edits.size(), batch, collectEachBatch ? 1u : 0u,
+                            st.data(), gpuMs));
     return st;
 }
 

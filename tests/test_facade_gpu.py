@@ -32,6 +32,11 @@ def test_facade_demo_matches_oracle(tmp_path):
     o.edit(workloads.BOX_REMOVE, 0, (310, 530, 410), 20)
     o.edit(workloads.PAINT, 7, (600, 500, 600), 50)
     o.edit(workloads.COPY, 0, (300, 520, 400), 30, src=(300, 520, 400), dst=(700, 530, 200))
+    # The demo's edit stream.
+    o.edit(workloads.SPHERE_ADD, 5, (150, 560, 800), 24)
+    o.edit(workloads.SPHERE_REMOVE, 0, (820, 480, 150), 18)
+    o.edit(workloads.BOX_ADD, 3, (500, 600, 500), 12)
+    o.edit(workloads.PAINT, 9, (400, 510, 700), 30)
     assert gpu_hash == o.canonical()[0]
     hits, lit, unoccluded = (int(v) for v in r.stdout.split()[1:4])
     assert hits > 0 and 0 < lit < hits and 0 < unoccluded <= hits
