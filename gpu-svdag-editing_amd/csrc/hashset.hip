@@ -765,6 +765,7 @@ struct svdag_hashset {
     TableSet ts;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t regionStart = nullptr, regionEnd = nullptr; // svdag_hashset_timer
     uint32_t* segBuf = nullptr;
     BinScratch bins;
 };
@@ -832,6 +833,8 @@ int svdag_hashset_create(uint32_t minWords, uint32_t maxWords, const uint32_t* b
     SVDAG_HIP(hipStreamCreateWithFlags(&hs->stream, hipStreamNonBlocking));
     SVDAG_HIP(hipEventCreate(&hs->ev0));
     SVDAG_HIP(hipEventCreate(&hs->ev1));
+    SVDAG_HIP(hipEventCreate(&hs->regionStart));
+    SVDAG_HIP(hipEventCreate(&hs->regionEnd));
     SVDAG_HIP(hipMalloc(&hs->segBuf, 64 * 5 * 4));
     hs->ts.minWords = minWords;
     hs->ts.maxWords = maxWords;
@@ -868,6 +871,10 @@ int svdag_hashset_destroy(svdag_hashset* hs)
     (void)hipFree(hs->segBuf);
     (void)hipEventDestroy(hs->ev0);
     (void)hipEventDestroy(hs->ev1);
+    if (hs->regionStart)
+        (void)hipEventDestroy(hs->regionStart);
+    if (hs->regionEnd)
+        (void)hipEventDestroy(hs->regionEnd);
     (void)hipStreamDestroy(hs->stream);
     delete hs;
     return SVDAG_OK;
@@ -880,6 +887,10 @@ static int hashset_bulk(svdag_hashset* hs, const uint32_t* words, const svdag_se
         return SVDAG_E_INVALID;
     SVDAG_TRY(launch_bulk(hs->ts, words, segs, nseg, out, variant, insert, hs->stream, hs->segBuf, &hs->bins,
                           ms ? hs->ev0 : nullptr, ms ? hs->ev1 : nullptr));
+    // An untimed find (not XCD-binned, which checks its bins afterwards) is
+    // only queued: later calls on the set run after it on its stream.
+    if (!ms && !insert && variant < 1000)
+        return SVDAG_OK;
     SVDAG_HIP(hipStreamSynchronize(hs->stream));
     if (ms)
         SVDAG_HIP(hipEventElapsedTime(ms, hs->ev0, hs->ev1));
@@ -914,6 +925,20 @@ int svdag_hashset_find_dev(svdag_hashset* hs, const uint32_t* words, const svdag
                            uint32_t* out, int variant, float* ms)
 {
     return hashset_bulk(hs, words, segs, nseg, out, variant, ms, false);
+}
+
+int svdag_hashset_timer(svdag_hashset* hs, uint32_t stop, float* elapsed_ms)
+{
+    if (!hs || (stop && !elapsed_ms))
+        return SVDAG_E_INVALID;
+    if (!stop) {
+        SVDAG_HIP(hipEventRecord(hs->regionStart, hs->stream));
+        return SVDAG_OK;
+    }
+    SVDAG_HIP(hipEventRecord(hs->regionEnd, hs->stream));
+    SVDAG_HIP(hipEventSynchronize(hs->regionEnd));
+    SVDAG_HIP(hipEventElapsedTime(elapsed_ms, hs->regionStart, hs->regionEnd));
+    return SVDAG_OK;
 }
 
 int svdag_hashset_read(svdag_hashset* hs, uint32_t s, const uint32_t* ptrs, uint32_t n, uint32_t* outWords)

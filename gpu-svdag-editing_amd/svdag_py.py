@@ -124,7 +124,7 @@ EXPORTED = [
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_build_terrain_banded", "svdag_terrain_heights", "svdag_root",
-    "svdag_edit", "svdag_edit_batch", "svdag_edit_stream", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
+    "svdag_edit", "svdag_edit_batch", "svdag_edit_stream", "svdag_hashset_timer", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
     "svdag_query_voxels_noalloc", "svdag_edit_on_stream", "svdag_autotune_all",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
@@ -247,8 +247,20 @@ class HashSet:
     def add_dev(self, words_dev, segs, out_dev, variant=16):
         return self._bulk(lib().svdag_hashset_add_dev, words_dev, segs, out_dev, variant)
 
-    def find_dev(self, words_dev, segs, out_dev, variant=16):
+    def find_dev(self, words_dev, segs, out_dev, variant=16, timed=True):
+        """timed=False (variants below 1000): only queue the find; returns None."""
+        if not timed:
+            check(lib().svdag_hashset_find_dev(self.h, ctypes.cast(words_dev.ptr, u32p), segs, len(segs),
+                                               ctypes.cast(out_dev.ptr, u32p), int(variant), None))
+            return None
         return self._bulk(lib().svdag_hashset_find_dev, words_dev, segs, out_dev, variant)
+
+    def timer(self, stop):
+        """svdag_hashset_timer: timer(False) marks a region's start; timer(True)
+        waits for its end and returns the device milliseconds in between."""
+        ms = ctypes.c_float(0)
+        check(lib().svdag_hashset_timer(self.h, 1 if stop else 0, ctypes.byref(ms) if stop else None))
+        return ms.value if stop else None
 
     def read(self, item_words, ptrs):
         ptrs = np.ascontiguousarray(ptrs, dtype=np.uint32)

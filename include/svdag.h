@@ -91,9 +91,17 @@ int svdag_hashset_destroy(svdag_hashset* hs);
 int svdag_hashset_add_dev(svdag_hashset* hs, const uint32_t* words_dev, const svdag_segment* segs, uint32_t nseg,
                           uint32_t* out_ptrs_dev, int variant, float* elapsed_ms);
 /* Bulk find; out_ptrs_dev[i] = pointer of a stored equal item or 0xFFFFFFFF.
- * reference: find / findAsWarp (compact_acceleration_hash_table.h:205-280).          */
+ * reference: find / findAsWarp (compact_acceleration_hash_table.h:205-280).
+ * elapsed_ms NULL (variants below 1000): the call returns once the find is
+ * queued on the set's stream, as the reference's kernel launch does; later
+ * calls on the set run after it, and svdag_hashset_timer or
+ * svdag_device_synchronize waits for it.  Otherwise the call waits.          */
 int svdag_hashset_find_dev(svdag_hashset* hs, const uint32_t* words_dev, const svdag_segment* segs, uint32_t nseg,
                            uint32_t* out_ptrs_dev, int variant, float* elapsed_ms);
+/* Device time of a region of calls on the set's stream, by HIP events (no
+ * reference counterpart): stop = 0 marks the start; stop = 1 marks the end,
+ * waits for it and returns the milliseconds in between.                      */
+int svdag_hashset_timer(svdag_hashset* hs, uint32_t stop, float* elapsed_ms);
 /* Decode pointers to item words (host out buffer n*item_words).
  * reference: decodePointer (compact_acceleration_hash_table.h:398-402).              */
 int svdag_hashset_read(svdag_hashset* hs, uint32_t item_words, const uint32_t* ptrs, uint32_t n, uint32_t* out_words);

@@ -124,6 +124,34 @@ def test_long_chains_and_duplicates(variant):
 
 
 @pytest.mark.gpu
+def test_queued_finds_equal_waited_finds():
+    # Untimed finds are only queued on the set's stream: several back to back
+    # into different result buffers, then the region timer's wait, give what
+    # a waited find gives, and the timer reports a positive device time.
+    items = workloads.slab_hash_items(1 << 18)
+    finds = workloads.slab_hash_finds(items, 1 << 18, hit_rate=0.5)
+    hs, _ = _build(items)
+    groups = sorted(items.items())
+    words, segs, n = svdag_py.make_segments(groups)
+    wdev = svdag_py.DeviceBuffer.from_array(words)
+    odev = svdag_py.DeviceBuffer(4 * n)
+    hs.add_dev(wdev, segs, odev, 16)
+    fwords, fsegs, fn = svdag_py.make_segments([(s, finds[s]) for s, _ in groups])
+    fdev = svdag_py.DeviceBuffer.from_array(fwords)
+    ref = svdag_py.DeviceBuffer(4 * fn)
+    hs.find_dev(fdev, fsegs, ref, 1)
+    outs = [svdag_py.DeviceBuffer(4 * fn) for _ in range(3)]
+    hs.timer(False)
+    for o in outs:
+        assert hs.find_dev(fdev, fsegs, o, 1, timed=False) is None
+    assert hs.timer(True) > 0
+    expect = ref.to_array(np.uint32, fn)
+    for o in outs:
+        assert np.array_equal(o.to_array(np.uint32, fn), expect)
+    hs.close()
+
+
+@pytest.mark.gpu
 def test_empty_batch_and_oom_is_reported():
     hs = svdag_py.HashSet(2, 2, [1], [0])
     usage = hs.usage(2)
