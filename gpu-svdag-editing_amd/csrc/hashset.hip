@@ -628,7 +628,10 @@ int launch_bulk(const TableSet& ts,const uint32_t* words, const svdag_segment* s
         set_error("too many items");
         return SVDAG_E_INVALID;
     }
-    SVDAG_HIP(hipMemcpyAsync(segBuf, host, nseg * 5 * 4, hipMemcpyHostToDevice, stream));
+    // The segment-uniform lane variants (1, 2, 3) take their segments as
+    // kernel arguments; the others read them from segBuf.
+    if (!(variant >= 1 && variant <= 3))
+        SVDAG_HIP(hipMemcpyAsync(segBuf, host, nseg * 5 * 4, hipMemcpyHostToDevice, stream));
     const uint32_t n = (uint32_t)total;
     if (n == 0)
         return empty();
