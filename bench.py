@@ -467,7 +467,8 @@ def editing_bench(args):
                "throughput_req_per_s": len(reqs) / span, "voxel_edits_per_s": float(vox.sum() / span),
                "batches": st.batches, "max_batch": st.max_batch_seen, "frames": st.frames,
                "frames_skipped": st.frames_skipped, "trace_ms_avg_1920x1080": round(st.trace_ms_avg, 3),
-               "gcs": st.gcs, "grows": st.grows}
+               "gcs": st.gcs, "grows": st.grows, "sequential_spine_edits": st.sequential_spine_edits,
+               "handed_back_edits": st.handed_back_edits}
         log(f"[bench] serving: {row}")
         serving.append(row)
         g.garbage_collect()

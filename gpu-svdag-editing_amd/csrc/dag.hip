@@ -1126,6 +1126,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
                 break;
             }
         for (uint32_t k = first; k < bad; ++k) {
+            ctx->seqSpineEdits += hSt[k].seqSpine ? 1u : 0u;
             root_log(ctx);
             ctx->root = hSt[k].rootAfter;
             if (stats) {
@@ -1151,6 +1152,7 @@ int run_edits(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, svdag_edit_stats
         ctx->root = s.rootBefore;
         SVDAG_HIP(hipMemcpyAsync(ctx->dRoot, &ctx->root, 4, hipMemcpyHostToDevice, ctx->stream));
         if (s.fallback) {
+            ++ctx->handedBackEdits;
             alone[bad] = 1;
             why = "fused spine fallback";
         } else if (s.overflow) {
@@ -1924,6 +1926,7 @@ static int run_edit_stream(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, uin
     // of the first edit that did not.
     const uint32_t jb = bad == n ? nb : bad / batch;
     for (uint32_t k = 0; k < bad; ++k) {
+        ctx->seqSpineEdits += hSt[k].seqSpine ? 1u : 0u;
         root_log(ctx);
         ctx->root = hSt[k].rootAfter;
         svdag_edit_stats& o = stats[k];

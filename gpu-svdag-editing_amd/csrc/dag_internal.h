@@ -28,7 +28,8 @@ struct EditStatus {
     uint32_t fallback;  // the fused pass could not take this edit: run it alone
     uint32_t build;     // a DAG build (terrain, voxel grid): merged and published
                         // even without leaves (fully-filled children only)
-    uint32_t pad[3];
+    uint32_t seqSpine;  // a fused wave's edit whose levels above S the sequential spine built
+    uint32_t pad[2];
 };
 
 // Edit workspace as seen by the kernels.
@@ -293,6 +294,10 @@ struct svdag_ctx {
     // Collections run so far: between two collections no slot is freed, so
     // a root handle names one DAG content.
     uint64_t gcSerial = 0;
+    // Fused-wave edits whose spine the one-wave sequential kernel built (the
+    // parallel spine declined the wave), and edits a wave handed back to run
+    // on their own, since the context was made.
+    uint64_t seqSpineEdits = 0, handedBackEdits = 0;
     // Items per released root in each release round of recent counting
     // collections, a decaying maximum (sizes the next collection's grids).
     std::vector<double> gcPerRoot;

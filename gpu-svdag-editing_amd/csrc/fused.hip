@@ -747,6 +747,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
             if (lane == 0) {
                 EditStatus st {};
                 st.done = 1u;
+                st.seqSpine = 1u;
                 st.rootBefore = st.rootAfter = root;
                 status[k] = st;
             }
@@ -886,6 +887,7 @@ __global__ void __launch_bounds__(64) fused_spine(WsDev ws, DagDev dag, const Ed
             EditStatus st {};
             st.done = 1u;
             st.applied = 1u;
+            st.seqSpine = 1u;
             st.rootBefore = root;
             st.rootAfter = newRoot;
             st.leafCount = leafCount;

@@ -47,6 +47,7 @@ struct Server {
     svdag_server_config cfg {};
     std::vector<svdag_camera> cameras;
     Clock::time_point t0;
+    uint64_t seqSpine0 = 0, handedBack0 = 0; // the context's counters when the server started
     std::mutex qMu;
     std::condition_variable qCv;      // new request or stop
     std::condition_variable doneCv;   // a batch finished
@@ -418,6 +419,8 @@ int svdag_server_start(svdag_ctx* ctx, const svdag_server_config* cfg)
         return rc;
     }
     srv->t0 = Clock::now();
+    srv->seqSpine0 = ctx->seqSpineEdits;
+    srv->handedBack0 = ctx->handedBackEdits;
     ctx->server = srv;
     srv->batchThread = std::thread(batch_loop, ctx, srv);
     if (cfg->frame_width)
@@ -495,6 +498,8 @@ int svdag_server_stop(svdag_ctx* ctx, svdag_server_stats* stats)
     srv->stats.wall_s = seconds_since(srv->t0);
     srv->stats.trace_ms_avg = srv->traceMsCount ? srv->traceMsSum / (double)srv->traceMsCount : 0.0;
     srv->stats.last_frame_root = srv->lastFrameRoot;
+    srv->stats.sequential_spine_edits = ctx->seqSpineEdits - srv->seqSpine0;
+    srv->stats.handed_back_edits = ctx->handedBackEdits - srv->handedBack0;
     if (stats)
         *stats = srv->stats;
     const int err = srv->error.load();

@@ -108,7 +108,8 @@ class ServerStats(ctypes.Structure):
                 ("frames_skipped", ctypes.c_uint64), ("gcs", ctypes.c_uint64), ("grows", ctypes.c_uint64),
                 ("deferred", ctypes.c_uint64), ("wall_s", ctypes.c_double), ("edit_busy_s", ctypes.c_double),
                 ("trace_ms_avg", ctypes.c_double), ("last_frame_root", ctypes.c_uint32),
-                ("max_batch_seen", ctypes.c_uint32)]
+                ("max_batch_seen", ctypes.c_uint32), ("sequential_spine_edits", ctypes.c_uint64),
+                ("handed_back_edits", ctypes.c_uint64)]
 
 
 REQUEST_DTYPE = np.dtype([("status", "<i4"), ("applied", "<u4"), ("root_after", "<u4"), ("batch", "<u4"),

@@ -529,6 +529,9 @@ typedef struct {
     double trace_ms_avg;           /* device time per frame trace */
     uint32_t last_frame_root;
     uint32_t max_batch_seen;
+    uint64_t sequential_spine_edits; /* fused-wave edits whose top levels the one-wave sequential
+                                        spine built (the parallel spine declined their wave) */
+    uint64_t handed_back_edits;      /* edits a fused wave handed back to run on their own */
 } svdag_server_stats;
 
 int svdag_server_start(svdag_ctx* ctx, const svdag_server_config* cfg);
