@@ -2279,6 +2279,24 @@ static int set_root(svdag_ctx* ctx, uint32_t r)
     return SVDAG_OK;
 }
 
+int svdag_add_frame(svdag_ctx* ctx, uint32_t root)
+{
+    if (!ctx)
+        return SVDAG_E_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const bool held = root == ctx->root || std::find(ctx->undo.begin(), ctx->undo.end(), root) != ctx->undo.end()
+                      || std::find(ctx->redo.begin(), ctx->redo.end(), root) != ctx->redo.end();
+    if (!held) {
+        set_error("svdag_add_frame: not a root this context holds");
+        return SVDAG_E_INVALID;
+    }
+    if (!ctx->undo.empty() && ctx->undo.back() == root)
+        return SVDAG_OK;
+    ctx->redo.clear();
+    ctx->undo.push_back(root);
+    return SVDAG_OK;
+}
+
 int svdag_undo(svdag_ctx* ctx)
 {
     if (!ctx)

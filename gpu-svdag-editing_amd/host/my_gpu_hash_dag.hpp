@@ -70,10 +70,14 @@ public:
     {
         o.ctx_ = nullptr;
     }
-    ~MyGPUHashDAG()
+    ~MyGPUHashDAG() { free(); }
+    // MyGPUHashDAG::free (my_gpu_hash_dag.cu:68-76): releases the device
+    // memory now; the destructor does it otherwise.  Idempotent.
+    void free()
     {
         if (ctx_)
             svdag_destroy(ctx_);
+        ctx_ = nullptr;
     }
 
     uint32_t levels() const { return levels_; }
@@ -217,6 +221,17 @@ struct MyGpuCopyEditor {
 // ------------------------------------------------------------------ undo / redo / GC
 class MyGPUHashDAGUndoRedo {
 public:
+    struct Frame {
+        uint32_t firstNodeIndex = 0;
+    };
+    // MyGPUHashDAGUndoRedo::add_frame (my_gpu_base_dag.h:23-33): the frame's
+    // root goes on the undo stack (not again if it is the last frame there)
+    // and the redo stack is cleared.  The history lives in the DAG's context,
+    // so the DAG is named.  Edits add their own frames, as editMyHashDag does.
+    void add_frame(MyGPUHashDAG& dag, const Frame& frame)
+    {
+        check(svdag_add_frame(dag.context(), frame.firstNodeIndex));
+    }
     void undo(MyGPUHashDAG& dag) { check(svdag_undo(dag.context())); }
     void redo(MyGPUHashDAG& dag) { check(svdag_redo(dag.context())); }
     // Clears the history, then frees every stored item the root and the

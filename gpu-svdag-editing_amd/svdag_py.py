@@ -125,7 +125,7 @@ EXPORTED = [
     "svdag_hashset_create", "svdag_hashset_destroy", "svdag_hashset_add_dev", "svdag_hashset_find_dev",
     "svdag_hashset_read", "svdag_hashset_usage",
     "svdag_create", "svdag_destroy", "svdag_build_terrain", "svdag_build_terrain_banded", "svdag_terrain_heights", "svdag_root",
-    "svdag_edit", "svdag_edit_batch", "svdag_edit_stream", "svdag_hashset_timer", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
+    "svdag_edit", "svdag_edit_batch", "svdag_edit_stream", "svdag_hashset_timer", "svdag_autotune", "svdag_set_batch_mode", "svdag_undo", "svdag_redo", "svdag_add_frame", "svdag_history", "svdag_gc", "svdag_gc_mark", "svdag_query_voxels",
     "svdag_query_voxels_noalloc", "svdag_edit_on_stream", "svdag_autotune_all",
     "svdag_trace_primary", "svdag_memory_usage", "svdag_server_start", "svdag_server_submit", "svdag_server_wait",
     "svdag_server_stop", "svdag_server_last_frame", "svdag_serve_replay", "svdag_export_canonical",
@@ -424,6 +424,10 @@ class MyGPUHashDAG:
         """0: edits of a batch one after another; 1: fused waves (default);
         2: fused waves replayed from captured HIP graphs."""
         check(lib().svdag_set_batch_mode(self.h, int(mode)))
+
+    def add_frame(self, root):
+        """svdag_add_frame: the root on the undo stack (not twice in a row); clears redo."""
+        check(lib().svdag_add_frame(self.h, ctypes.c_uint32(root)))
 
     def undo(self):
         check(lib().svdag_undo(self.h))

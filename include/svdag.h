@@ -306,6 +306,12 @@ int svdag_set_load_factor(svdag_ctx* ctx, uint32_t load_factor);
 int svdag_autotune_load_factor(svdag_ctx* ctx, const svdag_edit_desc* sample, uint32_t n, const uint32_t* candidates,
                                uint32_t num_candidates, uint32_t* chosen, float* ms);
 
+/* reference: MyGPUHashDAGUndoRedo::add_frame (my_gpu_base_dag.h:23-33): root
+ * goes on the undo stack unless it is the last frame there, and the redo stack
+ * is cleared.  Every edit already adds its own frame (editMyHashDag does).
+ * root must be a root this context held (the current one or one still on a
+ * stack): a collection frees what only other roots reach. */
+int svdag_add_frame(svdag_ctx* ctx, uint32_t root);
 /* reference: MyGPUHashDAGUndoRedo::undo / redo (my_gpu_base_dag.h:35-64). */
 int svdag_undo(svdag_ctx* ctx);
 int svdag_redo(svdag_ctx* ctx);

@@ -926,6 +926,29 @@ def test_edit_stream_matches_batches_and_oracle(collect, mode):
 
 
 @pytest.mark.gpu
+def test_add_frame_follows_the_reference_undo_stack():
+    # MyGPUHashDAGUndoRedo::add_frame: the frame goes on the undo stack unless
+    # it is the last one there, and redo is cleared; undo then returns to it.
+    g, _ = _pair()
+    r0 = g.root
+    g.edit(svdag_py.SPHERE_ADD, 3, (300.0, 520.0, 300.0), 30.0)
+    ra = g.root
+    assert g.history() == (1, 0)
+    g.add_frame(ra)
+    g.add_frame(ra)  # the last frame already: not pushed again
+    assert g.history() == (2, 0)
+    g.undo()
+    assert g.root == ra and g.history() == (1, 1)
+    g.undo()
+    assert g.root == r0 and g.history() == (0, 2)
+    g.add_frame(r0)  # clears redo
+    assert g.history() == (1, 0)
+    with pytest.raises(svdag_py.SvdagError):
+        g.add_frame(0x1234567)
+    g.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", [1, 2])
 def test_edit_stream_recreates_what_the_previous_collection_frees(mode):
     # Batches that add, remove and add again the same spheres: each
