@@ -926,6 +926,28 @@ def test_edit_stream_matches_batches_and_oracle(collect, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("batch", [1, 5, 64])
+def test_edit_stream_batch_sizes(batch):
+    # Batches of one edit, a short last batch, and one batch larger than the
+    # stream: the DAG equals the oracle's, every released root was collected
+    # (a collection afterwards frees nothing), and the stored items are what
+    # the root reaches.
+    levels = L10
+    edits = _fusable_stream(levels, 18, seed=61)
+    g, o = _pair(levels=levels)
+    g.garbage_collect()
+    st, _ = g.edit_stream(edits, batch=batch, collect=True)
+    assert len(st) == len(edits)
+    for e in edits:
+        o.edit(*e)
+    _same(g, o)
+    after = g.garbage_collect()
+    assert after.items_freed == 0 and after.items_live == _stored_items(g)
+    assert svdag_py.lib().svdag_edit_stream(g.h, None, 0, batch, 1, None, None) == 0
+    g.close()
+
+
+@pytest.mark.gpu
 def test_add_frame_follows_the_reference_undo_stack():
     # MyGPUHashDAGUndoRedo::add_frame: the frame goes on the undo stack unless
     # it is the last one there, and redo is cleared; undo then returns to it.
