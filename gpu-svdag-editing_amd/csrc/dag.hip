@@ -1790,6 +1790,9 @@ static void push_undo_frames(svdag_ctx* ctx, const svdag_edit_stats* st, uint32_
 static int run_edit_stream(svdag_ctx* ctx, const EditorDev* eds, uint32_t n, uint32_t batch, bool collect,
                            svdag_edit_stats* stats, float* totalMs)
 {
+    // A batch past the stream is the whole stream (and keeps the batch
+    // arithmetic below within 32 bits).
+    batch = std::min(batch, n);
     float ms = 0.0f;
     // Batch by batch from edit k0, waiting for each.
     auto serial = [&](uint32_t k0) -> int {
