@@ -1000,7 +1000,8 @@ def test_edit_stream_recreates_what_the_previous_collection_frees(mode):
 
 
 @pytest.mark.gpu
-def test_edit_stream_recovers_when_an_arena_runs_out():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_edit_stream_recovers_when_an_arena_runs_out(mode):
     # Arenas far too small: an edit of the pipelined stream runs out on the
     # device, the stream's later collections release nothing, and the rest
     # runs batch by batch after the arenas grow.  The DAG equals the
@@ -1009,6 +1010,7 @@ def test_edit_stream_recovers_when_an_arena_runs_out():
     levels = L10
     edits = _fusable_stream(levels, 96, seed=29)
     g, o = _pair(levels=levels, expected_items=20000, arena_slabs=64)
+    g.set_batch_mode(mode)
     g.garbage_collect()
     cap0 = g.memory_usage().slab_capacity
     g.edit_stream(edits, batch=8, collect=True)
